@@ -1,0 +1,168 @@
+/*
+ * vision_amd — MI355X-native megakernel path tracer behind Vision's scene
+ * format.  Plain C ABI: opaque handles, caller-owned arrays, integer status
+ * codes and a per-context error string; nothing throws across this boundary.
+ *
+ * The reference's only C ABI is the SDK vtable returned by create()/destroy()
+ * (src/sdk/vision.h:66-99, src/sdk/vision.cpp:222-227).  Each entry point below
+ * names the reference interface it stands in for.
+ */
+#ifndef VISION_AMD_H
+#define VISION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t vs_status;
+#define VS_OK 0
+#define VS_ERROR_INVALID_ARGUMENT 1
+#define VS_ERROR_PARSE 2
+#define VS_ERROR_UNSUPPORTED 3
+#define VS_ERROR_DEVICE 4
+#define VS_ERROR_OUT_OF_MEMORY 5
+#define VS_ERROR_STATE 6
+
+typedef struct vs_context vs_context;
+typedef struct vs_scene vs_scene;
+
+/* 32-byte ray: world origin, t_min, direction (need not be unit), t_max.
+ * Valid hits satisfy t_min < t < t_max.  Mirrors ocarina's Ray as used by
+ * Geometry::trace_closest (src/base/mgr/geometry.cpp:167-173). */
+typedef struct vs_ray {
+    float org[3];
+    float t_min;
+    float dir[3];
+    float t_max;
+} vs_ray;
+
+/* 24-byte hit: TriangleHit{inst_id, prim_id, bary} of the reference
+ * (used at src/base/mgr/geometry.cpp:78-96) plus t and the scene-global
+ * triangle id.  inst == 0xFFFFFFFF marks a miss.  bary weights (v1, v2):
+ * p = (1 - b0 - b1) * v0 + b0 * v1 + b1 * v2. */
+typedef struct vs_hit {
+    uint32_t inst;
+    uint32_t prim;
+    float bary[2];
+    float t;
+    uint32_t tri;
+} vs_hit;
+
+typedef struct vs_scene_info {
+    uint32_t num_triangles;
+    uint32_t num_vertices;
+    uint32_t num_instances;
+    uint32_t num_materials;
+    uint32_t num_lights;
+    uint32_t width;
+    uint32_t height;
+    uint32_t output_spp;
+    uint32_t max_depth;
+    uint32_t spectrum_dim;
+    uint32_t bvh_nodes;
+    uint32_t bvh_depth;
+    uint32_t frame_index;
+    uint32_t accel_built;
+    double bvh_build_ms;
+    double bvh_sah_cost;
+} vs_scene_info;
+
+typedef struct vs_render_stats {
+    double kernel_ms;      /* HIP-event time of the last vs_render call's kernels */
+    uint64_t rays;         /* rays cast (only when counting was enabled) */
+    uint64_t node_fetches; /* BVH8 node fetches (counting runs) */
+    uint64_t tri_fetches;  /* leaf triangle fetches (counting runs) */
+    uint32_t launches;     /* kernel launches of the last call */
+    uint32_t samples_per_pixel;
+} vs_render_stats;
+
+/* Flattened world-space scene handed to tests and the CPU oracle.
+ * All pointers are owned by the scene and stay valid until vs_scene_destroy. */
+typedef struct vs_scene_arrays {
+    const float *positions;     /* [num_vertices][3] */
+    const float *normals;       /* [num_vertices][3], unnormalized, may be 0 */
+    const float *uvs;           /* [num_vertices][2] */
+    const uint32_t *triangles;  /* [num_triangles][3] */
+    const uint32_t *tri_instance;/* [num_triangles] */
+    const uint32_t *instances;  /* [num_instances][4]: material, light, tri_base, 0 */
+    const uint32_t *materials;  /* [num_materials][32]: type, flags, 30 float params */
+    const uint32_t *lights;     /* [num_lights][20]: light records (80 B) */
+    const float *camera;        /* [24]: c2w rows [R|t] (12), screen lower_x, upper_y,
+                                   raster step x, y, tan(fov_y/2), lens_radius,
+                                   focal_distance, width, height, filter (0 box,
+                                   1 triangle), filter_radius, thin_lens flag */
+    const float *render;        /* [8]: max_depth, min_depth, rr_threshold, mis_mode,
+                                   light_sampler(0 uniform,1 power), ray_offset_factor,
+                                   accumulation, exposure */
+    uint32_t num_vertices, num_triangles, num_instances, num_materials, num_lights;
+} vs_scene_arrays;
+
+/* library version (major*10000 + minor*100 + patch) */
+int32_t vs_version(void);
+
+/* Device context on HIP device `device` with its own stream; device = -1
+ * makes a host-only context that can parse, build the BVH on the host and
+ * export, but not render.  Stands in for the SDK's create()
+ * (src/sdk/vision.cpp:222-224) and device selection (src/core/cli_parser.cpp:21-22). */
+vs_status vs_create(int32_t device, vs_context **out);
+void vs_destroy(vs_context *ctx);
+/* last error message of the context ("" if none) */
+const char *vs_last_error(const vs_context *ctx);
+
+/* Parse a Vision scene (JSON text, comments allowed) and resolve it to a
+ * flattened world-space scene.  base_dir resolves relative mesh paths.
+ * Replaces SceneDesc::from_json + Scene::init
+ * (src/core/scene_desc.cpp:37-61, src/base/mgr/scene.cpp:16-34). */
+vs_status vs_scene_load(vs_context *ctx, const char *json_text, const char *base_dir, vs_scene **out);
+vs_status vs_scene_load_file(vs_context *ctx, const char *path, vs_scene **out);
+void vs_scene_destroy(vs_scene *scene);
+
+/* Build the BVH8 (on the host) and, on a device context, upload geometry,
+ * materials, lights and film to HBM.
+ * Replaces Geometry::build_accel + Pipeline upload
+ * (src/base/mgr/geometry.cpp:36-70) and the SDK's build_accel. */
+vs_status vs_scene_build_accel(vs_scene *scene);
+vs_status vs_scene_info_get(const vs_scene *scene, vs_scene_info *out);
+vs_status vs_scene_export(const vs_scene *scene, vs_scene_arrays *out);
+/* The host copy of the BVH8: 80-byte nodes (20 uint32 each) and 48-byte leaf
+ * triangles (12 floats: v0, id bits, v1, 0, v2, 0), valid until destroy. */
+vs_status vs_scene_bvh_export(const vs_scene *scene, const uint32_t **nodes, uint32_t *num_nodes,
+                              const float **leaf_tris, uint32_t *num_leaf_tris);
+
+/* Override film resolution (SDK update_resolution, src/sdk/vision.h:78). */
+vs_status vs_scene_set_resolution(vs_scene *scene, uint32_t width, uint32_t height);
+/* Override max_depth / min_depth / rr_threshold of the integrator. */
+vs_status vs_scene_set_integrator(vs_scene *scene, uint32_t max_depth, uint32_t min_depth, float rr_threshold);
+
+/* Closest-hit / any-hit for a batch of rays in host memory (the parity hook
+ * for Geometry::trace_closest / trace_occlusion, geometry.cpp:167-184). */
+vs_status vs_trace_closest(vs_scene *scene, const vs_ray *rays, uint32_t n, vs_hit *hits);
+vs_status vs_trace_any(vs_scene *scene, const vs_ray *rays, uint32_t n, uint8_t *occluded);
+/* Same on device-resident buffers, enqueued on the context stream (or on
+ * `stream` when non-null, a hipStream_t); returns without synchronizing. */
+vs_status vs_trace_closest_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, vs_hit *d_hits, void *stream);
+vs_status vs_trace_any_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, uint8_t *d_occluded, void *stream);
+
+/* Add `spp` progressive frames to the film (frame indices continue from the
+ * current one).  fused = 0: one launch per frame, as
+ * PathTracingIntegrator::render does (src/render_core/integrators/pt.cpp:90-99);
+ * fused = 1: one launch adding all frames with the running mean in registers.
+ * Both produce the same film.  Synchronizes before returning. */
+vs_status vs_render(vs_scene *scene, uint32_t spp, int32_t fused);
+/* Enable per-ray counters (rays, node and triangle fetches) for later renders. */
+vs_status vs_set_counting(vs_scene *scene, int32_t enable);
+vs_status vs_last_render_stats(const vs_scene *scene, vs_render_stats *out);
+/* Reset frame index and film (Integrator::invalidation, src/base/integrator.cpp:13-19). */
+vs_status vs_reset_film(vs_scene *scene);
+/* Download the film: which = 0 running-mean radiance buffer (rt_buffer),
+ * 1 exposure/tone-mapped output buffer (normal.cpp:87-101); rgba float[W*H*4]. */
+vs_status vs_film_download(vs_scene *scene, int32_t which, float *rgba);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VISION_AMD_H */

@@ -1,0 +1,261 @@
+"""vision_amd — MI355X-native megakernel path tracer behind Vision's scene format.
+
+Python mirror of the C ABI in include/vision_amd.h (ctypes over
+libvision_amd.so).  The flow follows the reference's pipeline
+(src/base/mgr/pipeline.cpp): load a scene JSON, prepare (BVH build + upload),
+render progressive frames, read the film.
+
+The HIP library is required: importing works without it, but every call
+raises ``LibraryMissing`` when libvision_amd.so has not been built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import json
+import os
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIB_PATH = _PKG / "libvision_amd.so"
+
+VS_OK = 0
+STATUS_NAMES = {
+    0: "ok", 1: "invalid argument", 2: "parse error", 3: "unsupported", 4: "device error",
+    5: "out of memory", 6: "invalid state",
+}
+
+
+class LibraryMissing(RuntimeError):
+    pass
+
+
+class VisionError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(status, status)}: {message}")
+        self.status = status
+
+
+class Ray(C.Structure):
+    _fields_ = [("org", C.c_float * 3), ("t_min", C.c_float), ("dir", C.c_float * 3), ("t_max", C.c_float)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("inst", C.c_uint32), ("prim", C.c_uint32), ("bary", C.c_float * 2), ("t", C.c_float),
+                ("tri", C.c_uint32)]
+
+
+RAY_DTYPE = np.dtype([("org", "<f4", 3), ("t_min", "<f4"), ("dir", "<f4", 3), ("t_max", "<f4")])
+HIT_DTYPE = np.dtype([("inst", "<u4"), ("prim", "<u4"), ("bary", "<f4", 2), ("t", "<f4"), ("tri", "<u4")])
+
+
+class SceneInfo(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in (
+        "num_triangles", "num_vertices", "num_instances", "num_materials", "num_lights", "width", "height",
+        "output_spp", "max_depth", "spectrum_dim", "bvh_nodes", "bvh_depth", "frame_index", "accel_built")] + [
+        ("bvh_build_ms", C.c_double), ("bvh_sah_cost", C.c_double)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("kernel_ms", C.c_double), ("rays", C.c_uint64), ("node_fetches", C.c_uint64),
+                ("tri_fetches", C.c_uint64), ("launches", C.c_uint32), ("samples_per_pixel", C.c_uint32)]
+
+
+class SceneArrays(C.Structure):
+    _fields_ = [("positions", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
+                ("uvs", C.POINTER(C.c_float)), ("triangles", C.POINTER(C.c_uint32)),
+                ("tri_instance", C.POINTER(C.c_uint32)), ("instances", C.POINTER(C.c_uint32)),
+                ("materials", C.POINTER(C.c_uint32)), ("lights", C.POINTER(C.c_uint32)),
+                ("camera", C.POINTER(C.c_float)), ("render", C.POINTER(C.c_float)),
+                ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_instances", C.c_uint32),
+                ("num_materials", C.c_uint32), ("num_lights", C.c_uint32)]
+
+
+# every entry point of include/vision_amd.h: name -> (restype, argtypes)
+_P = C.c_void_p
+SIGNATURES = {
+    "vs_version": (C.c_int32, []),
+    "vs_create": (C.c_int32, [C.c_int32, C.POINTER(_P)]),
+    "vs_destroy": (None, [_P]),
+    "vs_last_error": (C.c_char_p, [_P]),
+    "vs_scene_load": (C.c_int32, [_P, C.c_char_p, C.c_char_p, C.POINTER(_P)]),
+    "vs_scene_load_file": (C.c_int32, [_P, C.c_char_p, C.POINTER(_P)]),
+    "vs_scene_destroy": (None, [_P]),
+    "vs_scene_build_accel": (C.c_int32, [_P]),
+    "vs_scene_info_get": (C.c_int32, [_P, C.POINTER(SceneInfo)]),
+    "vs_scene_export": (C.c_int32, [_P, C.POINTER(SceneArrays)]),
+    "vs_scene_bvh_export": (C.c_int32, [_P, C.POINTER(C.POINTER(C.c_uint32)), C.POINTER(C.c_uint32),
+                                        C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32)]),
+    "vs_scene_set_resolution": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    "vs_scene_set_integrator": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_float]),
+    "vs_trace_closest": (C.c_int32, [_P, _P, C.c_uint32, _P]),
+    "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
+    "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
+    "vs_trace_any_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
+    "vs_render": (C.c_int32, [_P, C.c_uint32, C.c_int32]),
+    "vs_set_counting": (C.c_int32, [_P, C.c_int32]),
+    "vs_last_render_stats": (C.c_int32, [_P, C.POINTER(RenderStats)]),
+    "vs_reset_film": (C.c_int32, [_P]),
+    "vs_film_download": (C.c_int32, [_P, C.c_int32, _P]),
+}
+
+_lib_handle: Optional[C.CDLL] = None
+
+
+def lib() -> C.CDLL:
+    """Load libvision_amd.so (built in-tree by vision_amd/build.py)."""
+    global _lib_handle
+    if _lib_handle is None:
+        if not LIB_PATH.exists():
+            raise LibraryMissing(f"{LIB_PATH} is not built; run `python -m vision_amd.build`")
+        h = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(h, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib_handle = h
+    return _lib_handle
+
+
+class Context:
+    """A HIP device context (device=-1: host-only, for parsing and export)."""
+
+    def __init__(self, device: int = 0):
+        self._h = _P()
+        st = lib().vs_create(device, C.byref(self._h))
+        if st != VS_OK:
+            raise VisionError(st, f"vs_create(device={device}) failed")
+        self.device = device
+
+    def _check(self, st: int):
+        if st != VS_OK:
+            raise VisionError(st, lib().vs_last_error(self._h).decode())
+
+    def load(self, path: str | os.PathLike) -> "Scene":
+        s = _P()
+        self._check(lib().vs_scene_load_file(self._h, str(path).encode(), C.byref(s)))
+        return Scene(self, s)
+
+    def load_text(self, text: str | dict, base_dir: str = ".") -> "Scene":
+        if isinstance(text, dict):
+            text = json.dumps(text)
+        s = _P()
+        self._check(lib().vs_scene_load(self._h, text.encode(), str(base_dir).encode(), C.byref(s)))
+        return Scene(self, s)
+
+    def close(self):
+        if self._h:
+            lib().vs_destroy(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Scene:
+    """A loaded scene (the reference's Scene + Pipeline for one render)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._h = handle
+
+    def close(self):
+        if self._h:
+            lib().vs_scene_destroy(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- preparation
+    def build_accel(self) -> "Scene":
+        self.ctx._check(lib().vs_scene_build_accel(self._h))
+        return self
+
+    def info(self) -> dict:
+        i = SceneInfo()
+        self.ctx._check(lib().vs_scene_info_get(self._h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in SceneInfo._fields_}
+
+    def set_resolution(self, w: int, h: int):
+        self.ctx._check(lib().vs_scene_set_resolution(self._h, w, h))
+
+    def set_integrator(self, max_depth: int, min_depth: int, rr_threshold: float):
+        self.ctx._check(lib().vs_scene_set_integrator(self._h, max_depth, min_depth, rr_threshold))
+
+    def export(self) -> dict:
+        """Flattened scene arrays (copies) — the input of the CPU oracle."""
+        a = SceneArrays()
+        self.ctx._check(lib().vs_scene_export(self._h, C.byref(a)))
+
+        def arr(ptr, n, dtype):
+            if n == 0:
+                return np.zeros(0, dtype)
+            return np.ctypeslib.as_array(ptr, shape=(n,)).astype(dtype, copy=True)
+
+        return {
+            "positions": arr(a.positions, 3 * a.num_vertices, np.float32).reshape(-1, 3),
+            "normals": arr(a.normals, 3 * a.num_vertices, np.float32).reshape(-1, 3),
+            "uvs": arr(a.uvs, 2 * a.num_vertices, np.float32).reshape(-1, 2),
+            "triangles": arr(a.triangles, 3 * a.num_triangles, np.uint32).reshape(-1, 3),
+            "tri_instance": arr(a.tri_instance, a.num_triangles, np.uint32),
+            "instances": arr(a.instances, 4 * a.num_instances, np.uint32).reshape(-1, 4),
+            "materials": arr(a.materials, 32 * a.num_materials, np.uint32).reshape(-1, 32),
+            "lights": arr(a.lights, 20 * a.num_lights, np.uint32).reshape(-1, 20),
+            "camera": arr(a.camera, 24, np.float32),
+            "render": arr(a.render, 8, np.float32),
+        }
+
+    def bvh(self) -> tuple[np.ndarray, np.ndarray]:
+        """Host copy of the BVH8: (nodes uint32[N,20], leaf triangles float32[M,12])."""
+        pn = C.POINTER(C.c_uint32)()
+        pt = C.POINTER(C.c_float)()
+        nn = C.c_uint32()
+        nt = C.c_uint32()
+        self.ctx._check(lib().vs_scene_bvh_export(self._h, C.byref(pn), C.byref(nn), C.byref(pt), C.byref(nt)))
+        nodes = np.ctypeslib.as_array(pn, shape=(nn.value * 20,)).copy().reshape(-1, 20)
+        tris = np.ctypeslib.as_array(pt, shape=(nt.value * 12,)).copy().reshape(-1, 12)
+        return nodes, tris
+
+    # -- ray queries (Geometry::trace_closest / trace_occlusion)
+    def trace_closest(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        self.ctx._check(lib().vs_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
+        return hits
+
+    def trace_any(self, rays: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        occ = np.zeros(len(rays), dtype=np.uint8)
+        self.ctx._check(lib().vs_trace_any(self._h, rays.ctypes.data, len(rays), occ.ctypes.data))
+        return occ
+
+    # -- rendering (PathTracingIntegrator::render, one frame = one spp)
+    def render(self, spp: int, fused: bool = True) -> dict:
+        self.ctx._check(lib().vs_render(self._h, spp, 1 if fused else 0))
+        return self.stats()
+
+    def set_counting(self, on: bool):
+        self.ctx._check(lib().vs_set_counting(self._h, 1 if on else 0))
+
+    def stats(self) -> dict:
+        s = RenderStats()
+        self.ctx._check(lib().vs_last_render_stats(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in RenderStats._fields_}
+
+    def reset_film(self):
+        self.ctx._check(lib().vs_reset_film(self._h))
+
+    def film(self, which: int = 0) -> np.ndarray:
+        info = self.info()
+        out = np.zeros((info["height"], info["width"], 4), dtype=np.float32)
+        self.ctx._check(lib().vs_film_download(self._h, which, out.ctypes.data))
+        return out

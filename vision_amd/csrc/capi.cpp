@@ -1,0 +1,550 @@
+// extern "C" boundary of libvision_amd.so (declared in include/vision_amd.h).
+#include "../../include/vision_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "dev_scene.h"
+#include "launch.h"
+#include "scene.h"
+#include "tables.h"
+
+struct vs_context {
+    int device{0};// -1: host-only context (scene parsing, BVH build, export)
+    hipStream_t stream{nullptr};
+    hipEvent_t ev0{nullptr}, ev1{nullptr};
+    std::string last_error;
+};
+
+namespace {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevBuf {
+    void *ptr{nullptr};
+    size_t bytes{0};
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw DeviceError(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <typename T>
+    void upload(const std::vector<T> &v, hipStream_t s) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) hip_check(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+}// namespace
+
+struct vs_scene {
+    vs_context *ctx{nullptr};
+    vs::SceneHost host;
+    vs::Bvh8 bvh;
+    bool host_bvh{false};// BVH built on the host
+    bool built{false};   // BVH and scene resident in HBM
+    double build_ms{0.0};
+    std::vector<vs::MaterialRecord> mats;
+    vs::LightTables lights;
+    // device buffers
+    DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, mat, light, alias, func;
+    DevBuf film_rt, film_out, counters;
+    DevBuf scratch_rays, scratch_hits;
+    vs::SceneView view{};
+    vs::CameraView cam{};
+    uint32_t frame_index{0};
+    bool counting{false};
+    vs_render_stats stats{};
+    // exported flat arrays
+    std::vector<uint32_t> exp_inst;
+    std::vector<float> exp_camera, exp_render;
+};
+
+namespace {
+
+vs_status fail(vs_context *ctx, vs_status code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+template <typename F>
+vs_status guarded(vs_context *ctx, F &&f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return VS_OK;
+    } catch (const vs::JsonError &e) {
+        return fail(ctx, VS_ERROR_PARSE, e.what());
+    } catch (const DeviceError &e) {
+        return fail(ctx, VS_ERROR_DEVICE, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, VS_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::invalid_argument &e) {
+        return fail(ctx, VS_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::logic_error &e) {
+        return fail(ctx, VS_ERROR_STATE, e.what());
+    } catch (const std::exception &e) {
+        std::string m = e.what();
+        vs_status code = (m.find("unsupported") != std::string::npos || m.find("not supported") != std::string::npos)
+                             ? VS_ERROR_UNSUPPORTED
+                             : VS_ERROR_INVALID_ARGUMENT;
+        return fail(ctx, code, m);
+    }
+}
+
+void fill_camera(vs_scene *s) {
+    const vs::CameraHost &c = s->host.camera;
+    vs::CameraView &v = s->cam;
+    // c2w rows [R | t]
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 4; ++col) v.c2w[r * 4 + col] = vs::m4_at(c.c2w, r, col);
+    // RadianceCollector::update_screen_window + Sensor::update_resolution_
+    float ratio = float(c.width) / float(c.height);
+    float lower_x = -1.f, upper_x = 1.f, lower_y = -1.f, upper_y = 1.f;
+    if (ratio > 1.f) {
+        lower_x = -ratio;
+        upper_x = ratio;
+    } else {
+        lower_y = -1.f / ratio;
+        upper_y = 1.f / ratio;
+    }
+    v.lower_x = lower_x;
+    v.upper_y = upper_y;
+    v.raster_sx = (upper_x - lower_x) / float(c.width);
+    v.raster_sy = (upper_y - lower_y) / float(c.height);
+    v.tan_half_fov = std::tan(c.fov_y * 0.5f * 3.14159265358979323846f / 180.f);
+    v.lens_radius = c.lens_radius;
+    v.focal_distance = c.focal_distance;
+    v.width = c.width;
+    v.height = c.height;
+    v.filter = c.filter_type == "box" ? 0u : 1u;
+    v.filter_radius = c.filter_radius;
+    v.thin_lens = c.type == "thin_lens" ? 1u : 0u;
+    s->exp_camera.assign(v.c2w, v.c2w + 12);
+    float rest[12] = {v.lower_x, v.upper_y, v.raster_sx, v.raster_sy, v.tan_half_fov, v.lens_radius,
+                      v.focal_distance, float(v.width), float(v.height), float(v.filter), v.filter_radius,
+                      float(v.thin_lens)};
+    s->exp_camera.insert(s->exp_camera.end(), rest, rest + 12);
+}
+
+void fill_render_export(vs_scene *s) {
+    const vs::SceneHost &h = s->host;
+    s->exp_render = {float(h.integrator.max_depth), float(h.integrator.min_depth), h.integrator.rr_threshold,
+                     float(h.integrator.mis_mode), h.light_sampler_type == "power" ? 1.f : 0.f, h.ray_offset_factor,
+                     float(h.camera.accumulation), h.camera.exposure};
+}
+
+void alloc_film(vs_scene *s) {
+    size_t px = size_t(s->host.camera.width) * s->host.camera.height;
+    s->film_rt.alloc(px * 16);
+    s->film_out.alloc(px * 16);
+    hip_check(hipMemsetAsync(s->film_rt.ptr, 0, px * 16, s->ctx->stream), "film clear");
+    hip_check(hipMemsetAsync(s->film_out.ptr, 0, px * 16, s->ctx->stream), "film clear");
+    s->frame_index = 0;
+}
+
+void require_built(vs_scene *s) {
+    if (s->ctx->device < 0) throw std::logic_error("host-only context: no device work possible");
+    if (!s->built) throw std::logic_error("accel not built: call vs_scene_build_accel first");
+}
+
+}// namespace
+
+extern "C" {
+
+int32_t vs_version(void) { return 100; }
+
+vs_status vs_create(int32_t device, vs_context **out) {
+    if (!out) return VS_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    if (device == -1) {
+        auto *host_ctx = new (std::nothrow) vs_context();
+        if (!host_ctx) return VS_ERROR_OUT_OF_MEMORY;
+        host_ctx->device = -1;
+        *out = host_ctx;
+        return VS_OK;
+    }
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return VS_ERROR_DEVICE;
+    if (device < 0 || device >= count) return VS_ERROR_INVALID_ARGUMENT;
+    auto *ctx = new (std::nothrow) vs_context();
+    if (!ctx) return VS_ERROR_OUT_OF_MEMORY;
+    ctx->device = device;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        vs_destroy(ctx);
+        return VS_ERROR_DEVICE;
+    }
+    *out = ctx;
+    return VS_OK;
+}
+
+void vs_destroy(vs_context *ctx) {
+    if (!ctx) return;
+    if (ctx->device < 0) {
+        delete ctx;
+        return;
+    }
+    (void)hipSetDevice(ctx->device);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *vs_last_error(const vs_context *ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+vs_status vs_scene_load(vs_context *ctx, const char *json_text, const char *base_dir, vs_scene **out) {
+    if (!ctx || !json_text || !out) return fail(ctx, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    return guarded(ctx, [&] {
+        auto s = std::make_unique<vs_scene>();
+        s->ctx = ctx;
+        s->host = vs::load_scene(json_text, base_dir ? base_dir : "");
+        s->mats = vs::pack_materials(s->host);
+        s->lights = vs::pack_lights(s->host);
+        fill_camera(s.get());
+        fill_render_export(s.get());
+        for (const auto &ir : s->host.instRecords) {
+            s->exp_inst.push_back(ir.mat);
+            s->exp_inst.push_back(ir.light);
+            s->exp_inst.push_back(ir.tri_base);
+            s->exp_inst.push_back(0);
+        }
+        *out = s.release();
+    });
+}
+
+vs_status vs_scene_load_file(vs_context *ctx, const char *path, vs_scene **out) {
+    if (!ctx || !path || !out) return fail(ctx, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    std::string text;
+    vs_status st = guarded(ctx, [&] { text = vs::read_text_file(path); });
+    if (st != VS_OK) return st;
+    std::string p(path);
+    size_t slash = p.find_last_of('/');
+    std::string dir = slash == std::string::npos ? "." : p.substr(0, slash);
+    return vs_scene_load(ctx, text.c_str(), dir.c_str(), out);
+}
+
+void vs_scene_destroy(vs_scene *scene) {
+    if (!scene) return;
+    if (scene->ctx && scene->ctx->device >= 0) {
+        (void)hipSetDevice(scene->ctx->device);
+        (void)hipStreamSynchronize(scene->ctx->stream);
+    }
+    delete scene;
+}
+
+vs_status vs_scene_build_accel(vs_scene *s) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        auto t0 = std::chrono::steady_clock::now();
+        s->bvh = vs::build_bvh8(s->host.wP, s->host.wTri);
+        auto t1 = std::chrono::steady_clock::now();
+        s->build_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+        if (s->bvh.max_depth > 64) throw std::runtime_error("bvh deeper than the 64-entry traversal stack");
+        s->host_bvh = true;
+        if (s->ctx->device < 0) return;// host-only context: no upload
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hipStream_t st = s->ctx->stream;
+        const vs::SceneHost &h = s->host;
+        s->nodes.upload(s->bvh.nodes, st);
+        s->ltris.upload(s->bvh.tris, st);
+        s->P.upload(h.wP, st);
+        s->N.upload(h.wN, st);
+        s->UV.upload(h.wUV, st);
+        s->tri.upload(h.wTri, st);
+        s->tri_inst.upload(h.triInst, st);
+        s->inst.upload(h.instRecords, st);
+        s->mat.upload(s->mats, st);
+        s->light.upload(s->lights.records, st);
+        s->alias.upload(s->lights.alias, st);
+        s->func.upload(s->lights.func, st);
+        s->counters.alloc(64);
+        alloc_film(s);
+        hip_check(hipStreamSynchronize(st), "upload sync");
+        vs::SceneView &v = s->view;
+        v.nodes = s->nodes.as<uint32_t>();
+        v.leaf_tris = s->ltris.as<float>();
+        v.P = s->P.as<vs::f3>();
+        v.N = s->N.as<vs::f3>();
+        v.UV = s->UV.as<vs::f2>();
+        v.tri = s->tri.as<vs::u3>();
+        v.tri_inst = s->tri_inst.as<uint32_t>();
+        v.inst = s->inst.as<vs::InstanceRecord>();
+        v.mat = s->mat.as<vs::MaterialRecord>();
+        v.lights = s->light.as<vs::LightRecord>();
+        v.alias = s->alias.as<vs::AliasEntry>();
+        v.alias_func = s->func.as<float>();
+        v.num_lights = uint32_t(h.lights.size());
+        v.light_sampler = h.light_sampler_type == "power" ? 1u : 0u;
+        v.select_offset = s->lights.select_offset;
+        v.select_integral = s->lights.select_integral;
+        v.stack_size = s->bvh.max_depth;
+        v.ray_offset_factor = h.ray_offset_factor;
+        v.num_tris = uint32_t(h.wTri.size());
+        s->built = true;
+    });
+}
+
+vs_status vs_scene_info_get(const vs_scene *s, vs_scene_info *out) {
+    if (!s || !out) return VS_ERROR_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof(*out));
+    const vs::SceneHost &h = s->host;
+    out->num_triangles = uint32_t(h.wTri.size());
+    out->num_vertices = uint32_t(h.wP.size());
+    out->num_instances = uint32_t(h.instances.size());
+    out->num_materials = uint32_t(h.materials.size());
+    out->num_lights = uint32_t(h.lights.size());
+    out->width = h.camera.width;
+    out->height = h.camera.height;
+    out->output_spp = h.output_spp;
+    out->max_depth = h.integrator.max_depth;
+    out->spectrum_dim = h.spectrum_dim;
+    out->bvh_nodes = uint32_t(s->bvh.nodes.size());
+    out->bvh_depth = s->bvh.max_depth;
+    out->frame_index = s->frame_index;
+    out->accel_built = s->built ? 1u : 0u;
+    out->bvh_build_ms = s->build_ms;
+    out->bvh_sah_cost = s->bvh.sah_cost;
+    return VS_OK;
+}
+
+vs_status vs_scene_export(const vs_scene *s, vs_scene_arrays *out) {
+    if (!s || !out) return VS_ERROR_INVALID_ARGUMENT;
+    const vs::SceneHost &h = s->host;
+    out->positions = reinterpret_cast<const float *>(h.wP.data());
+    out->normals = reinterpret_cast<const float *>(h.wN.data());
+    out->uvs = reinterpret_cast<const float *>(h.wUV.data());
+    out->triangles = reinterpret_cast<const uint32_t *>(h.wTri.data());
+    out->tri_instance = h.triInst.data();
+    out->instances = s->exp_inst.data();
+    out->materials = reinterpret_cast<const uint32_t *>(s->mats.data());
+    out->lights = reinterpret_cast<const uint32_t *>(s->lights.records.data());
+    out->camera = s->exp_camera.data();
+    out->render = s->exp_render.data();
+    out->num_vertices = uint32_t(h.wP.size());
+    out->num_triangles = uint32_t(h.wTri.size());
+    out->num_instances = uint32_t(h.instances.size());
+    out->num_materials = uint32_t(s->mats.size());
+    out->num_lights = uint32_t(s->lights.records.size());
+    return VS_OK;
+}
+
+vs_status vs_scene_bvh_export(const vs_scene *s, const uint32_t **nodes, uint32_t *num_nodes,
+                              const float **leaf_tris, uint32_t *num_leaf_tris) {
+    if (!s || !nodes || !num_nodes || !leaf_tris || !num_leaf_tris) return VS_ERROR_INVALID_ARGUMENT;
+    if (!s->host_bvh) return fail(s->ctx, VS_ERROR_STATE, "bvh not built");
+    *nodes = reinterpret_cast<const uint32_t *>(s->bvh.nodes.data());
+    *num_nodes = uint32_t(s->bvh.nodes.size());
+    *leaf_tris = reinterpret_cast<const float *>(s->bvh.tris.data());
+    *num_leaf_tris = uint32_t(s->bvh.tris.size());
+    return VS_OK;
+}
+
+vs_status vs_scene_set_resolution(vs_scene *s, uint32_t width, uint32_t height) {
+    if (!s || width == 0 || height == 0 || width > 16384 || height > 16384)
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "bad resolution");
+    return guarded(s->ctx, [&] {
+        s->host.camera.width = width;
+        s->host.camera.height = height;
+        fill_camera(s);
+        if (s->built && s->ctx->device >= 0) {
+            hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+            alloc_film(s);
+            hip_check(hipStreamSynchronize(s->ctx->stream), "film sync");
+        }
+    });
+}
+
+vs_status vs_scene_set_integrator(vs_scene *s, uint32_t max_depth, uint32_t min_depth, float rr_threshold) {
+    if (!s || max_depth > 1024) return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "bad integrator settings");
+    s->host.integrator.max_depth = max_depth;
+    s->host.integrator.min_depth = min_depth;
+    s->host.integrator.rr_threshold = rr_threshold;
+    fill_render_export(s);
+    return VS_OK;
+}
+
+static vs_status trace_host(vs_scene *s, const vs_ray *rays, uint32_t n, vs_hit *hits, uint8_t *occ, bool any) {
+    if (!s || (n && !rays) || (n && !any && !hits) || (n && any && !occ))
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (n == 0) return;
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hipStream_t st = s->ctx->stream;
+        DevBuf dr, dh;
+        dr.alloc(size_t(n) * sizeof(vs::RayRec));
+        dh.alloc(size_t(n) * (any ? 1 : sizeof(vs::HitRec)));
+        hip_check(hipMemcpyAsync(dr.ptr, rays, size_t(n) * sizeof(vs::RayRec), hipMemcpyHostToDevice, st), "rays H2D");
+        hip_check(vs::launch_trace(s->view, dr.as<vs::RayRec>(), any ? nullptr : dh.as<vs::HitRec>(),
+                                   any ? dh.as<uint8_t>() : nullptr, n, any, nullptr, st),
+                  "trace launch");
+        if (any) hip_check(hipMemcpyAsync(occ, dh.ptr, n, hipMemcpyDeviceToHost, st), "occ D2H");
+        else hip_check(hipMemcpyAsync(hits, dh.ptr, size_t(n) * sizeof(vs::HitRec), hipMemcpyDeviceToHost, st), "hits D2H");
+        hip_check(hipStreamSynchronize(st), "trace sync");
+    });
+}
+
+vs_status vs_trace_closest(vs_scene *s, const vs_ray *rays, uint32_t n, vs_hit *hits) {
+    static_assert(sizeof(vs_ray) == sizeof(vs::RayRec), "ray layout");
+    static_assert(sizeof(vs_hit) == sizeof(vs::HitRec), "hit layout");
+    return trace_host(s, rays, n, hits, nullptr, false);
+}
+
+vs_status vs_trace_any(vs_scene *s, const vs_ray *rays, uint32_t n, uint8_t *occluded) {
+    return trace_host(s, rays, n, nullptr, occluded, true);
+}
+
+static vs_status trace_device(vs_scene *s, const vs_ray *rays, uint32_t n, vs_hit *hits, uint8_t *occ, bool any,
+                              void *stream) {
+    if (!s || (n && !rays) || (n && !any && !hits) || (n && any && !occ))
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hipStream_t st = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+        unsigned long long *ctr = nullptr;
+        if (s->counting) {
+            ctr = s->counters.as<unsigned long long>();
+            hip_check(hipMemsetAsync(ctr, 0, 64, st), "counter clear");
+        }
+        hip_check(vs::launch_trace(s->view, reinterpret_cast<const vs::RayRec *>(rays),
+                                   reinterpret_cast<vs::HitRec *>(hits), occ, n, any, ctr, st),
+                  "trace launch");
+    });
+}
+
+vs_status vs_trace_closest_device(vs_scene *s, const vs_ray *d_rays, uint32_t n, vs_hit *d_hits, void *stream) {
+    return trace_device(s, d_rays, n, d_hits, nullptr, false, stream);
+}
+
+vs_status vs_trace_any_device(vs_scene *s, const vs_ray *d_rays, uint32_t n, uint8_t *d_occ, void *stream) {
+    return trace_device(s, d_rays, n, nullptr, d_occ, true, stream);
+}
+
+vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (spp == 0) return;
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hipStream_t st = s->ctx->stream;
+        const vs::SceneHost &h = s->host;
+        vs::RenderView rv{};
+        rv.max_depth = h.integrator.max_depth;
+        rv.min_depth = h.integrator.min_depth;
+        rv.rr_threshold = h.integrator.rr_threshold;
+        rv.mis_mode = h.integrator.mis_mode;
+        rv.accumulation = h.camera.accumulation;
+        rv.exposure = h.camera.exposure;
+        rv.tone_mapper = h.camera.tone_mapper == "aces" ? 1u : (h.camera.tone_mapper == "reinhard" ? 2u : 0u);
+        rv.write_output = 1;
+        unsigned long long *ctr = nullptr;
+        if (s->counting) {
+            ctr = s->counters.as<unsigned long long>();
+            hip_check(hipMemsetAsync(ctr, 0, 64, st), "counter clear");
+        }
+        uint32_t launches = 0;
+        hip_check(hipEventRecord(s->ctx->ev0, st), "event");
+        if (fused) {
+            rv.frame_start = s->frame_index;
+            rv.spp = spp;
+            hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr, st),
+                      "pt launch");
+            ++launches;
+        } else {
+            for (uint32_t f = 0; f < spp; ++f) {
+                rv.frame_start = s->frame_index + f;
+                rv.spp = 1;
+                hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr, st),
+                          "pt launch");
+                ++launches;
+            }
+        }
+        hip_check(hipEventRecord(s->ctx->ev1, st), "event");
+        hip_check(hipEventSynchronize(s->ctx->ev1), "render sync");
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1), "event time");
+        s->frame_index += spp;
+        s->stats = vs_render_stats{};
+        s->stats.kernel_ms = ms;
+        s->stats.launches = launches;
+        s->stats.samples_per_pixel = spp;
+        if (ctr) {
+            unsigned long long c[3];
+            hip_check(hipMemcpy(c, ctr, sizeof(c), hipMemcpyDeviceToHost), "counters D2H");
+            s->stats.rays = c[0];
+            s->stats.node_fetches = c[1];
+            s->stats.tri_fetches = c[2];
+        }
+    });
+}
+
+vs_status vs_set_counting(vs_scene *s, int32_t enable) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    s->counting = enable != 0;
+    return VS_OK;
+}
+
+vs_status vs_last_render_stats(const vs_scene *s, vs_render_stats *out) {
+    if (!s || !out) return VS_ERROR_INVALID_ARGUMENT;
+    *out = s->stats;
+    return VS_OK;
+}
+
+vs_status vs_reset_film(vs_scene *s) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        alloc_film(s);
+        hip_check(hipStreamSynchronize(s->ctx->stream), "film sync");
+    });
+}
+
+vs_status vs_film_download(vs_scene *s, int32_t which, float *rgba) {
+    if (!s || !rgba || (which != 0 && which != 1)) return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        size_t bytes = size_t(s->host.camera.width) * s->host.camera.height * 16;
+        const DevBuf &b = which == 0 ? s->film_rt : s->film_out;
+        hip_check(hipMemcpyAsync(rgba, b.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream), "film D2H");
+        hip_check(hipStreamSynchronize(s->ctx->stream), "film sync");
+    });
+}
+
+}// extern "C"

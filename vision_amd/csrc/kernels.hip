@@ -1,0 +1,118 @@
+// HIP kernels for gfx950: BVH8 trace entry points and the PT megakernel.
+#include <hip/hip_runtime.h>
+
+#include "dev_pt.h"
+#include "launch.h"
+
+namespace vs {
+
+using namespace dev;
+
+template <bool ANY>
+__global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *__restrict__ rays,
+                                                     HitRec *__restrict__ hits, uint8_t *__restrict__ occ,
+                                                     uint32_t n, unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    RayRec r = rays[i];
+    uint32_t nf = 0, tf = 0;
+    TraceHit h = traverse<ANY>(sc, mk3(r.o[0], r.o[1], r.o[2]), mk3(r.d[0], r.d[1], r.d[2]), r.tmin, r.tmax,
+                               lds_stack + threadIdx.x, blockDim.x, nf, tf);
+    if (ANY) {
+        occ[i] = h.tri != kInvalid ? 1 : 0;
+    } else {
+        HitRec out;
+        if (h.tri == kInvalid) {
+            out.inst = kInvalid;
+            out.prim = kInvalid;
+            out.bary[0] = out.bary[1] = 0.f;
+            out.t = r.tmax;
+            out.tri = kInvalid;
+        } else {
+            uint32_t inst = sc.tri_inst[h.tri];
+            out.inst = inst;
+            out.prim = h.tri - sc.inst[inst].tri_base;
+            out.bary[0] = h.b1;
+            out.bary[1] = h.b2;
+            out.t = h.t;
+            out.tri = h.tri;
+        }
+        hits[i] = out;
+    }
+    if (counters) {
+        atomicAdd(counters + 0, 1ull);
+        atomicAdd(counters + 1, (unsigned long long)nf);
+        atomicAdd(counters + 2, (unsigned long long)tf);
+    }
+}
+
+// one thread per pixel; all `spp` frames of the launch accumulate in registers
+// with the film's running-mean arithmetic (normal.cpp:87-101)
+__global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
+                                                  float4 *__restrict__ rt, float4 *__restrict__ out,
+                                                  unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= cam.width || py >= cam.height) return;
+    uint32_t idx = py * cam.width + px;
+    float4 acc = rt[idx];
+    PathCounters pc{0, 0, 0};
+    for (uint32_t s = 0; s < rv.spp; ++s) {
+        uint32_t frame = rv.frame_start + s;
+        Sampler sampler;
+        sampler.start(px, py, frame, 0u);
+        f3 o, d;
+        float fw;
+        camera_ray(cam, px, py, sampler, o, d, fw);
+        f3 L = trace_path(sc, rv, o, d, sampler, lds_stack + threadIdx.x, blockDim.x, pc) * fw;
+        float4 v = make_float4(zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f);
+        if (rv.accumulation) {
+            float a = 1.f / float(frame + 1u);
+            acc = make_float4(acc.x + a * (v.x - acc.x), acc.y + a * (v.y - acc.y), acc.z + a * (v.z - acc.z),
+                              acc.w + a * (v.w - acc.w));
+        } else {
+            acc = v;
+        }
+    }
+    rt[idx] = acc;
+    if (rv.write_output) {
+        float4 o4;
+        o4.x = tone_map(1.f - expf(-acc.x * rv.exposure), rv.tone_mapper);
+        o4.y = tone_map(1.f - expf(-acc.y * rv.exposure), rv.tone_mapper);
+        o4.z = tone_map(1.f - expf(-acc.z * rv.exposure), rv.tone_mapper);
+        o4.w = 1.f;
+        out[idx] = o4;
+    }
+    if (counters) {
+        atomicAdd(counters + 0, (unsigned long long)pc.rays);
+        atomicAdd(counters + 1, (unsigned long long)pc.nodes);
+        atomicAdd(counters + 2, (unsigned long long)pc.tris);
+    }
+}
+
+hipError_t launch_trace(const SceneView &sc, const RayRec *rays, HitRec *hits, uint8_t *occ, uint32_t n, bool any,
+                        unsigned long long *counters, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const uint32_t block = 128;
+    size_t lds = size_t(sc.stack_size) * block * sizeof(uint2);
+    dim3 grid((n + block - 1) / block);
+    if (any) {
+        hipLaunchKernelGGL(trace_kernel<true>, grid, dim3(block), lds, stream, sc, rays, hits, occ, n, counters);
+    } else {
+        hipLaunchKernelGGL(trace_kernel<false>, grid, dim3(block), lds, stream, sc, rays, hits, occ, n, counters);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderView &rv, float4 *rt, float4 *out,
+                     unsigned long long *counters, hipStream_t stream) {
+    dim3 block(64);
+    dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8);
+    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+    hipLaunchKernelGGL(pt_kernel, grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+    return hipGetLastError();
+}
+
+}// namespace vs

@@ -1,0 +1,16 @@
+// Host-callable launchers for the kernels in kernels.hip.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "dev_scene.h"
+
+namespace vs {
+
+hipError_t launch_trace(const SceneView &sc, const RayRec *rays, HitRec *hits, uint8_t *occ, uint32_t n, bool any,
+                        unsigned long long *counters, hipStream_t stream);
+
+hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderView &rv, float4 *rt, float4 *out,
+                     unsigned long long *counters, hipStream_t stream);
+
+}// namespace vs

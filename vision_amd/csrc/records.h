@@ -1,0 +1,64 @@
+// Plain-data records laid out identically on host and device.
+#pragma once
+
+#include <cstdint>
+
+namespace vs {
+
+enum MaterialType : uint32_t {
+    MAT_DIFFUSE = 0,   // Lambert (diffuse.cpp:21-30, bxdf.h:82-96)
+    MAT_OREN_NAYAR = 1,// diffuse with "sigma" (bxdf.cpp:94-121)
+    MAT_BLACK = 2,     // black_body on light quads (zero BSDF)
+    MAT_METAL = 3,     // GGX conductor (metal.cpp:39-157)
+    MAT_GLASS = 4,     // rough dielectric (glass.cpp:171-258)
+    MAT_PRINCIPLED = 5,// principled_bsdf lobe stack
+};
+
+enum LightType : uint32_t {
+    LIGHT_AREA = 0,
+    LIGHT_POINT = 1,
+    LIGHT_SPOT = 2,
+    LIGHT_ENV = 3,
+};
+
+// 128 B material record; meaning of p[] depends on type (see scene.cpp pack)
+struct MaterialRecord {
+    uint32_t type;
+    uint32_t flags;
+    float p[30];
+};
+static_assert(sizeof(MaterialRecord) == 128, "MaterialRecord layout");
+
+struct LightRecord {
+    uint32_t type;
+    uint32_t two_sided;
+    uint32_t inst;        // emitting instance (area)
+    uint32_t tri_base;    // first global triangle of the emitter (area)
+    uint32_t tri_count;   // triangles of the emitter (area)
+    uint32_t alias_offset;// offset of its per-triangle alias table
+    float alias_integral; // sum(area)/n as float (alias.h:86-122)
+    float pad0;
+    float color[3];
+    float scale;
+    float position[3];
+    float angle;// spot: radians
+    float direction[3];
+    float falloff;// spot: radians
+};
+static_assert(sizeof(LightRecord) == 80, "LightRecord layout");
+
+struct InstanceRecord {
+    uint32_t mat;  // material index, 0xFFFFFFFF = none
+    uint32_t light;// light index, 0xFFFFFFFF = not emissive
+    uint32_t tri_base;
+    uint32_t pad;
+};
+
+struct AliasEntry {
+    float prob;
+    uint32_t alias;
+};
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+}// namespace vs

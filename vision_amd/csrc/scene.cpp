@@ -1,0 +1,609 @@
+#include "scene.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace vs {
+
+namespace {
+
+constexpr float kPi = 3.14159265358979323846f;
+
+float radians(float deg) { return deg * (kPi / 180.f); }
+float degrees(float rad) { return rad * (180.f / kPi); }
+
+m4 mat_mul(const m4 &a, const m4 &b) {
+    m4 r;
+    for (int col = 0; col < 4; ++col) {
+        float v[4];
+        for (int row = 0; row < 4; ++row) {
+            float s = 0.f;
+            for (int k = 0; k < 4; ++k) s += m4_at(a, row, k) * m4_at(b, k, col);
+            v[row] = s;
+        }
+        r.c[col] = mk4(v[0], v[1], v[2], v[3]);
+    }
+    return r;
+}
+
+m4 translation(f3 t) {
+    m4 m = m4_identity();
+    m.c[3] = mk4(t.x, t.y, t.z, 1.f);
+    return m;
+}
+m4 scale_m(f3 s) {
+    m4 m = m4_identity();
+    m.c[0].x = s.x;
+    m.c[1].y = s.y;
+    m.c[2].z = s.z;
+    return m;
+}
+// right-handed rotations about the principal axes, angle in degrees
+m4 rotation_x(float deg) {
+    float c = std::cos(radians(deg)), s = std::sin(radians(deg));
+    m4 m = m4_identity();
+    m.c[1] = mk4(0, c, s, 0);
+    m.c[2] = mk4(0, -s, c, 0);
+    return m;
+}
+m4 rotation_y(float deg) {
+    float c = std::cos(radians(deg)), s = std::sin(radians(deg));
+    m4 m = m4_identity();
+    m.c[0] = mk4(c, 0, -s, 0);
+    m.c[2] = mk4(s, 0, c, 0);
+    return m;
+}
+m4 rotation_z(float deg) {
+    float c = std::cos(radians(deg)), s = std::sin(radians(deg));
+    m4 m = m4_identity();
+    m.c[0] = mk4(c, s, 0, 0);
+    m.c[1] = mk4(-s, c, 0, 0);
+    return m;
+}
+m4 rotation_axis(f3 axis, float deg) {
+    f3 a = normalize(axis);
+    float c = std::cos(radians(deg)), s = std::sin(radians(deg)), t = 1 - c;
+    m4 m = m4_identity();
+    m.c[0] = mk4(t * a.x * a.x + c, t * a.x * a.y + s * a.z, t * a.x * a.z - s * a.y, 0);
+    m.c[1] = mk4(t * a.x * a.y - s * a.z, t * a.y * a.y + c, t * a.y * a.z + s * a.x, 0);
+    m.c[2] = mk4(t * a.x * a.z + s * a.y, t * a.y * a.z - s * a.x, t * a.z * a.z + c, 0);
+    return m;
+}
+
+f3 as_f3(const Json &j, f3 def) {
+    auto v = j.as_numbers();
+    if (v.size() >= 3) return mk3(float(v[0]), float(v[1]), float(v[2]));
+    if (v.size() == 1) return mk3(float(v[0]));
+    return def;
+}
+
+// look_at: camera columns (right, up, forward, position) with
+// forward = normalize(target - position), right = normalize(forward x up).
+m4 look_at(f3 pos, f3 target, f3 up) {
+    f3 fwd = normalize(target - pos);
+    f3 right = normalize(cross(fwd, up));
+    f3 up2 = cross(right, fwd);
+    m4 m;
+    m.c[0] = mk4(right.x, right.y, right.z, 0);
+    m.c[1] = mk4(up2.x, up2.y, up2.z, 0);
+    m.c[2] = mk4(fwd.x, fwd.y, fwd.z, 0);
+    m.c[3] = mk4(pos.x, pos.y, pos.z, 1);
+    return m;
+}
+
+// TransformDesc::init (src/core/node_desc.cpp:31-58)
+m4 parse_transform(const Json &t) {
+    if (t.is_null()) return m4_identity();
+    std::string type = t["type"].as_string("matrix4x4");
+    const Json &p = t["param"];
+    if (type == "look_at") {
+        return look_at(as_f3(p["position"], mk3(0.f)), as_f3(p["target_pos"], mk3(0, 0, 1)),
+                       as_f3(p["up"], mk3(0, 1, 0)));
+    }
+    if (type == "Euler") {
+        m4 yaw = rotation_y(float(p["yaw"].as_float(0)));
+        m4 roll = rotation_z(float(p["roll"].as_float(0)));
+        m4 pitch = rotation_x(float(p["pitch"].as_float(0)));
+        m4 tt = translation(as_f3(p["position"], mk3(0.f)));
+        return mat_mul(mat_mul(mat_mul(tt, pitch), roll), yaw);
+    }
+    if (type == "trs") {
+        f3 tr = as_f3(p["t"], mk3(0.f));
+        auto r = p["r"].as_numbers();
+        f3 s = as_f3(p["s"], mk3(1.f));
+        m4 rot = m4_identity();
+        if (r.size() == 4 && (r[0] != 0 || r[1] != 0 || r[2] != 0))
+            rot = rotation_axis(mk3(float(r[0]), float(r[1]), float(r[2])), float(r[3]));
+        return mat_mul(mat_mul(translation(tr), rot), scale_m(s));
+    }
+    if (type == "matrix4x4") {
+        const Json &mj = p["matrix4x4"];
+        m4 m = m4_identity();
+        if (mj.is_array() && mj.size() == 4) {
+            for (int col = 0; col < 4; ++col) {
+                auto v = mj.at(col).as_numbers();
+                if (v.size() != 4) throw std::runtime_error("matrix4x4 column must have 4 entries");
+                m.c[col] = mk4(float(v[0]), float(v[1]), float(v[2]), float(v[3]));
+            }
+        }
+        return m;
+    }
+    throw std::runtime_error("transform type error " + type);
+}
+
+// Slot value resolution (SlotDesc::init, ShaderNodeDesc::init,
+// src/core/node_desc.cpp:126-147,318-345).  Only constant ("number") nodes
+// are supported; image/texture nodes raise.
+bool slot_values(const Json &slot, std::vector<float> &out) {
+    if (slot.is_null()) return false;
+    const Json *node = &slot;
+    if (slot.is_object() && slot.contains("channels")) node = &slot["node"];
+    std::vector<double> v;
+    if (node->is_array() || node->is_number()) {
+        v = node->as_numbers();
+    } else if (node->is_object()) {
+        if (node->contains("param")) {
+            std::string t = (*node)["type"].as_string("");
+            if (t != "number") throw std::runtime_error("unsupported shader node type: " + t);
+            v = (*node)["param"]["value"].as_numbers();
+        } else {
+            std::string t = (*node)["type"].as_string("image");
+            if (t == "number") v = (*node)["value"].as_numbers();
+            else throw std::runtime_error("unsupported shader node (textures are not supported): " + t);
+        }
+    } else {
+        return false;
+    }
+    out.clear();
+    for (double d : v) out.push_back(float(d));
+    return !out.empty();
+}
+
+f3 slot_f3(const Json &slot, f3 def) {
+    std::vector<float> v;
+    if (!slot_values(slot, v)) return def;
+    if (v.size() >= 3) return mk3(v[0], v[1], v[2]);
+    return mk3(v[0]);
+}
+
+float slot_f1(const Json &slot, float def) {
+    std::vector<float> v;
+    if (!slot_values(slot, v)) return def;
+    return v[0];
+}
+
+// NumberArray::normalize (src/render_core/shadernodes/number.cpp:40-49):
+// divides by the max component when it is >= 1 and returns the factor.
+float normalize_color(f3 &c) {
+    float m = std::max(c.x, std::max(c.y, c.z));
+    if (m < 1.f) return 1.f;
+    c = mk3(c.x / m, c.y / m, c.z / m);
+    return m;
+}
+
+void add_vertex(MeshHost &m, f3 p, f3 n, f2 uv) {
+    m.P.push_back(p);
+    m.N.push_back(n);
+    m.UV.push_back(uv);
+}
+
+// Quad (src/render_core/shapes/quad.cpp:21-50)
+MeshHost make_quad(float width, float height) {
+    MeshHost m;
+    float w = width / 2, h = height / 2;
+    f3 P[4] = {mk3(w, 0, h), mk3(w, 0, -h), mk3(-w, 0, h), mk3(-w, 0, -h)};
+    f2 UV[4] = {mk2(1, 1), mk2(1, 0), mk2(0, 1), mk2(0, 0)};
+    f3 ng_un = cross(P[0] - P[2], P[1] - P[2]);
+    for (int i = 0; i < 4; ++i) add_vertex(m, P[i], ng_un, UV[i]);
+    m.tris = {{0, 1, 2}, {2, 1, 3}};
+    return m;
+}
+
+// Cube (src/render_core/shapes/cube.cpp:20-71)
+MeshHost make_cube(float x, float y, float z) {
+    y = y == 0 ? x : y;
+    z = z == 0 ? y : z;
+    x /= 2.f;
+    y /= 2.f;
+    z /= 2.f;
+    f3 P[24] = {
+        mk3(-x, -y, z), mk3(x, -y, z), mk3(-x, y, z), mk3(x, y, z),
+        mk3(-x, y, -z), mk3(x, y, -z), mk3(-x, -y, -z), mk3(x, -y, -z),
+        mk3(-x, y, z), mk3(x, y, z), mk3(-x, y, -z), mk3(x, y, -z),
+        mk3(-x, -y, z), mk3(x, -y, z), mk3(-x, -y, -z), mk3(x, -y, -z),
+        mk3(x, -y, z), mk3(x, y, z), mk3(x, y, -z), mk3(x, -y, -z),
+        mk3(-x, -y, z), mk3(-x, y, z), mk3(-x, y, -z), mk3(-x, -y, -z)};
+    f3 N[6] = {mk3(0, 0, 1), mk3(0, 0, -1), mk3(0, 1, 0), mk3(0, -1, 0), mk3(1, 0, 0), mk3(-1, 0, 0)};
+    f2 UV[24] = {
+        mk2(0, 0), mk2(1, 0), mk2(0, 1), mk2(1, 1),
+        mk2(0, 1), mk2(1, 1), mk2(0, 0), mk2(1, 0),
+        mk2(0, 1), mk2(1, 1), mk2(0, 0), mk2(1, 0),
+        mk2(0, 1), mk2(1, 1), mk2(0, 0), mk2(1, 0),
+        mk2(0, 1), mk2(1, 1), mk2(1, 0), mk2(0, 0),
+        mk2(0, 1), mk2(1, 1), mk2(1, 0), mk2(0, 0)};
+    MeshHost m;
+    for (int i = 0; i < 24; ++i) add_vertex(m, P[i], N[i / 4], UV[i]);
+    m.tris = {{0, 1, 3}, {0, 3, 2}, {6, 5, 7}, {4, 5, 6}, {10, 9, 11}, {8, 9, 10},
+              {13, 14, 15}, {13, 12, 14}, {18, 17, 19}, {17, 16, 19}, {21, 22, 23}, {20, 21, 23}};
+    return m;
+}
+
+MeshHost load_obj(const std::string &path) {
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open mesh " + path);
+    std::vector<f3> v, vn;
+    std::vector<f2> vt;
+    MeshHost m;
+    struct Key {
+        int v, t, n;
+        bool operator==(const Key &o) const { return v == o.v && t == o.t && n == o.n; }
+    };
+    struct KeyHash {
+        size_t operator()(const Key &k) const {
+            return (size_t(k.v) * 73856093u) ^ (size_t(k.t) * 19349663u) ^ (size_t(k.n) * 83492791u);
+        }
+    };
+    std::unordered_map<Key, uint32_t, KeyHash> remap;
+    std::string line;
+    auto resolve = [](int idx, size_t n) -> int { return idx < 0 ? int(n) + idx : idx - 1; };
+    while (std::getline(in, line)) {
+        if (line.size() < 2) continue;
+        std::istringstream ss(line);
+        std::string tag;
+        ss >> tag;
+        if (tag == "v") {
+            f3 p;
+            ss >> p.x >> p.y >> p.z;
+            v.push_back(p);
+        } else if (tag == "vn") {
+            f3 n;
+            ss >> n.x >> n.y >> n.z;
+            vn.push_back(n);
+        } else if (tag == "vt") {
+            f2 t;
+            ss >> t.x >> t.y;
+            vt.push_back(t);
+        } else if (tag == "f") {
+            std::vector<uint32_t> poly;
+            std::string tok;
+            while (ss >> tok) {
+                int vi = 0, ti = 0, ni = 0;
+                const char *c = tok.c_str();
+                vi = std::atoi(c);
+                const char *s1 = std::strchr(c, '/');
+                if (s1) {
+                    if (s1[1] != '/') ti = std::atoi(s1 + 1);
+                    const char *s2 = std::strchr(s1 + 1, '/');
+                    if (s2) ni = std::atoi(s2 + 1);
+                }
+                Key k{resolve(vi, v.size()), ti ? resolve(ti, vt.size()) : -1, ni ? resolve(ni, vn.size()) : -1};
+                auto it = remap.find(k);
+                uint32_t id;
+                if (it == remap.end()) {
+                    id = uint32_t(m.P.size());
+                    if (k.v < 0 || size_t(k.v) >= v.size()) throw std::runtime_error("bad obj vertex index in " + path);
+                    add_vertex(m, v[k.v], k.n >= 0 ? vn.at(k.n) : mk3(0.f), k.t >= 0 ? vt.at(k.t) : mk2(0, 0));
+                    remap.emplace(k, id);
+                } else {
+                    id = it->second;
+                }
+                poly.push_back(id);
+            }
+            for (size_t i = 2; i < poly.size(); ++i) m.tris.push_back({poly[0], poly[i - 1], poly[i]});
+        }
+    }
+    return m;
+}
+
+// Binary mesh: "VSMESH01", u32 nv, u32 nt, u32 flags (1: normals, 2: uvs),
+// f32 P[nv*3], [f32 N[nv*3]], [f32 UV[nv*2]], u32 tri[nt*3]
+MeshHost load_vsmesh(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open mesh " + path);
+    char magic[8];
+    in.read(magic, 8);
+    if (std::memcmp(magic, "VSMESH01", 8) != 0) throw std::runtime_error("bad vsmesh magic in " + path);
+    uint32_t hdr[3];
+    in.read(reinterpret_cast<char *>(hdr), sizeof(hdr));
+    uint32_t nv = hdr[0], nt = hdr[1], flags = hdr[2];
+    MeshHost m;
+    m.P.resize(nv);
+    in.read(reinterpret_cast<char *>(m.P.data()), std::streamsize(nv) * 12);
+    m.N.assign(nv, mk3(0.f));
+    if (flags & 1) in.read(reinterpret_cast<char *>(m.N.data()), std::streamsize(nv) * 12);
+    m.UV.assign(nv, mk2(0, 0));
+    if (flags & 2) in.read(reinterpret_cast<char *>(m.UV.data()), std::streamsize(nv) * 8);
+    m.tris.resize(nt);
+    in.read(reinterpret_cast<char *>(m.tris.data()), std::streamsize(nt) * 12);
+    if (!in) throw std::runtime_error("truncated vsmesh " + path);
+    for (const u3 &t : m.tris)
+        if (t.x >= nv || t.y >= nv || t.z >= nv) throw std::runtime_error("vsmesh index out of range in " + path);
+    return m;
+}
+
+std::string join_path(const std::string &dir, const std::string &fn) {
+    if (fn.empty() || fn[0] == '/' || dir.empty()) return fn;
+    return dir + "/" + fn;
+}
+
+LightHost parse_light(const Json &desc) {
+    LightHost l;
+    std::string type = desc["type"].as_string("area");
+    const Json &p = desc["param"];
+    l.name = desc["name"].as_string("");
+    l.color = slot_f3(p["color"], mk3(1.f));
+    l.scale = float(p["scale"].as_float(1.0));
+    // Light::Light folds the colour normalization factor into scale
+    l.scale *= normalize_color(l.color);
+    if (type == "area") {
+        l.type = LIGHT_AREA;
+        l.two_sided = p["two_sided"].as_bool(false) ? 1u : 0u;
+    } else if (type == "point") {
+        l.type = LIGHT_POINT;
+        l.position = as_f3(p["position"], mk3(0.f));
+    } else if (type == "spot") {
+        // SpotLight ctor (src/render_core/lights/spot.cpp:34-40); the falloff
+        // clamp bound is the angle already converted to radians
+        l.type = LIGHT_SPOT;
+        l.position = as_f3(p["position"], mk3(0.f));
+        l.angle = radians(std::clamp(float(p["angle"].as_float(45.0)), 1.f, 89.f));
+        l.falloff = radians(std::clamp(float(p["falloff"].as_float(10.0)), 0.f, l.angle));
+        l.direction = normalize(as_f3(p["direction"], mk3(0, 0, 1)));
+    } else if (type == "spherical") {
+        l.type = LIGHT_ENV;
+        l.env_fn = p["color"].is_object() ? p["color"]["node"]["fn"].as_string("") : "";
+    } else {
+        throw std::runtime_error("unsupported light type: " + type);
+    }
+    return l;
+}
+
+}// namespace
+
+m4 camera_to_world(f3 position, float yaw, float pitch) {
+    m4 rot = mat_mul(mat_mul(scale_m(mk3(1, 1, -1)), rotation_y(yaw)), rotation_x(-pitch));
+    return mat_mul(translation(position), rot);
+}
+
+std::string read_text_file(const std::string &path) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    std::ostringstream ss;
+    ss << in.rdbuf();
+    return ss.str();
+}
+
+SceneHost load_scene(const std::string &text, const std::string &base_dir) {
+    Json root = Json::parse(text);
+    SceneHost s;
+    s.base_dir = base_dir;
+
+    // integrator (IntegratorDesc::init; IlluminationIntegrator ctor
+    // src/base/integrator.cpp:60-67)
+    {
+        const Json &d = root["integrator"];
+        s.integrator.type = d["type"].as_string("pt");
+        const Json &p = d["param"];
+        s.integrator.param = p;
+        s.integrator.max_depth = p["max_depth"].as_uint(16);
+        s.integrator.min_depth = p["min_depth"].as_uint(5);
+        s.integrator.rr_threshold = float(p["rr_threshold"].as_float(1.0));
+        s.integrator.mis_mode = p["mis_mode"].as_int(0);
+        if (s.integrator.type != "pt" && s.integrator.type != "rt")
+            throw std::runtime_error("unsupported integrator type: " + s.integrator.type);
+    }
+    // spectrum (SpectrumDesc::init; hero dimension default 3, hero.cpp:239)
+    {
+        const Json &d = root["spectrum"];
+        s.spectrum_type = d["type"].as_string("srgb");
+        if (s.spectrum_type == "hero") s.spectrum_dim = d["param"]["dimension"].as_uint(3);
+        else if (s.spectrum_type == "srgb") s.spectrum_dim = 3;
+        else throw std::runtime_error("unsupported spectrum type: " + s.spectrum_type);
+    }
+    // sampler
+    {
+        const Json &d = root["sampler"];
+        s.sampler_type = d["type"].as_string("independent");
+        if (s.sampler_type != "independent") throw std::runtime_error("unsupported sampler: " + s.sampler_type);
+    }
+    // render_setting (RenderSettingDesc::init)
+    s.ray_offset_factor = float(root["render_setting"]["ray_offset_factor"].as_float(1.0));
+    // output (OutputDesc::init)
+    s.output_spp = root["output"]["spp"].as_uint(0);
+    s.output_fn = root["output"]["fn"].as_string("output.png");
+
+    // light sampler + its light list (LightSamplerDesc::init, LightSampler ctor
+    // src/base/illumination/lightsampler.cpp:11-26)
+    {
+        const Json &d = root["light_sampler"];
+        s.light_sampler_type = d["type"].as_string("uniform");
+        if (s.light_sampler_type != "uniform" && s.light_sampler_type != "power")
+            throw std::runtime_error("unsupported light sampler: " + s.light_sampler_type);
+        const Json &p = d["param"];
+        s.env_separate = p["env_separate"].as_bool(false);
+        s.env_prob = std::clamp(float(p["env_prob"].as_float(0.5)), 0.01f, 0.99f);
+        const Json &lights = p["lights"];
+        if (lights.is_array())
+            for (const Json &l : lights.elements()) s.lights.push_back(parse_light(l));
+    }
+
+    // materials (MaterialDesc::init)
+    {
+        const Json &mats = root["materials"];
+        if (mats.is_array()) {
+            for (const Json &m : mats.elements()) {
+                MaterialHost mh;
+                mh.type = m["type"].as_string("diffuse");
+                mh.name = m["name"].as_string("");
+                mh.param = m["param"];
+                s.materials.push_back(mh);
+            }
+        }
+    }
+
+    // camera (SensorDesc::init, Sensor::init, RadianceCollector ctor)
+    {
+        const Json &d = root["camera"];
+        CameraHost &c = s.camera;
+        c.type = d["type"].as_string("thin_lens");
+        if (c.type != "pinhole" && c.type != "thin_lens") throw std::runtime_error("unsupported camera: " + c.type);
+        const Json &p = d["param"];
+        c.fov_y = float(p["fov_y"].as_float(20.0));
+        m4 m = parse_transform(p["transform"]);
+        // Sensor::update_mat re-derives pitch/yaw/position (sensor.cpp:78-83)
+        c.pitch = degrees(std::atan2(m.c[1].z, m.c[1].y));
+        c.yaw = degrees(std::atan2(m.c[2].x, m.c[0].x));
+        c.position = mk3(m.c[3].x, m.c[3].y, m.c[3].z);
+        c.c2w = camera_to_world(c.position, c.yaw, c.pitch);
+        if (c.type == "thin_lens") {
+            c.focal_distance = float(p["focal_distance"].as_float(5.0));
+            c.lens_radius = float(p["lens_radius"].as_float(0.0));
+        }
+        const Json &rc = p["radiance_collector"]["param"];
+        auto res = rc["resolution"].as_numbers();
+        if (res.size() >= 2) {
+            c.width = uint32_t(res[0]);
+            c.height = uint32_t(res[1]);
+        }
+        c.exposure = float(rc["exposure"].as_float(1.0));
+        c.accumulation = rc["accumulation"].as_uint(1);
+        c.tone_mapper = rc["tone_mapper"]["type"].as_string("linear");
+        const Json &f = p["filter"];
+        c.filter_type = f["type"].as_string("gaussian");
+        const Json &fr = f["param"]["radius"];
+        auto frv = fr.as_numbers();
+        c.filter_radius = frv.empty() ? 0.5f : float(frv[0]);
+        if (c.filter_type != "box" && c.filter_type != "triangle")
+            throw std::runtime_error("unsupported filter: " + c.filter_type);
+        if (c.width == 0 || c.height == 0) throw std::runtime_error("resolution must be positive");
+    }
+
+    // shapes (ShapeDesc::init, Scene::add_shape src/base/mgr/scene.cpp:118-154)
+    {
+        const Json &shapes = root["shapes"];
+        if (shapes.is_array()) {
+            for (const Json &sh : shapes.elements()) {
+                std::string type = sh["type"].as_string("");
+                const Json &p = sh["param"];
+                MeshHost mesh;
+                if (type == "quad") {
+                    mesh = make_quad(float(p["width"].as_float(1.0)), float(p["height"].as_float(1.0)));
+                } else if (type == "cube") {
+                    mesh = make_cube(float(p["x"].as_float(1.0)), float(p["y"].as_float(1.0)), float(p["z"].as_float(1.0)));
+                } else if (type == "model") {
+                    std::string fn = join_path(base_dir, p["fn"].as_string(""));
+                    if (fn.size() >= 7 && fn.compare(fn.size() - 7, 7, ".vsmesh") == 0) mesh = load_vsmesh(fn);
+                    else mesh = load_obj(fn);
+                } else {
+                    throw std::runtime_error("unsupported shape type: " + type);
+                }
+                InstanceHost inst;
+                inst.mesh = uint32_t(s.meshes.size());
+                inst.o2w = parse_transform(p["transform"]);
+                inst.name = sh["name"].as_string("");
+                inst.material_name = p["material"].as_string("");
+                for (size_t i = 0; i < s.materials.size(); ++i) {
+                    if (s.materials[i].name == inst.material_name) {
+                        inst.material = int32_t(i);
+                        break;
+                    }
+                }
+                if (p.contains("emission")) {
+                    LightHost l = parse_light(p["emission"]);
+                    if (l.type != LIGHT_AREA) throw std::runtime_error("shape emission must be an area light");
+                    l.inst = uint32_t(s.instances.size());
+                    inst.light = int32_t(s.lights.size());
+                    s.lights.push_back(l);
+                }
+                s.meshes.push_back(std::move(mesh));
+                s.instances.push_back(inst);
+            }
+        }
+    }
+    s.flatten();
+    return s;
+}
+
+namespace {
+
+// inverse-transpose of the upper 3x3 (normal transform)
+void normal_matrix(const m4 &m, float out[3][3]) {
+    double a[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) a[r][c] = m4_at(m, r, c);
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) -
+                 a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    double inv[3][3];
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    // out = transpose(inv)
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) out[r][c] = float(inv[c][r]);
+}
+
+}// namespace
+
+void SceneHost::flatten() {
+    wP.clear();
+    wN.clear();
+    wUV.clear();
+    wTri.clear();
+    triInst.clear();
+    triPrim.clear();
+    instTriBase.clear();
+    instRecords.clear();
+    for (size_t i = 0; i < instances.size(); ++i) {
+        const InstanceHost &inst = instances[i];
+        const MeshHost &mesh = meshes[inst.mesh];
+        uint32_t vbase = uint32_t(wP.size());
+        instTriBase.push_back(uint32_t(wTri.size()));
+        float nm[3][3];
+        normal_matrix(inst.o2w, nm);
+        for (size_t v = 0; v < mesh.P.size(); ++v) {
+            wP.push_back(xform_point(inst.o2w, mesh.P[v]));
+            f3 n = mesh.N[v];
+            wN.push_back(mk3(nm[0][0] * n.x + nm[0][1] * n.y + nm[0][2] * n.z,
+                             nm[1][0] * n.x + nm[1][1] * n.y + nm[1][2] * n.z,
+                             nm[2][0] * n.x + nm[2][1] * n.y + nm[2][2] * n.z));
+            wUV.push_back(mesh.UV[v]);
+        }
+        for (size_t t = 0; t < mesh.tris.size(); ++t) {
+            u3 tr = mesh.tris[t];
+            wTri.push_back({tr.x + vbase, tr.y + vbase, tr.z + vbase});
+            triInst.push_back(uint32_t(i));
+            triPrim.push_back(uint32_t(t));
+        }
+        InstanceRecord ir{};
+        ir.mat = inst.material >= 0 ? uint32_t(inst.material) : kInvalid;
+        ir.light = inst.light >= 0 ? uint32_t(inst.light) : kInvalid;
+        ir.tri_base = instTriBase.back();
+        instRecords.push_back(ir);
+    }
+    // per-triangle areas of emitters (ShapeInstance::surface_areas)
+    for (LightHost &l : lights) {
+        if (l.type != LIGHT_AREA) continue;
+        l.tri_base = instTriBase[l.inst];
+        size_t n = meshes[instances[l.inst].mesh].tris.size();
+        l.tri_areas.resize(n);
+        for (size_t t = 0; t < n; ++t) {
+            u3 tr = wTri[l.tri_base + t];
+            f3 v0 = wP[tr.x], v1 = wP[tr.y], v2 = wP[tr.z];
+            l.tri_areas[t] = 0.5f * length(cross(v1 - v0, v2 - v0));
+        }
+    }
+}
+
+}// namespace vs

@@ -1,0 +1,119 @@
+// Host-side scene model: Vision scene JSON -> resolved descriptors ->
+// world-space triangle soup plus material / light / camera records.
+//
+// Descriptor keys and defaults follow the reference's *Desc::init functions
+// (src/core/node_desc.cpp:31-369) and plugin constructors cited per field.
+#pragma once
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "json.h"
+#include "records.h"
+#include "vmath.h"
+
+namespace vs {
+
+struct MeshHost {
+    std::vector<f3> P, N;
+    std::vector<f2> UV;
+    std::vector<u3> tris;
+};
+
+struct InstanceHost {
+    uint32_t mesh{0};
+    m4 o2w{m4_identity()};
+    int32_t material{-1};// index into SceneHost::materials, -1 = none
+    int32_t light{-1};   // index into SceneHost::lights, -1 = not emissive
+    std::string name;
+    std::string material_name;
+};
+
+// A material with its resolved slot values; packed into MaterialRecord.
+struct MaterialHost {
+    std::string type;// diffuse | black_body | metal | glass | principled_bsdf
+    std::string name;
+    Json param;
+};
+
+struct LightHost {
+    uint32_t type{LIGHT_AREA};
+    std::string name;
+    f3 color{1, 1, 1};// normalized by its max component (light.cpp:10-17)
+    float scale{1.f};
+    uint32_t two_sided{0};
+    uint32_t inst{0xFFFFFFFFu};
+    f3 position{0, 0, 0};
+    f3 direction{0, 0, 1};
+    float angle{0.f}, falloff{0.f};
+    std::string env_fn;
+    // per-emitter triangle area weights (area lights) filled at flatten time
+    std::vector<float> tri_areas;
+    uint32_t tri_base{0};
+};
+
+struct CameraHost {
+    std::string type{"thin_lens"};
+    float fov_y{20.f};
+    f3 position{0, 0, 0};
+    float yaw{0.f}, pitch{0.f};// degrees, re-derived from the transform
+    m4 c2w{m4_identity()};
+    float lens_radius{0.f}, focal_distance{5.f};
+    uint32_t width{1280}, height{720};
+    std::string filter_type{"gaussian"};
+    float filter_radius{0.5f};
+    float exposure{1.f};
+    uint32_t accumulation{1};
+    std::string tone_mapper{"linear"};
+};
+
+struct IntegratorHost {
+    std::string type{"pt"};
+    uint32_t max_depth{16}, min_depth{5};
+    float rr_threshold{1.f};
+    int mis_mode{0};
+    Json param;
+};
+
+struct SceneHost {
+    std::string base_dir;
+    std::vector<MeshHost> meshes;
+    std::vector<InstanceHost> instances;
+    std::vector<MaterialHost> materials;
+    std::vector<LightHost> lights;
+    CameraHost camera;
+    IntegratorHost integrator;
+    std::string spectrum_type{"srgb"};
+    uint32_t spectrum_dim{3};
+    std::string light_sampler_type{"uniform"};
+    float env_prob{0.5f};
+    bool env_separate{false};
+    std::string sampler_type{"independent"};
+    float ray_offset_factor{1.f};
+    uint32_t output_spp{0};
+    std::string output_fn{"output.png"};
+
+    // flattened world-space geometry (filled by flatten())
+    std::vector<f3> wP, wN;
+    std::vector<f2> wUV;
+    std::vector<u3> wTri;         // global vertex indices
+    std::vector<uint32_t> triInst;// instance id per global triangle
+    std::vector<uint32_t> triPrim;// mesh-local primitive id per global triangle
+    std::vector<uint32_t> instTriBase;
+    std::vector<MaterialRecord> matRecords;
+    std::vector<LightRecord> lightRecords;
+    std::vector<InstanceRecord> instRecords;
+
+    void flatten();
+};
+
+// Parse a scene from JSON text; base_dir resolves relative mesh/texture paths.
+SceneHost load_scene(const std::string &text, const std::string &base_dir);
+std::string read_text_file(const std::string &path);
+
+// c2w of a camera from the reference's yaw/pitch/position re-derivation
+// (src/base/sensor/sensor.cpp:78-83,163-172).
+m4 camera_to_world(f3 position, float yaw_deg, float pitch_deg);
+
+}// namespace vs
