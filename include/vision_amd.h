@@ -89,7 +89,7 @@ typedef struct vs_scene_arrays {
     const uint32_t *tri_instance;/* [num_triangles] */
     const uint32_t *instances;  /* [num_instances][4]: material, light, tri_base, 0 */
     const uint32_t *materials;  /* [num_materials][32]: type, flags, 30 float params */
-    const uint32_t *lights;     /* [num_lights][20]: light records (80 B) */
+    const uint32_t *lights;     /* [num_lights][24]: light records (96 B) */
     const float *camera;        /* [24]: c2w rows [R|t] (12), screen lower_x, upper_y,
                                    raster step x, y, tan(fov_y/2), lens_radius,
                                    focal_distance, width, height, filter (0 box,

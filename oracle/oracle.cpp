@@ -163,10 +163,10 @@ struct Scene {
     uint32_t mat_type(uint32_t m) const { return mat[32 * m]; }
     float lightf(uint32_t l, int word) const {
         float f;
-        std::memcpy(&f, &light[20 * l + word], 4);
+        std::memcpy(&f, &light[24 * l + word], 4);
         return f;
     }
-    uint32_t lightu(uint32_t l, int word) const { return light[20 * l + word]; }
+    uint32_t lightu(uint32_t l, int word) const { return light[24 * l + word]; }
 };
 
 // ---------------------------------------------------------------- intersection
@@ -513,7 +513,7 @@ LSample sample_light(const Scene &s, const LightTables &lt, const Surf &it, Samp
             V3 dir = v3(s.lightf(l, 16), s.lightf(l, 17), s.lightf(l, 18));
             float angle = s.lightf(l, 15), fall = s.lightf(l, 19);
             float start = std::max(0.f, angle - fall);
-            float ca = std::cos(angle), cs = std::cos(start);
+            float ca = std::cos(angle), cs = std::cos(start);// spot.cpp:51-59
             float ct = std::min(std::max(dotp(dir, unit(wu)), ca), cs);
             float f = (ct - ca) / (cs - ca);
             val = mul(val, f * f * f * f);

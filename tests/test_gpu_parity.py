@@ -24,8 +24,9 @@ ABS_FLOOR = 1e-6
 
 def _soup_scene(n, seed, d):
     rng = np.random.default_rng(seed)
-    c = rng.random((n, 1, 3)) * 4 - 2
-    tri = (c + rng.normal(scale=0.1, size=(n, 3, 3))).reshape(-1, 3).astype(np.float32)
+    # a triangle soup inside the Cornell box ([-1, 1] x [0, 2] x [-1, 1])
+    c = rng.random((n, 1, 3)) * 1.6 - 0.8 + np.array([0, 1, 0])
+    tri = (c + rng.normal(scale=0.05, size=(n, 3, 3))).reshape(-1, 3).astype(np.float32)
     idx = np.arange(3 * n, dtype=np.uint32)
     fn = Path(d) / "soup.vsmesh"
     with open(fn, "wb") as f:
@@ -35,9 +36,10 @@ def _soup_scene(n, seed, d):
     return scene
 
 
-def _rays(rng, n, lo=-3, hi=3):
+def _rays(rng, n, lo=-0.95, hi=0.95):
+    # origins inside the box: most rays hit a wall or the soup
     r = np.zeros(n, va.RAY_DTYPE)
-    r["org"] = rng.random((n, 3)) * (hi - lo) + lo
+    r["org"] = rng.random((n, 3)) * (hi - lo) + lo + np.array([0, 1, 0])
     d = rng.normal(size=(n, 3))
     r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
     r["t_min"] = 0

@@ -209,7 +209,7 @@ class Scene:
             "tri_instance": arr(a.tri_instance, a.num_triangles, np.uint32),
             "instances": arr(a.instances, 4 * a.num_instances, np.uint32).reshape(-1, 4),
             "materials": arr(a.materials, 32 * a.num_materials, np.uint32).reshape(-1, 32),
-            "lights": arr(a.lights, 20 * a.num_lights, np.uint32).reshape(-1, 20),
+            "lights": arr(a.lights, 24 * a.num_lights, np.uint32).reshape(-1, 24),
             "camera": arr(a.camera, 24, np.float32),
             "render": arr(a.render, 8, np.float32),
         }

@@ -352,6 +352,7 @@ vs_status vs_scene_export(const vs_scene *s, vs_scene_arrays *out) {
     out->instances = s->exp_inst.data();
     out->materials = reinterpret_cast<const uint32_t *>(s->mats.data());
     out->lights = reinterpret_cast<const uint32_t *>(s->lights.records.data());
+    static_assert(sizeof(vs::LightRecord) == 24 * 4, "exported light record is 24 words");
     out->camera = s->exp_camera.data();
     out->render = s->exp_render.data();
     out->num_vertices = uint32_t(h.wP.size());

@@ -142,9 +142,8 @@ __device__ __forceinline__ LightSampleD sample_light(const SceneView &sc, const 
         Spec3 val = color / length_squared(w_un);
         if (L.type == LIGHT_SPOT) {
             f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
-            float falloff_start = fmaxf(0.f, L.angle - L.falloff);
-            float cos_angle = cosf(L.angle);
-            float cos_start = cosf(falloff_start);
+            float cos_angle = L.cos_angle;
+            float cos_start = L.cos_falloff_start;
             float ct = fminf(fmaxf(dot(dir, normalize(w_un)), cos_angle), cos_start);
             float f = (ct - cos_angle) / (cos_start - cos_angle);
             float f2v = f * f;

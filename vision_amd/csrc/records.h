@@ -44,8 +44,11 @@ struct LightRecord {
     float angle;// spot: radians
     float direction[3];
     float falloff;// spot: radians
+    float cos_angle;        // spot: cos(angle), computed once on the host
+    float cos_falloff_start;// spot: cos(max(0, angle - falloff))
+    float pad1, pad2;
 };
-static_assert(sizeof(LightRecord) == 80, "LightRecord layout");
+static_assert(sizeof(LightRecord) == 96, "LightRecord layout");
 
 struct InstanceRecord {
     uint32_t mat;  // material index, 0xFFFFFFFF = none

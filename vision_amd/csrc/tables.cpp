@@ -1,6 +1,7 @@
 #include "tables.h"
 
 #include <algorithm>
+#include <cmath>
 #include <numeric>
 #include <stdexcept>
 
@@ -128,6 +129,8 @@ LightTables pack_lights(const SceneHost &s) {
         r.direction[2] = l.direction.z;
         r.angle = l.angle;
         r.falloff = l.falloff;
+        r.cos_angle = std::cos(l.angle);
+        r.cos_falloff_start = std::cos(std::max(0.f, l.angle - l.falloff));
         if (l.type == LIGHT_AREA) {
             AliasTableHost a = build_alias(l.tri_areas);
             r.tri_base = l.tri_base;
