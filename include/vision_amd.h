@@ -89,7 +89,7 @@ typedef struct vs_scene_arrays {
     const uint32_t *tri_instance;/* [num_triangles] */
     const uint32_t *instances;  /* [num_instances][4]: material, light, tri_base, 0 */
     const uint32_t *materials;  /* [num_materials][32]: type, flags, 30 float params */
-    const uint32_t *lights;     /* [num_lights][24]: light records (96 B) */
+    const uint32_t *lights;     /* [num_lights][28]: light records (112 B) */
     const float *camera;        /* [24]: c2w rows [R|t] (12), screen lower_x, upper_y,
                                    raster step x, y, tan(fov_y/2), lens_radius,
                                    focal_distance, width, height, filter (0 box,
@@ -98,10 +98,26 @@ typedef struct vs_scene_arrays {
                                    light_sampler(0 uniform,1 power), ray_offset_factor,
                                    accumulation, exposure */
     uint32_t num_vertices, num_triangles, num_instances, num_materials, num_lights;
+    const float *spd;           /* [num_spd]: CIE X, Y, Z, D65 (4 x 94) then material curves */
+    uint32_t num_spd;
+    uint32_t spectrum_rgb;      /* 1: sRGB mode, 0: hero wavelengths */
+    uint32_t spectrum_dim;
+    uint32_t env_light;         /* index of the environment light, 0xFFFFFFFF if none */
+    uint32_t env_w, env_h;
+    const float *env_rgb;       /* [env_h][env_w][3] linear RGB, top row first */
+    float env_w2o[9], env_o2w[9];/* row-major 3x3 */
+    float env_scale;
+    float world_radius;
 } vs_scene_arrays;
 
 /* library version (major*10000 + minor*100 + patch) */
 int32_t vs_version(void);
+
+/* RGB -> sigmoid-polynomial spectrum coefficients of hero-wavelength mode
+ * (RGBToSpectrumTable::decode_albedo / decode_unbound,
+ * src/render_core/spectra/hero.cpp:91-139).  unbound = 0: albedo (rgb clamped
+ * to [0, 1]; out[3] = luminance); unbound = 1: illumination (out[3] = scale). */
+vs_status vs_rgb_to_spectrum(const float *rgb, uint32_t n, int32_t unbound, float *coeffs);
 
 /* Device context on HIP device `device` with its own stream; device = -1
  * makes a host-only context that can parse, build the BVH on the host and

@@ -163,10 +163,10 @@ struct Scene {
     uint32_t mat_type(uint32_t m) const { return mat[32 * m]; }
     float lightf(uint32_t l, int word) const {
         float f;
-        std::memcpy(&f, &light[24 * l + word], 4);
+        std::memcpy(&f, &light[28 * l + word], 4);
         return f;
     }
-    uint32_t lightu(uint32_t l, int word) const { return light[24 * l + word]; }
+    uint32_t lightu(uint32_t l, int word) const { return light[28 * l + word]; }
 };
 
 // ---------------------------------------------------------------- intersection

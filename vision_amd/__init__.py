@@ -70,13 +70,18 @@ class SceneArrays(C.Structure):
                 ("materials", C.POINTER(C.c_uint32)), ("lights", C.POINTER(C.c_uint32)),
                 ("camera", C.POINTER(C.c_float)), ("render", C.POINTER(C.c_float)),
                 ("num_vertices", C.c_uint32), ("num_triangles", C.c_uint32), ("num_instances", C.c_uint32),
-                ("num_materials", C.c_uint32), ("num_lights", C.c_uint32)]
+                ("num_materials", C.c_uint32), ("num_lights", C.c_uint32),
+                ("spd", C.POINTER(C.c_float)), ("num_spd", C.c_uint32), ("spectrum_rgb", C.c_uint32),
+                ("spectrum_dim", C.c_uint32), ("env_light", C.c_uint32), ("env_w", C.c_uint32),
+                ("env_h", C.c_uint32), ("env_rgb", C.POINTER(C.c_float)), ("env_w2o", C.c_float * 9),
+                ("env_o2w", C.c_float * 9), ("env_scale", C.c_float), ("world_radius", C.c_float)]
 
 
 # every entry point of include/vision_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 SIGNATURES = {
     "vs_version": (C.c_int32, []),
+    "vs_rgb_to_spectrum": (C.c_int32, [_P, C.c_uint32, C.c_int32, _P]),
     "vs_create": (C.c_int32, [C.c_int32, C.POINTER(_P)]),
     "vs_destroy": (None, [_P]),
     "vs_last_error": (C.c_char_p, [_P]),
@@ -117,6 +122,17 @@ def lib() -> C.CDLL:
             fn.argtypes = args
         _lib_handle = h
     return _lib_handle
+
+
+def rgb_to_spectrum(rgb: np.ndarray, unbound: bool = False) -> np.ndarray:
+    """Sigmoid-polynomial coefficients (c0, c1, c2, scale-or-luminance) of
+    hero-wavelength mode for float32 RGB[n, 3] (hero.cpp:91-139)."""
+    rgb = np.ascontiguousarray(rgb, dtype=np.float32).reshape(-1, 3)
+    out = np.zeros((rgb.shape[0], 4), np.float32)
+    st = lib().vs_rgb_to_spectrum(rgb.ctypes.data, rgb.shape[0], 1 if unbound else 0, out.ctypes.data)
+    if st != 0:
+        raise VisionError(st, "rgb_to_spectrum failed (spectrum table missing?)")
+    return out
 
 
 class Context:
@@ -209,9 +225,18 @@ class Scene:
             "tri_instance": arr(a.tri_instance, a.num_triangles, np.uint32),
             "instances": arr(a.instances, 4 * a.num_instances, np.uint32).reshape(-1, 4),
             "materials": arr(a.materials, 32 * a.num_materials, np.uint32).reshape(-1, 32),
-            "lights": arr(a.lights, 24 * a.num_lights, np.uint32).reshape(-1, 24),
+            "lights": arr(a.lights, 28 * a.num_lights, np.uint32).reshape(-1, 28),
             "camera": arr(a.camera, 24, np.float32),
             "render": arr(a.render, 8, np.float32),
+            "spd": arr(a.spd, a.num_spd, np.float32),
+            "spectrum_rgb": int(a.spectrum_rgb),
+            "spectrum_dim": int(a.spectrum_dim),
+            "env_light": int(a.env_light),
+            "env_rgb": arr(a.env_rgb, 3 * a.env_w * a.env_h, np.float32).reshape(a.env_h, a.env_w, 3),
+            "env_w2o": np.array(a.env_w2o[:], np.float32).reshape(3, 3),
+            "env_o2w": np.array(a.env_o2w[:], np.float32).reshape(3, 3),
+            "env_scale": float(a.env_scale),
+            "world_radius": float(a.world_radius),
         }
 
     def bvh(self) -> tuple[np.ndarray, np.ndarray]:

@@ -1,6 +1,6 @@
 """Build libvision_amd.so (host C++ + HIP kernels for gfx950) in-tree.
 
-Every source is compiled with hipcc; kernels target gfx950 only.  Object
+Host sources are compiled with g++, HIP sources with hipcc for gfx950 only.  Object
 files go to vision_amd/_build/, the shared library to vision_amd/.
 """
 from __future__ import annotations
@@ -38,8 +38,36 @@ def _compile(src: Path) -> Path:
     return obj
 
 
+DATA = PKG / "data"
+RGB2SPEC = DATA / "rgb2spec_srgb.f32"
+
+
+def build_rgb2spec(verbose: bool = False) -> Path:
+    """Generate the sRGB -> sigmoid-spectrum table (gen/rgb2spec.cpp) unless
+    it is present and newer than its generator and the CIE data."""
+    src = PKG / "gen" / "rgb2spec.cpp"
+    cie = DATA / "cie_xyz_d65.f32"
+    if RGB2SPEC.exists() and RGB2SPEC.stat().st_mtime > max(src.stat().st_mtime, cie.stat().st_mtime):
+        return RGB2SPEC
+    BUILD.mkdir(exist_ok=True)
+    exe = BUILD / "rgb2spec"
+    res = subprocess.run([CXX, "-O2", "-std=c++17", "-o", str(exe), str(src), "-lpthread"],
+                         capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"rgb2spec compile failed\n{res.stderr}")
+    tmp = RGB2SPEC.with_suffix(".tmp")
+    res = subprocess.run([str(exe), str(cie), str(tmp), "64"], capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"rgb2spec generation failed\n{res.stderr}")
+    os.replace(tmp, RGB2SPEC)
+    if verbose:
+        print(f"generated {RGB2SPEC}")
+    return RGB2SPEC
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
+    build_rgb2spec(verbose)
     sources = sorted(CSRC.glob("*.cpp")) + sorted(CSRC.glob("*.hip"))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))

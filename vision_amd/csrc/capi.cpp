@@ -16,6 +16,7 @@
 #include "dev_scene.h"
 #include "launch.h"
 #include "scene.h"
+#include "data_tables.h"
 #include "tables.h"
 
 struct vs_context {
@@ -75,10 +76,10 @@ struct vs_scene {
     bool host_bvh{false};// BVH built on the host
     bool built{false};   // BVH and scene resident in HBM
     double build_ms{0.0};
-    std::vector<vs::MaterialRecord> mats;
-    vs::LightTables lights;
+    vs::PackedScene packed;
     // device buffers
     DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, mat, light, alias, func;
+    DevBuf spd, lut, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
     DevBuf film_rt, film_out, counters;
     DevBuf scratch_rays, scratch_hits;
     vs::SceneView view{};
@@ -185,6 +186,23 @@ extern "C" {
 
 int32_t vs_version(void) { return 100; }
 
+vs_status vs_rgb_to_spectrum(const float *rgb, uint32_t n, int32_t unbound, float *coeffs) {
+    if ((n && (!rgb || !coeffs))) return VS_ERROR_INVALID_ARGUMENT;
+    try {
+        for (uint32_t i = 0; i < n; ++i) {
+            vs::f3 c = vs::mk3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+            vs::f4 r = unbound ? vs::rgb_to_unbound_coeffs(c) : vs::rgb_to_albedo_coeffs(c);
+            coeffs[4 * i] = r.x;
+            coeffs[4 * i + 1] = r.y;
+            coeffs[4 * i + 2] = r.z;
+            coeffs[4 * i + 3] = r.w;
+        }
+    } catch (const std::exception &) {
+        return VS_ERROR_STATE;
+    }
+    return VS_OK;
+}
+
 vs_status vs_create(int32_t device, vs_context **out) {
     if (!out) return VS_ERROR_INVALID_ARGUMENT;
     *out = nullptr;
@@ -232,8 +250,7 @@ vs_status vs_scene_load(vs_context *ctx, const char *json_text, const char *base
         auto s = std::make_unique<vs_scene>();
         s->ctx = ctx;
         s->host = vs::load_scene(json_text, base_dir ? base_dir : "");
-        s->mats = vs::pack_materials(s->host);
-        s->lights = vs::pack_lights(s->host);
+        s->packed = vs::pack_scene(s->host);
         fill_camera(s.get());
         fill_render_export(s.get());
         for (const auto &ir : s->host.instRecords) {
@@ -287,10 +304,18 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         s->tri.upload(h.wTri, st);
         s->tri_inst.upload(h.triInst, st);
         s->inst.upload(h.instRecords, st);
-        s->mat.upload(s->mats, st);
-        s->light.upload(s->lights.records, st);
-        s->alias.upload(s->lights.alias, st);
-        s->func.upload(s->lights.func, st);
+        const vs::PackedScene &pk = s->packed;
+        s->mat.upload(pk.mats, st);
+        s->light.upload(pk.lights, st);
+        s->alias.upload(pk.alias, st);
+        s->func.upload(pk.func, st);
+        s->spd.upload(pk.spd, st);
+        s->lut.upload(vs::lut_tables(), st);
+        s->env_tex.upload(pk.env_tex, st);
+        s->env_cond.upload(pk.env_cond, st);
+        s->env_cond_func.upload(pk.env_cond_func, st);
+        s->env_marg.upload(pk.env_marg, st);
+        s->env_marg_func.upload(pk.env_marg_func, st);
         s->counters.alloc(64);
         alloc_film(s);
         hip_check(hipStreamSynchronize(st), "upload sync");
@@ -309,8 +334,28 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         v.alias_func = s->func.as<float>();
         v.num_lights = uint32_t(h.lights.size());
         v.light_sampler = h.light_sampler_type == "power" ? 1u : 0u;
-        v.select_offset = s->lights.select_offset;
-        v.select_integral = s->lights.select_integral;
+        v.select_offset = pk.select_offset;
+        v.select_integral = pk.select_integral;
+        v.spd = s->spd.as<float>();
+        v.lut = s->lut.as<float>();
+        v.cie_interval = pk.cie_interval;
+        v.cie_y_integral = pk.cie_y_integral;
+        v.spectrum_rgb = pk.rgb ? 1u : 0u;
+        v.env_light = pk.env_light;
+        v.env_tex = s->env_tex.as<float>();
+        v.env_cond = s->env_cond.as<vs::AliasEntry>();
+        v.env_cond_func = s->env_cond_func.as<float>();
+        v.env_marg = s->env_marg.as<vs::AliasEntry>();
+        v.env_marg_func = s->env_marg_func.as<float>();
+        v.env_w = pk.env_w;
+        v.env_h = pk.env_h;
+        v.env_marg_integral = pk.env_marg_integral;
+        v.world_diameter = 2.f * pk.world_radius;
+        for (int i = 0; i < 9; ++i) {
+            v.env_w2o[i] = pk.env_w2o[i];
+            v.env_o2w[i] = pk.env_o2w[i];
+        }
+        v.env_scale = pk.env_scale;
         v.stack_size = s->bvh.max_depth;
         v.ray_offset_factor = h.ray_offset_factor;
         v.num_tris = uint32_t(h.wTri.size());
@@ -350,16 +395,32 @@ vs_status vs_scene_export(const vs_scene *s, vs_scene_arrays *out) {
     out->triangles = reinterpret_cast<const uint32_t *>(h.wTri.data());
     out->tri_instance = h.triInst.data();
     out->instances = s->exp_inst.data();
-    out->materials = reinterpret_cast<const uint32_t *>(s->mats.data());
-    out->lights = reinterpret_cast<const uint32_t *>(s->lights.records.data());
-    static_assert(sizeof(vs::LightRecord) == 24 * 4, "exported light record is 24 words");
+    out->materials = reinterpret_cast<const uint32_t *>(s->packed.mats.data());
+    out->lights = reinterpret_cast<const uint32_t *>(s->packed.lights.data());
+    static_assert(sizeof(vs::LightRecord) == 28 * 4, "exported light record is 28 words");
     out->camera = s->exp_camera.data();
     out->render = s->exp_render.data();
     out->num_vertices = uint32_t(h.wP.size());
     out->num_triangles = uint32_t(h.wTri.size());
     out->num_instances = uint32_t(h.instances.size());
-    out->num_materials = uint32_t(s->mats.size());
-    out->num_lights = uint32_t(s->lights.records.size());
+    out->num_materials = uint32_t(s->packed.mats.size());
+    out->num_lights = uint32_t(s->packed.lights.size());
+    const vs::PackedScene &pk = s->packed;
+    out->spd = pk.spd.data();
+    out->num_spd = uint32_t(pk.spd.size());
+    out->spectrum_rgb = pk.rgb ? 1u : 0u;
+    out->spectrum_dim = s->host.spectrum_dim;
+    out->env_light = pk.env_light;
+    out->env_w = pk.env_w;
+    out->env_h = pk.env_h;
+    const vs::LightHost *env = pk.env_light != vs::kInvalid ? &s->host.lights[pk.env_light] : nullptr;
+    out->env_rgb = env ? env->env_rgb.data() : nullptr;
+    for (int i = 0; i < 9; ++i) {
+        out->env_w2o[i] = pk.env_w2o[i];
+        out->env_o2w[i] = pk.env_o2w[i];
+    }
+    out->env_scale = pk.env_scale;
+    out->world_radius = pk.world_radius;
     return VS_OK;
 }
 
@@ -473,6 +534,7 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         rv.exposure = h.camera.exposure;
         rv.tone_mapper = h.camera.tone_mapper == "aces" ? 1u : (h.camera.tone_mapper == "reinhard" ? 2u : 0u);
         rv.write_output = 1;
+        rv.spectrum_dim = s->packed.rgb ? 3u : h.spectrum_dim;
         unsigned long long *ctr = nullptr;
         if (s->counting) {
             ctr = s->counters.as<unsigned long long>();

@@ -1,0 +1,489 @@
+// Per-hit material evaluator: MaterialEvaluator with individual shading
+// normals (src/base/scattering/material.cpp:132-184) over the lobes of
+// diffuse.cpp, metal.cpp, glass.cpp and principled_bsdf.cpp.
+#pragma once
+
+#include "dev_bsdf.h"
+#include "dev_spectrum.h"
+
+namespace vs {
+namespace dev {
+
+// material record layouts (packed by the host, tables.cpp)
+//  diffuse / oren-nayar: p[0..3] colour slot, p[4] sigma
+//  metal:  p[0..2] eta rgb, p[3..5] k rgb, p[6] ax, p[7] ay, p[8] spd eta offset,
+//          p[9] spd k offset, p[10] spd interval
+//  glass:  p[0..3] colour slot, p[4] ior, p[5] ax, p[6] ay, p[7] spd offset,
+//          p[8] spd interval; flags bit 0: dispersive ior curve
+//  principled: p[0..3] color, p[4..7] spec_tint, p[8..11] sheen_tint,
+//          p[12..15] coat_tint, p[16] metallic, p[17] ior, p[18] roughness,
+//          p[19] anisotropic, p[20] sheen_weight, p[21] sheen_roughness,
+//          p[22] coat_weight, p[23] coat_roughness, p[24] coat_ior,
+//          p[25] transmission_weight
+// A colour slot holds linear sRGB in sRGB mode and the sigmoid-polynomial
+// coefficients of the RGB albedo spectrum in hero mode.
+constexpr uint32_t MATF_DISPERSIVE = 1u;
+constexpr uint32_t FLAG_TRANSMISSION = 4u;// BxDFFlag::Transmission
+
+template <int NS, bool RGB>
+__device__ __forceinline__ Spec<NS> eval_albedo(const float *slot, const Swl<NS> &swl) {
+    Spec<NS> r;
+    if (RGB) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) r.v[i] = slot[i < 3 ? i : 2];
+    } else {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) r.v[i] = sigmoid_poly(slot, swl.lambda[i]);
+    }
+    return r;
+}
+
+template <int NS>
+struct MatCtx {
+    uint32_t type;
+    Frame fr;
+    f3 ng, wo_w, wo;
+    Ggx ggx;
+    // diffuse / metal / glass
+    Spec<NS> a0;// kr (diffuse), eta (metal), kt (glass)
+    Spec<NS> a1;// k (metal), per-channel eta (glass)
+    float onA, onB;
+    float comp;
+    float eta0;
+    // principled
+    float sh_a, sh_b;
+    Spec<NS> sh_tint, cc_kr, mt_kr, mt_F0, mt_B, tr_kt, tr_F0, sp_kr, sp_F0, df_kr;
+    float cc_eta, cc_alpha, mt_comp, tr_eta, tr_weight, tr_comp, sp_eta;
+    float sw[6];// normalized sample weights: sheen, coat, metal, trans, spec, diffuse
+};
+
+template <int NS>
+struct EvalN {
+    Spec<NS> f;// BSDF * |cos wi|
+    float pdf;
+    uint32_t flags;
+};
+
+__device__ __forceinline__ float layering_max(float a, float w) { return w != 0.f ? a / w : 0.f; }
+template <int NS>
+__device__ __forceinline__ Spec<NS> layering_weight(const Spec<NS> &albedo, const Spec<NS> &weight) {
+    float m = layering_max(albedo.v[0], weight.v[0]);
+#pragma unroll
+    for (int i = 1; i < NS; ++i) m = fmaxf(m, layering_max(albedo.v[i], weight.v[i]));
+    return weight * fminf(fmaxf(1.f - m, 0.f), 1.f);
+}
+
+__device__ __forceinline__ float dielectric_comp(const float *lut, float ax, float ay, float abs_cos_o, float eta) {
+    // DielectricLobe::sample_lut / refl_compensate (lobe.cpp:263-306)
+    const float *t = lut + (eta > 1.f ? LUT_DIEL : LUT_DIEL_INV);
+    float x = sqrtf(sqrtf(ax * ay));
+    float e = eta > 1.f ? eta : 1.f / eta;
+    float z = (e - 1.003f) / (5.f - 1.003f);
+    float v[2];
+    lut3<2>(t, 32, x, abs_cos_o, z, v);
+    return 1.f / v[0];
+}
+
+__device__ __forceinline__ float pure_reflection_comp(const float *lut, float ax, float ay, float cos_o) {
+    // PureReflectionLobe::compensate_factor (lobe.cpp:716-721)
+    float v;
+    lut2<1>(lut + LUT_PURE, 32, sqrtf(ax * ay), cos_o, &v);
+    return 1.f / v;
+}
+
+// Build the evaluator for material `m` at `it` (create_lobe_set of each
+// material).  `dispersive` reports a wavelength-dependent ior (glass with an
+// ior curve), on which the caller drops the secondary wavelengths.
+template <int NS, bool RGB>
+__device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const MaterialRecord &m, const Frame &fr,
+                                                    f3 ng, f3 wo_w, const Swl<NS> &swl, bool &dispersive) {
+    MatCtx<NS> c;
+    c.type = m.type;
+    c.fr = fr;
+    c.ng = ng;
+    c.wo_w = wo_w;
+    c.wo = fr.to_local(wo_w);
+    dispersive = false;
+    const float *p = m.p;
+    if (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR) {
+        c.a0 = eval_albedo<NS, RGB>(p, swl);
+        if (m.type == MAT_OREN_NAYAR) {
+            float sig = p[4] * (kPi * 0.5f);
+            float s2 = (sig * sig) * (sig * sig);
+            c.onA = 1.f - (s2 / (2.f * (s2 + 0.33f)));
+            c.onB = 0.45f * s2 / (s2 + 0.09f);
+        }
+    } else if (m.type == MAT_METAL) {
+        c.ggx = {p[6], p[7]};
+        if (RGB) {
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                c.a0.v[i] = p[i < 3 ? i : 2];
+                c.a1.v[i] = p[3 + (i < 3 ? i : 2)];
+            }
+        } else {
+            const float *se = sc.spd + __float_as_uint(p[8]);
+            const float *sk = sc.spd + __float_as_uint(p[9]);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                c.a0.v[i] = spd_eval(se, p[10], swl.lambda[i]);
+                c.a1.v[i] = spd_eval(sk, p[10], swl.lambda[i]);
+            }
+        }
+        c.comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
+    } else if (m.type == MAT_GLASS) {
+        c.ggx = {p[5], p[6]};
+        c.a0 = eval_albedo<NS, RGB>(p, swl);
+        bool front = dot(wo_w, ng) > 0.f;// Interaction::correct_eta
+        if (!RGB && (m.flags & MATF_DISPERSIVE)) {
+            dispersive = true;
+            const float *s = sc.spd + __float_as_uint(p[7]);
+#pragma unroll
+            for (int i = 0; i < NS; ++i) {
+                float ior = spd_eval(s, p[8], swl.lambda[i]);
+                c.a1.v[i] = front ? ior : 1.f / ior;
+            }
+        } else {
+            float e = front ? p[4] : 1.f / p[4];
+            c.a1 = Spec<NS>::fill(e);
+        }
+        c.eta0 = c.a1.v[0];
+        c.comp = dielectric_comp(sc.lut, c.ggx.ax, c.ggx.ay, fabsf(c.wo.z), c.eta0);
+    } else if (m.type == MAT_PRINCIPLED) {
+        // PrincipledBSDF::create_lobe_set (principled_bsdf.cpp:352-461)
+        Spec<NS> color = eval_albedo<NS, RGB>(p, swl);
+        Spec<NS> spec_tint = eval_albedo<NS, RGB>(p + 4, swl);
+        float ior = p[17];
+        float rough = fminf(fmaxf(p[18], 0.0001f), 1.f);
+        float aniso = p[19];
+        float aspect = sqrtf(1.f - aniso * 0.9f);
+        float r2 = rough * rough;
+        c.ggx = {fmaxf(0.001f, r2 / aspect), fmaxf(0.001f, r2 * aspect)};
+        Spec<NS> weight = Spec<NS>::fill(1.f);
+        float cos_theta = dot(wo_w, ng);
+        float front = cos_theta > 0.f ? 1.f : 0.f;
+        float sw[6];
+        {   // sheen (SheenLTC, approximate mode)
+            Spec<NS> tint = eval_albedo<NS, RGB>(p + 8, swl) * (p[20] * front) * weight;
+            float ltc[4];
+            lut2<4>(sc.lut + LUT_SHEEN, 32, cos_theta, p[21], ltc);
+            c.sh_a = ltc[0];
+            c.sh_b = ltc[1];
+            c.sh_tint = tint * ltc[2];
+            sw[0] = c.sh_tint.average();
+            weight = layering_weight(c.sh_tint, weight);
+        }
+        {   // coat
+            float cw = p[22] * front;
+            float cr = fminf(fmaxf(p[23], 0.0001f), 1.f);
+            cr = cr * cr;
+            c.cc_alpha = cr;
+            c.cc_eta = p[24];
+            c.cc_kr = eval_albedo<NS, RGB>(p + 12, swl) * weight * cw;
+            float s;
+            lut3<1>(sc.lut + LUT_COAT, 32, sqrtf(sqrtf(cr * cr)), cos_theta, (c.cc_eta - 1.003f) / (4.f - 1.003f), &s);
+            Spec<NS> cc_albedo = c.cc_kr * s;
+            sw[1] = cc_albedo.average();
+            weight = layering_weight(cc_albedo, weight);
+        }
+        {   // metallic (F82-tint conductor with energy compensation)
+            float metallic = p[16] * front;
+            c.mt_F0 = color;
+            const float f = 6.f / 7.f;
+            const float f5 = f * f * f * f * f;
+            Spec<NS> fs = color + f5 * (1.f - color);
+            c.mt_B = fs * (7.f / (f5 * f)) * (1.f - spec_tint);
+            c.mt_kr = weight * metallic;
+            c.mt_comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
+            sw[2] = metallic * weight.average();
+            weight = weight * (1.f - metallic);
+        }
+        {   // transmission (dielectric with Schlick Fresnel)
+            float tw = p[25];
+            float eta = cos_theta > 0.f ? ior : 1.f / ior;
+            c.tr_eta = eta;
+            c.tr_kt = color;
+            c.tr_F0 = spec_tint * schlick_f0_from_ior(eta);
+            Spec<NS> t_weight = weight * tw;
+            c.tr_weight = t_weight.average();
+            c.tr_comp = dielectric_comp(sc.lut, c.ggx.ax, c.ggx.ay, fabsf(c.wo.z), eta);
+            sw[3] = c.tr_weight;
+            weight = weight * (1.f - tw);
+        }
+        {   // specular
+            c.sp_eta = ior;
+            c.sp_F0 = spec_tint * schlick_f0_from_ior(ior);
+            c.sp_kr = weight;
+            float s;
+            float z = sqrtf(fabsf((ior - 1.f) / (ior + 1.f)));
+            lut3<1>(sc.lut + LUT_SPEC, 32, sqrtf(sqrtf(c.ggx.ax * c.ggx.ay)), cos_theta, z, &s);
+            Spec<NS> alb = (c.sp_F0 + s * (1.f - c.sp_F0)) * c.sp_kr;
+            sw[4] = alb.average();
+            weight = layering_weight(alb, weight);
+        }
+        {   // diffuse
+            c.df_kr = color * weight * front;
+            sw[5] = c.df_kr.average();
+        }
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) sum += sw[i];
+#pragma unroll
+        for (int i = 0; i < 6; ++i) c.sw[i] = sw[i] / sum;
+    }
+    return c;
+}
+
+// ---- local lobe evaluations (f without the cosine factor applied)
+template <int NS>
+__device__ __forceinline__ void eval_conductor_like(const Ggx &g, f3 wo, f3 wi, bool f82, const Spec<NS> &a,
+                                                    const Spec<NS> &b, const Spec<NS> &kr, float comp,
+                                                    EvalN<NS> &e) {
+    // MicrofacetReflection::f / PDF through BxDF::safe_evaluate, then
+    // PureReflectionLobe compensation (bxdf.cpp:64-79, lobe.cpp:723-730)
+    bool s = wo.z * wi.z > 0.f;
+    f3 wh = normalize(wo + wi);
+    f3 whf = face_forward(wh, mk3(0.f, 0.f, 1.f));
+    float c = fabsf(dot(wo, whf));
+    Spec<NS> F;
+    if (f82) {
+        F = fresnel_f82(a, b, c);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) F.v[i] = fresnel_conductor(c, a.v[i], b.v[i]);
+    }
+    e.f = s ? F * kr * g.brdf(wo, whf, wi) : Spec<NS>::fill(0.f);
+    e.pdf = s ? g.pdf_reflection(wo, wh) : 0.f;
+    e.f = e.f * comp;
+    e.flags = 0;
+}
+
+template <int NS>
+__device__ __forceinline__ void eval_reflection_dielectric(const Ggx &g, f3 wo, f3 wi, float eta, bool schlick,
+                                                           const Spec<NS> &F0, const Spec<NS> &kr, EvalN<NS> &e) {
+    // MicrofacetReflection with a dielectric / Schlick Fresnel (coat, specular)
+    bool s = wo.z * wi.z > 0.f;
+    f3 wh = normalize(wo + wi);
+    f3 whf = face_forward(wh, mk3(0.f, 0.f, 1.f));
+    float c = fabsf(dot(wo, whf));
+    Spec<NS> F = schlick ? fresnel_schlick(F0, eta, c) : Spec<NS>::fill(fresnel_dielectric(c, eta));
+    e.f = s ? F * kr * g.brdf(wo, whf, wi) : Spec<NS>::fill(0.f);
+    e.pdf = s ? g.pdf_reflection(wo, wh) : 0.f;
+    e.flags = 0;
+}
+
+// DielectricLobe::evaluate_local_impl with eta (lobe.cpp:318-415)
+template <int NS>
+__device__ __forceinline__ void eval_dielectric(const Ggx &g, f3 wo, f3 wi, const Spec<NS> &eta_s, float eta0,
+                                                bool schlick, const Spec<NS> &F0, const Spec<NS> &kt, float comp,
+                                                EvalN<NS> &e, float *eta_out) {
+    bool reflect = wo.z * wi.z > 0.f;
+    float eta_p = reflect ? 1.f : eta0;
+    if (eta_out) *eta_out = eta_p;
+    f3 wh = face_forward(normalize(wo + wi * eta_p), wo);
+    float c = fabsf(dot(wh, wo));
+    Spec<NS> F;
+    if (schlick) {
+        F = fresnel_schlick(F0, eta0, c);
+    } else {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) F.v[i] = fresnel_dielectric(c, eta_s.v[i]);
+    }
+    Spec<NS> T = 1.f - F;
+    float rp = F.average() / (T * kt + F).average();
+    if (reflect) {
+        e.f = F * g.brdf(wo, wh, wi);
+        e.pdf = g.pdf_reflection(wo, wh) * rp;
+        e.flags = 0;
+    } else {
+        f3 wh2 = normalize(wo + wi * eta0);
+        e.f = T * kt * g.btdf(wo, wh2, wi, eta0);
+        e.pdf = g.pdf_transmission(wo, wh, wi, eta0) * (1.f - rp);
+        e.flags = FLAG_TRANSMISSION;
+    }
+    e.f = e.f * comp;
+}
+
+// SheenLTC::evaluate_local_impl (principled_bsdf.cpp:59-71)
+template <int NS>
+__device__ __forceinline__ void eval_sheen(float a, float b, const Spec<NS> &tint, f3 wo, f3 wi, EvalN<NS> &e) {
+    f3 w = mk3(a * wi.x + b * wi.z, a * wi.y, wi.z);
+    float l = length(w);
+    w = w / l;
+    float jac = (a * a) / (l * l * l);
+    float ltc = w.z * kInvPi * jac;// cosine_hemisphere_PDF(cos_theta(w)) * jacobian
+    e.f = (wi.z < 0.f || wo.z < 0.f) ? Spec<NS>::fill(0.f) : tint * (ltc / wi.z);
+    e.pdf = ltc;
+    e.flags = 0;
+}
+
+template <int NS>
+__device__ __forceinline__ void eval_lambert(const Spec<NS> &kr, f3 wo, f3 wi, float A, float B, bool oren,
+                                             EvalN<NS> &e) {
+    bool s = wo.z * wi.z > 0.f;
+    float k = kInvPi;
+    if (oren) {
+        float sti = sqrtf(fmaxf(0.f, 1.f - wi.z * wi.z)), sto = sqrtf(fmaxf(0.f, 1.f - wo.z * wo.z));
+        float cpi = sti == 0.f ? 1.f : fminf(fmaxf(wi.x / sti, -1.f), 1.f);
+        float spi = sti == 0.f ? 0.f : fminf(fmaxf(wi.y / sti, -1.f), 1.f);
+        float cpo = sto == 0.f ? 1.f : fminf(fmaxf(wo.x / sto, -1.f), 1.f);
+        float spo = sto == 0.f ? 0.f : fminf(fmaxf(wo.y / sto, -1.f), 1.f);
+        float maxc = fmaxf(0.f, cpi * cpo + spi * spo);
+        bool cond = fabsf(wi.z) > fabsf(wo.z);
+        float sin_a = cond ? sto : sti;
+        float tan_b = cond ? sti / fabsf(wi.z) : sto / fabsf(wo.z);
+        k = kInvPi * (A + B * maxc * sin_a * tan_b);
+    }
+    e.f = s ? kr * k : Spec<NS>::fill(0.f);
+    e.pdf = s ? fabsf(wi.z) * kInvPi : 0.f;
+    e.flags = 0;
+}
+
+// full evaluation in world space: f * |cos wi|, pdf, with the geometric
+// hemisphere check of MaterialEvaluator::evaluate
+template <int NS>
+__device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, float *eta_out) {
+    EvalN<NS> e;
+    f3 wo = c.wo, wi = c.fr.to_local(wi_w);
+    if (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR) {
+        eval_lambert(c.a0, wo, wi, c.onA, c.onB, c.type == MAT_OREN_NAYAR, e);
+        e.f = e.f * fabsf(wi.z);
+    } else if (c.type == MAT_METAL) {
+        eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), c.comp, e);
+        e.f = e.f * fabsf(wi.z);
+    } else if (c.type == MAT_GLASS) {
+        eval_dielectric(c.ggx, wo, wi, c.a1, c.eta0, false, c.a1, c.a0, c.comp, e, eta_out);
+        e.f = e.f * fabsf(wi.z);
+    } else if (c.type == MAT_PRINCIPLED) {
+        // LobeSet::evaluate_impl (lobe.cpp:668-688): each lobe weighted, its pdf
+        // scaled by its sample weight, both masked by the lobe's hemisphere factor
+        float acos = fabsf(wi.z);
+        float same = wo.z * wi.z > 0.f ? 1.f : 0.f;
+        EvalN<NS> t;
+        e.f = Spec<NS>::fill(0.f);
+        e.pdf = 0.f;
+        e.flags = 0;
+        eval_sheen(c.sh_a, c.sh_b, c.sh_tint, wo, wi, t);
+        e.f = e.f + t.f * (acos * same);
+        e.pdf += t.pdf * c.sw[0] * same;
+        eval_reflection_dielectric(Ggx{c.cc_alpha, c.cc_alpha}, wo, wi, c.cc_eta, false, c.cc_kr, c.cc_kr, t);
+        e.f = e.f + t.f * (acos * same);
+        e.pdf += t.pdf * c.sw[1] * same;
+        eval_conductor_like(c.ggx, wo, wi, true, c.mt_F0, c.mt_B, c.mt_kr, c.mt_comp, t);
+        e.f = e.f + t.f * (acos * same);
+        e.pdf += t.pdf * c.sw[2] * same;
+        eval_dielectric(c.ggx, wo, wi, c.tr_F0, c.tr_eta, true, c.tr_F0, c.tr_kt, c.tr_comp, t, eta_out);
+        e.f = e.f + t.f * (acos * c.tr_weight);
+        e.pdf += t.pdf * c.sw[3];
+        e.flags |= t.flags;
+        eval_reflection_dielectric(c.ggx, wo, wi, c.sp_eta, true, c.sp_F0, c.sp_kr, t);
+        e.f = e.f + t.f * (acos * same);
+        e.pdf += t.pdf * c.sw[4] * same;
+        eval_lambert(c.df_kr, wo, wi, 0.f, 0.f, false, t);
+        e.f = e.f + t.f * (acos * same);
+        e.pdf += t.pdf * c.sw[5] * same;
+    } else {
+        // black body: f = 0, pdf = 1 (black_body.cpp:36-43)
+        e.f = Spec<NS>::fill(0.f);
+        e.pdf = 1.f;
+        e.flags = 0;
+    }
+    bool transmission = (e.flags & FLAG_TRANSMISSION) != 0;
+    bool same_geo = dot(c.wo_w, c.ng) * dot(wi_w, c.ng) > 0.f;
+    if (same_geo == transmission) e.pdf = 0.f;
+    return e;
+}
+
+template <int NS>
+struct SampleN {
+    EvalN<NS> e;
+    f3 wi;
+    float eta;
+};
+
+// local direction sampling of one lobe; returns validity
+template <int NS>
+__device__ __forceinline__ bool sample_dielectric_dir(const Ggx &g, f3 wo, const Spec<NS> &eta_s, float eta0,
+                                                      bool schlick, const Spec<NS> &F0, const Spec<NS> &kt,
+                                                      Sampler &sp, f3 &wi) {
+    // DielectricLobe::sample_wi_local_impl (lobe.cpp:452-470): wh 2, uc 1
+    f3 wh = g.sample_wh(wo, sp.next2d());
+    float d = dot(wo, wh);
+    Spec<NS> F;
+    if (schlick) {
+        F = fresnel_schlick(F0, eta0, fabsf(d));
+    } else {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) F.v[i] = fresnel_dielectric(fabsf(d), eta_s.v[i]);
+    }
+    float rp = F.average() / ((1.f - F) * kt + F).average();
+    float uc = sp.next1d();
+    if (uc < rp) {
+        wi = reflect_dir(wo, wh);
+        return wo.z * wi.z > 0.f;
+    }
+    bool ok = refract_dir(wo, wh, eta0, wi);
+    return ok && !(wo.z * wi.z > 0.f);
+}
+
+template <int NS>
+__device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &sp) {
+    SampleN<NS> s;
+    s.eta = 1.f;
+    f3 wo = c.wo, wi = mk3(0.f, 0.f, 1.f);
+    bool valid = true;
+    if (c.type == MAT_BLACK) {
+        // BlackBodyLobe::sample_local: wi = wo, no draws
+        s.wi = c.wo_w;
+        s.e.f = Spec<NS>::fill(0.f);
+        s.e.pdf = 1.f;
+        s.e.flags = 0;
+        bool same_geo = dot(c.wo_w, c.ng) * dot(s.wi, c.ng) > 0.f;
+        if (!same_geo) s.e.pdf = 0.f;
+        return s;
+    }
+    if (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR) {
+        wi = square_to_cosine_hemisphere(sp.next2d());
+        if (wo.z < 0.f) wi.z = -wi.z;
+    } else if (c.type == MAT_METAL) {
+        f3 wh = c.ggx.sample_wh(wo, sp.next2d());
+        wi = reflect_dir(wo, wh);
+        valid = wo.z * wi.z > 0.f;
+    } else if (c.type == MAT_GLASS) {
+        valid = sample_dielectric_dir(c.ggx, wo, c.a1, c.eta0, false, c.a1, c.a0, sp, wi);
+    } else {
+        // LobeSet::sample_wi_impl (lobe.cpp:597-627): uc 1 + an unused 2D draw,
+        // then the chosen lobe's draws
+        float uc = sp.next1d();
+        (void)sp.next2d();
+        int k = 0;
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            k = uc > sum ? i : k;
+            sum += c.sw[i];
+        }
+        if (k == 0) {
+            f3 w = square_to_cosine_hemisphere(sp.next2d());
+            w = mk3(w.x / c.sh_a - w.z * c.sh_b / c.sh_a, w.y / c.sh_a, w.z);
+            wi = normalize(w);
+        } else if (k == 1 || k == 2 || k == 4) {
+            Ggx g = k == 1 ? Ggx{c.cc_alpha, c.cc_alpha} : c.ggx;
+            f3 wh = g.sample_wh(wo, sp.next2d());
+            wi = reflect_dir(wo, wh);
+            valid = wo.z * wi.z > 0.f;
+        } else if (k == 3) {
+            valid = sample_dielectric_dir(c.ggx, wo, c.tr_F0, c.tr_eta, true, c.tr_F0, c.tr_kt, sp, wi);
+        } else {
+            wi = square_to_cosine_hemisphere(sp.next2d());
+            if (wo.z < 0.f) wi.z = -wi.z;
+        }
+    }
+    s.wi = c.fr.to_world(wi);
+    s.e = mat_eval(c, s.wi, &s.eta);
+    if (!valid) s.e.pdf = 0.f;
+    return s;
+}
+
+}// namespace dev
+}// namespace vs

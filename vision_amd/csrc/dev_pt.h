@@ -1,27 +1,18 @@
-// Megakernel path tracer (sRGB mode): one thread per pixel, all samples of a
-// launch processed in-thread with the film's running mean kept in registers.
+// Megakernel path tracer: one thread per pixel, all samples of a launch
+// processed in-thread with the film's running mean kept in registers.
 //
 // Restates PathTracingIntegrator::compile (src/render_core/integrators/pt.cpp:50-75)
-// and IlluminationIntegrator::Li (src/base/integrator.cpp:134-283), with the
-// sampler-dimension order of the reference (see Sampler draws noted inline).
+// and IlluminationIntegrator::Li (src/base/integrator.cpp:134-283) with the
+// reference's sampler-dimension order (draw counts noted inline).  Template
+// parameters: NS spectral channels, RGB = sRGB mode (else hero wavelengths).
 #pragma once
 
 #include "dev_common.h"
+#include "dev_material.h"
 #include "dev_traverse.h"
 
 namespace vs {
 namespace dev {
-
-struct Spec3 {
-    float v[3];
-};
-__device__ __forceinline__ Spec3 spec(float s) { return {{s, s, s}}; }
-__device__ __forceinline__ Spec3 spec(f3 c) { return {{c.x, c.y, c.z}}; }
-__device__ __forceinline__ Spec3 operator*(Spec3 a, Spec3 b) { return {{a.v[0] * b.v[0], a.v[1] * b.v[1], a.v[2] * b.v[2]}}; }
-__device__ __forceinline__ Spec3 operator*(Spec3 a, float s) { return {{a.v[0] * s, a.v[1] * s, a.v[2] * s}}; }
-__device__ __forceinline__ Spec3 operator/(Spec3 a, float s) { return {{a.v[0] / s, a.v[1] / s, a.v[2] / s}}; }
-__device__ __forceinline__ Spec3 operator+(Spec3 a, Spec3 b) { return {{a.v[0] + b.v[0], a.v[1] + b.v[1], a.v[2] + b.v[2]}}; }
-__device__ __forceinline__ float smax(Spec3 a) { return fmaxf(a.v[0], fmaxf(a.v[1], a.v[2])); }
 
 // ---- surface interaction (Geometry::compute_surface_interaction,
 //      src/base/mgr/geometry.cpp:78-165); barycentrics weight (v1, v2)
@@ -75,15 +66,19 @@ __device__ __forceinline__ Surface surface_at(const SceneView &sc, uint32_t tri,
     return s;
 }
 
-// ---- alias table (src/render_core/warpers/alias.h:141-158)
-__device__ __forceinline__ uint32_t alias_sample(const SceneView &sc, uint32_t off, uint32_t n, float u, float *u_remapped) {
+// ---- alias tables (src/render_core/warpers/alias.h:141-158)
+__device__ __forceinline__ uint32_t alias_pick(const AliasEntry *tab, uint32_t n, float u, float *u_remapped) {
     u = u * float(n);
     uint32_t idx = min(uint32_t(u), n - 1u);
     u = fminf(u - float(idx), kOneMinusEpsilon);
-    AliasEntry e = sc.alias[off + idx];
+    AliasEntry e = tab[idx];
     bool keep = u < e.prob;
-    if (u_remapped) *u_remapped = keep ? fminf(u / e.prob, kOneMinusEpsilon) : fminf((1.f - u) / (1.f - e.prob), kOneMinusEpsilon);
+    if (u_remapped)
+        *u_remapped = keep ? fminf(u / e.prob, kOneMinusEpsilon) : fminf((1.f - u) / (1.f - e.prob), kOneMinusEpsilon);
     return keep ? idx : e.alias;
+}
+__device__ __forceinline__ uint32_t alias_sample(const SceneView &sc, uint32_t off, uint32_t n, float u, float *u_remapped) {
+    return alias_pick(sc.alias + off, n, u, u_remapped);
 }
 __device__ __forceinline__ float alias_pmf(const SceneView &sc, uint32_t off, uint32_t n, float integral, uint32_t i) {
     return integral > 0.f ? sc.alias_func[off + i] / (integral * float(n)) : 0.f;
@@ -95,16 +90,78 @@ __device__ __forceinline__ float light_select_pmf(const SceneView &sc, uint32_t 
     return alias_pmf(sc, sc.select_offset, sc.num_lights, sc.select_integral, li);
 }
 
-struct LightSampleD {
-    Spec3 L;
+// ---- spectra of lights
+template <int NS, bool RGB>
+__device__ __forceinline__ Spec<NS> illum_spectrum(const SceneView &sc, const float *rgb, const float *coef,
+                                                   float scale, const Swl<NS> &swl) {
+    Spec<NS> r;
+    if (RGB) {
+#pragma unroll
+        for (int i = 0; i < NS; ++i) r.v[i] = rgb[i < 3 ? i : 2] * scale;
+    } else {
+        // RGBIlluminationSpectrum: sigmoid(lambda) * unbound scale * D65(lambda)
+#pragma unroll
+        for (int i = 0; i < NS; ++i)
+            r.v[i] = sigmoid_poly(coef, swl.lambda[i]) * coef[3] * spd_eval(sc.spd + 282, sc.cie_interval, swl.lambda[i]) *
+                     scale;
+    }
+    return r;
+}
+
+template <int NS, bool RGB>
+__device__ __forceinline__ Spec<NS> light_color(const SceneView &sc, const LightRecord &L, const Swl<NS> &swl) {
+    return illum_spectrum<NS, RGB>(sc, L.color, L.illum, L.scale, swl);
+}
+
+// ---- environment map (SphericalMap, src/render_core/lights/environments/spherical.cpp)
+__device__ __forceinline__ f3 mat3_apply(const float *m, f3 v) {
+    return mk3(m[0] * v.x + m[1] * v.y + m[2] * v.z, m[3] * v.x + m[4] * v.y + m[5] * v.z,
+               m[6] * v.x + m[7] * v.y + m[8] * v.z);
+}
+__device__ __forceinline__ f2 spherical_uv(f3 local) {
+    float theta = acosf(fminf(fmaxf(local.z, -1.f), 1.f));
+    float phi = atan2f(local.y, local.x);
+    if (phi < 0.f) phi += 2.f * kPi;
+    return mk2(phi * (0.5f * kInvPi), theta * kInvPi);
+}
+template <int NS, bool RGB>
+__device__ __forceinline__ Spec<NS> env_texel(const SceneView &sc, f2 uv, const Swl<NS> &swl) {
+    uint32_t x = min(uint32_t(fmaxf(uv.x, 0.f) * float(sc.env_w)), sc.env_w - 1u);
+    uint32_t y = min(uint32_t(fmaxf(uv.y, 0.f) * float(sc.env_h)), sc.env_h - 1u);
+    const float *t = sc.env_tex + 4 * (size_t(y) * sc.env_w + x);
+    return illum_spectrum<NS, RGB>(sc, t, t, sc.env_scale, swl);
+}
+// AliasTable2D::PDF (alias2d.cpp:101-105)
+__device__ __forceinline__ float env_map_pdf(const SceneView &sc, f2 uv) {
+    uint32_t iu = min(uint32_t(fmaxf(uv.x * float(sc.env_w), 0.f)), sc.env_w - 1u);
+    uint32_t iv = min(uint32_t(fmaxf(uv.y * float(sc.env_h), 0.f)), sc.env_h - 1u);
+    return sc.env_marg_integral > 0.f ? sc.env_cond_func[size_t(iv) * sc.env_w + iu] / sc.env_marg_integral : 0.f;
+}
+// SphericalMap::evaluate_wi for a direction (pdf in solid angle, before light selection)
+template <int NS, bool RGB>
+__device__ __forceinline__ void env_eval(const SceneView &sc, f3 dir, const Swl<NS> &swl, Spec<NS> &L, float &pdf) {
+    f3 local = mat3_apply(sc.env_w2o, dir);
+    float theta = acosf(fminf(fmaxf(local.z, -1.f), 1.f));
+    float sin_t = sinf(theta);
+    f2 uv = spherical_uv(local);
+    L = env_texel<NS, RGB>(sc, uv, swl);
+    float p = env_map_pdf(sc, uv) / (2.f * kPi * kPi * sin_t);
+    pdf = sin_t == 0.f ? 0.f : p;
+}
+
+template <int NS>
+struct LightSampleN {
+    Spec<NS> L;
     float pdf;
     f3 p_light;
 };
 
 // LightSampler::sample_wi (src/base/illumination/lightsampler.cpp:199-216)
 // draws: select 1, surface 2
-__device__ __forceinline__ LightSampleD sample_light(const SceneView &sc, const Surface &it, Sampler &sampler) {
-    LightSampleD ls;
+template <int NS, bool RGB>
+__device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3 ref_pos, Sampler &sampler,
+                                                         const Swl<NS> &swl) {
+    LightSampleN<NS> ls;
     float u_light = sampler.next1d();
     f2 u = sampler.next2d();
     uint32_t li;
@@ -117,7 +174,6 @@ __device__ __forceinline__ LightSampleD sample_light(const SceneView &sc, const 
         sel_pmf = alias_pmf(sc, sc.select_offset, sc.num_lights, sc.select_integral, li);
     }
     const LightRecord &L = sc.lights[li];
-    Spec3 color = spec(mk3(L.color[0], L.color[1], L.color[2])) * L.scale;
     if (L.type == LIGHT_AREA) {
         // AreaLight::sample_wi (src/render_core/lights/area.cpp:120-149)
         float ux;
@@ -126,26 +182,48 @@ __device__ __forceinline__ LightSampleD sample_light(const SceneView &sc, const 
         f2 bary = square_to_triangle(mk2(ux, u.y));
         Surface p = surface_at(sc, L.tri_base + prim, bary.x, bary.y, false);
         float pdf_pos = (1.f / p.prim_area) * pmf;
-        f3 w = it.pos - p.pos;
+        f3 w = ref_pos - p.pos;
         bool front = dot(w, p.ng) > 0.f || L.two_sided;
-        ls.L = front ? color : spec(0.f);
+        ls.L = front ? light_color<NS, RGB>(sc, L, swl) : Spec<NS>::fill(0.f);
         float cos_t = fabsf(dot(p.ng, normalize(w)));
         float pdf = pdf_pos * length_squared(w) / cos_t;
         ls.pdf = (isinf(pdf) || isnan(pdf)) ? 0.f : pdf;
         float factor = dot(p.ng, w) > 0.f ? 1.f : -1.f;
         ls.p_light = offset_ray_origin(p.pos, p.ng * factor * sc.ray_offset_factor);
+    } else if (L.type == LIGHT_ENV) {
+        // SphericalMap::sample_wi / evaluate (spherical.cpp:94-124, 157-163);
+        // AliasTable2D::sample_continuous (alias2d.cpp:109-129)
+        float uvr;
+        uint32_t iv = alias_pick(sc.env_marg, sc.env_h, u.y, &uvr);
+        float fv = (float(iv) + uvr) / float(sc.env_h);
+        float pdf_v = sc.env_marg_integral > 0.f ? sc.env_marg_func[iv] / sc.env_marg_integral : 0.f;
+        size_t row = size_t(sc.env_w) * iv;
+        float uur;
+        uint32_t iu = alias_pick(sc.env_cond + row, sc.env_w, u.x, &uur);
+        float fu = (float(iu) + uur) / float(sc.env_w);
+        float integral_u = sc.env_marg_func[iv];
+        float pdf_u = integral_u > 0.f ? sc.env_cond_func[row + iu] / integral_u : 0.f;
+        float pdf_map = pdf_u * pdf_v;
+        float theta = fv * kPi, phi = fu * 2.f * kPi;
+        float sin_t = sinf(theta), cos_t = cosf(theta);
+        float sp, cp;
+        sincosf(phi, &sp, &cp);
+        f3 local = mk3(sin_t * cp, sin_t * sp, cos_t);
+        f3 world = normalize(mat3_apply(sc.env_o2w, local));
+        float pdf_dir = pdf_map / (2.f * kPi * kPi * sin_t);
+        ls.pdf = isinf(pdf_dir) ? 0.f : pdf_dir;
+        ls.L = env_texel<NS, RGB>(sc, spherical_uv(local), swl);
+        ls.p_light = ref_pos + world * sc.world_diameter;
     } else {
         // IPointLight::sample_wi (src/base/illumination/light.cpp:42-51);
         // point.cpp:39-43, spot.cpp:51-70
         f3 pos = mk3(L.position[0], L.position[1], L.position[2]);
-        f3 w_un = it.pos - pos;
-        Spec3 val = color / length_squared(w_un);
+        f3 w_un = ref_pos - pos;
+        Spec<NS> val = light_color<NS, RGB>(sc, L, swl) / length_squared(w_un);
         if (L.type == LIGHT_SPOT) {
             f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
-            float cos_angle = L.cos_angle;
-            float cos_start = L.cos_falloff_start;
-            float ct = fminf(fmaxf(dot(dir, normalize(w_un)), cos_angle), cos_start);
-            float f = (ct - cos_angle) / (cos_start - cos_angle);
+            float ct = fminf(fmaxf(dot(dir, normalize(w_un)), L.cos_angle), L.cos_falloff_start);
+            float f = (ct - L.cos_angle) / (L.cos_falloff_start - L.cos_angle);
             float f2v = f * f;
             val = val * (f2v * f2v);
         }
@@ -159,94 +237,46 @@ __device__ __forceinline__ LightSampleD sample_light(const SceneView &sc, const 
 
 // LightSampler::evaluate_hit_wi for an area emitter hit by a BSDF ray
 // (src/base/illumination/lightsampler.cpp:252-267)
+template <int NS, bool RGB>
 __device__ __forceinline__ void eval_hit_light(const SceneView &sc, const Surface &it, uint32_t li, uint32_t tri,
-                                               f3 ref_pos, Spec3 &L, float &pdf) {
+                                               f3 ref_pos, const Swl<NS> &swl, Spec<NS> &L, float &pdf) {
     const LightRecord &Lr = sc.lights[li];
     uint32_t prim = tri - Lr.tri_base;
     float pdf_pos = (1.f / it.prim_area) * alias_pmf(sc, Lr.alias_offset, Lr.tri_count, Lr.alias_integral, prim);
     f3 w = ref_pos - it.pos;
     bool front = dot(w, it.ng) > 0.f || Lr.two_sided;
-    L = front ? spec(mk3(Lr.color[0], Lr.color[1], Lr.color[2])) * Lr.scale : spec(0.f);
+    L = front ? light_color<NS, RGB>(sc, Lr, swl) : Spec<NS>::fill(0.f);
     float cos_t = fabsf(dot(it.ng, normalize(w)));
     float p = pdf_pos * length_squared(w) / cos_t;
     p = (isinf(p) || isnan(p)) ? 0.f : p;
     pdf = p * light_select_pmf(sc, li);
 }
 
-// ---- materials
-struct BsdfEval {
-    Spec3 f;// BSDF * |cos wi|
-    float pdf;
-};
-
-__device__ __forceinline__ bool same_hemisphere(f3 a, f3 b, f3 n) { return dot(a, n) * dot(b, n) > 0.f; }
-
-// MaterialEvaluator::evaluate with individual shading normals
-// (src/base/scattering/material.cpp:132-148): lobe eval in the shading frame,
-// f *= |cos|, pdf zeroed when wo/wi straddle the geometric surface
-__device__ __forceinline__ BsdfEval eval_material(const MaterialRecord &m, const Surface &it, f3 wi_w) {
-    BsdfEval e;
-    f3 wo = it.frame.to_local(it.wo), wi = it.frame.to_local(wi_w);
-    if (m.type == MAT_BLACK) {
-        e.f = spec(0.f);
-        e.pdf = 1.f;
-    } else {
-        // Lambert (bxdf.h:82-96, bxdf.cpp:13-52); Oren-Nayar (bxdf.cpp:94-121),
-        // whose sigma2 is sigma^4 as written there
-        bool s = wo.z * wi.z > 0.f;
-        Spec3 kr = spec(mk3(m.p[0], m.p[1], m.p[2]));
-        float k = kInvPi;
-        if (m.type == MAT_OREN_NAYAR) {
-            float sig = m.p[3] * (kPi * 0.5f);
-            float s2 = (sig * sig) * (sig * sig);
-            float A = 1.f - (s2 / (2.f * (s2 + 0.33f)));
-            float B = 0.45f * s2 / (s2 + 0.09f);
-            float sti = sqrtf(fmaxf(0.f, 1.f - wi.z * wi.z)), sto = sqrtf(fmaxf(0.f, 1.f - wo.z * wo.z));
-            float cpi = sti == 0.f ? 1.f : fminf(fmaxf(wi.x / sti, -1.f), 1.f);
-            float spi = sti == 0.f ? 0.f : fminf(fmaxf(wi.y / sti, -1.f), 1.f);
-            float cpo = sto == 0.f ? 1.f : fminf(fmaxf(wo.x / sto, -1.f), 1.f);
-            float spo = sto == 0.f ? 0.f : fminf(fmaxf(wo.y / sto, -1.f), 1.f);
-            float maxc = fmaxf(0.f, cpi * cpo + spi * spo);
-            bool cond = fabsf(wi.z) > fabsf(wo.z);
-            float sin_a = cond ? sto : sti;
-            float tan_b = cond ? sti / fabsf(wi.z) : sto / fabsf(wo.z);
-            k = kInvPi * (A + B * maxc * sin_a * tan_b);
-        }
-        e.f = s ? kr * k : spec(0.f);
-        e.pdf = s ? fabsf(wi.z) * kInvPi : 0.f;
-        e.f = e.f * fabsf(wi.z);
+// Spectrum::linear_srgb: identity in sRGB mode; in hero mode the CIE XYZ
+// estimate of HeroWavelengthSpectrum::cie_xyz (hero.cpp:262-274) mapped to
+// linear sRGB (cie.h:414-420)
+template <int NS, bool RGB>
+__device__ __forceinline__ f3 to_linear_srgb(const SceneView &sc, const Spec<NS> &s, const Swl<NS> &swl) {
+    if (RGB) return mk3(s.v[0], s.v[1], s.v[2]);
+    float X = 0.f, Y = 0.f, Z = 0.f;
+    uint32_t valid = 0;
+#pragma unroll
+    for (int i = 0; i < NS; ++i) {
+        float l = swl.lambda[i], p = swl.pdf[i];
+        valid += p > 0.f ? 1u : 0u;
+        float x = spd_eval(sc.spd + 0, sc.cie_interval, l) * s.v[i];
+        float y = spd_eval(sc.spd + 94, sc.cie_interval, l) * s.v[i];
+        float z = spd_eval(sc.spd + 188, sc.cie_interval, l) * s.v[i];
+        X += p == 0.f ? 0.f : x / p;
+        Y += p == 0.f ? 0.f : y / p;
+        Z += p == 0.f ? 0.f : z / p;
     }
-    bool discard = !same_hemisphere(it.wo, wi_w, it.ng);
-    if (discard) e.pdf = 0.f;
-    return e;
-}
-
-struct BsdfSample {
-    BsdfEval eval;
-    f3 wi;
-    float eta;
-};
-
-// MaterialEvaluator::sample (material.cpp:162-184); Lambert draws 2
-__device__ __forceinline__ BsdfSample sample_material(const MaterialRecord &m, const Surface &it, Sampler &sampler) {
-    BsdfSample bs;
-    bs.eta = 1.f;
-    if (m.type == MAT_BLACK) {
-        // BlackBodyLobe::sample_local: wi = wo, pdf 1, no draws
-        bs.wi = it.wo;
-        bs.eval.f = spec(0.f);
-        bs.eval.pdf = 1.f;
-    } else {
-        f3 wo = it.frame.to_local(it.wo);
-        f3 wi = square_to_cosine_hemisphere(sampler.next2d());
-        if (wo.z < 0.f) wi.z = -wi.z;
-        bs.wi = it.frame.to_world(wi);
-        bs.eval = eval_material(m, it, bs.wi);
-        return bs;
-    }
-    bool discard = !same_hemisphere(it.wo, bs.wi, it.ng);
-    if (discard) bs.eval.pdf = 0.f;
-    return bs;
+    float factor = 1.f / (float(valid) * sc.cie_y_integral);
+    X *= factor;
+    Y *= factor;
+    Z *= factor;
+    return mk3(3.240479f * X - 1.537150f * Y - 0.498535f * Z, -0.969256f * X + 1.875991f * Y + 0.041556f * Z,
+               0.055648f * X - 0.204043f * Y + 1.057311f * Z);
 }
 
 struct PathCounters {
@@ -256,18 +286,34 @@ struct PathCounters {
 };
 
 // one camera path; returns linear sRGB radiance (before the filter weight)
+template <int NS, bool RGB>
 __device__ f3 trace_path(const SceneView &sc, const RenderView &rv, f3 ray_o, f3 ray_d, Sampler &sampler,
-                         uint2 *stack, uint32_t stride, PathCounters &pc) {
-    Spec3 ret = spec(0.f);
-    Spec3 throughput = spec(1.f);
+                         Swl<NS> &swl, uint2 *stack, uint32_t stride, PathCounters &pc) {
+    f3 ret = mk3(0.f, 0.f, 0.f);
+    Spec<NS> throughput = Spec<NS>::fill(1.f);
     float scatter_pdf = 1e16f;
     float eta_scale = 1.f;
-    f3 prev_ng = ray_d;
+    f3 primary_dir = ray_d;
     uint32_t skips = 0;
+    bool has_env = sc.env_light != kInvalid;
     for (uint32_t bounces = 0; bounces < rv.max_depth; ++bounces) {
         TraceHit h = traverse<false>(sc, ray_o, ray_d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
         ++pc.rays;
-        if (h.tri == kInvalid) break;// no environment light in this mode
+        if (h.tri == kInvalid) {
+            // IlluminationIntegrator::evaluate_miss (integrator.cpp:111-132)
+            if (has_env) {
+                f3 wdir = normalize((ray_o + ray_d) - ray_o);
+                Spec<NS> L;
+                float lpdf;
+                env_eval<NS, RGB>(sc, wdir, swl, L, lpdf);
+                lpdf *= light_select_pmf(sc, sc.env_light);
+                float w = mis_weight(scatter_pdf, lpdf);
+                if (rv.mis_mode == 2) w = 1.f;
+                else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
+                ret += to_linear_srgb<NS, RGB>(sc, L * throughput * w, swl);
+            }
+            break;
+        }
         Surface it = surface_at(sc, h.tri, h.b1, h.b2, true);
         it.wo = normalize(-ray_d);
         const InstanceRecord inst = sc.inst[it.inst];
@@ -280,27 +326,36 @@ __device__ f3 trace_path(const SceneView &sc, const RenderView &rv, f3 ray_o, f3
             continue;
         }
         if (inst.light != kInvalid) {
-            Spec3 Le;
+            Spec<NS> Le;
             float lpdf;
-            eval_hit_light(sc, it, inst.light, h.tri, ray_o, Le, lpdf);
+            eval_hit_light<NS, RGB>(sc, it, inst.light, h.tri, ray_o, swl, Le, lpdf);
             float w = mis_weight(scatter_pdf, lpdf);
             if (rv.mis_mode == 2) w = 1.f;
             else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
-            ret = ret + Le * throughput * w;
+            ret += to_linear_srgb<NS, RGB>(sc, Le * throughput * w, swl);
         }
-        prev_ng = it.ng;
-        (void)prev_ng;
         // next-event estimation
-        LightSampleD ls = sample_light(sc, it, sampler);
-        const MaterialRecord &m = sc.mat[inst.mat];
+        LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, sampler, swl);
+        bool dispersive;
+        MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, swl, dispersive);
+        if (!RGB && dispersive) {
+            // SampledWavelengths::check_dispersive (spectrum.cpp:21-39)
+            bool secondary_valid = true;
+#pragma unroll
+            for (int i = 1; i < NS; ++i) secondary_valid = secondary_valid && swl.pdf[i] != 0.f;
+            if (secondary_valid) {
+#pragma unroll
+                for (int i = 1; i < NS; ++i) swl.pdf[i] = 0.f;
+            }
+        }
         f3 wi_l = normalize(ls.p_light - it.pos);
-        BsdfEval se = eval_material(m, it, wi_l);
-        BsdfSample bs = sample_material(m, it, sampler);
+        EvalN<NS> se = mat_eval(mc, wi_l, nullptr);
+        SampleN<NS> bs = mat_sample(mc, sampler);
         bool delta = ls.pdf < 0.f;
         float w = delta ? 1.f : mis_weight(ls.pdf, se.pdf);
         if (rv.mis_mode == 1) w = 1.f;
         float lpdf = delta ? -ls.pdf : ls.pdf;
-        if (se.pdf > 0.f && lpdf > 0.f) {
+        if (se.pdf > 0.f && lpdf > 0.f && rv.mis_mode != 2) {
             f3 dir = ls.p_light - it.pos;
             f3 n = dot(it.ng, dir) > 0.f ? it.ng : -it.ng;
             uint32_t dn = 0, dt = 0;
@@ -309,27 +364,28 @@ __device__ f3 trace_path(const SceneView &sc, const RenderView &rv, f3 ray_o, f3
             pc.nodes += dn;
             pc.tris += dt;
             ++pc.rays;
-            if (sh.tri == kInvalid && rv.mis_mode != 2) {
-                Spec3 Ld = ls.L * se.f * w / lpdf;
-                ret = ret + throughput * Ld;
+            if (sh.tri == kInvalid) {
+                Spec<NS> Ld = ls.L * se.f * w / lpdf;
+                ret += to_linear_srgb<NS, RGB>(sc, throughput * Ld, swl);
             }
         }
         eta_scale *= bs.eta * bs.eta;
-        float lum = smax(throughput);
-        if (!(bs.eval.pdf > 0.f) || lum == 0.f) break;
-        throughput = throughput * (bs.eval.f / bs.eval.pdf);
+        float lum = throughput.maxc();
+        if (!(bs.e.pdf > 0.f) || lum == 0.f) break;
+        throughput = throughput * (bs.e.f / bs.e.pdf);
         if (eta_scale * lum < rv.rr_threshold && bounces >= rv.min_depth) {
             float q = fminf(0.95f, lum);
             float rr = sampler.next1d();
             if (q < rr) break;
             throughput = throughput / q;
         }
-        scatter_pdf = bs.eval.pdf;
+        scatter_pdf = bs.e.pdf;
         f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
         ray_o = offset_ray_origin(it.pos, n);
         ray_d = bs.wi;
     }
-    return mk3(ret.v[0], ret.v[1], ret.v[2]);
+    (void)primary_dir;
+    return ret;
 }
 
 // Sensor::generate_ray (src/base/sensor/sensor.cpp:44-56) + filter sample
@@ -366,6 +422,35 @@ __device__ __forceinline__ void camera_ray(const CameraView &cam, uint32_t px, u
             m[8] * oc.x + m[9] * oc.y + m[10] * oc.z + m[11]);
     d = mk3(m[0] * dc.x + m[1] * dc.y + m[2] * dc.z, m[4] * dc.x + m[5] * dc.y + m[6] * dc.z,
             m[8] * dc.x + m[9] * dc.y + m[10] * dc.z);
+}
+
+// RenderEnv::initial (src/base/integrator.cpp:49-58): wavelengths come from
+// the stream (pixel, frame, 0xFFFFFFFF) without advancing the main stream
+template <int NS, bool RGB>
+__device__ __forceinline__ Swl<NS> sample_wavelengths(uint32_t px, uint32_t py, uint32_t frame) {
+    Swl<NS> swl;
+    if (RGB) {
+        const float peaks[3] = {602.785f, 539.285f, 445.772f};
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            swl.lambda[i] = peaks[i < 3 ? i : 2];
+            swl.pdf[i] = 1.f;
+        }
+    } else {
+        Sampler t;
+        t.start(px, py, frame, 0xFFFFFFFFu);
+        float u = t.next1d();
+#pragma unroll
+        for (int i = 0; i < NS; ++i) {
+            float offset = float(i) * (1.f / float(NS));
+            float up = u + offset;
+            up = up - floorf(up);
+            float l = sample_visible_wavelength(up);
+            swl.lambda[i] = l;
+            swl.pdf[i] = visible_wavelength_pdf(l);
+        }
+    }
+    return swl;
 }
 
 __device__ __forceinline__ float zero_if_nan_inf(float v) { return (isnan(v) || isinf(v)) ? 0.f : v; }

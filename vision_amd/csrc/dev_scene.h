@@ -30,6 +30,28 @@ struct SceneView {
     float ray_offset_factor;
     uint32_t num_tris;
     uint32_t pad;
+    // spectra: CIE X, Y, Z and D65 tables (94 samples each, at offsets 0, 94,
+    // 188, 282) followed by the materials' ior / eta / k curves
+    const float *spd;
+    const float *lut;// energy-compensation tables (offsets in dev_bsdf.h)
+    float cie_interval;
+    float cie_y_integral;
+    uint32_t spectrum_rgb;// 1: sRGB mode, 0: hero-wavelength mode
+    uint32_t env_light;   // light index of the environment map, 0xFFFFFFFF if none
+    // environment map (spherical light): texels hold linear sRGB (sRGB mode)
+    // or sigmoid coefficients + unbound scale (hero mode)
+    const float *env_tex;
+    const AliasEntry *env_cond;// per-row alias tables, W*H
+    const float *env_cond_func;
+    const AliasEntry *env_marg;// row-marginal alias table, H
+    const float *env_marg_func;
+    uint32_t env_w, env_h;
+    float env_marg_integral;
+    float world_diameter;
+    float env_w2o[9];// row-major 3x3
+    float env_o2w[9];
+    float env_scale;
+    float pad2;
 };
 
 struct CameraView {
@@ -55,7 +77,7 @@ struct RenderView {
     uint32_t tone_mapper;// 0 linear, 1 aces, 2 reinhard
     uint32_t write_output;
     uint32_t count_rays;
-    uint32_t pad;
+    uint32_t spectrum_dim;// hero wavelengths per path (3 in sRGB mode)
 };
 
 // device ray / hit records of the trace entry points (32 B / 24 B)

@@ -48,7 +48,9 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 }
 
 // one thread per pixel; all `spp` frames of the launch accumulate in registers
-// with the film's running-mean arithmetic (normal.cpp:87-101)
+// with the film's running-mean arithmetic (normal.cpp:87-101).  NS: wavelengths
+// carried per path; RGB: sRGB mode (three fixed channels) vs hero wavelengths.
+template <int NS, bool RGB>
 __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
                                                   float4 *__restrict__ rt, float4 *__restrict__ out,
                                                   unsigned long long *__restrict__ counters) {
@@ -61,12 +63,13 @@ __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, Re
     PathCounters pc{0, 0, 0};
     for (uint32_t s = 0; s < rv.spp; ++s) {
         uint32_t frame = rv.frame_start + s;
+        Swl<NS> swl = sample_wavelengths<NS, RGB>(px, py, frame);
         Sampler sampler;
         sampler.start(px, py, frame, 0u);
         f3 o, d;
         float fw;
         camera_ray(cam, px, py, sampler, o, d, fw);
-        f3 L = trace_path(sc, rv, o, d, sampler, lds_stack + threadIdx.x, blockDim.x, pc) * fw;
+        f3 L = trace_path<NS, RGB>(sc, rv, o, d, sampler, swl, lds_stack + threadIdx.x, blockDim.x, pc) * fw;
         float4 v = make_float4(zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f);
         if (rv.accumulation) {
             float a = 1.f / float(frame + 1u);
@@ -111,7 +114,15 @@ hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderVie
     dim3 block(64);
     dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8);
     size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
-    hipLaunchKernelGGL(pt_kernel, grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+    if (sc.spectrum_rgb) {
+        hipLaunchKernelGGL((pt_kernel<3, true>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+    } else if (rv.spectrum_dim == 4) {
+        hipLaunchKernelGGL((pt_kernel<4, false>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+    } else if (rv.spectrum_dim == 3) {
+        hipLaunchKernelGGL((pt_kernel<3, false>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -47,8 +47,11 @@ struct LightRecord {
     float cos_angle;        // spot: cos(angle), computed once on the host
     float cos_falloff_start;// spot: cos(max(0, angle - falloff))
     float pad1, pad2;
+    // hero mode: RGBIlluminationSpectrum of the colour (sigmoid coefficients,
+    // unbound scale), evaluated times D65 and `scale` (hero.cpp:134-152)
+    float illum[4];
 };
-static_assert(sizeof(LightRecord) == 96, "LightRecord layout");
+static_assert(sizeof(LightRecord) == 112, "LightRecord layout");
 
 struct InstanceRecord {
     uint32_t mat;  // material index, 0xFFFFFFFF = none

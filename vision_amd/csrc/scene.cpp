@@ -12,6 +12,8 @@ namespace vs {
 
 namespace {
 
+std::string join_path(const std::string &dir, const std::string &fn);
+
 constexpr float kPi = 3.14159265358979323846f;
 
 float radians(float deg) { return deg * (kPi / 180.f); }
@@ -331,7 +333,7 @@ std::string join_path(const std::string &dir, const std::string &fn) {
     return dir + "/" + fn;
 }
 
-LightHost parse_light(const Json &desc) {
+LightHost parse_light(const Json &desc, const std::string &base_dir) {
     LightHost l;
     std::string type = desc["type"].as_string("area");
     const Json &p = desc["param"];
@@ -355,8 +357,19 @@ LightHost parse_light(const Json &desc) {
         l.falloff = radians(std::clamp(float(p["falloff"].as_float(10.0)), 0.f, l.angle));
         l.direction = normalize(as_f3(p["direction"], mk3(0, 0, 1)));
     } else if (type == "spherical") {
+        // SphericalMap (src/render_core/lights/environments/spherical.cpp:33-45)
         l.type = LIGHT_ENV;
-        l.env_fn = p["color"].is_object() ? p["color"]["node"]["fn"].as_string("") : "";
+        const Json &c = p["color"];
+        const Json &node = c.contains("node") ? c["node"] : c;
+        std::string fn = node["fn"].as_string("");
+        if (fn.empty() && node.contains("param")) fn = node["param"]["fn"].as_string("");
+        if (fn.empty()) throw std::runtime_error("spherical light needs an image (color.node.fn)");
+        l.env_fn = join_path(base_dir, fn);
+        load_pfm(l.env_fn, l.env_rgb, l.env_w, l.env_h);
+        l.color = mk3(1.f);
+        l.scale = float(p["scale"].as_float(1.0));
+        l.flip_u = p["flip_u"].as_bool(false);
+        l.env_o2w = parse_transform(p["o2w"]);
     } else {
         throw std::runtime_error("unsupported light type: " + type);
     }
@@ -364,6 +377,40 @@ LightHost parse_light(const Json &desc) {
 }
 
 }// namespace
+
+void load_pfm(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h) {
+    std::ifstream in(path, std::ios::binary);
+    if (!in) throw std::runtime_error("cannot open image " + path);
+    std::string magic;
+    in >> magic;
+    if (magic != "PF" && magic != "Pf") throw std::runtime_error("unsupported image format (PFM expected): " + path);
+    int iw = 0, ih = 0;
+    float sc = 0.f;
+    in >> iw >> ih >> sc;
+    in.get();
+    if (iw <= 0 || ih <= 0 || iw > 65536 || ih > 65536) throw std::runtime_error("bad PFM size in " + path);
+    int ch = magic == "PF" ? 3 : 1;
+    std::vector<float> raw(size_t(iw) * ih * ch);
+    in.read(reinterpret_cast<char *>(raw.data()), std::streamsize(raw.size() * 4));
+    if (!in) throw std::runtime_error("truncated PFM " + path);
+    if (sc > 0.f) {// big-endian
+        for (float &f : raw) {
+            uint32_t u;
+            std::memcpy(&u, &f, 4);
+            u = __builtin_bswap32(u);
+            std::memcpy(&f, &u, 4);
+        }
+    }
+    w = uint32_t(iw);
+    h = uint32_t(ih);
+    rgb.assign(size_t(w) * h * 3, 0.f);
+    for (uint32_t y = 0; y < h; ++y) {
+        uint32_t src_row = h - 1 - y;// PFM rows run bottom to top
+        for (uint32_t x = 0; x < w; ++x)
+            for (int c = 0; c < 3; ++c)
+                rgb[(size_t(y) * w + x) * 3 + c] = raw[(size_t(src_row) * w + x) * ch + (ch == 3 ? c : 0)];
+    }
+}
 
 m4 camera_to_world(f3 position, float yaw, float pitch) {
     m4 rot = mat_mul(mat_mul(scale_m(mk3(1, 1, -1)), rotation_y(yaw)), rotation_x(-pitch));
@@ -401,7 +448,12 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
     {
         const Json &d = root["spectrum"];
         s.spectrum_type = d["type"].as_string("srgb");
-        if (s.spectrum_type == "hero") s.spectrum_dim = d["param"]["dimension"].as_uint(3);
+        if (s.spectrum_type == "hero") {
+            s.spectrum_dim = d["param"]["dimension"].as_uint(3);
+            if (s.spectrum_dim != 3 && s.spectrum_dim != 4)
+                throw std::runtime_error("unsupported hero spectrum dimension (3 or 4 supported): " +
+                                         std::to_string(s.spectrum_dim));
+        }
         else if (s.spectrum_type == "srgb") s.spectrum_dim = 3;
         else throw std::runtime_error("unsupported spectrum type: " + s.spectrum_type);
     }
@@ -413,6 +465,7 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
     }
     // render_setting (RenderSettingDesc::init)
     s.ray_offset_factor = float(root["render_setting"]["ray_offset_factor"].as_float(1.0));
+    s.min_world_radius = float(root["render_setting"]["min_world_radius"].as_float(10.0));
     // output (OutputDesc::init)
     s.output_spp = root["output"]["spp"].as_uint(0);
     s.output_fn = root["output"]["fn"].as_string("output.png");
@@ -429,7 +482,7 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         s.env_prob = std::clamp(float(p["env_prob"].as_float(0.5)), 0.01f, 0.99f);
         const Json &lights = p["lights"];
         if (lights.is_array())
-            for (const Json &l : lights.elements()) s.lights.push_back(parse_light(l));
+            for (const Json &l : lights.elements()) s.lights.push_back(parse_light(l, base_dir));
     }
 
     // materials (MaterialDesc::init)
@@ -514,7 +567,7 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                     }
                 }
                 if (p.contains("emission")) {
-                    LightHost l = parse_light(p["emission"]);
+                    LightHost l = parse_light(p["emission"], base_dir);
                     if (l.type != LIGHT_AREA) throw std::runtime_error("shape emission must be an area light");
                     l.inst = uint32_t(s.instances.size());
                     inst.light = int32_t(s.lights.size());
@@ -591,6 +644,12 @@ void SceneHost::flatten() {
         ir.light = inst.light >= 0 ? uint32_t(inst.light) : kInvalid;
         ir.tri_base = instTriBase.back();
         instRecords.push_back(ir);
+    }
+    aabb_lo = mk3(3.4e38f);
+    aabb_hi = mk3(-3.4e38f);
+    for (const f3 &p : wP) {
+        aabb_lo = vmin(aabb_lo, p);
+        aabb_hi = vmax(aabb_hi, p);
     }
     // per-triangle areas of emitters (ShapeInstance::surface_areas)
     for (LightHost &l : lights) {

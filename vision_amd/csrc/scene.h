@@ -47,7 +47,12 @@ struct LightHost {
     f3 position{0, 0, 0};
     f3 direction{0, 0, 1};
     float angle{0.f}, falloff{0.f};
+    // environment map (spherical): linear RGB texels, top row first
     std::string env_fn;
+    std::vector<float> env_rgb;
+    uint32_t env_w{0}, env_h{0};
+    m4 env_o2w{m4_identity()};
+    bool flip_u{false};
     // per-emitter triangle area weights (area lights) filled at flatten time
     std::vector<float> tri_areas;
     uint32_t tri_base{0};
@@ -91,6 +96,8 @@ struct SceneHost {
     bool env_separate{false};
     std::string sampler_type{"independent"};
     float ray_offset_factor{1.f};
+    float min_world_radius{10.f};
+    f3 aabb_lo{0, 0, 0}, aabb_hi{0, 0, 0};// world bounds of all triangles (after flatten)
     uint32_t output_spp{0};
     std::string output_fn{"output.png"};
 
@@ -107,6 +114,9 @@ struct SceneHost {
 
     void flatten();
 };
+
+// Read a PFM image (linear RGB float) into top-row-first RGB texels.
+void load_pfm(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h);
 
 // Parse a scene from JSON text; base_dir resolves relative mesh/texture paths.
 SceneHost load_scene(const std::string &text, const std::string &base_dir);

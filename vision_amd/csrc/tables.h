@@ -18,19 +18,33 @@ struct AliasTableHost {
 // (AliasTable::build, src/render_core/warpers/alias.h:86-122)
 AliasTableHost build_alias(const std::vector<float> &weights);
 
-// light power per LightType (area.cpp:87-89, point.cpp:32-34, spot.cpp:44-46)
-f3 light_power(const LightHost &l);
+// light power per LightType (area.cpp:87-89, point.cpp:32-34, spot.cpp:44-46,
+// spherical.cpp:57-60)
+f3 light_power(const LightHost &l, float world_radius);
 
-// Pack materials into 128-byte device records.
-std::vector<MaterialRecord> pack_materials(const SceneHost &s);
-
-struct LightTables {
-    std::vector<LightRecord> records;
+// scene records in device layout
+struct PackedScene {
+    bool rgb{true};
+    std::vector<MaterialRecord> mats;
+    std::vector<float> spd;// CIE X, Y, Z, D65 (4 x 94) then material curves
+    float cie_interval{0.f}, cie_y_integral{0.f};
+    std::vector<LightRecord> lights;
     std::vector<AliasEntry> alias;
     std::vector<float> func;
     uint32_t select_offset{0};
     float select_integral{0.f};
+    // environment map
+    uint32_t env_light{kInvalid};
+    uint32_t env_w{0}, env_h{0};
+    std::vector<float> env_tex;// W*H*4
+    std::vector<AliasEntry> env_cond, env_marg;
+    std::vector<float> env_cond_func, env_marg_func;
+    float env_marg_integral{0.f};
+    float env_w2o[9]{}, env_o2w[9]{};
+    float env_scale{1.f};
+    float world_radius{10.f};
 };
-LightTables pack_lights(const SceneHost &s);
+
+PackedScene pack_scene(const SceneHost &s);
 
 }// namespace vs
