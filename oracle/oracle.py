@@ -13,7 +13,11 @@ import numpy as np
 
 HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
+# numeric tables (data files, not code): energy-compensation LUTs, CIE 1931
+# and D65 at 1 nm, and the generated sRGB -> sigmoid coefficient table
+DATA = HERE.parent / "vision_amd" / "data"
 _lib = None
+_data: dict = {}
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
 _f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
@@ -48,7 +52,19 @@ def lib():
                                _u32p, _f32p, _f32p]
         h.or_render.restype = C.c_uint64
         h.or_render.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
-                                [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, _f32p])
+                                [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, C.c_void_p, _f32p])
+        h.or_cie_tables.argtypes = [_f32p, _f32p, C.POINTER(C.c_float)]
+        h.or_spd_eval.argtypes = [_f32p, C.c_float, _f32p, C.c_uint32, _f32p]
+        h.or_wavelengths.argtypes = [_f32p, C.c_uint32, _f32p, _f32p]
+        h.or_wavelength_pdf.argtypes = [_f32p, C.c_uint32, _f32p]
+        h.or_sigmoid.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, _f32p]
+        h.or_rgb_to_spectrum.argtypes = [_f32p, _f32p, C.c_uint32, C.c_int, _f32p]
+        h.or_fresnel.argtypes = [C.c_int, _f32p, C.c_uint32, C.c_float, C.c_float, _f32p]
+        h.or_ggx.argtypes = [C.c_float, C.c_float, _f32p, _f32p, C.c_uint32, _f32p, _f32p, _f32p]
+        h.or_ggx_sample.argtypes = [C.c_float, C.c_float, _f32p, _f32p, C.c_uint32, _f32p]
+        h.or_bsdf_eval.argtypes = [C.c_void_p, _u32p, _f32p, _f32p, C.c_void_p, C.c_uint32, _f32p, _f32p]
+        h.or_bsdf_sample.argtypes = [C.c_void_p, _u32p, _f32p, C.c_void_p, C.c_uint32, C.c_uint32, _f32p, _f32p,
+                                     _f32p]
         _lib = h
     return _lib
 
@@ -153,15 +169,200 @@ def render(scene: dict, frame_start: int = 0, spp: int = 1, window=None, bvh: bo
            film: np.ndarray | None = None) -> np.ndarray:
     """Render the exported scene (vision_amd.Scene.export()) on the CPU.
     Returns the running-mean film float32[H, W, 4] (updated in place if given)."""
+    spec = Spectral(scene) if "spd" in scene else None
     cam = np.ascontiguousarray(scene["camera"], np.float32)
     W, H = int(cam[19]), int(cam[20])
     if film is None:
         film = np.zeros((H, W, 4), np.float32)
     x0, y0, x1, y1 = window if window is not None else (0, 0, W, H)
-    a = {k: np.ascontiguousarray(v) for k, v in scene.items()}
+    a = {k: np.ascontiguousarray(v) for k, v in scene.items() if isinstance(v, np.ndarray)}
     lib().or_render(a["positions"].reshape(-1), a["normals"].reshape(-1), a["uvs"].reshape(-1),
                     a["triangles"].reshape(-1), a["tri_instance"], a["instances"].reshape(-1),
                     a["materials"].reshape(-1), a["lights"].reshape(-1), cam, a["render"],
                     len(a["positions"]), len(a["triangles"]), len(a["instances"]), len(a["materials"]),
-                    len(a["lights"]), x0, y0, x1, y1, frame_start, spp, 1 if bvh else 0, threads, film)
+                    len(a["lights"]), x0, y0, x1, y1, frame_start, spp, 1 if bvh else 0, threads,
+                    None if spec is None else spec.ptr, film)
     return film
+
+
+# ---------------------------------------------------------------- spectral inputs
+class OrSpectral(C.Structure):
+    """Mirror of `or_spectral` in oracle.cpp."""
+    _fields_ = [("rgb", C.c_uint32), ("ns", C.c_uint32), ("spd", C.c_void_p), ("cie_y_integral", C.c_float),
+                ("lut_diel", C.c_void_p), ("lut_diel_inv", C.c_void_p), ("lut_spec", C.c_void_p),
+                ("lut_coat", C.c_void_p), ("lut_pure", C.c_void_p), ("lut_sheen", C.c_void_p),
+                ("rgb2spec", C.c_void_p), ("env_light", C.c_uint32), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
+                ("env_rgb", C.c_void_p), ("env_w2o", C.c_float * 9), ("env_o2w", C.c_float * 9),
+                ("env_scale", C.c_float), ("world_radius", C.c_float)]
+
+
+def data(name: str, count: int | None = None) -> np.ndarray:
+    """A float32 data table from vision_amd/data (cached)."""
+    if name not in _data:
+        arr = np.fromfile(DATA / name, dtype="<f4")
+        if count is not None and arr.size != count:
+            raise ValueError(f"{name}: {arr.size} floats, expected {count}")
+        _data[name] = np.ascontiguousarray(arr, np.float32)
+    return _data[name]
+
+
+LUTS = {"lut_diel": ("lut_dielectric.f32", 65536), "lut_diel_inv": ("lut_dielectric_inv.f32", 65536),
+        "lut_spec": ("lut_specular.f32", 32768), "lut_coat": ("lut_coat.f32", 32768),
+        "lut_pure": ("lut_pure_reflection.f32", 1024), "lut_sheen": ("lut_sheen_approx.f32", 4096)}
+
+
+def cie_tables() -> tuple[np.ndarray, float]:
+    """(CIE X, Y, Z, D65 downsampled to 4 x 94, CIE Y integral) from the 1 nm data."""
+    raw = data("cie_xyz_d65.f32", 4 * 471)
+    out = np.zeros(4 * 94, np.float32)
+    yi = C.c_float()
+    lib().or_cie_tables(raw, out, C.byref(yi))
+    return out, yi.value
+
+
+def rgb2spec_table() -> np.ndarray:
+    return data("rgb2spec_srgb.f32", 3 * 64 * 64 * 64 * 3)
+
+
+class Spectral:
+    """Holds the arrays an OrSpectral points at (keep it alive during calls)."""
+
+    def __init__(self, scene: dict | None = None, rgb: bool = True, ns: int = 3, spd: np.ndarray | None = None):
+        if scene is not None:
+            rgb = bool(scene["spectrum_rgb"])
+            ns = 3 if rgb else int(scene["spectrum_dim"])
+            spd = scene["spd"]
+        cie, y_integral = cie_tables()
+        self.spd = np.ascontiguousarray(spd if spd is not None else cie, np.float32)
+        self.luts = {k: data(*v) for k, v in LUTS.items()}
+        self.table = rgb2spec_table() if not rgb else np.zeros(1, np.float32)
+        s = OrSpectral()
+        s.rgb = 1 if rgb else 0
+        s.ns = ns
+        s.spd = self.spd.ctypes.data
+        s.cie_y_integral = y_integral
+        for k, v in self.luts.items():
+            setattr(s, k, v.ctypes.data)
+        s.rgb2spec = self.table.ctypes.data
+        s.env_light = 0xFFFFFFFF
+        s.env_scale = 1.0
+        s.world_radius = 1.0
+        if scene is not None:
+            s.world_radius = scene["world_radius"]
+            if scene["env_light"] != 0xFFFFFFFF:
+                self.env_rgb = np.ascontiguousarray(scene["env_rgb"], np.float32)
+                s.env_light = scene["env_light"]
+                s.env_h, s.env_w = self.env_rgb.shape[:2]
+                s.env_rgb = self.env_rgb.ctypes.data
+                s.env_w2o[:] = [float(x) for x in np.asarray(scene["env_w2o"]).reshape(-1)]
+                s.env_o2w[:] = [float(x) for x in np.asarray(scene["env_o2w"]).reshape(-1)]
+                s.env_scale = scene["env_scale"]
+        self.s = s
+
+    @property
+    def ptr(self):
+        return C.byref(self.s)
+
+
+def spd_eval(tab: np.ndarray, interval: float, lam: np.ndarray) -> np.ndarray:
+    tab = np.ascontiguousarray(tab, np.float32)
+    lam = np.ascontiguousarray(lam, np.float32).reshape(-1)
+    out = np.zeros_like(lam)
+    lib().or_spd_eval(tab, interval, lam, len(lam), out)
+    return out
+
+
+def wavelengths(u: np.ndarray) -> tuple[np.ndarray, np.ndarray]:
+    u = np.ascontiguousarray(u, np.float32).reshape(-1)
+    lam = np.zeros_like(u)
+    pdf = np.zeros_like(u)
+    lib().or_wavelengths(u, len(u), lam, pdf)
+    return lam, pdf
+
+
+def wavelength_pdf(lam: np.ndarray) -> np.ndarray:
+    lam = np.ascontiguousarray(lam, np.float32).reshape(-1)
+    out = np.zeros_like(lam)
+    lib().or_wavelength_pdf(lam, len(lam), out)
+    return out
+
+
+def sigmoid(coef: np.ndarray, lam: np.ndarray) -> np.ndarray:
+    """Sigmoid spectra float32[ncoef, nlambda] of coefficient rows (c0, c1, c2, s)."""
+    coef = np.ascontiguousarray(coef, np.float32).reshape(-1, 4)
+    lam = np.ascontiguousarray(lam, np.float32).reshape(-1)
+    out = np.zeros((len(coef), len(lam)), np.float32)
+    lib().or_sigmoid(coef, len(coef), lam, len(lam), out)
+    return out
+
+
+def rgb_to_spectrum(rgb: np.ndarray, unbound: bool = False) -> np.ndarray:
+    rgb = np.ascontiguousarray(rgb, np.float32).reshape(-1, 3)
+    out = np.zeros((len(rgb), 4), np.float32)
+    lib().or_rgb_to_spectrum(rgb2spec_table(), rgb, len(rgb), 1 if unbound else 0, out)
+    return out
+
+
+def fresnel(cos_t: np.ndarray, eta: float, k: float | None = None) -> np.ndarray:
+    cos_t = np.ascontiguousarray(cos_t, np.float32).reshape(-1)
+    out = np.zeros_like(cos_t)
+    lib().or_fresnel(0 if k is not None else 1, cos_t, len(cos_t), eta, 0.0 if k is None else k, out)
+    return out
+
+
+def ggx(ax: float, ay: float, wo: np.ndarray, wh: np.ndarray):
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wh = np.ascontiguousarray(wh, np.float32).reshape(-1, 3)
+    n = len(wo)
+    D, G1, pdf = (np.zeros(n, np.float32) for _ in range(3))
+    lib().or_ggx(ax, ay, wo, wh, n, D, G1, pdf)
+    return D, G1, pdf
+
+
+def ggx_sample(ax: float, ay: float, wo: np.ndarray, u: np.ndarray) -> np.ndarray:
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    out = np.zeros_like(wo)
+    lib().or_ggx_sample(ax, ay, wo, u, len(wo), out)
+    return out
+
+
+def material_record(type_: int, params, flags: int = 0, uint_slots: dict | None = None) -> np.ndarray:
+    """A 32-word material record: type, flags, 30 float parameters; uint_slots
+    maps a parameter index to an integer stored bitwise (SPD offsets)."""
+    rec = np.zeros(32, np.uint32)
+    rec[0] = type_
+    rec[1] = flags
+    p = np.zeros(30, np.float32)
+    p[:len(params)] = params
+    rec[2:] = p.view(np.uint32)
+    for k, v in (uint_slots or {}).items():
+        rec[2 + k] = v
+    return rec
+
+
+def bsdf_eval(spec: Spectral, rec: np.ndarray, wo: np.ndarray, wi: np.ndarray, lam: np.ndarray | None = None):
+    """f * |cos wi| (float32[n, 4]) and pdf of material `rec` in a frame whose
+    shading and geometric normal is +z."""
+    rec = np.ascontiguousarray(rec, np.uint32)
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    wi = np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+    n = len(wo)
+    lam_a = None if lam is None else np.ascontiguousarray(lam, np.float32).reshape(n, 4)
+    f = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().or_bsdf_eval(spec.ptr, rec, wo, wi, None if lam_a is None else lam_a.ctypes.data, n, f, pdf)
+    return f, pdf
+
+
+def bsdf_sample(spec: Spectral, rec: np.ndarray, wo: np.ndarray, frame: int = 0, lam: np.ndarray | None = None):
+    """Sampled wi (float32[n, 3]), f * |cos wi| and pdf for each wo."""
+    rec = np.ascontiguousarray(rec, np.uint32)
+    wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+    n = len(wo)
+    lam_a = None if lam is None else np.ascontiguousarray(lam, np.float32).reshape(n, 4)
+    wi = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    lib().or_bsdf_sample(spec.ptr, rec, wo, None if lam_a is None else lam_a.ctypes.data, n, frame, wi, f, pdf)
+    return wi, f, pdf

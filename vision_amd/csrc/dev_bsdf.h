@@ -188,7 +188,7 @@ __device__ __forceinline__ Spec<NS> fresnel_f82(const Spec<NS> &F0, const Spec<N
     float mu2 = mu * mu;
     float mu5 = mu2 * mu2 * mu;
     Spec<NS> fs = F0 + mu5 * (1.f - F0);
-    return saturate(fs - B * (cos_t * mu5 * mu));
+    return saturate(fs - ((B * cos_t) * mu5) * mu);
 }
 
 }// namespace dev

@@ -179,7 +179,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
             cr = cr * cr;
             c.cc_alpha = cr;
             c.cc_eta = p[24];
-            c.cc_kr = eval_albedo<NS, RGB>(p + 12, swl) * weight * cw;
+            c.cc_kr = (cw * weight) * eval_albedo<NS, RGB>(p + 12, swl);
             float s;
             lut3<1>(sc.lut + LUT_COAT, 32, sqrtf(sqrtf(cr * cr)), cos_theta, (c.cc_eta - 1.003f) / (4.f - 1.003f), &s);
             Spec<NS> cc_albedo = c.cc_kr * s;
@@ -252,7 +252,7 @@ __device__ __forceinline__ void eval_conductor_like(const Ggx &g, f3 wo, f3 wi, 
 #pragma unroll
         for (int i = 0; i < NS; ++i) F.v[i] = fresnel_conductor(c, a.v[i], b.v[i]);
     }
-    e.f = s ? F * kr * g.brdf(wo, whf, wi) : Spec<NS>::fill(0.f);
+    e.f = s ? (g.brdf(wo, whf, wi) * F) * kr : Spec<NS>::fill(0.f);
     e.pdf = s ? g.pdf_reflection(wo, wh) : 0.f;
     e.f = e.f * comp;
     e.flags = 0;
@@ -267,7 +267,7 @@ __device__ __forceinline__ void eval_reflection_dielectric(const Ggx &g, f3 wo, 
     f3 whf = face_forward(wh, mk3(0.f, 0.f, 1.f));
     float c = fabsf(dot(wo, whf));
     Spec<NS> F = schlick ? fresnel_schlick(F0, eta, c) : Spec<NS>::fill(fresnel_dielectric(c, eta));
-    e.f = s ? F * kr * g.brdf(wo, whf, wi) : Spec<NS>::fill(0.f);
+    e.f = s ? (g.brdf(wo, whf, wi) * F) * kr : Spec<NS>::fill(0.f);
     e.pdf = s ? g.pdf_reflection(wo, wh) : 0.f;
     e.flags = 0;
 }
@@ -297,7 +297,7 @@ __device__ __forceinline__ void eval_dielectric(const Ggx &g, f3 wo, f3 wi, cons
         e.flags = 0;
     } else {
         f3 wh2 = normalize(wo + wi * eta0);
-        e.f = T * kt * g.btdf(wo, wh2, wi, eta0);
+        e.f = (g.btdf(wo, wh2, wi, eta0) * T) * kt;
         e.pdf = g.pdf_transmission(wo, wh, wi, eta0) * (1.f - rp);
         e.flags = FLAG_TRANSMISSION;
     }
@@ -312,7 +312,7 @@ __device__ __forceinline__ void eval_sheen(float a, float b, const Spec<NS> &tin
     w = w / l;
     float jac = (a * a) / (l * l * l);
     float ltc = w.z * kInvPi * jac;// cosine_hemisphere_PDF(cos_theta(w)) * jacobian
-    e.f = (wi.z < 0.f || wo.z < 0.f) ? Spec<NS>::fill(0.f) : tint * (ltc / wi.z);
+    e.f = (wi.z < 0.f || wo.z < 0.f) ? Spec<NS>::fill(0.f) : (ltc * tint) / wi.z;
     e.pdf = ltc;
     e.flags = 0;
 }
@@ -321,7 +321,7 @@ template <int NS>
 __device__ __forceinline__ void eval_lambert(const Spec<NS> &kr, f3 wo, f3 wi, float A, float B, bool oren,
                                              EvalN<NS> &e) {
     bool s = wo.z * wi.z > 0.f;
-    float k = kInvPi;
+    Spec<NS> f = kr * kInvPi;
     if (oren) {
         float sti = sqrtf(fmaxf(0.f, 1.f - wi.z * wi.z)), sto = sqrtf(fmaxf(0.f, 1.f - wo.z * wo.z));
         float cpi = sti == 0.f ? 1.f : fminf(fmaxf(wi.x / sti, -1.f), 1.f);
@@ -332,9 +332,9 @@ __device__ __forceinline__ void eval_lambert(const Spec<NS> &kr, f3 wo, f3 wi, f
         bool cond = fabsf(wi.z) > fabsf(wo.z);
         float sin_a = cond ? sto : sti;
         float tan_b = cond ? sti / fabsf(wi.z) : sto / fabsf(wo.z);
-        k = kInvPi * (A + B * maxc * sin_a * tan_b);
+        f = f * (A + B * maxc * sin_a * tan_b);
     }
-    e.f = s ? kr * k : Spec<NS>::fill(0.f);
+    e.f = s ? f : Spec<NS>::fill(0.f);
     e.pdf = s ? fabsf(wi.z) * kInvPi : 0.f;
     e.flags = 0;
 }
@@ -373,7 +373,7 @@ __device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, floa
         e.f = e.f + t.f * (acos * same);
         e.pdf += t.pdf * c.sw[2] * same;
         eval_dielectric(c.ggx, wo, wi, c.tr_F0, c.tr_eta, true, c.tr_F0, c.tr_kt, c.tr_comp, t, eta_out);
-        e.f = e.f + t.f * (acos * c.tr_weight);
+        e.f = e.f + (t.f * acos) * c.tr_weight;
         e.pdf += t.pdf * c.sw[3];
         e.flags |= t.flags;
         eval_reflection_dielectric(c.ggx, wo, wi, c.sp_eta, true, c.sp_F0, c.sp_kr, t);

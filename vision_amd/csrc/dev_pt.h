@@ -310,7 +310,7 @@ __device__ f3 trace_path(const SceneView &sc, const RenderView &rv, f3 ray_o, f3
                 float w = mis_weight(scatter_pdf, lpdf);
                 if (rv.mis_mode == 2) w = 1.f;
                 else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
-                ret += to_linear_srgb<NS, RGB>(sc, L * throughput * w, swl);
+                ret += to_linear_srgb<NS, RGB>(sc, (L * w) * throughput, swl);
             }
             break;
         }

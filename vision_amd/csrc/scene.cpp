@@ -338,10 +338,12 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
     std::string type = desc["type"].as_string("area");
     const Json &p = desc["param"];
     l.name = desc["name"].as_string("");
-    l.color = slot_f3(p["color"], mk3(1.f));
     l.scale = float(p["scale"].as_float(1.0));
-    // Light::Light folds the colour normalization factor into scale
-    l.scale *= normalize_color(l.color);
+    if (type != "spherical") {
+        // Light::Light folds the colour normalization factor into scale
+        l.color = slot_f3(p["color"], mk3(1.f));
+        l.scale *= normalize_color(l.color);
+    }
     if (type == "area") {
         l.type = LIGHT_AREA;
         l.two_sided = p["two_sided"].as_bool(false) ? 1u : 0u;

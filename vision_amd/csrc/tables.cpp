@@ -233,8 +233,9 @@ f3 light_power(const LightHost &l, float world_radius) {
             size_t n = size_t(l.env_w) * l.env_h;
             for (size_t i = 0; i < n; ++i)
                 for (int c = 0; c < 3; ++c) acc[c] += l.env_rgb[3 * i + c];
-            f3 mean = n ? mk3(float(acc[0] / n), float(acc[1] / n), float(acc[2] / n)) : mk3(0.f);
-            return mean * (l.scale * kPiF * world_radius * world_radius);
+            // Pi * sqr(world_radius) * average(), average = mean texel * scale
+            f3 mean = n ? mk3(float(acc[0] / double(n)), float(acc[1] / double(n)), float(acc[2] / double(n))) : mk3(0.f);
+            return (mean * l.scale) * (kPiF * (world_radius * world_radius));
         }
         default: return mk3(0.f);
     }

@@ -150,6 +150,28 @@ def lit_plane(light: dict, albedo=0.5, cam_height=1.0, fov=2.0, width=16, height
     }
 
 
+def write_pfm(path: str | Path, rgb: np.ndarray) -> Path:
+    """Write float32 RGB[h, w, 3] (top row first) as a little-endian PFM,
+    whose rows run bottom to top."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    img = np.ascontiguousarray(np.asarray(rgb, np.float32)[::-1])
+    h, w = img.shape[:2]
+    with open(path, "wb") as f:
+        f.write(f"PF\n{w} {h}\n-1.0\n".encode())
+        f.write(img.astype("<f4").tobytes())
+    return path
+
+
+def env_light(fn: str, scale: float = 1.0, o2w: dict | None = None, flip_u: bool = False) -> dict:
+    """A spherical (HDRI) environment light over image file `fn`."""
+    p = {"color": {"channels": "xyz", "node": {"fn": fn, "color_space": "linear"}}, "scale": scale,
+         "flip_u": flip_u}
+    if o2w is not None:
+        p["o2w"] = o2w
+    return {"type": "spherical", "param": p}
+
+
 def write_scene(scene: dict, path: str | Path) -> Path:
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
