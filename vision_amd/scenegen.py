@@ -172,6 +172,180 @@ def env_light(fn: str, scale: float = 1.0, o2w: dict | None = None, flip_u: bool
     return {"type": "spherical", "param": p}
 
 
+# ---------------------------------------------------------------- procedural scenes
+def icosphere(level: int) -> tuple[np.ndarray, np.ndarray]:
+    """Unit icosphere: 20 * 4^level triangles (counter-clockwise seen from outside)."""
+    t = (1 + 5 ** 0.5) / 2
+    P = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    F = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    P /= np.linalg.norm(P, axis=1, keepdims=True)
+    for _ in range(level):
+        e = np.concatenate([F[:, [0, 1]], F[:, [1, 2]], F[:, [2, 0]]])
+        key = np.sort(e, axis=1)
+        uniq, inv = np.unique(key, axis=0, return_inverse=True)
+        mid = P[uniq[:, 0]] + P[uniq[:, 1]]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = len(P) + inv.reshape(3, -1)  # midpoint index of edges (01, 12, 20) per face
+        a, b, c = F[:, 0], F[:, 1], F[:, 2]
+        m01, m12, m20 = m[0], m[1], m[2]
+        F = np.concatenate([np.stack([a, m01, m20], 1), np.stack([b, m12, m01], 1), np.stack([c, m20, m12], 1),
+                            np.stack([m01, m12, m20], 1)])
+        P = np.concatenate([P, mid])
+    return P, F
+
+
+def value_noise3(p: np.ndarray, seed: int) -> np.ndarray:
+    """Seeded 3D lattice value noise in [-1, 1] with smoothstep interpolation."""
+    i = np.floor(p).astype(np.int64)
+    f = p - i
+    w = f * f * (3 - 2 * f)
+
+    def h(ix, iy, iz):
+        x = (ix * 73856093) ^ (iy * 19349663) ^ (iz * 83492791) ^ (seed * 2654435761)
+        x = (x ^ (x >> 13)) * 1274126177
+        x = x ^ (x >> 16)
+        return (x & 0xFFFFFF).astype(np.float64) / float(0xFFFFFF) * 2 - 1
+
+    out = 0.0
+    for dz in (0, 1):
+        for dy in (0, 1):
+            for dx in (0, 1):
+                wx = w[:, 0] if dx else 1 - w[:, 0]
+                wy = w[:, 1] if dy else 1 - w[:, 1]
+                wz = w[:, 2] if dz else 1 - w[:, 2]
+                out = out + wx * wy * wz * h(i[:, 0] + dx, i[:, 1] + dy, i[:, 2] + dz)
+    return out
+
+
+def vertex_normals(P: np.ndarray, F: np.ndarray) -> np.ndarray:
+    n = np.cross(P[F[:, 1]] - P[F[:, 0]], P[F[:, 2]] - P[F[:, 0]])
+    N = np.zeros_like(P)
+    for k in range(3):
+        np.add.at(N, F[:, k], n)
+    return N / np.linalg.norm(N, axis=1, keepdims=True)
+
+
+def write_vsmesh(path: str | Path, P: np.ndarray, F: np.ndarray, N: np.ndarray | None = None) -> Path:
+    """Binary mesh: "VSMESH01", u32 nv, nt, flags (1: normals), f32 P, [f32 N], u32 tri."""
+    path = Path(path)
+    path.parent.mkdir(parents=True, exist_ok=True)
+    with open(path, "wb") as f:
+        f.write(b"VSMESH01")
+        f.write(np.array([len(P), len(F), 1 if N is not None else 0], "<u4").tobytes())
+        f.write(np.asarray(P, "<f4").tobytes())
+        if N is not None:
+            f.write(np.asarray(N, "<f4").tobytes())
+        f.write(np.asarray(F, "<u4").tobytes())
+    return path
+
+
+def sky_hdri(width: int = 2048, height: int = 1024, seed: int = 0, sun_peak: float = 5e4,
+             sun_sigma_deg: float = 0.75) -> tuple[np.ndarray, tuple[float, float]]:
+    """Seeded sky gradient plus one Gaussian sun lobe (peak radiance sun_peak)
+    in the equirectangular layout of the spherical light: row v = theta / pi
+    from the zenith (world +y), column u = phi / 2pi with phi measured from
+    world +x toward +z.  Returns (float32 RGB[h, w, 3], (elevation, azimuth))."""
+    rng = np.random.default_rng(seed)
+    theta = (np.arange(height) + 0.5) / height * np.pi
+    phi = (np.arange(width) + 0.5) / width * 2 * np.pi
+    T, Ph = np.meshgrid(theta, phi, indexing="ij")
+    up = np.cos(T)
+    zenith = np.array([0.25, 0.45, 0.95]) * rng.uniform(0.9, 1.2)
+    horizon = np.array([0.95, 0.95, 1.0]) * rng.uniform(1.5, 2.5)
+    ground = np.array([0.12, 0.10, 0.08])
+    k = np.clip(up, 0, 1)[..., None] ** 0.5
+    sky = horizon * (1 - k) + zenith * k
+    img = np.where(up[..., None] > 0, sky, ground * (1 + 0.3 * np.clip(-up, 0, 1))[..., None])
+    elev = math.radians(rng.uniform(25, 50))
+    azim = rng.uniform(0, 2 * np.pi)
+    d = np.stack([np.sin(T) * np.cos(Ph), np.cos(T), np.sin(T) * np.sin(Ph)], -1)
+    s = np.array([math.cos(elev) * math.cos(azim), math.sin(elev), math.cos(elev) * math.sin(azim)])
+    ang = np.arccos(np.clip(d @ s, -1, 1))
+    sig = math.radians(sun_sigma_deg)
+    img = img + sun_peak * np.exp(-0.5 * (ang / sig) ** 2)[..., None] * np.array([1.0, 0.95, 0.85])
+    return img.astype(np.float32), (elev, azim)
+
+
+MATERIAL_FAMILIES = ("matte", "metal", "glass", "disney")
+
+
+def procedural_scene(out_dir: str | Path, seed: int = 0, n_instances: int = 64, level: int = 5,
+                     width: int = 1920, height: int = 1080, spp: int = 64, max_depth: int = 16,
+                     spectrum: str = "hero", dimension: int = 4, hdri=(2048, 1024),
+                     displacement: float = 0.1) -> dict:
+    """Seeded instances of a displaced level-`level` icosphere (20 * 4^level
+    triangles each, value-noise displacement of the given amplitude) on a
+    2-triangle ground plane, materials drawn uniformly from matte, metal (GGX
+    roughness U[0.05, 0.5], Cu / Au / Al), glass (Cauchy A = 1.5,
+    B = 0.004 um^2) and Disney (principled), lit by a sky + sun HDRI.
+    Meshes and the HDRI are written into out_dir; returns the scene dict
+    (load it with base_dir = out_dir)."""
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    rng = np.random.default_rng(seed)
+    P0, F = icosphere(level)
+    side = int(math.ceil(math.sqrt(n_instances)))
+    spacing = 2.6
+    shapes = [{"type": "quad", "name": "ground",
+               "param": {"material": "Ground", "transform": xform(scale(400, 1, 400))}}]
+    materials = [diffuse("Ground", (0.45, 0.45, 0.42))]
+    for i in range(n_instances):
+        mesh = out / f"sphere_{seed}_{level}_{i}.vsmesh"
+        if not mesh.exists():
+            noise = value_noise3(P0 * 2.5 + rng.uniform(-100, 100, 3), seed * 1000 + i)
+            noise += 0.5 * value_noise3(P0 * 5.0 + rng.uniform(-100, 100, 3), seed * 1000 + i + 500)
+            P = P0 * (1 + displacement * noise / 1.5)[:, None]
+            write_vsmesh(mesh, P, F, vertex_normals(P, F))
+        else:
+            rng.uniform(-100, 100, 6)  # keep the stream aligned with a fresh generation
+        gx, gz = i % side, i // side
+        r = rng.uniform(0.7, 1.1)
+        cx = (gx - (side - 1) / 2) * spacing + rng.uniform(-0.3, 0.3)
+        cz = (gz - (side - 1) / 2) * spacing + rng.uniform(-0.3, 0.3)
+        m = translate(cx, r * (1 + displacement), cz) @ rot_y(rng.uniform(0, 360)) @ scale(r, r, r)
+        fam = MATERIAL_FAMILIES[int(rng.integers(0, 4))]
+        name = f"M{i}"
+        if fam == "matte":
+            mat = {"type": "diffuse", "name": name, "param": {"color": [float(c) for c in rng.uniform(0.1, 0.9, 3)]}}
+        elif fam == "metal":
+            mat = {"type": "metal", "name": name,
+                   "param": {"material_name": ["Cu", "Au", "Al"][int(rng.integers(0, 3))],
+                             "roughness": float(rng.uniform(0.05, 0.5))}}
+        elif fam == "glass":
+            mat = {"type": "glass", "name": name,
+                   "param": {"material_name": "Cauchy", "cauchy_a": 1.5, "cauchy_b": 0.004, "roughness": 0.05,
+                             "color": [1.0, 1.0, 1.0]}}
+        else:
+            mat = {"type": "principled_bsdf", "name": name,
+                   "param": {"color": [float(c) for c in rng.uniform(0.1, 0.9, 3)],
+                             "metallic": float(rng.uniform(0, 1)), "roughness": float(rng.uniform(0.1, 0.8)),
+                             "ior": 1.5, "spec_tint": [1.0, 1.0, 1.0],
+                             "sheen_weight": float(rng.uniform(0, 0.5)),
+                             "coat_weight": float(rng.uniform(0, 0.5)), "coat_roughness": 0.1,
+                             "transmission_weight": 0.0}}
+        materials.append(mat)
+        shapes.append({"type": "model", "name": f"sphere{i}",
+                       "param": {"fn": mesh.name, "material": name, "transform": xform(m)}})
+    hdr = out / f"sky_{seed}_{hdri[0]}x{hdri[1]}.pfm"
+    if not hdr.exists():
+        img, _ = sky_hdri(hdri[0], hdri[1], seed)
+        write_pfm(hdr, img)
+    spec = {"type": "srgb"} if spectrum == "srgb" else {"type": "hero", "param": {"dimension": dimension}}
+    return {
+        "shapes": shapes,
+        "materials": materials,
+        "light_sampler": {"type": "power", "param": {"lights": [env_light(hdr.name)]}},
+        "sampler": {"type": "independent", "param": {"spp": 1}},
+        "spectrum": spec,
+        "integrator": {"type": "pt", "param": {"min_depth": 5, "max_depth": max_depth, "rr_threshold": 1}},
+        "camera": camera((0.0, 9.0, 23.0), (0.0, 0.6, 0.0), 38.0, width, height),
+        "output": {"spp": spp, "fn": "procedural.exr"},
+    }
+
+
 def write_scene(scene: dict, path: str | Path) -> Path:
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
