@@ -162,6 +162,16 @@ vs_status vs_trace_any(vs_scene *scene, const vs_ray *rays, uint32_t n, uint8_t 
 vs_status vs_trace_closest_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, vs_hit *d_hits, void *stream);
 vs_status vs_trace_any_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, uint8_t *d_occluded, void *stream);
 
+/* Material-level parity hook: MaterialEvaluator::evaluate (sample = 0) or
+ * ::sample (sample = 1) of scene material `mat` at a surface whose shading
+ * frame is the world axes and whose geometric normal is +z
+ * (src/base/scattering/material.cpp:132-184).  Host arrays: wo, wi, wi_out
+ * [n][3]; lambda [n][4] (hero mode only: the sampled wavelengths); f [n][4]
+ * (f * |cos wi| per wavelength, unused channels 0); pdf [n].  Sampling draws
+ * from the sampler stream (pixel (0, i), index `frame`, dim 0). */
+vs_status vs_scene_bsdf(vs_scene *scene, uint32_t mat, uint32_t n, const float *wo, const float *wi,
+                        const float *lambda, int32_t sample, uint32_t frame, float *wi_out, float *f, float *pdf);
+
 /* Add `spp` progressive frames to the film (frame indices continue from the
  * current one).  fused = 0: one launch per frame, as
  * PathTracingIntegrator::render does (src/render_core/integrators/pt.cpp:90-99);

@@ -2,10 +2,14 @@
 the HDRI environment light (MI355X).
 
 Same seeded sampler streams on both sides: per-pixel radiance within 1e-4
-relative, except the few paths where a float rounding difference of a
-transcendental function (device vs host libm) flips a discrete decision
-(Russian roulette, lobe choice, reflect vs refract); those are counted and
-bounded, and the film means must agree closely.
+relative.  The device's sin/cos/acos/atan2 differ from the host libm's in
+the last bit; along a path such differences stay in the last bits until a
+discrete decision flips (reflect vs refract, Russian roulette, an edge hit),
+after which the two paths are independent samples.  Glass makes this
+compound with depth.  So: the material layer alone matches direction by
+direction, shallow (depth 2) renders match pixel for pixel, and full-depth
+renders match on nearly all pixels, with the diverged remainder consistent
+with zero mean difference (a z-test on the per-pixel differences).
 """
 import numpy as np
 import pytest
@@ -25,6 +29,13 @@ def _compare(gpu, ref):
     err = np.abs(g - r)
     ok = (err <= REL_TOL * np.maximum(np.abs(g), np.abs(r)) + ABS_FLOOR).all(axis=-1)
     return ok.mean(), g.mean(), r.mean()
+
+
+def _mean_z(gpu, ref):
+    """|mean difference| in units of its standard error over pixels."""
+    d = (gpu[..., :3].astype(np.float64) - ref[..., :3].astype(np.float64)).mean(-1).ravel()
+    se = d.std() / np.sqrt(d.size)
+    return 0.0 if se == 0 else abs(d.mean()) / se
 
 
 TEST_MATERIALS = {
@@ -69,11 +80,16 @@ def _render_both(ctx, scene, spp, base_dir="."):
 @pytest.mark.parametrize("mat", list(TEST_MATERIALS))
 def test_material_parity(gpu_ctx, mat, spectrum):
     print(f"case: material {mat} spectrum {spectrum}", flush=True)
-    gpu, ref = _render_both(gpu_ctx, _material_box(mat, spectrum), 2)
+    shallow, sref = _render_both(gpu_ctx, _material_box(mat, spectrum, depth=2), 4)
+    sfrac, _, _ = _compare(shallow, sref)
+    gpu, ref = _render_both(gpu_ctx, _material_box(mat, spectrum), 4)
     frac, gm, rm = _compare(gpu, ref)
-    print(f"{mat}/{spectrum}: {frac * 100:.3f}% pixels within {REL_TOL}; mean gpu {gm:.6f} oracle {rm:.6f}")
-    assert frac >= 0.99
-    assert abs(gm / rm - 1) < 5e-3
+    z = _mean_z(gpu, ref)
+    print(f"{mat}/{spectrum}: depth 2 {sfrac * 100:.3f}%, depth 8 {frac * 100:.3f}% pixels within {REL_TOL}; "
+          f"mean gpu {gm:.6f} oracle {rm:.6f} (z {z:.2f})")
+    assert sfrac >= 0.999
+    assert frac >= 0.96
+    assert z < 4.5
 
 
 @pytest.mark.parametrize("spectrum", ["srgb", "hero"])
@@ -85,9 +101,11 @@ def test_procedural_scene_parity(gpu_ctx, tmp_path, spectrum):
                                       max_depth=8, spectrum=spectrum, hdri=(256, 128))
     gpu, ref = _render_both(gpu_ctx, scene, 2, str(tmp_path))
     frac, gm, rm = _compare(gpu, ref)
-    print(f"procedural/{spectrum}: {frac * 100:.3f}% pixels within {REL_TOL}; mean gpu {gm:.6f} oracle {rm:.6f}")
-    assert frac >= 0.99
-    assert abs(gm / rm - 1) < 1e-2
+    z = _mean_z(gpu, ref)
+    print(f"procedural/{spectrum}: {frac * 100:.3f}% pixels within {REL_TOL}; mean gpu {gm:.6f} oracle {rm:.6f} "
+          f"(z {z:.2f})")
+    assert frac >= 0.96
+    assert z < 4.5
 
 
 def test_gpu_hero_furnace_matches_oracle_closed_form(gpu_ctx):
@@ -112,3 +130,37 @@ def test_gpu_hero_furnace_matches_oracle_closed_form(gpu_ctx):
                   [0.055648, -0.204043, 1.057311]])
     want = m @ (xyz / 0.5)
     assert np.allclose(got, want, rtol=0.01), (got, want)
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero4"])
+@pytest.mark.parametrize("mat", list(TEST_MATERIALS))
+def test_device_bsdf_matches_oracle(gpu_ctx, mat, spectrum):
+    # the material layer alone (MaterialEvaluator::evaluate / ::sample) on the
+    # device vs the oracle, for directions on both sides of the surface
+    print(f"case: bsdf {mat} {spectrum}", flush=True)
+    scene = _material_box(mat, spectrum)
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    ex = sc.export()
+    m = len(ex["materials"]) - 1  # the test material is appended last
+    spec = oracle.Spectral(ex)
+    rng = np.random.default_rng(4)
+    n = 20000
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi = rng.normal(size=(n, 3)).astype(np.float32)
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    lam = None
+    if spectrum != "srgb":
+        lam = oracle.wavelengths(rng.random(4 * n, dtype=np.float32))[0].reshape(n, 4)
+    _, fd, pd = sc.bsdf(m, wo, wi, lam)
+    fo, po = oracle.bsdf_eval(spec, ex["materials"][m], wo, wi, lam)
+    ok = np.isclose(fd, fo, rtol=1e-5, atol=1e-7).all(1) & np.isclose(pd, po, rtol=1e-5, atol=1e-7)
+    print(f"bsdf eval {mat}/{spectrum}: {ok.mean() * 100:.3f}% within 1e-5")
+    assert ok.mean() >= 0.999
+    wsd, fsd, psd = sc.bsdf(m, wo, None, lam, frame=7)
+    wso, fso, pso = oracle.bsdf_sample(spec, ex["materials"][m], wo, frame=7, lam=lam)
+    ok = np.isclose(wsd, wso, atol=1e-5).all(1) & np.isclose(psd, pso, rtol=1e-4, atol=1e-7)
+    print(f"bsdf sample {mat}/{spectrum}: {ok.mean() * 100:.3f}% within 1e-5")
+    assert ok.mean() >= 0.999
+    sc.close()

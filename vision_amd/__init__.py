@@ -99,6 +99,7 @@ SIGNATURES = {
     "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
     "vs_trace_any_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
+    "vs_scene_bsdf": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_int32, C.c_uint32, _P, _P, _P]),
     "vs_render": (C.c_int32, [_P, C.c_uint32, C.c_int32]),
     "vs_set_counting": (C.c_int32, [_P, C.c_int32]),
     "vs_last_render_stats": (C.c_int32, [_P, C.POINTER(RenderStats)]),
@@ -264,6 +265,23 @@ class Scene:
         return occ
 
     # -- rendering (PathTracingIntegrator::render, one frame = one spp)
+    def bsdf(self, mat: int, wo: np.ndarray, wi: np.ndarray | None = None, lam: np.ndarray | None = None,
+             frame: int = 0):
+        """Evaluate (wi given) or sample (wi None) scene material `mat` on the
+        device; returns (wi, f[n, 4], pdf[n])."""
+        wo = np.ascontiguousarray(wo, np.float32).reshape(-1, 3)
+        n = len(wo)
+        sample = wi is None
+        wi_a = np.zeros((n, 3), np.float32) if sample else np.ascontiguousarray(wi, np.float32).reshape(-1, 3)
+        lam_a = None if lam is None else np.ascontiguousarray(lam, np.float32).reshape(n, 4)
+        wi_out = np.zeros((n, 3), np.float32)
+        f = np.zeros((n, 4), np.float32)
+        pdf = np.zeros(n, np.float32)
+        self.ctx._check(lib().vs_scene_bsdf(self._h, mat, n, wo.ctypes.data, wi_a.ctypes.data,
+                                            None if lam_a is None else lam_a.ctypes.data, 1 if sample else 0, frame,
+                                            wi_out.ctypes.data, f.ctypes.data, pdf.ctypes.data))
+        return (wi_out if sample else wi_a), f, pdf
+
     def render(self, spp: int, fused: bool = True) -> dict:
         self.ctx._check(lib().vs_render(self._h, spp, 1 if fused else 0))
         return self.stats()

@@ -517,6 +517,39 @@ vs_status vs_trace_any_device(vs_scene *s, const vs_ray *d_rays, uint32_t n, uin
     return trace_device(s, d_rays, n, nullptr, d_occ, true, stream);
 }
 
+vs_status vs_scene_bsdf(vs_scene *s, uint32_t mat, uint32_t n, const float *wo, const float *wi, const float *lambda,
+                        int32_t sample, uint32_t frame, float *wi_out, float *f, float *pdf) {
+    if (!s || (n && (!wo || !f || !pdf || (sample ? !wi_out : !wi))))
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (mat >= s->packed.mats.size()) throw std::invalid_argument("material index out of range");
+        if (!s->packed.rgb && n && !lambda) throw std::invalid_argument("hero mode needs 4 wavelengths per direction");
+        if (n == 0) return;
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hipStream_t st = s->ctx->stream;
+        DevBuf d_wo, d_wi, d_lam, d_wio, d_f, d_pdf;
+        d_wo.alloc(size_t(n) * 12);
+        d_wi.alloc(size_t(n) * 12);
+        d_lam.alloc(size_t(n) * 16);
+        d_wio.alloc(size_t(n) * 12);
+        d_f.alloc(size_t(n) * 16);
+        d_pdf.alloc(size_t(n) * 4);
+        hip_check(hipMemcpyAsync(d_wo.ptr, wo, size_t(n) * 12, hipMemcpyHostToDevice, st), "copy wo");
+        if (!sample) hip_check(hipMemcpyAsync(d_wi.ptr, wi, size_t(n) * 12, hipMemcpyHostToDevice, st), "copy wi");
+        if (!s->packed.rgb)
+            hip_check(hipMemcpyAsync(d_lam.ptr, lambda, size_t(n) * 16, hipMemcpyHostToDevice, st), "copy lambda");
+        uint32_t dim = s->packed.rgb ? 3u : s->host.spectrum_dim;
+        hip_check(vs::launch_bsdf(s->view, dim, mat, d_wo.as<float>(), d_wi.as<float>(), d_lam.as<float>(), n,
+                                  sample ? 1 : 0, frame, d_wio.as<float>(), d_f.as<float>(), d_pdf.as<float>(), st),
+                  "bsdf launch");
+        if (sample) hip_check(hipMemcpyAsync(wi_out, d_wio.ptr, size_t(n) * 12, hipMemcpyDeviceToHost, st), "copy wi");
+        hip_check(hipMemcpyAsync(f, d_f.ptr, size_t(n) * 16, hipMemcpyDeviceToHost, st), "copy f");
+        hip_check(hipMemcpyAsync(pdf, d_pdf.ptr, size_t(n) * 4, hipMemcpyDeviceToHost, st), "copy pdf");
+        hip_check(hipStreamSynchronize(st), "bsdf sync");
+    });
+}
+
 vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
     if (!s) return VS_ERROR_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] {

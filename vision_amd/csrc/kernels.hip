@@ -95,6 +95,66 @@ __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, Re
     }
 }
 
+// material-level parity hook: evaluate or sample material `mat` at a surface
+// whose shading frame is the world axes and whose geometric normal is +z
+template <int NS, bool RGB>
+__global__ void __launch_bounds__(256) bsdf_kernel(SceneView sc, uint32_t mat, const float *__restrict__ wo,
+                                                    const float *__restrict__ wi, const float *__restrict__ lambda,
+                                                    uint32_t n, int sample, uint32_t frame, float *__restrict__ wi_out,
+                                                    float *__restrict__ f, float *__restrict__ pdf) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Swl<NS> swl;
+#pragma unroll
+    for (int k = 0; k < NS; ++k) {
+        swl.lambda[k] = RGB ? (k == 0 ? 602.785f : (k == 1 ? 539.285f : 445.772f)) : lambda[4 * i + k];
+        swl.pdf[k] = RGB ? 1.f : visible_wavelength_pdf(swl.lambda[k]);
+    }
+    Frame fr{mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
+    bool dispersive;
+    MatCtx<NS> c = make_material<NS, RGB>(sc, sc.mat[mat], fr, mk3(0.f, 0.f, 1.f),
+                                          mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), swl, dispersive);
+    Spec<NS> fv;
+    float p;
+    if (sample) {
+        Sampler sp;
+        sp.start(0u, i, frame, 0u);
+        SampleN<NS> s = mat_sample(c, sp);
+        wi_out[3 * i] = s.wi.x;
+        wi_out[3 * i + 1] = s.wi.y;
+        wi_out[3 * i + 2] = s.wi.z;
+        fv = s.e.f;
+        p = s.e.pdf;
+    } else {
+        EvalN<NS> e = mat_eval(c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr);
+        fv = e.f;
+        p = e.pdf;
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? fv.v[k] : 0.f;
+    pdf[i] = p;
+}
+
+hipError_t launch_bsdf(const SceneView &sc, uint32_t spectrum_dim, uint32_t mat, const float *wo, const float *wi,
+                       const float *lambda, uint32_t n, int sample, uint32_t frame, float *wi_out, float *f,
+                       float *pdf, hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    dim3 block(256), grid((n + 255) / 256);
+    if (sc.spectrum_rgb) {
+        hipLaunchKernelGGL((bsdf_kernel<3, true>), grid, block, 0, stream, sc, mat, wo, wi, lambda, n, sample, frame,
+                           wi_out, f, pdf);
+    } else if (spectrum_dim == 4) {
+        hipLaunchKernelGGL((bsdf_kernel<4, false>), grid, block, 0, stream, sc, mat, wo, wi, lambda, n, sample,
+                           frame, wi_out, f, pdf);
+    } else if (spectrum_dim == 3) {
+        hipLaunchKernelGGL((bsdf_kernel<3, false>), grid, block, 0, stream, sc, mat, wo, wi, lambda, n, sample,
+                           frame, wi_out, f, pdf);
+    } else {
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_trace(const SceneView &sc, const RayRec *rays, HitRec *hits, uint8_t *occ, uint32_t n, bool any,
                         unsigned long long *counters, hipStream_t stream) {
     if (n == 0) return hipSuccess;

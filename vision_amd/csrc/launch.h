@@ -10,6 +10,9 @@ namespace vs {
 hipError_t launch_trace(const SceneView &sc, const RayRec *rays, HitRec *hits, uint8_t *occ, uint32_t n, bool any,
                         unsigned long long *counters, hipStream_t stream);
 
+hipError_t launch_bsdf(const SceneView &sc, uint32_t spectrum_dim, uint32_t mat, const float *wo, const float *wi,
+                       const float *lambda, uint32_t n, int sample, uint32_t frame, float *wi_out, float *f,
+                       float *pdf, hipStream_t stream);
 hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderView &rv, float4 *rt, float4 *out,
                      unsigned long long *counters, hipStream_t stream);
 
