@@ -88,6 +88,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
+    ap.add_argument("--chunk-frames", type=int, default=8)
+    ap.add_argument("--trace-block", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU scaling is not part of it", file=sys.stderr)
@@ -100,6 +103,7 @@ def main():
     t0 = time.perf_counter()
     sc.build_accel()
     build_s = time.perf_counter() - t0
+    sc.set_kernel(args.kernel, args.chunk_frames, args.trace_block)
     info = sc.info()
     W, H = info["width"], info["height"]
 

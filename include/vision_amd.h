@@ -172,6 +172,17 @@ vs_status vs_trace_any_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n,
 vs_status vs_scene_bsdf(vs_scene *scene, uint32_t mat, uint32_t n, const float *wo, const float *wi,
                         const float *lambda, int32_t sample, uint32_t frame, float *wi_out, float *f, float *pdf);
 
+/* Kernel form of vs_render.  MEGAKERNEL: one thread per pixel runs whole
+ * paths (fused or per-frame launches).  WAVEFRONT: the bounce loop split into
+ * closest-hit / shade / any-hit kernels over queues of live paths held in
+ * HBM, `chunk_frames` frames of all pixels per chunk, traversal blocks of
+ * `trace_block` (64, 128 or 256) threads.  Both give the same film.  No
+ * reference counterpart: the reference runs one kernel per frame
+ * (pt.cpp:90-99); this is the choice the autotuner makes per scene. */
+#define VS_KERNEL_MEGAKERNEL 0
+#define VS_KERNEL_WAVEFRONT 1
+vs_status vs_scene_set_kernel(vs_scene *scene, int32_t mode, uint32_t chunk_frames, uint32_t trace_block);
+
 /* Add `spp` progressive frames to the film (frame indices continue from the
  * current one).  fused = 0: one launch per frame, as
  * PathTracingIntegrator::render does (src/render_core/integrators/pt.cpp:90-99);

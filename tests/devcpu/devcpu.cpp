@@ -1,0 +1,152 @@
+// Host build of the device path-tracing code (tests only): the same headers
+// the HIP kernels use, compiled by g++ against tests/devcpu/hip/hip_runtime.h,
+// driven like the kernels drive them, for bit-level comparison with the oracle.
+#include "dev_pt.h"
+#include "data_tables.h"
+#include <vector>
+using namespace vs;
+using namespace vs::dev;
+
+template <int NS, bool RGB>
+static void run(const uint32_t *mat, const float *spd, const float *wo, const float *wi, const float *lambda,
+                uint32_t n, int sample, uint32_t frame, float *wi_out, float *f, float *pdf) {
+    SceneView sc{};
+    sc.spd = spd;
+    sc.lut = lut_tables().data();
+    MaterialRecord rec;
+    std::memcpy(&rec, mat, sizeof(rec));
+    for (uint32_t i = 0; i < n; ++i) {
+        Swl<NS> swl;
+        for (int k = 0; k < NS; ++k) {
+            swl.lambda[k] = RGB ? (k == 0 ? 602.785f : (k == 1 ? 539.285f : 445.772f)) : lambda[4 * i + k];
+            swl.pdf[k] = RGB ? 1.f : visible_wavelength_pdf(swl.lambda[k]);
+        }
+        Frame fr{mk3(1, 0, 0), mk3(0, 1, 0), mk3(0, 0, 1)};
+        f3 wo3 = mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
+        bool disp;
+        MatCtx<NS> c = make_material<NS, RGB>(sc, rec, fr, mk3(0, 0, 1), wo3, swl, disp);
+        if (!sample) {
+            EvalN<NS> e = mat_eval(c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr);
+            for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? e.f.v[k] : 0.f;
+            pdf[i] = e.pdf;
+        } else {
+            Sampler sp;
+            sp.start(0u, i, frame, 0u);
+            SampleN<NS> s = mat_sample(c, sp);
+            wi_out[3 * i] = s.wi.x, wi_out[3 * i + 1] = s.wi.y, wi_out[3 * i + 2] = s.wi.z;
+            for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? s.e.f.v[k] : 0.f;
+            pdf[i] = s.e.pdf;
+        }
+    }
+}
+
+extern "C" void dc_bsdf(int rgb, int ns, const uint32_t *mat, const float *spd, const float *wo, const float *wi,
+                        const float *lambda, uint32_t n, int sample, uint32_t frame, float *wi_out, float *f, float *pdf) {
+    if (rgb) run<3, true>(mat, spd, wo, wi, lambda, n, sample, frame, wi_out, f, pdf);
+    else if (ns == 4) run<4, false>(mat, spd, wo, wi, lambda, n, sample, frame, wi_out, f, pdf);
+    else run<3, false>(mat, spd, wo, wi, lambda, n, sample, frame, wi_out, f, pdf);
+}
+
+// ---- full-path CPU render of the device code
+#include "bvh.h"
+#include "scene.h"
+#include "tables.h"
+#include <string>
+
+struct DcScene {
+    SceneHost h;
+    PackedScene pk;
+    Bvh8 bvh;
+    SceneView v{};
+    CameraView cam{};
+};
+
+extern "C" void *dc_load(const char *text, const char *base) {
+    auto *s = new DcScene();
+    s->h = load_scene(text, base);
+    s->pk = pack_scene(s->h);
+    s->bvh = build_bvh8(s->h.wP, s->h.wTri);
+    SceneView &v = s->v;
+    const SceneHost &h = s->h;
+    const PackedScene &pk = s->pk;
+    v.nodes = reinterpret_cast<const uint32_t *>(s->bvh.nodes.data());
+    v.leaf_tris = reinterpret_cast<const float *>(s->bvh.tris.data());
+    v.P = h.wP.data();
+    v.N = h.wN.data();
+    v.UV = h.wUV.data();
+    v.tri = h.wTri.data();
+    v.tri_inst = h.triInst.data();
+    v.inst = h.instRecords.data();
+    v.mat = pk.mats.data();
+    v.lights = pk.lights.data();
+    v.alias = pk.alias.data();
+    v.alias_func = pk.func.data();
+    v.num_lights = uint32_t(h.lights.size());
+    v.light_sampler = h.light_sampler_type == "power" ? 1u : 0u;
+    v.select_offset = pk.select_offset;
+    v.select_integral = pk.select_integral;
+    v.spd = pk.spd.data();
+    v.lut = lut_tables().data();
+    v.cie_interval = pk.cie_interval;
+    v.cie_y_integral = pk.cie_y_integral;
+    v.spectrum_rgb = pk.rgb ? 1u : 0u;
+    v.env_light = pk.env_light;
+    v.env_tex = pk.env_tex.data();
+    v.env_cond = pk.env_cond.data();
+    v.env_cond_func = pk.env_cond_func.data();
+    v.env_marg = pk.env_marg.data();
+    v.env_marg_func = pk.env_marg_func.data();
+    v.env_w = pk.env_w;
+    v.env_h = pk.env_h;
+    v.env_marg_integral = pk.env_marg_integral;
+    v.world_diameter = 2.f * pk.world_radius;
+    for (int i = 0; i < 9; ++i) v.env_w2o[i] = pk.env_w2o[i], v.env_o2w[i] = pk.env_o2w[i];
+    v.env_scale = pk.env_scale;
+    v.stack_size = s->bvh.max_depth;
+    v.ray_offset_factor = h.ray_offset_factor;
+    v.num_tris = uint32_t(h.wTri.size());
+    return s;
+}
+
+extern "C" void dc_set_camera(void *p, const float *c) {
+    auto *s = static_cast<DcScene *>(p);
+    CameraView &v = s->cam;
+    for (int i = 0; i < 12; ++i) v.c2w[i] = c[i];
+    v.lower_x = c[12]; v.upper_y = c[13]; v.raster_sx = c[14]; v.raster_sy = c[15]; v.tan_half_fov = c[16];
+    v.lens_radius = c[17]; v.focal_distance = c[18]; v.width = uint32_t(c[19]); v.height = uint32_t(c[20]);
+    v.filter = uint32_t(c[21]); v.filter_radius = c[22]; v.thin_lens = uint32_t(c[23]);
+}
+
+template <int NS, bool RGB>
+static void render_px(DcScene *s, const RenderView &rv, uint32_t px, uint32_t py, float *out) {
+    std::vector<uint2> stack(64);
+    float acc[4] = {0, 0, 0, 0};
+    PathCounters pc{0, 0, 0};
+    for (uint32_t f = rv.frame_start; f < rv.frame_start + rv.spp; ++f) {
+        f3 L = trace_path<NS, RGB>(s->v, s->cam, rv, px, py, f, stack.data(), 1, pc);
+        float v[4] = {zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f};
+        float a = 1.f / float(f + 1u);
+        for (int k = 0; k < 4; ++k) acc[k] = acc[k] + a * (v[k] - acc[k]);
+    }
+    for (int k = 0; k < 4; ++k) out[k] = acc[k];
+}
+
+extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *film) {
+    auto *s = static_cast<DcScene *>(p);
+    RenderView rv{};
+    rv.max_depth = uint32_t(render[0]);
+    rv.min_depth = uint32_t(render[1]);
+    rv.rr_threshold = render[2];
+    rv.mis_mode = int(render[3]);
+    rv.frame_start = 0;
+    rv.spp = spp;
+    rv.accumulation = 1;
+    rv.spectrum_dim = s->pk.rgb ? 3u : s->h.spectrum_dim;
+    for (uint32_t y = 0; y < s->cam.height; ++y)
+        for (uint32_t x = 0; x < s->cam.width; ++x) {
+            float *o = film + 4 * (size_t(y) * s->cam.width + x);
+            if (s->pk.rgb) render_px<3, true>(s, rv, x, y, o);
+            else if (rv.spectrum_dim == 4) render_px<4, false>(s, rv, x, y, o);
+            else render_px<3, false>(s, rv, x, y, o);
+        }
+}

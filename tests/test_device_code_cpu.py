@@ -1,0 +1,129 @@
+"""The device path-tracing source, compiled for the host, against the oracle.
+
+tests/devcpu builds the very headers the HIP kernels include (dev_pt.h and
+what it pulls in) with g++ against a small host stand-in for the HIP
+runtime header, and drives them as the kernels do.  With identical inputs the
+result must equal the oracle bit for bit: BSDF values, sampled directions
+and whole films.  On the GPU the same source differs from the oracle only
+through the device math library's last-bit rounding of sin/cos/acos/atan2
+(test_gpu_materials.py); this test shows that the arithmetic around those
+calls is the oracle's.
+"""
+import ctypes as C
+import json
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import vision_amd as va
+from oracle import oracle
+from vision_amd import build as vbuild
+from vision_amd import scenegen
+
+HERE = Path(__file__).resolve().parent
+SRC = HERE / "devcpu" / "devcpu.cpp"
+BUILD = HERE / "devcpu" / "_build"
+LIB = BUILD / "libdevcpu.so"
+OBJS = ["data_tables.cpp.o", "json.cpp.o", "scene.cpp.o", "bvh.cpp.o", "tables.cpp.o"]
+
+_f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_u32 = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+
+
+def _harness():
+    deps = [SRC, HERE / "devcpu" / "hip" / "hip_runtime.h", *vbuild.CSRC.glob("*.h")]
+    objs = [vbuild.BUILD / o for o in OBJS]
+    if not all(o.exists() for o in objs):
+        vbuild.build()
+    if not LIB.exists() or LIB.stat().st_mtime < max(p.stat().st_mtime for p in deps + objs):
+        BUILD.mkdir(parents=True, exist_ok=True)
+        data = BUILD / "data"
+        if not data.exists():
+            data.symlink_to(vbuild.DATA)
+        tmp = LIB.with_suffix(".tmp")
+        cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-w", "-I", str(HERE / "devcpu"),
+               "-I", str(vbuild.CSRC), "-o", str(tmp), str(SRC), *map(str, objs), "-lpthread"]
+        subprocess.run(cmd, check=True, capture_output=True)
+        tmp.replace(LIB)
+    h = C.CDLL(str(LIB))
+    h.dc_bsdf.argtypes = [C.c_int, C.c_int, _u32, _f32, _f32, _f32, C.c_void_p, C.c_uint32, C.c_int, C.c_uint32,
+                          _f32, _f32, _f32]
+    h.dc_load.restype = C.c_void_p
+    h.dc_load.argtypes = [C.c_char_p, C.c_char_p]
+    h.dc_set_camera.argtypes = [C.c_void_p, _f32]
+    h.dc_render.argtypes = [C.c_void_p, _f32, C.c_uint32, _f32]
+    return h
+
+
+@pytest.fixture(scope="module")
+def dc():
+    return _harness()
+
+
+def _dev_bsdf(h, rec, spd, wo, wi, sample, frame, rgb, ns, lam):
+    n = len(wo)
+    wi_out = np.zeros((n, 3), np.float32)
+    f = np.zeros((n, 4), np.float32)
+    pdf = np.zeros(n, np.float32)
+    h.dc_bsdf(1 if rgb else 0, ns, rec, spd, wo, wi, None if lam is None else lam.ctypes.data, n, sample, frame,
+              wi_out, f, pdf)
+    return wi_out, f, pdf
+
+
+def _bits_equal(a, b):
+    return np.array_equal(np.asarray(a, np.float32).view(np.uint32), np.asarray(b, np.float32).view(np.uint32))
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero4"])
+def test_device_bsdf_source_equals_oracle(dc, spectrum):
+    import test_gpu_materials as G
+    rng = np.random.default_rng(8)
+    n = 4000
+    wo = rng.normal(size=(n, 3)).astype(np.float32)
+    wo /= np.linalg.norm(wo, axis=1, keepdims=True)
+    wi = rng.normal(size=(n, 3)).astype(np.float32)
+    wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    lam = None if spectrum == "srgb" else oracle.wavelengths(rng.random(4 * n, dtype=np.float32))[0].reshape(n, 4)
+    for mat in G.TEST_MATERIALS:
+        with va.Context(-1) as ctx:
+            s = ctx.load_text(G._material_box(mat, spectrum)).build_accel()
+            ex = s.export()
+        rec = np.ascontiguousarray(ex["materials"][-1])
+        spec = oracle.Spectral(ex)
+        rgb, ns = bool(ex["spectrum_rgb"]), 3 if ex["spectrum_rgb"] else ex["spectrum_dim"]
+        _, fd, pd = _dev_bsdf(dc, rec, spec.spd, wo, wi, 0, 0, rgb, ns, lam)
+        fo, po = oracle.bsdf_eval(spec, rec, wo, wi, lam)
+        assert _bits_equal(fd, fo) and _bits_equal(pd, po), mat
+        wsd, fsd, psd = _dev_bsdf(dc, rec, spec.spd, wo, wi, 1, 3, rgb, ns, lam)
+        wso, fso, pso = oracle.bsdf_sample(spec, rec, wo, frame=3, lam=lam)
+        assert _bits_equal(wsd, wso) and _bits_equal(fsd, fso) and _bits_equal(psd, pso), mat
+
+
+def _dev_render(h, scene, spp, base="."):
+    with va.Context(-1) as ctx:
+        ex = ctx.load_text(scene, base).build_accel().export()
+    p = h.dc_load(json.dumps(scene).encode(), base.encode())
+    cam = np.ascontiguousarray(ex["camera"], np.float32)
+    h.dc_set_camera(p, cam)
+    W, H = int(cam[19]), int(cam[20])
+    film = np.zeros((H, W, 4), np.float32)
+    h.dc_render(p, np.ascontiguousarray(ex["render"], np.float32), spp, film)
+    return film, oracle.render(ex, 0, spp)
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero4", "hero3"])
+def test_device_path_source_equals_oracle_material_boxes(dc, spectrum):
+    import test_gpu_materials as G
+    for mat in G.TEST_MATERIALS:
+        film, ref = _dev_render(dc, G._material_box(mat, spectrum, res=24), 2)
+        assert _bits_equal(film, ref), (mat, spectrum)
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero"])
+def test_device_path_source_equals_oracle_procedural(dc, tmp_path, spectrum):
+    scene = scenegen.procedural_scene(tmp_path, seed=3, n_instances=9, level=3, width=40, height=24, spp=2,
+                                      max_depth=8, spectrum=spectrum, hdri=(256, 128))
+    film, ref = _dev_render(dc, scene, 2, str(tmp_path))
+    assert _bits_equal(film, ref)

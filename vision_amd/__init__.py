@@ -100,6 +100,7 @@ SIGNATURES = {
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
     "vs_trace_any_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
     "vs_scene_bsdf": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_int32, C.c_uint32, _P, _P, _P]),
+    "vs_scene_set_kernel": (C.c_int32, [_P, C.c_int32, C.c_uint32, C.c_uint32]),
     "vs_render": (C.c_int32, [_P, C.c_uint32, C.c_int32]),
     "vs_set_counting": (C.c_int32, [_P, C.c_int32]),
     "vs_last_render_stats": (C.c_int32, [_P, C.POINTER(RenderStats)]),
@@ -281,6 +282,11 @@ class Scene:
                                             None if lam_a is None else lam_a.ctypes.data, 1 if sample else 0, frame,
                                             wi_out.ctypes.data, f.ctypes.data, pdf.ctypes.data))
         return (wi_out if sample else wi_a), f, pdf
+
+    def set_kernel(self, mode: str = "megakernel", chunk_frames: int = 8, trace_block: int = 128):
+        """Kernel form of render(): "megakernel" or "wavefront"."""
+        m = {"megakernel": 0, "wavefront": 1}[mode]
+        self.ctx._check(lib().vs_scene_set_kernel(self._h, m, chunk_frames, trace_block))
 
     def render(self, spp: int, fused: bool = True) -> dict:
         self.ctx._check(lib().vs_render(self._h, spp, 1 if fused else 0))

@@ -3,6 +3,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -18,6 +19,7 @@
 #include "scene.h"
 #include "data_tables.h"
 #include "tables.h"
+#include "wavefront.h"
 
 struct vs_context {
     int device{0};// -1: host-only context (scene parsing, BVH build, export)
@@ -82,6 +84,14 @@ struct vs_scene {
     DevBuf spd, lut, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
     DevBuf film_rt, film_out, counters;
     DevBuf scratch_rays, scratch_hits;
+    // wavefront form: per-path state, queues, append counters
+    uint32_t kernel_mode{VS_KERNEL_MEGAKERNEL};
+    uint32_t wf_chunk_frames{8};
+    uint32_t wf_trace_block{128};
+    size_t wf_capacity{0};
+    DevBuf wf_o, wf_d, wf_thr, wf_L, wf_meta, wf_lam, wf_lpdf, wf_hit, wf_sh_o, wf_sh_d, wf_sh_c, wf_q0, wf_q1,
+        wf_counts;
+    uint32_t *wf_host_counts{nullptr};// pinned
     vs::SceneView view{};
     vs::CameraView cam{};
     uint32_t frame_index{0};
@@ -280,7 +290,22 @@ void vs_scene_destroy(vs_scene *scene) {
         (void)hipSetDevice(scene->ctx->device);
         (void)hipStreamSynchronize(scene->ctx->stream);
     }
+    if (scene->wf_host_counts) (void)hipHostFree(scene->wf_host_counts);
     delete scene;
+}
+
+vs_status vs_scene_set_kernel(vs_scene *s, int32_t mode, uint32_t chunk_frames, uint32_t trace_block) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    if (mode != VS_KERNEL_MEGAKERNEL && mode != VS_KERNEL_WAVEFRONT)
+        return fail(s->ctx, VS_ERROR_INVALID_ARGUMENT, "unknown kernel mode");
+    if (chunk_frames == 0 || chunk_frames > 1024)
+        return fail(s->ctx, VS_ERROR_INVALID_ARGUMENT, "chunk_frames must be in 1..1024");
+    if (trace_block != 64 && trace_block != 128 && trace_block != 256)
+        return fail(s->ctx, VS_ERROR_INVALID_ARGUMENT, "trace_block must be 64, 128 or 256");
+    s->kernel_mode = uint32_t(mode);
+    s->wf_chunk_frames = chunk_frames;
+    s->wf_trace_block = trace_block;
+    return VS_OK;
 }
 
 vs_status vs_scene_build_accel(vs_scene *s) {
@@ -550,6 +575,79 @@ vs_status vs_scene_bsdf(vs_scene *s, uint32_t mat, uint32_t n, const float *wo, 
     });
 }
 
+namespace {
+
+// Wavefront render of `spp` frames starting at s->frame_index: chunks of
+// `wf_chunk_frames` frames, each a loop of trace / shade / shadow over the
+// live-path queue until it empties, then the chunk folds into the film.
+uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr) {
+    hipStream_t st = s->ctx->stream;
+    const uint32_t P = s->cam.width * s->cam.height;
+    const uint32_t chunk = std::min<uint32_t>(s->wf_chunk_frames, spp);
+    const size_t cap = size_t(P) * chunk;
+    if (cap > 0xFFFFFFFFull) throw std::invalid_argument("wavefront chunk exceeds 2^32 paths");
+    if (s->wf_capacity < cap) {
+        for (DevBuf *b : {&s->wf_o, &s->wf_d, &s->wf_thr, &s->wf_L, &s->wf_meta, &s->wf_lam, &s->wf_lpdf, &s->wf_hit,
+                          &s->wf_sh_o, &s->wf_sh_d, &s->wf_sh_c})
+            b->alloc(cap * 16);
+        s->wf_q0.alloc(cap * 4);
+        s->wf_q1.alloc(cap * 4);
+        s->wf_counts.alloc(64);
+        if (!s->wf_host_counts) {
+            void *hp = nullptr;
+            hip_check(hipHostMalloc(&hp, 64, hipHostMallocDefault), "pinned counts");
+            s->wf_host_counts = static_cast<uint32_t *>(hp);
+        }
+        s->wf_capacity = cap;
+    }
+    vs::WavefrontView w{};
+    w.o = s->wf_o.as<float4>();
+    w.d = s->wf_d.as<float4>();
+    w.thr = s->wf_thr.as<float4>();
+    w.L = s->wf_L.as<float4>();
+    w.meta = s->wf_meta.as<uint4>();
+    w.lam = s->wf_lam.as<float4>();
+    w.lpdf = s->wf_lpdf.as<float4>();
+    w.hit = s->wf_hit.as<float4>();
+    w.sh_o = s->wf_sh_o.as<float4>();
+    w.sh_d = s->wf_sh_d.as<float4>();
+    w.sh_c = s->wf_sh_c.as<float4>();
+    w.pixels = P;
+    w.width = s->cam.width;
+    const bool rgb = s->packed.rgb;
+    const uint32_t dim = rv0.spectrum_dim;
+    uint32_t *q[2] = {s->wf_q0.as<uint32_t>(), s->wf_q1.as<uint32_t>()};
+    uint32_t *counts = s->wf_counts.as<uint32_t>();
+    uint32_t launches = 0;
+    for (uint32_t f0 = 0; f0 < spp; f0 += chunk) {
+        const uint32_t fc = std::min(chunk, spp - f0);
+        const uint32_t n = P * fc;
+        const uint32_t frame0 = s->frame_index + f0;
+        hip_check(vs::wf_begin(s->cam, rv0, w, dim, rgb, frame0, n, q[0], st), "wf begin");
+        ++launches;
+        uint32_t count = rv0.max_depth > 0 ? n : 0u;
+        int cur = 0;
+        while (count) {
+            hip_check(vs::wf_zero(counts, st), "wf zero");
+            hip_check(vs::wf_trace(s->view, w, q[cur], count, ctr, s->wf_trace_block, st), "wf trace");
+            hip_check(vs::wf_shade(s->view, rv0, w, dim, rgb, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
+            hip_check(vs::wf_shadow(s->view, w, counts, count, ctr, s->wf_trace_block, st), "wf shadow");
+            launches += 4;
+            hip_check(hipMemcpyAsync(s->wf_host_counts, counts, 8, hipMemcpyDeviceToHost, st), "wf counts");
+            hip_check(hipStreamSynchronize(st), "wf sync");
+            count = s->wf_host_counts[vs::WF_LIVE];
+            if (count > n) throw std::logic_error("wavefront queue overflow");
+            cur ^= 1;
+        }
+        hip_check(vs::wf_accumulate(rv0, w, frame0, fc, s->film_rt.as<float4>(), s->film_out.as<float4>(), st),
+                  "wf accumulate");
+        ++launches;
+    }
+    return launches;
+}
+
+}// namespace
+
 vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
     if (!s) return VS_ERROR_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] {
@@ -575,7 +673,9 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         }
         uint32_t launches = 0;
         hip_check(hipEventRecord(s->ctx->ev0, st), "event");
-        if (fused) {
+        if (s->kernel_mode == VS_KERNEL_WAVEFRONT) {
+            launches = render_wavefront(s, rv, spp, ctr);
+        } else if (fused) {
             rv.frame_start = s->frame_index;
             rv.spp = spp;
             hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr, st),

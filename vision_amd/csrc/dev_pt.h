@@ -285,107 +285,128 @@ struct PathCounters {
     uint32_t tris;
 };
 
-// one camera path; returns linear sRGB radiance (before the filter weight)
+// State of one camera path between bounces.  The bounce loop of
+// IlluminationIntegrator::Li is split at its traversals: path_shade runs one
+// loop iteration after the closest hit is known and hands back the NEE
+// shadow ray, whose (already weighted) contribution the caller adds when the
+// ray is unoccluded.  The megakernel runs begin / trace / shade / shadow in
+// one thread; the wavefront kernels run each step over a queue of paths.
+// Both execute the same arithmetic in the same order.
+template <int NS>
+struct PathState {
+    f3 o, d;          // next ray (t in (0, inf))
+    Spec<NS> thr;     // throughput
+    f3 L;             // linear sRGB radiance gathered so far
+    float scatter_pdf;// pdf of the last BSDF sample (1e16 before the first, pt.cpp:69)
+    float eta_scale;
+    uint32_t bounce;  // scattering events so far
+    uint32_t skips;   // material-less surfaces passed through
+    uint32_t alive;
+    float fw;         // film weight of the sensor sample
+    Sampler sampler;
+    Swl<NS> swl;
+};
+
+struct ShadowRay {
+    f3 o, d;// t in (0, 1 - ShadowEpsilon)
+    f3 c;   // linear sRGB contribution if unoccluded
+    uint32_t valid;
+};
+
+// one bounce of IlluminationIntegrator::Li (src/base/integrator.cpp:150-273)
+// given the closest hit of ps.o + t ps.d
 template <int NS, bool RGB>
-__device__ f3 trace_path(const SceneView &sc, const RenderView &rv, f3 ray_o, f3 ray_d, Sampler &sampler,
-                         Swl<NS> &swl, uint2 *stack, uint32_t stride, PathCounters &pc) {
-    f3 ret = mk3(0.f, 0.f, 0.f);
-    Spec<NS> throughput = Spec<NS>::fill(1.f);
-    float scatter_pdf = 1e16f;
-    float eta_scale = 1.f;
-    f3 primary_dir = ray_d;
-    uint32_t skips = 0;
-    bool has_env = sc.env_light != kInvalid;
-    for (uint32_t bounces = 0; bounces < rv.max_depth; ++bounces) {
-        TraceHit h = traverse<false>(sc, ray_o, ray_d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
-        ++pc.rays;
-        if (h.tri == kInvalid) {
-            // IlluminationIntegrator::evaluate_miss (integrator.cpp:111-132)
-            if (has_env) {
-                f3 wdir = normalize((ray_o + ray_d) - ray_o);
-                Spec<NS> L;
-                float lpdf;
-                env_eval<NS, RGB>(sc, wdir, swl, L, lpdf);
-                lpdf *= light_select_pmf(sc, sc.env_light);
-                float w = mis_weight(scatter_pdf, lpdf);
-                if (rv.mis_mode == 2) w = 1.f;
-                else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
-                ret += to_linear_srgb<NS, RGB>(sc, (L * w) * throughput, swl);
-            }
-            break;
-        }
-        Surface it = surface_at(sc, h.tri, h.b1, h.b2, true);
-        it.wo = normalize(-ray_d);
-        const InstanceRecord inst = sc.inst[it.inst];
-        if (inst.mat == kInvalid) {
-            // material-less surface: continue straight through (integrator.cpp:167-173)
-            f3 n = dot(it.ng, ray_d) > 0.f ? it.ng : -it.ng;
-            ray_o = offset_ray_origin(it.pos, n);
-            if (++skips > 64u) break;
-            --bounces;
-            continue;
-        }
-        if (inst.light != kInvalid) {
-            Spec<NS> Le;
+__device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<NS> &ps, const TraceHit &h,
+                           ShadowRay &sr) {
+    sr.valid = 0u;
+    const uint32_t bounces = ps.bounce;
+    if (h.tri == kInvalid) {
+        // IlluminationIntegrator::evaluate_miss (integrator.cpp:111-132)
+        if (sc.env_light != kInvalid) {
+            f3 wdir = normalize((ps.o + ps.d) - ps.o);
+            Spec<NS> L;
             float lpdf;
-            eval_hit_light<NS, RGB>(sc, it, inst.light, h.tri, ray_o, swl, Le, lpdf);
-            float w = mis_weight(scatter_pdf, lpdf);
+            env_eval<NS, RGB>(sc, wdir, ps.swl, L, lpdf);
+            lpdf *= light_select_pmf(sc, sc.env_light);
+            float w = mis_weight(ps.scatter_pdf, lpdf);
             if (rv.mis_mode == 2) w = 1.f;
             else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
-            ret += to_linear_srgb<NS, RGB>(sc, Le * throughput * w, swl);
+            ps.L += to_linear_srgb<NS, RGB>(sc, (L * w) * ps.thr, ps.swl);
         }
-        // next-event estimation
-        LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, sampler, swl);
-        bool dispersive;
-        MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, swl, dispersive);
-        if (!RGB && dispersive) {
-            // SampledWavelengths::check_dispersive (spectrum.cpp:21-39)
-            bool secondary_valid = true;
-#pragma unroll
-            for (int i = 1; i < NS; ++i) secondary_valid = secondary_valid && swl.pdf[i] != 0.f;
-            if (secondary_valid) {
-#pragma unroll
-                for (int i = 1; i < NS; ++i) swl.pdf[i] = 0.f;
-            }
-        }
-        f3 wi_l = normalize(ls.p_light - it.pos);
-        EvalN<NS> se = mat_eval(mc, wi_l, nullptr);
-        SampleN<NS> bs = mat_sample(mc, sampler);
-        bool delta = ls.pdf < 0.f;
-        float w = delta ? 1.f : mis_weight(ls.pdf, se.pdf);
-        if (rv.mis_mode == 1) w = 1.f;
-        float lpdf = delta ? -ls.pdf : ls.pdf;
-        if (se.pdf > 0.f && lpdf > 0.f && rv.mis_mode != 2) {
-            f3 dir = ls.p_light - it.pos;
-            f3 n = dot(it.ng, dir) > 0.f ? it.ng : -it.ng;
-            uint32_t dn = 0, dt = 0;
-            TraceHit sh = traverse<true>(sc, offset_ray_origin(it.pos, n), dir, 0.f, 1.f - kShadowEpsilon, stack,
-                                         stride, dn, dt);
-            pc.nodes += dn;
-            pc.tris += dt;
-            ++pc.rays;
-            if (sh.tri == kInvalid) {
-                Spec<NS> Ld = ls.L * se.f * w / lpdf;
-                ret += to_linear_srgb<NS, RGB>(sc, throughput * Ld, swl);
-            }
-        }
-        eta_scale *= bs.eta * bs.eta;
-        float lum = throughput.maxc();
-        if (!(bs.e.pdf > 0.f) || lum == 0.f) break;
-        throughput = throughput * (bs.e.f / bs.e.pdf);
-        if (eta_scale * lum < rv.rr_threshold && bounces >= rv.min_depth) {
-            float q = fminf(0.95f, lum);
-            float rr = sampler.next1d();
-            if (q < rr) break;
-            throughput = throughput / q;
-        }
-        scatter_pdf = bs.e.pdf;
-        f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
-        ray_o = offset_ray_origin(it.pos, n);
-        ray_d = bs.wi;
+        ps.alive = 0u;
+        return;
     }
-    (void)primary_dir;
-    return ret;
+    Surface it = surface_at(sc, h.tri, h.b1, h.b2, true);
+    it.wo = normalize(-ps.d);
+    const InstanceRecord inst = sc.inst[it.inst];
+    if (inst.mat == kInvalid) {
+        // material-less surface: continue straight through (integrator.cpp:180-186)
+        f3 n = dot(it.ng, ps.d) > 0.f ? it.ng : -it.ng;
+        ps.o = offset_ray_origin(it.pos, n);
+        if (++ps.skips > 64u) ps.alive = 0u;
+        return;
+    }
+    if (inst.light != kInvalid) {
+        Spec<NS> Le;
+        float lpdf;
+        eval_hit_light<NS, RGB>(sc, it, inst.light, h.tri, ps.o, ps.swl, Le, lpdf);
+        float w = mis_weight(ps.scatter_pdf, lpdf);
+        if (rv.mis_mode == 2) w = 1.f;
+        else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
+        ps.L += to_linear_srgb<NS, RGB>(sc, Le * ps.thr * w, ps.swl);
+    }
+    // next-event estimation (direct_lighting, integrator.cpp:22-38)
+    LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, ps.sampler, ps.swl);
+    bool dispersive;
+    MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, ps.swl, dispersive);
+    if (!RGB && dispersive) {
+        // SampledWavelengths::check_dispersive (spectrum.cpp:21-39)
+        bool secondary_valid = NS > 1;
+#pragma unroll
+        for (int i = 1; i < NS; ++i) secondary_valid = secondary_valid && ps.swl.pdf[i] != 0.f;
+        if (secondary_valid) {
+#pragma unroll
+            for (int i = 1; i < NS; ++i) ps.swl.pdf[i] = 0.f;
+        }
+    }
+    f3 wi_l = normalize(ls.p_light - it.pos);
+    EvalN<NS> se = mat_eval(mc, wi_l, nullptr);
+    SampleN<NS> bs = mat_sample(mc, ps.sampler);
+    bool delta = ls.pdf < 0.f;
+    float w = delta ? 1.f : mis_weight(ls.pdf, se.pdf);
+    if (rv.mis_mode == 1) w = 1.f;
+    float lpdf = delta ? -ls.pdf : ls.pdf;
+    if (se.pdf > 0.f && lpdf > 0.f && rv.mis_mode != 2) {
+        f3 dir = ls.p_light - it.pos;
+        f3 n = dot(it.ng, dir) > 0.f ? it.ng : -it.ng;
+        sr.o = offset_ray_origin(it.pos, n);
+        sr.d = dir;
+        Spec<NS> Ld = ls.L * se.f * w / lpdf;
+        sr.c = to_linear_srgb<NS, RGB>(sc, ps.thr * Ld, ps.swl);
+        sr.valid = 1u;
+    }
+    ps.eta_scale *= bs.eta * bs.eta;
+    float lum = ps.thr.maxc();
+    if (!(bs.e.pdf > 0.f) || lum == 0.f) {
+        ps.alive = 0u;
+        return;
+    }
+    ps.thr = ps.thr * (bs.e.f / bs.e.pdf);
+    if (ps.eta_scale * lum < rv.rr_threshold && bounces >= rv.min_depth) {
+        float q = fminf(0.95f, lum);
+        float rr = ps.sampler.next1d();
+        if (q < rr) {
+            ps.alive = 0u;
+            return;
+        }
+        ps.thr = ps.thr / q;
+    }
+    ps.scatter_pdf = bs.e.pdf;
+    f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
+    ps.o = offset_ray_origin(it.pos, n);
+    ps.d = bs.wi;
+    ps.bounce = bounces + 1u;
+    if (ps.bounce >= rv.max_depth) ps.alive = 0u;
 }
 
 // Sensor::generate_ray (src/base/sensor/sensor.cpp:44-56) + filter sample
@@ -453,6 +474,46 @@ __device__ __forceinline__ Swl<NS> sample_wavelengths(uint32_t px, uint32_t py, 
     return swl;
 }
 
+// start of a camera path (pt.cpp:53-73): wavelengths from their own stream,
+// sensor sample, camera ray; the film weight of box and tent filters is 1
+template <int NS, bool RGB>
+__device__ __forceinline__ void path_begin(const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
+                                           uint32_t frame, PathState<NS> &ps) {
+    ps.swl = sample_wavelengths<NS, RGB>(px, py, frame);
+    ps.sampler.start(px, py, frame, 0u);
+    camera_ray(cam, px, py, ps.sampler, ps.o, ps.d, ps.fw);
+    ps.thr = Spec<NS>::fill(1.f);
+    ps.L = mk3(0.f, 0.f, 0.f);
+    ps.scatter_pdf = 1e16f;
+    ps.eta_scale = 1.f;
+    ps.bounce = 0u;
+    ps.skips = 0u;
+    ps.alive = rv.max_depth > 0u ? 1u : 0u;
+}
+
+// the whole path in one thread (the megakernel form)
+template <int NS, bool RGB>
+__device__ f3 trace_path(const SceneView &sc, const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
+                         uint32_t frame, uint2 *stack, uint32_t stride, PathCounters &pc) {
+    PathState<NS> ps;
+    path_begin<NS, RGB>(cam, rv, px, py, frame, ps);
+    while (ps.alive) {
+        TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
+        ++pc.rays;
+        ShadowRay sr;
+        path_shade<NS, RGB>(sc, rv, ps, h, sr);
+        if (sr.valid) {
+            uint32_t dn = 0, dt = 0;
+            TraceHit sh = traverse<true>(sc, sr.o, sr.d, 0.f, 1.f - kShadowEpsilon, stack, stride, dn, dt);
+            pc.nodes += dn;
+            pc.tris += dt;
+            ++pc.rays;
+            if (sh.tri == kInvalid) ps.L += sr.c;
+        }
+    }
+    return ps.L * ps.fw;
+}
+
 __device__ __forceinline__ float zero_if_nan_inf(float v) { return (isnan(v) || isinf(v)) ? 0.f : v; }
 
 __device__ __forceinline__ float tone_map(float v, uint32_t tm) {
@@ -462,6 +523,26 @@ __device__ __forceinline__ float tone_map(float v, uint32_t tm) {
     }
     if (tm == 2) return v / (v + 1.f);
     return v;
+}
+
+// one sample into the film's running mean (NormalRadianceCollector::add_sample,
+// src/render_core/radiancecollectors/normal.cpp:87-101)
+__device__ __forceinline__ float4 film_add(const RenderView &rv, float4 acc, f3 L, uint32_t frame) {
+    float4 v = make_float4(zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f);
+    if (!rv.accumulation) return v;
+    float a = 1.f / float(frame + 1u);
+    return make_float4(acc.x + a * (v.x - acc.x), acc.y + a * (v.y - acc.y), acc.z + a * (v.z - acc.z),
+                       acc.w + a * (v.w - acc.w));
+}
+
+// the output buffer: tone_map(1 - exp(-v * exposure)) (radiance_collector.cpp:24-26)
+__device__ __forceinline__ float4 film_output(const RenderView &rv, float4 acc) {
+    float4 o4;
+    o4.x = tone_map(1.f - expf(-acc.x * rv.exposure), rv.tone_mapper);
+    o4.y = tone_map(1.f - expf(-acc.y * rv.exposure), rv.tone_mapper);
+    o4.z = tone_map(1.f - expf(-acc.z * rv.exposure), rv.tone_mapper);
+    o4.w = 1.f;
+    return o4;
 }
 
 }// namespace dev

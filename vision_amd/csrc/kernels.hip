@@ -63,31 +63,11 @@ __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, Re
     PathCounters pc{0, 0, 0};
     for (uint32_t s = 0; s < rv.spp; ++s) {
         uint32_t frame = rv.frame_start + s;
-        Swl<NS> swl = sample_wavelengths<NS, RGB>(px, py, frame);
-        Sampler sampler;
-        sampler.start(px, py, frame, 0u);
-        f3 o, d;
-        float fw;
-        camera_ray(cam, px, py, sampler, o, d, fw);
-        f3 L = trace_path<NS, RGB>(sc, rv, o, d, sampler, swl, lds_stack + threadIdx.x, blockDim.x, pc) * fw;
-        float4 v = make_float4(zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f);
-        if (rv.accumulation) {
-            float a = 1.f / float(frame + 1u);
-            acc = make_float4(acc.x + a * (v.x - acc.x), acc.y + a * (v.y - acc.y), acc.z + a * (v.z - acc.z),
-                              acc.w + a * (v.w - acc.w));
-        } else {
-            acc = v;
-        }
+        f3 L = trace_path<NS, RGB>(sc, cam, rv, px, py, frame, lds_stack + threadIdx.x, blockDim.x, pc);
+        acc = film_add(rv, acc, L, frame);
     }
     rt[idx] = acc;
-    if (rv.write_output) {
-        float4 o4;
-        o4.x = tone_map(1.f - expf(-acc.x * rv.exposure), rv.tone_mapper);
-        o4.y = tone_map(1.f - expf(-acc.y * rv.exposure), rv.tone_mapper);
-        o4.z = tone_map(1.f - expf(-acc.z * rv.exposure), rv.tone_mapper);
-        o4.w = 1.f;
-        out[idx] = o4;
-    }
+    if (rv.write_output) out[idx] = film_output(rv, acc);
     if (counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
         atomicAdd(counters + 1, (unsigned long long)pc.nodes);
