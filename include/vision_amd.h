@@ -90,10 +90,13 @@ typedef struct vs_scene_arrays {
     const uint32_t *instances;  /* [num_instances][4]: material, light, tri_base, 0 */
     const uint32_t *materials;  /* [num_materials][32]: type, flags, 30 float params */
     const uint32_t *lights;     /* [num_lights][28]: light records (112 B) */
-    const float *camera;        /* [24]: c2w rows [R|t] (12), screen lower_x, upper_y,
+    const float *camera;        /* [32]: c2w rows [R|t] (12), screen lower_x, upper_y,
                                    raster step x, y, tan(fov_y/2), lens_radius,
                                    focal_distance, width, height, filter (0 box,
-                                   1 triangle), filter_radius, thin_lens flag */
+                                   1 triangle, 2 fitted curve), filter_radius,
+                                   thin_lens flag, filter kind (0 box, 1 triangle,
+                                   2 gaussian, 3 mitchell, 4 sinc), gaussian sigma,
+                                   mitchell b, c, sinc tau, 3 unused */
     const float *render;        /* [8]: max_depth, min_depth, rr_threshold, mis_mode,
                                    light_sampler(0 uniform,1 power), ray_offset_factor,
                                    accumulation, exposure */

@@ -1193,6 +1193,7 @@ V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w) {
         if (mis_mode == 1) return b == 0 ? wgt : 0.f;
         return wgt;
     };
+    const V3 primary = rd;
     for (uint32_t b = 0; b < max_depth; ++b) {
         Hit h = c.trace({ro, rd, 0.f, FLT_MAX}, false);
         if (h.tri == kInv) {
@@ -1263,20 +1264,157 @@ V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w) {
         ro = offset_origin(it.pos, nn);
         rd = bs.wi;
     }
+    // only_direct (max_depth < 2) with both MIS strategies: one more
+    // BSDF-sampled ray collects emitters and the environment
+    // (integrator.cpp:276-281 running mis_bsdf of :150-204 with inner = false):
+    // nothing when the ray is still the primary one, material-less surfaces
+    // passed through, the bounce index 0 for the MIS correction
+    if (max_depth < 2 && mis_mode == 0 && !(rd.x == primary.x && rd.y == primary.y && rd.z == primary.z)) {
+        while (true) {
+            Hit h = c.trace({ro, rd, 0.f, FLT_MAX}, false);
+            if (h.tri == kInv) {
+                if (lt.env.present) {
+                    Spec L;
+                    float lpdf;
+                    env_eval(sd, lt.env, unit(sub(add(ro, rd), ro)), w, L, lpdf);
+                    lpdf *= select_pmf(s, lt, lt.env.light);
+                    Spec d = (L * correct(mis_w(scatter_pdf, lpdf), 0)) * thr;
+                    ret = add(ret, linear_srgb(sd, d, w));
+                }
+                break;
+            }
+            Surf it = surface(s, h.tri, h.b1, h.b2, true);
+            uint32_t mat = s.inst[4 * it.inst], light = s.inst[4 * it.inst + 1];
+            if (mat == kInv) {
+                V3 nn = dotp(it.ng, rd) > 0.f ? it.ng : neg(it.ng);
+                ro = offset_origin(it.pos, nn);
+                if (++skips > 64u) break;
+                continue;
+            }
+            if (light != kInv) {
+                Spec Le;
+                float lpdf;
+                eval_hit(sd, s, lt, it, light, h.tri, ro, w, Le, lpdf);
+                Spec d = (Le * thr) * correct(mis_w(scatter_pdf, lpdf), 0);
+                ret = add(ret, linear_srgb(sd, d, w));
+            }
+            break;
+        }
+    }
     return ret;
+}
+
+// ---------------------------------------------------------------- pixel filters
+// The fitted-curve filters' evaluate: gaussian.cpp:38-42 (a normal density,
+// src/math/util.h:37-42, less its value at the radius, clamped at 0, per
+// axis), mitchell.cpp:29-45 (Mitchell-Netravali cubic of 2x/r per axis) and
+// sinc.cpp:21-25 (Lanczos: sinc(x) sinc(x/tau) inside the radius, per axis,
+// times 4).
+float normal_density(float x, float sigma) {
+    float var = sigma * sigma;
+    return 1.f / std::sqrt(2 * kPi * var) * std::exp(-(x * x) / (2 * var));
+}
+float mitchell(float x, float B, float C) {
+    x = std::fabs(x);
+    float x2 = x * x, x3 = x2 * x;
+    if (x <= 1) return ((12 - 9 * B - 6 * C) * x3 + (-18 + 12 * B + 6 * C) * x2 + (6 - 2 * B)) * (1.f / 6.f);
+    if (x <= 2) return ((-B - 6 * C) * x3 + (6 * B + 30 * C) * x2 + (-12 * B - 48 * C) * x + (8 * B + 24 * C)) * (1.f / 6.f);
+    return 0.f;
+}
+float sinc_pi(float x) {
+    float t = kPi * x;
+    return 1 + t * t == 1 ? 1.f : std::sin(t) / t;
+}
+float filter_value(const float *cam, float x, float y) {
+    const int kind = int(cam[24]);
+    const float r = cam[22];
+    if (kind == 2) {
+        float sigma = cam[25], edge = normal_density(r, sigma);
+        return std::max(0.f, normal_density(x, sigma) - edge) * std::max(0.f, normal_density(y, sigma) - edge);
+    }
+    if (kind == 3) return mitchell(2 * x / r, cam[26], cam[27]) * mitchell(2 * y / r, cam[26], cam[27]);
+    if (kind == 4) {
+        float tau = cam[28];
+        float fx = std::fabs(x) > r ? 0.f : sinc_pi(x) * sinc_pi(x / tau);
+        float fy = std::fabs(y) > r ? 0.f : sinc_pi(y) * sinc_pi(y / tau);
+        return fx * fy * 4;
+    }
+    return 1.f;
+}
+
+// FilterSampler::build (src/render_core/filters/fitted_curve.h:33-53): the
+// filter at the centres of a 20 x 20 grid over [0, r)^2, the table normalised
+// by its mean signed value (summed in float, as std::accumulate does), |f|
+// sampled by a 2D alias table (AliasTable2D::build, alias2d.cpp:32-71)
+constexpr uint32_t kFis = 20;
+struct FisTable {
+    bool on{false};
+    std::vector<float> lut;
+    std::vector<Alias> rows;
+    Alias marg;
+};
+FisTable fis_build(const float *cam) {
+    FisTable t;
+    if (cam[21] != 2.f) return t;
+    t.on = true;
+    const float r = cam[22];
+    std::vector<float> absf(kFis * kFis);
+    t.lut.resize(kFis * kFis);
+    for (uint32_t y = 0; y < kFis; ++y)
+        for (uint32_t x = 0; x < kFis; ++x) {
+            float v = filter_value(cam, (float(x) + 0.5f) / float(kFis) * r, (float(y) + 0.5f) / float(kFis) * r);
+            t.lut[y * kFis + x] = v;
+            absf[y * kFis + x] = std::fabs(v);
+        }
+    float total = 0.f;
+    for (float v : t.lut) total += v;
+    const float mean = total / float(kFis * kFis);
+    for (float &v : t.lut) v = v / mean;
+    std::vector<float> row_mass(kFis);
+    for (uint32_t y = 0; y < kFis; ++y) {
+        t.rows.push_back(alias_build(absf.data() + y * kFis, kFis));
+        row_mass[y] = t.rows.back().integral;
+    }
+    t.marg = alias_build(row_mass.data(), kFis);
+    return t;
+}
+
+// FilterSampler::sample (fitted_curve.h:67-73) with
+// AliasTable2D::sample_continuous (alias2d.cpp:106-129): u -> 2u - 1, the
+// table sampled at |u|, the point mirrored by the sign of u (sign(0) = +1),
+// weight = lut / pdf; FittedCurveFilter::sample (:90-94) scales by the radius
+void fis_sample(const FisTable &t, float radius, float ux, float uy, float &fx, float &fy, float &weight) {
+    float sx = ux * 2.f - 1.f, sy = uy * 2.f - 1.f;
+    float ry;
+    uint32_t iv = alias_sample(t.marg, std::fabs(sy), &ry);
+    float v = (float(iv) + ry) / float(kFis);
+    float pdf_v = t.marg.integral > 0.f ? t.marg.func[iv] / t.marg.integral : 0.f;
+    const Alias &row = t.rows[iv];
+    float rx;
+    uint32_t iu = alias_sample(row, std::fabs(sx), &rx);
+    float u = (float(iu) + rx) / float(kFis);
+    float row_integral = t.marg.func[iv];
+    float pdf_u = row_integral > 0.f ? row.func[iu] / row_integral : 0.f;
+    weight = t.lut[iv * kFis + iu] / (pdf_u * pdf_v);
+    fx = (sx < 0.f ? -u : u) * radius;
+    fy = (sy < 0.f ? -v : v) * radius;
 }
 
 // Sensor::generate_ray (src/base/sensor/sensor.cpp:44-56), thin lens
 // (src/render_core/sensors/thin_lens.cpp:34-42), Sampler::sensor_sample
 // (src/base/sampler.h:65-73), box / tent filters (box.cpp:16-20, triangle.cpp:16-18)
-void camera_ray(const Scene &s, uint32_t px, uint32_t py, Sampler &sp, V3 &o, V3 &d) {
+void camera_ray(const Scene &s, const FisTable &fis, uint32_t px, uint32_t py, Sampler &sp, V3 &o, V3 &d,
+                float &weight) {
     const float *c = s.cam;
     float ux = sp.next1d(), uy = sp.next1d();
     float r = c[22];
     float fx, fy;
+    weight = 1.f;
     if (c[21] == 0.f) {
         fx = -r + ux * (r + r);
         fy = -r + uy * (r + r);
+    } else if (fis.on) {
+        fis_sample(fis, r, ux, uy, fx, fy, weight);
     } else {
         fx = sample_tent(ux, r);
         fy = sample_tent(uy, r);
@@ -1409,6 +1547,26 @@ void or_alias_sample(const float *w, uint32_t nw, const float *u, uint32_t n, ui
     for (uint32_t i = 0; i < n; ++i) {
         idx[i] = alias_sample(a, u[i], u_rem ? &u_rem[i] : nullptr);
         if (pmf) pmf[i] = alias_pmf(a, idx[i]);
+    }
+}
+
+// pixel filter of a 32-float camera record (fields 21, 22, 24-28: sampling
+// mode, radius, filter kind, sigma, b, c, tau)
+void or_filter_eval(const float *cam, const float *xy, uint32_t n, float *out) {
+    for (uint32_t i = 0; i < n; ++i) out[i] = filter_value(cam, xy[2 * i], xy[2 * i + 1]);
+}
+void or_filter_table(const float *cam, float *lut) {
+    FisTable t = fis_build(cam);
+    for (uint32_t i = 0; i < kFis * kFis; ++i) lut[i] = t.on ? t.lut[i] : 0.f;
+}
+void or_filter_sample(const float *cam, const float *u, uint32_t n, float *p, float *weight) {
+    FisTable t = fis_build(cam);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!t.on) {
+            p[2 * i] = p[2 * i + 1] = weight[i] = 0.f;
+            continue;
+        }
+        fis_sample(t, cam[22], u[2 * i], u[2 * i + 1], p[2 * i], p[2 * i + 1], weight[i]);
     }
 }
 
@@ -1606,6 +1764,7 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
                                          spec->env_o2w, spec->env_scale, spec->world_radius)
                           : build_lights(s, sd, kInv, 0, 0, nullptr, nullptr, nullptr, 1.f, 1.f);
     Ctx c{&s, &b, &lt, &sd, mode == 0};
+    const FisTable fis = fis_build(cam);
     uint32_t W = uint32_t(cam[19]);
     bool accumulate = render[6] != 0.f;
     std::atomic<uint32_t> next_row{y0};
@@ -1620,8 +1779,9 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
                     Sampler sp;
                     sp.start(x, y, f, 0u);
                     V3 o, d;
-                    camera_ray(s, x, y, sp, o, d);
-                    V3 L = Li(c, o, d, sp, w);
+                    float fw;
+                    camera_ray(s, fis, x, y, sp, o, d, fw);
+                    V3 L = mul(Li(c, o, d, sp, w), fw);
                     float v[4] = {zero_nan_inf(L.x), zero_nan_inf(L.y), zero_nan_inf(L.z), 1.f};
                     if (accumulate) {
                         float a = 1.f / float(f + 1u);

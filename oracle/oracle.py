@@ -47,6 +47,9 @@ def lib():
         h.or_alias_build.argtypes = [_f32p, C.c_uint32, _f32p, _u32p, C.POINTER(C.c_float)]
         h.or_alias_sample.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, _u32p, C.c_void_p, C.c_void_p]
         h.or_offset_ray_origin.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
+        h.or_filter_eval.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
+        h.or_filter_table.argtypes = [_f32p, _f32p]
+        h.or_filter_sample.argtypes = [_f32p, _f32p, C.c_uint32, _f32p, _f32p]
         h.or_eval_diffuse.argtypes = [C.c_uint32, _f32p, C.c_float, _f32p, _f32p, C.c_uint32, _f32p, _f32p]
         h.or_trace.argtypes = [_f32p, _u32p, C.c_uint32, C.c_uint32, _f32p, C.c_uint32, C.c_int, C.c_int,
                                _u32p, _f32p, _f32p]
@@ -131,6 +134,41 @@ def alias_sample(w: np.ndarray, u: np.ndarray):
     pmf = np.zeros(len(u), np.float32)
     lib().or_alias_sample(w, len(w), u, len(u), idx, urem.ctypes.data, pmf.ctypes.data)
     return idx, urem, pmf
+
+
+FILTER_KINDS = {"box": 0, "triangle": 1, "gaussian": 2, "mitchell": 3, "sinc": 4}
+
+
+def filter_camera(kind: str, radius: float, sigma=1.0, b=1 / 3, c=1 / 3, tau=3.0) -> np.ndarray:
+    """A 32-float camera record carrying only the pixel-filter fields."""
+    cam = np.zeros(32, np.float32)
+    k = FILTER_KINDS[kind]
+    cam[21] = k if k <= 1 else 2
+    cam[22] = radius
+    cam[24:29] = [k, sigma, b, c, tau]
+    return cam
+
+
+def filter_eval(cam: np.ndarray, xy: np.ndarray) -> np.ndarray:
+    xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
+    out = np.zeros(len(xy), np.float32)
+    lib().or_filter_eval(np.ascontiguousarray(cam, np.float32), xy, len(xy), out)
+    return out
+
+
+def filter_table(cam: np.ndarray) -> np.ndarray:
+    lut = np.zeros(400, np.float32)
+    lib().or_filter_table(np.ascontiguousarray(cam, np.float32), lut)
+    return lut.reshape(20, 20)
+
+
+def filter_sample(cam: np.ndarray, u: np.ndarray):
+    """Filter importance sampling: film-plane offsets (n, 2) and weights (n,)."""
+    u = np.ascontiguousarray(u, np.float32).reshape(-1, 2)
+    p = np.zeros_like(u)
+    w = np.zeros(len(u), np.float32)
+    lib().or_filter_sample(np.ascontiguousarray(cam, np.float32), u, len(u), p, w)
+    return p, w
 
 
 def offset_ray_origin(p: np.ndarray, n: np.ndarray) -> np.ndarray:

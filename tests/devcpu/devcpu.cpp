@@ -59,6 +59,7 @@ struct DcScene {
     Bvh8 bvh;
     SceneView v{};
     CameraView cam{};
+    FilterTable fis;
 };
 
 extern "C" void *dc_load(const char *text, const char *base) {
@@ -115,6 +116,15 @@ extern "C" void dc_set_camera(void *p, const float *c) {
     v.lower_x = c[12]; v.upper_y = c[13]; v.raster_sx = c[14]; v.raster_sy = c[15]; v.tan_half_fov = c[16];
     v.lens_radius = c[17]; v.focal_distance = c[18]; v.width = uint32_t(c[19]); v.height = uint32_t(c[20]);
     v.filter = uint32_t(c[21]); v.filter_radius = c[22]; v.thin_lens = uint32_t(c[23]);
+    if (v.filter == 2u) {
+        s->fis = build_filter_table(s->h.camera);
+        v.fis_lut = s->fis.lut.data();
+        v.fis_cond = s->fis.cond.data();
+        v.fis_cond_func = s->fis.cond_func.data();
+        v.fis_marg = s->fis.marg.data();
+        v.fis_marg_func = s->fis.marg_func.data();
+        v.fis_marg_integral = s->fis.marg_integral;
+    }
 }
 
 template <int NS, bool RGB>

@@ -127,3 +127,21 @@ def test_device_path_source_equals_oracle_procedural(dc, tmp_path, spectrum):
                                       max_depth=8, spectrum=spectrum, hdri=(256, 128))
     film, ref = _dev_render(dc, scene, 2, str(tmp_path))
     assert _bits_equal(film, ref)
+
+
+@pytest.mark.parametrize("filt,param", [("gaussian", {"radius": 1.5, "sigma": 0.5}), ("mitchell", {"radius": 2.0}),
+                                        ("sinc", {"radius": 3.0, "tau": 3.0}), ("gaussian", {})])
+def test_device_path_source_equals_oracle_fitted_filters(dc, filt, param):
+    scene = scenegen.cornell_box(24, 24, max_depth=4)
+    scene["camera"]["param"]["filter"] = {"type": filt, "param": param}
+    film, ref = _dev_render(dc, scene, 3)
+    assert _bits_equal(film, ref), filt
+
+
+@pytest.mark.parametrize("max_depth", [0, 1])
+def test_device_path_source_equals_oracle_direct_only(dc, tmp_path, max_depth):
+    scene = scenegen.procedural_scene(tmp_path, seed=4, n_instances=9, level=3, width=32, height=20, spp=2,
+                                      max_depth=max_depth, spectrum="hero", hdri=(256, 128))
+    film, ref = _dev_render(dc, scene, 2, str(tmp_path))
+    assert _bits_equal(film, ref)
+    assert (film[..., :3].max() > 0) == (max_depth > 0)

@@ -196,3 +196,34 @@ def test_ray_counters(gpu_ctx):
     assert st["node_fetches"] >= st["rays"]
     sc.set_counting(False)
     sc.close()
+
+
+@pytest.mark.parametrize("filt,param", [("gaussian", {"radius": 1.5, "sigma": 0.5}), ("mitchell", {"radius": 2.0}),
+                                        ("sinc", {"radius": 3.0, "tau": 3.0})])
+def test_fitted_filter_parity(gpu_ctx, filt, param):
+    """Filter importance sampling of the fitted-curve filters (fitted_curve.h)."""
+    scene = scenegen.cornell_box(96, 96, max_depth=6)
+    scene["camera"]["param"]["filter"] = {"type": filt, "param": param}
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    sc.render(2)
+    gpu, ref = sc.film(0), oracle.render(sc.export(), 0, 2)
+    frac, _ = _compare(gpu, ref)
+    print(f"{filt}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
+    assert frac >= 0.999
+    sc.close()
+
+
+@pytest.mark.parametrize("kernel", ["megakernel", "wavefront"])
+def test_direct_only_bounce_parity(gpu_ctx, kernel):
+    """max_depth 1: the extra BSDF-sampled emitter ray (integrator.cpp:276-281),
+    in both the megakernel and the wavefront form."""
+    scene = scenegen.cornell_box(64, 64, max_depth=1)
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    sc.set_kernel(kernel, 2, 256)
+    sc.render(3)
+    gpu, ref = sc.film(0), oracle.render(sc.export(), 0, 3)
+    frac, _ = _compare(gpu, ref)
+    assert frac >= 0.999
+    sc.close()

@@ -228,7 +228,7 @@ class Scene:
             "instances": arr(a.instances, 4 * a.num_instances, np.uint32).reshape(-1, 4),
             "materials": arr(a.materials, 32 * a.num_materials, np.uint32).reshape(-1, 32),
             "lights": arr(a.lights, 28 * a.num_lights, np.uint32).reshape(-1, 28),
-            "camera": arr(a.camera, 24, np.float32),
+            "camera": arr(a.camera, 32, np.float32),
             "render": arr(a.render, 8, np.float32),
             "spd": arr(a.spd, a.num_spd, np.float32),
             "spectrum_rgb": int(a.spectrum_rgb),

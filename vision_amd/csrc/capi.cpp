@@ -82,6 +82,7 @@ struct vs_scene {
     // device buffers
     DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, mat, light, alias, func;
     DevBuf spd, lut, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
+    DevBuf fis_lut, fis_cond, fis_cond_func, fis_marg, fis_marg_func;
     DevBuf film_rt, film_out, counters;
     DevBuf scratch_rays, scratch_hits;
     // wavefront form: per-path state, queues, append counters
@@ -159,14 +160,16 @@ void fill_camera(vs_scene *s) {
     v.focal_distance = c.focal_distance;
     v.width = c.width;
     v.height = c.height;
-    v.filter = c.filter_type == "box" ? 0u : 1u;
+    const int kind = vs::filter_kind(c);
+    v.filter = kind <= 1 ? uint32_t(kind) : 2u;
     v.filter_radius = c.filter_radius;
     v.thin_lens = c.type == "thin_lens" ? 1u : 0u;
     s->exp_camera.assign(v.c2w, v.c2w + 12);
-    float rest[12] = {v.lower_x, v.upper_y, v.raster_sx, v.raster_sy, v.tan_half_fov, v.lens_radius,
+    float rest[20] = {v.lower_x, v.upper_y, v.raster_sx, v.raster_sy, v.tan_half_fov, v.lens_radius,
                       v.focal_distance, float(v.width), float(v.height), float(v.filter), v.filter_radius,
-                      float(v.thin_lens)};
-    s->exp_camera.insert(s->exp_camera.end(), rest, rest + 12);
+                      float(v.thin_lens), float(kind), c.filter_sigma, c.filter_b, c.filter_c, c.filter_tau,
+                      0.f, 0.f, 0.f};
+    s->exp_camera.insert(s->exp_camera.end(), rest, rest + 20);
 }
 
 void fill_render_export(vs_scene *s) {
@@ -341,6 +344,18 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         s->env_cond_func.upload(pk.env_cond_func, st);
         s->env_marg.upload(pk.env_marg, st);
         s->env_marg_func.upload(pk.env_marg_func, st);
+        vs::FilterTable ft = vs::build_filter_table(h.camera);
+        s->fis_lut.upload(ft.lut, st);
+        s->fis_cond.upload(ft.cond, st);
+        s->fis_cond_func.upload(ft.cond_func, st);
+        s->fis_marg.upload(ft.marg, st);
+        s->fis_marg_func.upload(ft.marg_func, st);
+        s->cam.fis_lut = s->fis_lut.as<float>();
+        s->cam.fis_cond = s->fis_cond.as<vs::AliasEntry>();
+        s->cam.fis_cond_func = s->fis_cond_func.as<float>();
+        s->cam.fis_marg = s->fis_marg.as<vs::AliasEntry>();
+        s->cam.fis_marg_func = s->fis_marg_func.as<float>();
+        s->cam.fis_marg_integral = ft.marg_integral;
         s->counters.alloc(64);
         alloc_film(s);
         hip_check(hipStreamSynchronize(st), "upload sync");

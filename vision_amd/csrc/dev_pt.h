@@ -302,6 +302,7 @@ struct PathState {
     uint32_t bounce;  // scattering events so far
     uint32_t skips;   // material-less surfaces passed through
     uint32_t alive;
+    uint32_t tail;    // the BSDF-only emitter bounce of max_depth < 2 (integrator.cpp:276-281)
     float fw;         // film weight of the sensor sample
     Sampler sampler;
     Swl<NS> swl;
@@ -319,7 +320,7 @@ template <int NS, bool RGB>
 __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<NS> &ps, const TraceHit &h,
                            ShadowRay &sr) {
     sr.valid = 0u;
-    const uint32_t bounces = ps.bounce;
+    const uint32_t bounces = ps.tail ? 0u : ps.bounce;
     if (h.tri == kInvalid) {
         // IlluminationIntegrator::evaluate_miss (integrator.cpp:111-132)
         if (sc.env_light != kInvalid) {
@@ -354,6 +355,10 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
         if (rv.mis_mode == 2) w = 1.f;
         else if (rv.mis_mode == 1) w = bounces == 0 ? w : 0.f;
         ps.L += to_linear_srgb<NS, RGB>(sc, Le * ps.thr * w, ps.swl);
+    }
+    if (ps.tail) {
+        ps.alive = 0u;
+        return;
     }
     // next-event estimation (direct_lighting, integrator.cpp:22-38)
     LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, ps.sampler, ps.swl);
@@ -403,10 +408,18 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
     }
     ps.scatter_pdf = bs.e.pdf;
     f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
+    // with max_depth < 2 the previous ray is the primary one; the extra
+    // bounce below is skipped when the new direction equals it (:154-158)
+    const bool same_dir = bs.wi.x == ps.d.x && bs.wi.y == ps.d.y && bs.wi.z == ps.d.z;
     ps.o = offset_ray_origin(it.pos, n);
     ps.d = bs.wi;
     ps.bounce = bounces + 1u;
-    if (ps.bounce >= rv.max_depth) ps.alive = 0u;
+    if (ps.bounce >= rv.max_depth) {
+        // with max_depth < 2 and full MIS, one more BSDF-sampled ray adds
+        // emitter / environment light only (integrator.cpp:276-281)
+        if (rv.max_depth < 2u && rv.mis_mode == 0 && !same_dir) ps.tail = 1u;
+        else ps.alive = 0u;
+    }
 }
 
 // Sensor::generate_ray (src/base/sensor/sensor.cpp:44-56) + filter sample
@@ -415,13 +428,32 @@ __device__ __forceinline__ void camera_ray(const CameraView &cam, uint32_t px, u
                                            f3 &o, f3 &d, float &weight) {
     f2 uf = sampler.next2d();
     f2 fs;
+    weight = 1.f;
     if (cam.filter == 0) {
         float r = cam.filter_radius;
         fs = mk2(-r + uf.x * (r + r), -r + uf.y * (r + r));
-    } else {
+    } else if (cam.filter == 1) {
         fs = mk2(sample_tent(uf.x, cam.filter_radius), sample_tent(uf.y, cam.filter_radius));
+    } else {
+        // FilterSampler::sample (src/render_core/filters/fitted_curve.h:67-73) and
+        // FittedCurveFilter::sample (:90-94): the 2D alias table over |f| on the
+        // positive quadrant, mirrored by the signs of u; weight = f / pdf
+        f2 u = mk2(uf.x * 2.f - 1.f, uf.y * 2.f - 1.f);
+        float uvr;
+        uint32_t iv = alias_pick(cam.fis_marg, 20u, fabsf(u.y), &uvr);
+        float fv = (float(iv) + uvr) / 20.f;
+        float pdf_v = cam.fis_marg_integral > 0.f ? cam.fis_marg_func[iv] / cam.fis_marg_integral : 0.f;
+        uint32_t row = 20u * iv;
+        float uur;
+        uint32_t iu = alias_pick(cam.fis_cond + row, 20u, fabsf(u.x), &uur);
+        float fu = (float(iu) + uur) / 20.f;
+        float integral_u = cam.fis_marg_func[iv];
+        float pdf_u = integral_u > 0.f ? cam.fis_cond_func[row + iu] / integral_u : 0.f;
+        float pdf = pdf_u * pdf_v;
+        f2 p = mk2(u.x < 0.f ? -fu : fu, u.y < 0.f ? -fv : fv);
+        weight = cam.fis_lut[row + iu] / pdf;
+        fs = mk2(p.x * cam.filter_radius, p.y * cam.filter_radius);
     }
-    weight = 1.f;
     f2 p_lens = sampler.next2d();
     (void)sampler.next1d();// time
     float fx = float(px) + 0.5f + fs.x;
@@ -488,6 +520,7 @@ __device__ __forceinline__ void path_begin(const CameraView &cam, const RenderVi
     ps.eta_scale = 1.f;
     ps.bounce = 0u;
     ps.skips = 0u;
+    ps.tail = 0u;
     ps.alive = rv.max_depth > 0u ? 1u : 0u;
 }
 

@@ -61,9 +61,17 @@ struct CameraView {
     float tan_half_fov;
     float lens_radius, focal_distance;
     uint32_t width, height;
-    uint32_t filter;// 0 box, 1 triangle
+    uint32_t filter;// 0 box, 1 triangle, 2 fitted curve (filter importance sampling)
     float filter_radius;
     uint32_t thin_lens;
+    // fitted-curve filters (FilterSampler): normalised 20x20 value table and
+    // the 2D alias table over |value|
+    const float *fis_lut;
+    const AliasEntry *fis_cond;
+    const float *fis_cond_func;
+    const AliasEntry *fis_marg;
+    const float *fis_marg_func;
+    float fis_marg_integral;
 };
 
 struct RenderView {

@@ -533,7 +533,14 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         const Json &fr = f["param"]["radius"];
         auto frv = fr.as_numbers();
         c.filter_radius = frv.empty() ? 0.5f : float(frv[0]);
-        if (c.filter_type != "box" && c.filter_type != "triangle")
+        // GaussianFilter / MitchellFilter / LanczosSincFilter parameters
+        // (gaussian.cpp:17-20, mitchell.cpp:16-18, sinc.cpp:16-17)
+        c.filter_sigma = float(f["param"]["sigma"].as_float(1.0));
+        c.filter_b = float(f["param"]["b"].as_float(1.0 / 3.0));
+        c.filter_c = float(f["param"]["c"].as_float(1.0 / 3.0));
+        c.filter_tau = float(f["param"]["tau"].as_float(3.0));
+        if (c.filter_type != "box" && c.filter_type != "triangle" && c.filter_type != "gaussian" &&
+            c.filter_type != "mitchell" && c.filter_type != "sinc")
             throw std::runtime_error("unsupported filter: " + c.filter_type);
         if (c.width == 0 || c.height == 0) throw std::runtime_error("resolution must be positive");
     }

@@ -68,6 +68,8 @@ struct CameraHost {
     uint32_t width{1280}, height{720};
     std::string filter_type{"gaussian"};
     float filter_radius{0.5f};
+    // fitted-curve filter parameters: gaussian sigma, mitchell b / c, sinc tau
+    float filter_sigma{1.f}, filter_b{1.f / 3.f}, filter_c{1.f / 3.f}, filter_tau{3.f};
     float exposure{1.f};
     uint32_t accumulation{1};
     std::string tone_mapper{"linear"};

@@ -382,4 +382,88 @@ PackedScene pack_scene(const SceneHost &s) {
     return out;
 }
 
+int filter_kind(const CameraHost &c) {
+    if (c.filter_type == "box") return 0;
+    if (c.filter_type == "triangle") return 1;
+    if (c.filter_type == "gaussian") return 2;
+    if (c.filter_type == "mitchell") return 3;
+    if (c.filter_type == "sinc") return 4;
+    throw std::runtime_error("unsupported filter: " + c.filter_type);
+}
+
+namespace {
+
+// gaussian (src/math/util.h:36-43)
+float gaussian(float x, float mu, float sigma) {
+    return 1.f / std::sqrt(2 * kPiF * (sigma * sigma)) * std::exp(-((x - mu) * (x - mu)) / (2 * (sigma * sigma)));
+}
+// MitchellFilter::mitchell_1d (mitchell.cpp:22-35)
+float mitchell_1d(float x, float b, float c) {
+    x = std::fabs(x);
+    if (x <= 1)
+        return ((12 - 9 * b - 6 * c) * (x * x * x) + (-18 + 12 * b + 6 * c) * (x * x) + (6 - 2 * b)) * (1.f / 6.f);
+    if (x <= 2)
+        return ((-b - 6 * c) * (x * x * x) + (6 * b + 30 * c) * (x * x) + (-12 * b - 48 * c) * x + (8 * b + 24 * c)) *
+               (1.f / 6.f);
+    return 0;
+}
+// windowed_sinc (src/math/util.h:51-66)
+float sin_x_over_x(float x) { return 1 + x * x == 1 ? 1.f : std::sin(x) / x; }
+float sinc(float x) { return sin_x_over_x(kPiF * x); }
+float windowed_sinc(float x, float radius, float tau) {
+    return std::fabs(x) > radius ? 0.f : sinc(x) * sinc(x / tau);
+}
+
+}// namespace
+
+// Filter::evaluate of gaussian.cpp:30-34, mitchell.cpp:37-40, sinc.cpp:21-24
+float filter_evaluate(const CameraHost &c, float x, float y) {
+    const float r = c.filter_radius;
+    switch (filter_kind(c)) {
+        case 2: {
+            float ex = gaussian(r, 0, c.filter_sigma);
+            float vx = gaussian(x, 0, c.filter_sigma) - ex;
+            float vy = gaussian(y, 0, c.filter_sigma) - ex;
+            return std::max(0.f, vx) * std::max(0.f, vy);
+        }
+        case 3: return mitchell_1d(2 * x / r, c.filter_b, c.filter_c) * mitchell_1d(2 * y / r, c.filter_b, c.filter_c);
+        case 4: return windowed_sinc(x, r, c.filter_tau) * windowed_sinc(y, r, c.filter_tau) * 4;
+        default: return 1.f;
+    }
+}
+
+// FilterSampler::build (fitted_curve.h:33-53)
+FilterTable build_filter_table(const CameraHost &c) {
+    FilterTable t;
+    const int n = kFilterTableSize, len = n * n;
+    t.lut.resize(len);
+    std::vector<float> func(len);
+    const float r = c.filter_radius;
+    for (int i = 0; i < len; ++i) {
+        int x = i % n, y = i / n;
+        float px = (x + 0.5f) / n * r, py = (y + 0.5f) / n * r;
+        float val = filter_evaluate(c, px, py);
+        func[i] = std::fabs(val);
+        t.lut[i] = val;
+    }
+    float sum = 0.f;
+    for (float v : t.lut) sum += v;
+    float integral = sum / len;
+    for (float &v : t.lut) v = v / integral;
+    // AliasTable2D::build (alias2d.cpp:32-71)
+    std::vector<float> marg(n);
+    for (int y = 0; y < n; ++y) {
+        std::vector<float> row(func.begin() + y * n, func.begin() + (y + 1) * n);
+        AliasTableHost a = build_alias(row);
+        marg[y] = a.integral;
+        t.cond.insert(t.cond.end(), a.table.begin(), a.table.end());
+        t.cond_func.insert(t.cond_func.end(), a.func.begin(), a.func.end());
+    }
+    AliasTableHost m = build_alias(marg);
+    t.marg = m.table;
+    t.marg_func = m.func;
+    t.marg_integral = m.integral;
+    return t;
+}
+
 }// namespace vs

@@ -47,4 +47,20 @@ struct PackedScene {
 
 PackedScene pack_scene(const SceneHost &s);
 
+// FilterSampler (src/render_core/filters/fitted_curve.h:16-73): the 20x20
+// table of a fitted-curve filter (gaussian, mitchell, sinc) over the positive
+// quadrant of its radius, normalised by its mean, and the 2D alias table of
+// its absolute values
+struct FilterTable {
+    std::vector<float> lut;
+    std::vector<AliasEntry> cond, marg;
+    std::vector<float> cond_func, marg_func;
+    float marg_integral{0.f};
+};
+constexpr int kFilterTableSize = 20;
+// 0 box, 1 triangle, 2 gaussian, 3 mitchell, 4 sinc
+int filter_kind(const CameraHost &c);
+float filter_evaluate(const CameraHost &c, float x, float y);
+FilterTable build_filter_table(const CameraHost &c);
+
 }// namespace vs

@@ -17,7 +17,7 @@ struct WavefrontView {
     float4 *d;    // ray direction xyz, eta scale
     float4 *thr;  // throughput (up to 4 wavelengths)
     float4 *L;    // radiance gathered xyz, film weight
-    uint4 *meta;  // sampler state, bounce | skips << 16, alive, unused
+    uint4 *meta;  // sampler state, bounce | skips << 16, alive, tail bounce
     float4 *lam;  // hero wavelengths
     float4 *lpdf; // their pdfs
     float4 *hit;  // t, triangle id bits, b1, b2

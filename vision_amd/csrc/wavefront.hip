@@ -32,6 +32,7 @@ __device__ __forceinline__ void load_state(const WavefrontView &w, uint32_t i, P
     ps.bounce = m.y & 0xFFFFu;
     ps.skips = m.y >> 16;
     ps.alive = m.z;
+    ps.tail = m.w;
     if (RGB) {
         ps.swl = sample_wavelengths<NS, true>(0u, 0u, 0u);
     } else {
@@ -54,7 +55,7 @@ __device__ __forceinline__ void store_state(const WavefrontView &w, uint32_t i, 
     for (int k = 0; k < NS; ++k) tv[k] = ps.thr.v[k];
     w.thr[i] = make_float4(tv[0], tv[1], tv[2], tv[3]);
     w.L[i] = make_float4(ps.L.x, ps.L.y, ps.L.z, ps.fw);
-    w.meta[i] = make_uint4(ps.sampler.state, ps.bounce | (ps.skips << 16), ps.alive, 0u);
+    w.meta[i] = make_uint4(ps.sampler.state, ps.bounce | (ps.skips << 16), ps.alive, ps.tail);
     if (!RGB) {
         float lv[4] = {0.f, 0.f, 0.f, 0.f}, pv[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll

@@ -59,7 +59,8 @@ def diffuse(name, color):
 
 
 def camera(position, target, fov_y, width, height, kind="pinhole", filter_type="box", filter_radius=0.5,
-           up=(0, 1, 0)):
+           up=(0, 1, 0), filter_param=None):
+    fparam = {"radius": filter_radius, **(filter_param or {})}
     return {
         "type": kind,
         "param": {
@@ -69,7 +70,7 @@ def camera(position, target, fov_y, width, height, kind="pinhole", filter_type="
             "radiance_collector": {"type": "normal", "param": {"resolution": [width, height], "exposure": 1.0,
                                                                "accumulation": 1,
                                                                "tone_mapper": {"type": "linear"}}},
-            "filter": {"type": filter_type, "param": {"radius": filter_radius}},
+            "filter": {"type": filter_type, "param": fparam},
         },
     }
 
