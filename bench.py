@@ -1,15 +1,26 @@
 #!/usr/bin/env python3
-"""Benchmark: megakernel path tracing throughput on one MI355X.
+"""Benchmark: rendering throughput on one MI355X.
 
-A step is one complete render job of the workload (film reset + all spp
-frames of the megakernel over the full film), with the scene, BVH and film
-already resident in HBM.  Prints one JSON line.
+Workloads:
+  procedural (default)  plain path tracing, 1920x1080, 64 spp, depth 16,
+                        hero-4 spectral, 1.31M triangles + HDRI; a step is
+                        one render job (film reset + all 64 frames).
+  cbox                  the 256x256 Cornell box, 16 spp, depth 8, sRGB.
+  restir                ReSTIR DI: the Cornell box lit by 10 000 emissive
+                        triangles and 256 point / spot lights, 1920x1080,
+                        1 spp per frame x 64 frames along a seeded camera
+                        jitter path; a step is one 64-frame job.
+Scene, BVH and film are resident in HBM before the timed region.  Prints one
+JSON line.
 
-The roofline entry is for the dominant kernel (the PT megakernel): its
-algorithmic bytes come from a counting run of the same job (BVH8 node
-fetches x 80 B + leaf-triangle fetches x 48 B + rays x 48 B for ray in /
-hit out + 32 B film read/write per pixel), divided by the kernel's average
-HIP-event duration in the timed region.
+The roofline entry is for the dominant kernel, with algorithmic bytes from a
+counting run of the same job (BVH8 node fetches x 80 B + leaf-triangle
+fetches x 48 B + rays x 48 B ray in / hit out, plus the kernel's own record
+traffic) divided by the kernel's HIP-event time in the timed region:
+  megakernel PT: the PT megakernel (+ 32 B film read / write per pixel);
+  wavefront PT: the closest-hit traversal kernel;
+  ReSTIR: shader0 (+ 48 B surface and 48 B reservoir written, the previous
+  frame's 48 B surface and 48 B reservoir read, per pixel).
 """
 from __future__ import annotations
 
@@ -51,6 +62,17 @@ def workload(name: str):
                         "hero spectral (4 wavelengths), box filter (plain megakernel PT)",
             "resolution": [1920, 1080], "spp": spp, "max_depth": 16, "triangles": 1310722,
             "spectrum": "hero4"}
+    if name == "restir":
+        frames = 64
+        return scenegen.many_lights_box(1920, 1080, seed=0), ".", frames, {
+            "workload": "ReSTIR DI: Cornell-box geometry with 10,000 emissive triangles (5,000 quads, radiance "
+                        "log-uniform [1,100]) + 256 point/spot lights, power light sampler; 32 light + 1 BSDF "
+                        "candidates, temporal reuse (history 20), 5 spatial neighbours within 30 px in 2 passes; "
+                        "1920x1080, 1 spp/frame x 64 frames along a seeded camera jitter path, sRGB, fp32 "
+                        "reservoirs",
+            "resolution": [1920, 1080], "frames": frames, "M_light": 32, "M_bsdf": 1, "spatial_passes": 2,
+            "spatial_neighbours": 5, "history_limit": 20, "emissive_triangles": 10000, "point_spot_lights": 256,
+            "spectrum": "srgb"}
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -81,15 +103,34 @@ def cpu_baseline(scene, spp: int, budget_s: float = 15.0) -> dict:
                       f"after {setup:.2f} s setup)"}
 
 
+def restir_cpu_baseline(scene, frames: int = 2) -> dict:
+    """The oracle's ReSTIR DI frames on the host cores (at most 16) over the
+    full image: the first `frames` frames of the same job."""
+    from oracle import oracle
+    from vision_amd import scenegen
+    cores = max(1, min(16, os.cpu_count() or 1))
+    r = oracle.Restir(scene.export(), threads=cores)
+    cams = scenegen.jitter_path(frames, seed=1)
+    t0 = time.perf_counter()
+    for c2w in cams:
+        scene.update_camera(c2w, 19.5)
+        r.render(1, scene=scene.export())
+    t = time.perf_counter() - t0
+    samples = r.W * r.H * frames
+    return {"value": round(samples / t / 1e6, 4), "unit": "Msamples/s", "cores": cores, "kind": "port",
+            "sample": f"the first {frames} frames of the job at {r.W}x{r.H} ({samples} samples, {t:.2f} s, "
+                      f"per-frame BVH2 / light-table setup included)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox"])
+    ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox", "restir"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--kernel", default="megakernel", choices=["megakernel", "wavefront"])
-    ap.add_argument("--chunk-frames", type=int, default=8)
+    ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
+    ap.add_argument("--chunk-frames", type=int, default=16)
     ap.add_argument("--trace-block", type=int, default=128)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -97,27 +138,44 @@ def main():
         sys.exit(2)
 
     import vision_amd as va
+    from vision_amd import scenegen
     scene_desc, base_dir, spp, config = workload(args.workload)
+    restir = args.workload == "restir"
     ctx = va.Context(0)
     sc = ctx.load_text(scene_desc, base_dir)
     t0 = time.perf_counter()
     sc.build_accel()
     build_s = time.perf_counter() - t0
-    sc.set_kernel(args.kernel, args.chunk_frames, args.trace_block)
+    if not restir:
+        sc.set_kernel(args.kernel, args.chunk_frames, args.trace_block)
+        config["kernel"] = args.kernel + (f" ({args.chunk_frames} frames per chunk)" if args.kernel == "wavefront"
+                                          else "")
     info = sc.info()
     W, H = info["width"], info["height"]
+    cams = scenegen.jitter_path(spp, seed=1) if restir else None
 
     def step():
+        """One job; returns the summed stats of its render calls."""
         sc.reset_film()
-        return sc.render(spp, fused=True)
+        if not restir:
+            return sc.render(spp, fused=True)
+        tot = None
+        for c2w in cams:
+            sc.update_camera(c2w, 19.5)
+            st = sc.render(1)
+            if tot is None:
+                tot = st
+            else:
+                for k, v in st.items():
+                    tot[k] = [a + b for a, b in zip(tot[k], v)] if isinstance(v, list) else tot[k] + v
+        return tot
 
     for _ in range(args.warmup):
         step()
-    kernel_ms = []
+    stats = []
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        st = step()
-        kernel_ms.append(st["kernel_ms"])
+        stats.append(step())
     wall = time.perf_counter() - t0
     ms_per_step = wall / args.steps * 1e3
     samples = W * H * spp
@@ -125,13 +183,26 @@ def main():
 
     # counting run of the same job: rays, node and triangle fetches
     sc.set_counting(True)
-    sc.reset_film()
-    cst = sc.render(spp, fused=True)
+    cst = step()
     sc.set_counting(False)
-    alg_bytes = (NODE_BYTES * cst["node_fetches"] + TRI_BYTES * cst["tri_fetches"] +
-                 RAY_BYTES * cst["rays"] + FILM_BYTES * W * H)
-    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    achieved = alg_bytes / avg_kernel_s / 1e9
+    trav = lambda rays, nodes, tris: NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays
+    avg_kernel_s = sum(s["kernel_ms"] for s in stats) / len(stats) / 1e3
+    staged = restir or args.kernel == "wavefront"
+    if staged:
+        stage_s = sum(s["stage_ms"][0] for s in stats) / len(stats) / 1e3
+        launches = cst["stage_launches"][0]
+        alg_bytes = trav(cst["stage_rays"][0], cst["stage_node_fetches"][0], cst["stage_tri_fetches"][0])
+        if restir:
+            alg_bytes += 4 * 48 * W * H * spp
+            kernel = "initial_kernel (shader0: candidates, visibility, temporal reuse)"
+        else:
+            kernel = "trace_queue_kernel (wavefront closest-hit traversal)"
+    else:
+        stage_s = avg_kernel_s
+        launches = cst["launches"]
+        alg_bytes = trav(cst["rays"], cst["node_fetches"], cst["tri_fetches"]) + FILM_BYTES * W * H
+        kernel = "pt_kernel (megakernel)"
+    achieved = alg_bytes / stage_s / 1e9
     mrays = cst["rays"] / avg_kernel_s / 1e6
 
     line = {
@@ -153,13 +224,16 @@ def main():
         "kernel_ms_per_step": round(avg_kernel_s * 1e3, 3),
         "bvh_build_s": round(build_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": int(alg_bytes),
-                     "node_fetches": int(cst["node_fetches"]), "tri_fetches": int(cst["tri_fetches"]),
-                     "rays": int(cst["rays"])},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+                     "launches_per_step": int(launches),
+                     "avg_launch_ms": round(stage_s / max(launches, 1) * 1e3, 4),
+                     "algorithmic_bytes_per_launch": int(alg_bytes / max(launches, 1)),
+                     "kernel_ms_per_step": round(stage_s * 1e3, 3)},
     }
+    if staged:
+        line["stages_ms_per_step"] = [round(sum(s["stage_ms"][k] for s in stats) / len(stats), 3) for k in range(2)]
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(sc, spp)
+        line["cpu_baseline"] = restir_cpu_baseline(sc) if restir else cpu_baseline(sc, spp)
     sc.close()
     ctx.close()
     print(json.dumps(line))

@@ -77,6 +77,14 @@ typedef struct vs_render_stats {
     uint64_t tri_fetches;  /* leaf triangle fetches (counting runs) */
     uint32_t launches;     /* kernel launches of the last call */
     uint32_t samples_per_pixel;
+    /* two kernels timed on their own (HIP events), with their launch counts
+     * and (counting runs) their rays / node / triangle fetches: in the
+     * wavefront form [0] closest-hit traversal, [1] any-hit traversal; with
+     * the "rt" integrator [0] shader0 (candidates, visibility, temporal
+     * reuse), [1] shader1 (spatial reuse, shading); zeros otherwise */
+    double stage_ms[2];
+    uint32_t stage_launches[2];
+    uint64_t stage_rays[2], stage_node_fetches[2], stage_tri_fetches[2];
 } vs_render_stats;
 
 /* Flattened world-space scene handed to tests and the CPU oracle.
@@ -111,6 +119,14 @@ typedef struct vs_scene_arrays {
     float env_w2o[9], env_o2w[9];/* row-major 3x3 */
     float env_scale;
     float world_radius;
+    /* ReSTIR DI settings of an "rt" integrator ([0] = 1) or zeros: M_light,
+     * M_bsdf, max_age, max_recursion, diffuse_factor, pairwise, spatial open,
+     * sample_num, cos(spatial theta), spatial depth, spatial radius, temporal
+     * open, history_limit, N, temporal radius, cos(temporal theta), temporal
+     * depth, temporal mis, extra spatial passes ([1..19]); world-to-camera
+     * 3x4 rows of the current camera ([20..31]) */
+    const float *restir;
+    uint32_t num_restir;
 } vs_scene_arrays;
 
 /* library version (major*10000 + minor*100 + patch) */
@@ -153,6 +169,12 @@ vs_status vs_scene_bvh_export(const vs_scene *scene, const uint32_t **nodes, uin
 
 /* Override film resolution (SDK update_resolution, src/sdk/vision.h:78). */
 vs_status vs_scene_set_resolution(vs_scene *scene, uint32_t width, uint32_t height);
+/* Move the camera: camera-to-world as 16 floats, four columns of four, and
+ * the vertical field of view in degrees.  Replaces the SDK's update_camera
+ * (src/sdk/vision.h:77, vision.cpp:204-210, which ignores fov_y and sets 45);
+ * the camera of the previous rendered frame stays the reference for ReSTIR's
+ * reprojection (Sensor::store_prev_data, src/base/sensor/sensor.cpp:95-103). */
+vs_status vs_scene_update_camera(vs_scene *scene, const float *c2w, float fov_y);
 /* Override max_depth / min_depth / rr_threshold of the integrator. */
 vs_status vs_scene_set_integrator(vs_scene *scene, uint32_t max_depth, uint32_t min_depth, float rr_threshold);
 
@@ -190,8 +212,20 @@ vs_status vs_scene_set_kernel(vs_scene *scene, int32_t mode, uint32_t chunk_fram
  * current one).  fused = 0: one launch per frame, as
  * PathTracingIntegrator::render does (src/render_core/integrators/pt.cpp:90-99);
  * fused = 1: one launch adding all frames with the running mean in registers.
- * Both produce the same film.  Synchronizes before returning. */
+ * Both produce the same film.  With the "rt" integrator each frame runs ReSTIR
+ * DI (RealTimeIntegrator::render, src/render_core/integrators/rt.cpp:144-160,
+ * direct half): initial candidates + temporal reuse, any extra spatial
+ * passes, spatial reuse + shading into the film.  Synchronizes before
+ * returning. */
 vs_status vs_render(vs_scene *scene, uint32_t spp, int32_t fused);
+/* ReSTIR DI state after the last rendered frame (the "rt" integrator): the
+ * reservoir buffers [4][3][pixels][4] (previous / current by frame parity,
+ * pass-through, second pass-through; fields weight_sum C W p_hat | light
+ * prim bary.xy | pos.xyz age) and the surface buffers [2][3][pixels][4]
+ * (triangle b1 b2 flag|replaced<<16 | normal depth | position
+ * diffuse_factor).  Integer fields are stored as their bit patterns.
+ * Test and inspection hook; no reference counterpart. */
+vs_status vs_scene_restir_state(vs_scene *scene, float *reservoirs, float *surfaces);
 /* Enable per-ray counters (rays, node and triangle fetches) for later renders. */
 vs_status vs_set_counting(vs_scene *scene, int32_t enable);
 vs_status vs_last_render_stats(const vs_scene *scene, vs_render_stats *out);

@@ -1404,7 +1404,7 @@ void fis_sample(const FisTable &t, float radius, float ux, float uy, float &fx, 
 // (src/render_core/sensors/thin_lens.cpp:34-42), Sampler::sensor_sample
 // (src/base/sampler.h:65-73), box / tent filters (box.cpp:16-20, triangle.cpp:16-18)
 void camera_ray(const Scene &s, const FisTable &fis, uint32_t px, uint32_t py, Sampler &sp, V3 &o, V3 &d,
-                float &weight) {
+                float &weight, float *p_film = nullptr) {
     const float *c = s.cam;
     float ux = sp.next1d(), uy = sp.next1d();
     float r = c[22];
@@ -1422,6 +1422,7 @@ void camera_ray(const Scene &s, const FisTable &fis, uint32_t px, uint32_t py, S
     float lx = sp.next1d(), ly = sp.next1d();
     (void)sp.next1d();
     float ffx = float(px) + 0.5f + fx, ffy = float(py) + 0.5f + fy;
+    if (p_film) p_film[0] = ffx, p_film[1] = ffy;
     float sx = c[12] + ffx * c[14], sy = c[13] - ffy * c[15];
     V3 dc = unit(v3(sx * c[16], sy * c[16], 1.f));
     V3 oc = v3(0, 0, 0);
@@ -1442,6 +1443,750 @@ void camera_ray(const Scene &s, const FisTable &fis, uint32_t px, uint32_t py, S
 }
 
 float zero_nan_inf(float v) { return (std::isnan(v) || std::isinf(v)) ? 0.f : v; }
+
+// ---------------------------------------------------------------- ReSTIR DI
+// The direct half of the real-time integrator
+// (src/render_core/integrators/ReSTIR/direct.cpp): shader0 (primary surface,
+// RIS over light and BSDF candidates, visibility, temporal reuse) and shader1
+// (spatial reuse, shading), DIReservoir (ReSTIR/reservoir.h:15-115) and the
+// point-measure light functions of LightSampler (lightsampler.cpp:218-313).
+// The state lives in float4 records laid out as the device buffers:
+// reservoirs [4][3][P] (weight_sum C W p_hat | light prim bary | pos age),
+// surfaces [2][3][P] (tri b1 b2 flag|replaced<<16 | normal depth | pos diff),
+// extensions [2][2][P] (throughput t_max | view position).
+struct RParams {
+    uint32_t W, H, P, frame;
+    uint32_t M_light, M_bsdf, max_age, max_recursion;
+    float diff;
+    bool pairwise;
+    bool spatial;
+    uint32_t s_num;
+    float s_dot, s_depth, s_radius;
+    bool temporal;
+    uint32_t t_limit, t_N;
+    float t_radius, t_dot, t_depth;
+    bool t_mis;
+    uint32_t extra;
+    float w2c[12], prev_w2c[12];
+    V3 cam_pos, prev_cam_pos;
+};
+
+struct RState {
+    float *rsv, *surf, *ext;
+    uint32_t P;
+    float *at(float *base, uint32_t buf, uint32_t nfield, uint32_t field, uint32_t i) const {
+        return base + 4 * ((size_t(buf) * nfield + field) * P + i);
+    }
+};
+
+constexpr uint32_t kFlagUnset = 1, kFlagRefl = 2, kFlagDiffuse = 8, kFlagGlossy = 16, kFlagNearSpec = 64;
+
+struct DSample {
+    uint32_t light{kInv}, prim{0};
+    float bx{0.f}, by{0.f};
+    uint32_t age{0};
+    float p_hat{0.f};
+    V3 pos{0.f, 0.f, 0.f};
+};
+struct Rsv {
+    float ws{0.f}, C{0.f}, W{0.f};
+    DSample s;
+    // DIReservoir::update / truncation / process_occluded / update_W (reservoir.h:80-103)
+    void update(float u, const DSample &v, float w, float c) {
+        ws += w;
+        C += c;
+        if (u * ws < w) s = v;
+    }
+    void truncation(float limit) { C = std::min(limit, C); }
+    void occlude(bool o) {
+        W = o ? 0.f : W;
+        ws = o ? 0.f : ws;
+    }
+    void update_W(float p) { W = p == 0.f ? 0.f : ws / p; }
+};
+float weight_of(float mis, float p_hat, float W) { return mis * p_hat * W; }
+float safe_weight_of(float mis, float p_hat, float W) {
+    float r = weight_of(mis, p_hat, W);
+    return std::isnan(r) ? 0.f : r;
+}
+
+struct SRec {
+    uint32_t tri{kInv};
+    float b1{0.f}, b2{0.f};
+    uint32_t flag{kFlagUnset}, replaced{0};
+    V3 n{0.f, 0.f, 0.f};
+    float depth{0.f};
+    V3 pos{0.f, 0.f, 0.f};
+    float diff{0.f};
+    bool hit() const { return tri != kInv; }
+};
+struct SExt {
+    V3 thr{1.f, 1.f, 1.f};
+    V3 view{0.f, 0.f, 0.f};
+    float t_max{0.f};
+};
+
+uint32_t fbits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+float bitsf(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+Rsv rsv_load(const RState &st, uint32_t buf, uint32_t i) {
+    const float *a = st.at(st.rsv, buf, 3, 0, i), *b = st.at(st.rsv, buf, 3, 1, i), *c = st.at(st.rsv, buf, 3, 2, i);
+    Rsv r;
+    r.ws = a[0], r.C = a[1], r.W = a[2], r.s.p_hat = a[3];
+    r.s.light = fbits(b[0]), r.s.prim = fbits(b[1]), r.s.bx = b[2], r.s.by = b[3];
+    r.s.pos = v3(c[0], c[1], c[2]);
+    r.s.age = fbits(c[3]);
+    return r;
+}
+void rsv_store(const RState &st, uint32_t buf, uint32_t i, const Rsv &r) {
+    float *a = st.at(st.rsv, buf, 3, 0, i), *b = st.at(st.rsv, buf, 3, 1, i), *c = st.at(st.rsv, buf, 3, 2, i);
+    a[0] = r.ws, a[1] = r.C, a[2] = r.W, a[3] = r.s.p_hat;
+    b[0] = bitsf(r.s.light), b[1] = bitsf(r.s.prim), b[2] = r.s.bx, b[3] = r.s.by;
+    c[0] = r.s.pos.x, c[1] = r.s.pos.y, c[2] = r.s.pos.z, c[3] = bitsf(r.s.age);
+}
+SRec srec_load(const RState &st, uint32_t buf, uint32_t i) {
+    const float *a = st.at(st.surf, buf, 3, 0, i), *b = st.at(st.surf, buf, 3, 1, i), *c = st.at(st.surf, buf, 3, 2, i);
+    SRec r;
+    r.tri = fbits(a[0]), r.b1 = a[1], r.b2 = a[2];
+    uint32_t fr = fbits(a[3]);
+    r.flag = fr & 0xFFFFu, r.replaced = fr >> 16;
+    r.n = v3(b[0], b[1], b[2]), r.depth = b[3];
+    r.pos = v3(c[0], c[1], c[2]), r.diff = c[3];
+    return r;
+}
+void srec_store(const RState &st, uint32_t buf, uint32_t i, const SRec &r) {
+    float *a = st.at(st.surf, buf, 3, 0, i), *b = st.at(st.surf, buf, 3, 1, i), *c = st.at(st.surf, buf, 3, 2, i);
+    a[0] = bitsf(r.tri), a[1] = r.b1, a[2] = r.b2, a[3] = bitsf(r.flag | (r.replaced << 16));
+    b[0] = r.n.x, b[1] = r.n.y, b[2] = r.n.z, b[3] = r.depth;
+    c[0] = r.pos.x, c[1] = r.pos.y, c[2] = r.pos.z, c[3] = r.diff;
+}
+SExt sext_load(const RState &st, uint32_t buf, uint32_t i) {
+    const float *a = st.at(st.ext, buf, 2, 0, i), *b = st.at(st.ext, buf, 2, 1, i);
+    SExt e;
+    e.thr = v3(a[0], a[1], a[2]), e.t_max = a[3];
+    e.view = v3(b[0], b[1], b[2]);
+    return e;
+}
+void sext_store(const RState &st, uint32_t buf, uint32_t i, const SExt &e) {
+    float *a = st.at(st.ext, buf, 2, 0, i), *b = st.at(st.ext, buf, 2, 1, i);
+    a[0] = e.thr.x, a[1] = e.thr.y, a[2] = e.thr.z, a[3] = e.t_max;
+    b[0] = e.view.x, b[1] = e.view.y, b[2] = e.view.z, b[3] = 0.f;
+}
+
+// the DSL's float -> integer casts, saturated
+uint32_t cast_u32(float v) { return v > 0.f ? (v < 4.0e9f ? uint32_t(v) : 0xFFFFFFFFu) : 0u; }
+int cast_i32(float v) { return v > -2.0e9f ? (v < 2.0e9f ? int(v) : 2000000000) : -2000000000; }
+V3 affine(const float *m, V3 p) {
+    return v3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+              m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+
+// luminance of f.vec3(): the first three channels
+float lum3(const Spec &f, int n) { return luminance(v3(f.v[0], n > 1 ? f.v[1] : 0.f, n > 2 ? f.v[2] : 0.f)); }
+
+// material flag and diffuse factor (MaterialEvaluator::flag / diffuse_factor,
+// material.cpp:110-130; near-specular below alpha 0.022, metal.cpp:46,147-148,
+// glass.cpp:178,251-252; MicrofacetLobe sqrt(alpha_average), lobe.cpp:212-214;
+// DielectricLobe sqrt(alpha_x), lobe.cpp:427-429); the principled lobe set
+// holds a diffuse lobe, never near-specular
+uint32_t mat_flag(const Scene &s, uint32_t m) {
+    if (m == kInv) return kFlagDiffuse | kFlagRefl;
+    uint32_t t = s.mat_type(m);
+    if (t == M_METAL) return std::min(s.matp(m, 6), s.matp(m, 7)) < 0.022f ? kFlagNearSpec : kFlagGlossy;
+    if (t == M_GLASS) return std::min(s.matp(m, 5), s.matp(m, 6)) < 0.022f ? kFlagNearSpec : kFlagGlossy;
+    if (t == M_PRINCIPLED) return kFlagDiffuse | kFlagRefl | kFlagGlossy;
+    return kFlagDiffuse | kFlagRefl;
+}
+float mat_diffuse_factor(const Scene &s, uint32_t m) {
+    if (m == kInv) return 1.f;
+    uint32_t t = s.mat_type(m);
+    if (t == M_METAL) return std::sqrt(std::sqrt(s.matp(m, 6) * s.matp(m, 7)));
+    if (t == M_GLASS) return std::sqrt(s.matp(m, 5));
+    return 1.f;
+}
+
+// create_evaluator at a surface, with check_dispersive (spectrum.cpp:21-39)
+Bsdf bsdf_at(const Spectral &sd, const Scene &s, const Surf &it, Swl &w, bool check) {
+    uint32_t m = s.inst[4 * it.inst];
+    if (m == kInv) {
+        Bsdf b;
+        b.black = true;
+        b.fr = it.fr;
+        b.ng = it.ng;
+        b.wo_w = it.wo;
+        return b;
+    }
+    Bsdf b = make_bsdf(sd, s, m, it, w);
+    if (check && !sd.rgb && b.dispersive) {
+        bool secondary_valid = w.n > 1;
+        for (int i = 1; i < w.n; ++i) secondary_valid = secondary_valid && w.pdf[i] != 0.f;
+        if (secondary_valid)
+            for (int i = 1; i < w.n; ++i) w.pdf[i] = 0.f;
+    }
+    return b;
+}
+
+// env map coordinate -> direction (SphericalMap::evaluate, spherical.cpp:104-124)
+void env_dir(const EnvMap &e, float u, float v, V3 &local, V3 &world, float &sin_t) {
+    float theta = v * kPi, phi = u * 2.f * kPi;
+    sin_t = std::sin(theta);
+    float cos_t = std::cos(theta);
+    local = v3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+    world = unit(m3(e.o2w, local));
+}
+// 2D alias sample of the env map (alias2d.cpp:106-129)
+void env_uv_sample(const EnvMap &e, float ux, float uy, float &u, float &v, float &pdf_map) {
+    float uvr;
+    uint32_t iv = alias_sample(e.marg, uy, &uvr);
+    v = (float(iv) + uvr) / float(e.h);
+    float pdf_v = e.marg.integral > 0.f ? e.marg.func[iv] / e.marg.integral : 0.f;
+    const Alias &row = e.rows[iv];
+    float uur;
+    uint32_t iu = alias_sample(row, ux, &uur);
+    u = (float(iu) + uur) / float(e.w);
+    float integral_u = e.marg.func[iv];
+    float pdf_u = integral_u > 0.f ? row.func[iu] / integral_u : 0.f;
+    pdf_map = pdf_u * pdf_v;
+}
+V3 robust_pos(V3 pos, V3 ng, V3 dir, float factor) {
+    float f = dotp(ng, dir) > 0.f ? 1.f : -1.f;
+    return offset_origin(pos, mul(mul(ng, f), factor));
+}
+
+// LightSampler::sample_only (lightsampler.cpp:218-228): select 1, surface 2
+DSample light_sample_only(const Scene &s, const LightTables &lt, Sampler &sp) {
+    DSample d;
+    float ul = sp.next1d();
+    uint32_t l = !lt.power ? uint32_t(std::min(ul * float(s.nl), float(s.nl) - 1.f)) : alias_sample(lt.select, ul, nullptr);
+    float ux = sp.next1d(), uy = sp.next1d();
+    uint32_t type = s.lightu(l, 0);
+    if (type == 0) {// AreaLight::sample_only (area.cpp:139-145)
+        float urem;
+        d.prim = alias_sample(lt.tri_alias[l], ux, &urem);
+        float su = std::sqrt(urem);
+        d.bx = 1.f - su, d.by = uy * su;
+    } else if (type == 3) {// SphericalMap::sample_only (spherical.cpp:126-131)
+        float pdf_map;
+        env_uv_sample(lt.env, ux, uy, d.bx, d.by, pdf_map);
+    } else {// Light::sample_only (light.h:177-182)
+        d.bx = ux, d.by = uy;
+    }
+    d.light = l;
+    return d;
+}
+
+struct PEval {
+    Spec L;
+    float pdf{0.f};
+    V3 p{0.f, 0.f, 0.f};
+};
+// LightSampler::evaluate_point (lightsampler.cpp:230-240): AreaLight Le * G
+// and area-measure pdf (area.cpp:96-118,157-168), point / spot Li and -1
+// (light.h:237-243), environment in solid angle (spherical.cpp:104-141)
+PEval light_eval_point(const Spectral &sd, const Scene &s, const LightTables &lt, V3 ref, const DSample &d,
+                       const Swl &w) {
+    PEval r;
+    uint32_t l = d.light, type = s.lightu(l, 0);
+    if (type == 0) {
+        Surf p = surface(s, s.lightu(l, 3) + d.prim, d.bx, d.by, false);
+        V3 wv = sub(ref, p.pos);
+        Spec Le = light_color(sd, s, l, w) * ((dotp(wv, p.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+        float G = dotp(unit(wv), p.ng) / dotp(wv, wv);
+        r.L = Le * G;
+        r.pdf = (1.f / p.area) * alias_pmf(lt.tri_alias[l], d.prim);
+        r.p = robust_pos(p.pos, p.ng, wv, s.render[5]);
+    } else if (type == 3) {
+        const EnvMap &e = lt.env;
+        float pdf_map = env_map_pdf(e, d.bx, d.by);
+        V3 local, world;
+        float sin_t;
+        env_dir(e, d.bx, d.by, local, world, sin_t);
+        float pdf_dir = pdf_map / (2.f * kPi * kPi * sin_t);
+        r.pdf = std::isinf(pdf_dir) ? 0.f : pdf_dir;
+        r.L = env_L(sd, e, local, w);
+        r.p = add(ref, mul(world, e.diameter));
+    } else {
+        V3 pos = v3(s.lightf(l, 12), s.lightf(l, 13), s.lightf(l, 14));
+        V3 wu = sub(ref, pos);
+        Spec val = light_color(sd, s, l, w) / dotp(wu, wu);
+        if (type == 2) {
+            V3 dir = v3(s.lightf(l, 16), s.lightf(l, 17), s.lightf(l, 18));
+            float angle = s.lightf(l, 15), fall = s.lightf(l, 19);
+            float start = std::max(0.f, angle - fall);
+            float ca = std::cos(angle), cs = std::cos(start);
+            float ct = std::min(std::max(dotp(dir, unit(wu)), ca), cs);
+            float f = (ct - ca) / (cs - ca);
+            float f2 = f * f;
+            val = val * (f2 * f2);
+        }
+        r.L = val;
+        r.pdf = -1.f;
+        r.p = pos;
+    }
+    r.pdf *= select_pmf(s, lt, l);
+    return r;
+}
+// LightSampler::PDF_point (lightsampler.cpp:242-250; Light::PDF_point,
+// light.h:141-151; warp.h:101-105; spherical.cpp:143-146)
+float light_pdf_point(const Scene &s, V3 ref, const DSample &d, float pdf_wi) {
+    uint32_t l = d.light, type = s.lightu(l, 0);
+    if (type == 3) return pdf_wi;
+    V3 lp, ln;
+    if (type == 0) {
+        Surf p = surface(s, s.lightu(l, 3) + d.prim, d.bx, d.by, false);
+        lp = p.pos, ln = p.ng;
+    } else {
+        lp = v3(s.lightf(l, 12), s.lightf(l, 13), s.lightf(l, 14));
+        ln = unit(sub(ref, lp));
+    }
+    V3 wv = sub(ref, lp);
+    float r = pdf_wi * std::fabs(dotp(ln, unit(wv))) / dotp(wv, wv);
+    return std::isinf(r) ? 0.f : r;
+}
+
+// ReSTIRDI::Li from the light (direct.cpp:123-159): f * L for a sample
+Spec light_contrib(const Spectral &sd, const Scene &s, const LightTables &lt, const Surf &it, const Bsdf &b,
+                   const DSample &d, const Swl &w) {
+    PEval le;
+    if (d.light != kInv) le = light_eval_point(sd, s, lt, it.pos, d, w);
+    else le.L = sfill(0.f);
+    V3 wi = unit(sub(le.p, it.pos));
+    BEval e = bsdf_eval(sd, b, wi, w.n, nullptr);
+    return e.f * le.L;
+}
+float p_hat_of(const Spectral &sd, const Scene &s, const LightTables &lt, const Surf &it, const Bsdf &b,
+               const DSample &d, const Swl &w) {
+    return lum3(light_contrib(sd, s, lt, it, b, d, w), w.n);
+}
+
+bool valid_neighbor(const SRec &c, const SRec &o, float dt, float depth_t, float diff_t) {
+    bool c0 = std::fabs(dotp(c.n, o.n)) > dt;
+    bool c1 = (std::fabs(c.depth - o.depth) / c.depth) < depth_t;
+    bool c2 = std::fabs(c.diff - o.diff) / c.diff < diff_t;
+    return c0 && c1 && c2 && c.hit() && o.hit();
+}
+
+bool occluded_to(const Ctx &c, const Surf &it, V3 p) {
+    V3 dir = sub(p, it.pos);
+    V3 nn = dotp(it.ng, dir) > 0.f ? it.ng : neg(it.ng);
+    return c.trace({offset_origin(it.pos, nn), dir, 0.f, 1.f - kShadowEps}, true).tri != kInv;
+}
+
+// delta sampling of a near-specular metal or glass (MaterialEvaluator::
+// sample_delta, material.cpp:154-160; lobe.cpp:223-230, 404-425): the mirror
+// or refracted direction and f |cos| / pdf there (1 draw for glass)
+// (the lobes evaluated without energy compensation, as BxDF::evaluate is)
+bool delta_sample(const Spectral &sd, const Bsdf &b, Sampler &sp, int n, V3 &wi_w, V3 &thr) {
+    if (b.black || b.set || (b.lobes[0].kind != LB_REFLECTION && b.lobes[0].kind != LB_DIELECTRIC)) {
+        wi_w = thr = v3(0.f, 0.f, 0.f);
+        return false;
+    }
+    Lobe L = b.lobes[0];
+    V3 wo = b.fr.local(b.wo_w);
+    V3 wi = v3(-wo.x, -wo.y, wo.z);
+    LEval e;
+    if (L.kind == LB_DIELECTRIC) {
+        Spec F = fresnel_eval(L.fr, std::fabs(wo.z), n);
+        float rp = refl_prob(F, L.kr, n);
+        float uc = sp.next1d();
+        float eta0 = L.fr.eta.v[0];
+        if (!(uc < rp)) {
+            V3 nz = v3(0.f, 0.f, 1.f);
+            refract_dir(wo, dotp(nz, wo) < 0.f ? neg(nz) : nz, eta0, wi);
+        }
+        bool same = wo.z * wi.z > 0.f;
+        float eta_p = same ? 1.f : eta0;
+        V3 wh = face_forward(unit(add(wo, mul(wi, eta_p))), wo);
+        Spec F2 = fresnel_eval(L.fr, std::fabs(dotp(wh, wo)), n);
+        float rp2 = refl_prob(F2, L.kr, n);
+        if (same) {
+            e.f = L.mf.brdf(wo, wh, wi) * F2;
+            e.pdf = L.mf.pdf_reflection(wo, wh) * rp2;
+        } else {
+            V3 wh2 = unit(add(wo, mul(wi, eta0)));
+            e.f = (L.mf.btdf(wo, wh2, wi, eta0) * (1.f - F2)) * L.kr;
+            e.pdf = L.mf.pdf_transmission(wo, face_forward(wh, wo), wi, eta0) * (1.f - rp2);
+        }
+    } else {
+        L.compensate = false;
+        e = lobe_eval_local(sd, L, wo, wi, n, nullptr);
+    }
+    Spec t = (e.f * std::fabs(wi.z)) / e.pdf;
+    thr = v3(t.v[0], t.v[n > 1 ? 1 : 0], t.v[n > 2 ? 2 : 0]);
+    wi_w = b.fr.world(wi);
+    return true;
+}
+
+// ReSTIRDI::Li from the BSDF (direct.cpp:57-121)
+Spec bsdf_candidate(const Ctx &c, const Surf &it, const Bsdf &b, Sampler &sp, const Swl &w, DSample &d, float &pdf,
+                    float &light_pdf) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    BSample bs = bsdf_sample(sd, b, sp, w.n);
+    V3 nn = dotp(it.ng, bs.wi) > 0.f ? it.ng : neg(it.ng);
+    Hit h = c.trace({offset_origin(it.pos, nn), bs.wi, 0.f, FLT_MAX}, false);
+    pdf = bs.e.pdf;
+    light_pdf = 0.f;
+    Spec f = sfill(0.f);
+    if (h.tri != kInv) {
+        Surf nx = surface(s, h.tri, h.b1, h.b2, false);
+        uint32_t l = s.inst[4 * nx.inst + 1];
+        if (l != kInv) {// LightSampler::evaluate_hit_point (lightsampler.cpp:269-288)
+            V3 wv = sub(it.pos, nx.pos);
+            Spec Le = light_color(sd, s, l, w) * ((dotp(wv, nx.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+            float G = dotp(unit(wv), nx.ng) / dotp(wv, wv);
+            float pp = pdf * std::fabs(dotp(nx.ng, unit(wv))) / dotp(wv, wv);
+            uint32_t prim = h.tri - s.lightu(l, 3);
+            light_pdf = (1.f / nx.area) * alias_pmf(lt.tri_alias[l], prim) * select_pmf(s, lt, l);
+            d.light = l, d.prim = prim, d.bx = h.b1, d.by = h.b2;
+            d.pos = robust_pos(nx.pos, nx.ng, wv, s.render[5]);
+            pdf = std::isinf(pp) ? 0.f : pp;
+            f = bs.e.f * (Le * G);
+        }
+    } else if (lt.env.present) {// LightSampler::evaluate_miss_point (lightsampler.cpp:302-313)
+        V3 wi = mul(bs.wi, lt.env.diameter);
+        V3 pl = add(it.pos, wi);
+        Spec Lv;
+        float epdf;
+        env_eval(sd, lt.env, unit(sub(pl, it.pos)), w, Lv, epdf);
+        light_pdf = epdf * select_pmf(s, lt, lt.env.light);
+        d.light = lt.env.light;
+        env_uv(unit(m3(lt.env.w2o, bs.wi)), d.bx, d.by);
+        d.pos = pl;
+        f = bs.e.f * Lv;
+    }
+    return f;
+}
+
+// ReSTIRDI::RIS (direct.cpp:161-233)
+Rsv ris(const Ctx &c, const RParams &rp, const Surf &it, Sampler &sp, Swl &w, V3 thr) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    Rsv r;
+    if (rp.M_light + rp.M_bsdf == 0) return r;
+    Bsdf b = bsdf_at(sd, s, it, w, true);
+    const float lum = luminance(thr);
+    const float Ml = float(rp.M_light), Mb = float(rp.M_bsdf);
+    for (uint32_t i = 0; i < rp.M_light; ++i) {
+        DSample d = light_sample_only(s, lt, sp);
+        PEval le = light_eval_point(sd, s, lt, it.pos, d, w);
+        V3 wi = unit(sub(le.p, it.pos));
+        BEval e = bsdf_eval(sd, b, wi, w.n, nullptr);
+        Spec f = e.f * le.L;
+        float bp = light_pdf_point(s, it.pos, d, e.pdf);
+        d.p_hat = lum3(f, w.n);
+        d.pos = le.p;
+        bool delta = le.pdf < 0.f;
+        float lpdf = delta ? -le.pdf : le.pdf;
+        float mis = delta ? 1.f / Ml : lpdf / (Ml * lpdf + Mb * bp);
+        r.update(sp.next1d(), d, safe_weight_of(mis, d.p_hat, 1.f / lpdf) * lum, 1.f);
+    }
+    for (uint32_t i = 0; i < rp.M_bsdf; ++i) {
+        DSample d;
+        float pdf, lpdf;
+        Spec f = bsdf_candidate(c, it, b, sp, w, d, pdf, lpdf);
+        d.p_hat = lum3(f, w.n);
+        r.update(sp.next1d(), d, safe_weight_of(pdf / (Ml * lpdf + Mb * pdf), d.p_hat, 1.f / pdf) * lum, 1.f);
+    }
+    return r;
+}
+
+// ReSTIRDI::combine_temporal (direct.cpp:327-380)
+Rsv combine_temporal(const Ctx &c, const RParams &rp, const Rsv &cur, const SRec &cs, Rsv other, V3 view,
+                     V3 prev_view, Sampler &sp, Swl &w) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    other.s.age += 1;
+    Surf it = surface(s, cs.tri, cs.b1, cs.b2, true);
+    float mis_cur, mis_prev, p_nn;
+    if (rp.t_mis) {
+        it.wo = unit(sub(prev_view, it.pos));
+        Bsdf bp = bsdf_at(sd, s, it, w, true);
+        float p_cn = p_hat_of(sd, s, lt, it, bp, cur.s, w);
+        p_nn = p_hat_of(sd, s, lt, it, bp, other.s, w);
+        it.wo = unit(sub(view, it.pos));
+        Bsdf bc = bsdf_at(sd, s, it, w, true);
+        float p_cc = p_hat_of(sd, s, lt, it, bc, cur.s, w);
+        float p_nc = p_hat_of(sd, s, lt, it, bc, other.s, w);
+        float nc = float(int(cur.C)), no = float(int(other.C));
+        mis_cur = (nc * p_cc) / (nc * p_cc + no * p_cn);
+        mis_prev = (no * p_nn) / (no * p_nn + nc * p_nc);
+    } else {
+        it.wo = unit(sub(prev_view, it.pos));
+        Bsdf bp = bsdf_at(sd, s, it, w, true);
+        p_nn = p_hat_of(sd, s, lt, it, bp, other.s, w);
+        mis_cur = cur.C / (cur.C + other.C);
+        mis_prev = other.C / (other.C + cur.C);
+    }
+    Rsv ret;
+    ret.update(0.5f, cur.s, safe_weight_of(mis_cur, cur.s.p_hat, cur.W), cur.C);
+    DSample os = other.s;
+    os.p_hat = p_nn;
+    ret.update(sp.next1d(), os, safe_weight_of(mis_prev, os.p_hat, other.W), other.C);
+    ret.update_W(ret.s.p_hat);
+    return ret;
+}
+
+// ReSTIRDI::temporal_reuse (direct.cpp:382-432)
+Rsv temporal_reuse(const Ctx &c, const RParams &rp, const RState &st, Rsv rsv, const SRec &cs, uint32_t pix,
+                   float mx, float my, const float *p_film, Sampler &sp, Swl &w) {
+    float prev_x = p_film[0] - mx, prev_y = p_film[1] - my;
+    uint32_t pslot = rp.frame & 1u, cslot = (rp.frame + 1u) & 1u;
+    float limit = rsv.C * float(rp.t_limit);
+    V3 view = cs.replaced ? sext_load(st, cslot, pix).view : rp.cam_pos;
+    V3 prev_view = rp.prev_cam_pos;
+    int ix = cast_i32(prev_x), iy = cast_i32(prev_y);
+    if (!(ix >= 0 && iy >= 0 && ix < int(rp.W) && iy < int(rp.H) && rp.temporal)) return rsv;
+    if (!cs.hit()) return rsv;
+    auto visit = [&](uint32_t j, Rsv &pr, SRec &ps) {
+        pr = rsv_load(st, pslot, j);
+        pr.truncation(limit);
+        ps = srec_load(st, pslot, j);
+        if (ps.replaced) prev_view = sext_load(st, pslot, j).view;
+    };
+    Rsv pr;
+    SRec ps;
+    visit(std::min(cast_u32(prev_y), rp.H - 1) * rp.W + std::min(cast_u32(prev_x), rp.W - 1), pr, ps);
+    if (valid_neighbor(cs, ps, rp.t_dot, rp.t_depth, rp.diff) && pr.s.age < rp.max_age)
+        return combine_temporal(c, rp, rsv, cs, pr, view, prev_view, sp, w);
+    for (uint32_t i = 0; i < rp.t_N; ++i) {
+        float ux = sp.next1d(), uy = sp.next1d();
+        float dx, dy;
+        square_to_disk(ux, uy, dx, dy);
+        float qx = dx * rp.t_radius + prev_x, qy = dy * rp.t_radius + prev_y;
+        visit(std::min(cast_u32(qy), rp.H - 1) * rp.W + std::min(cast_u32(qx), rp.W - 1), pr, ps);
+        if (valid_neighbor(cs, ps, rp.t_dot, rp.t_depth, rp.diff) && pr.s.age < rp.max_age)
+            return combine_temporal(c, rp, rsv, cs, pr, view, prev_view, sp, w);
+    }
+    return rsv;
+}
+
+// Sensor::prev_raster_coord (sensor.cpp:105-110): the point in the previous
+// camera's space divided by its depth, mapped to raster coordinates
+void prev_raster(const float *cam, const float *prev_w2c, V3 p, float &rx, float &ry) {
+    V3 q = affine(prev_w2c, p);
+    float qx = q.x / q.z, qy = q.y / q.z;
+    float sx = qx / cam[16], sy = qy / cam[16];
+    rx = (sx - cam[12]) / cam[14];
+    ry = (cam[13] - sy) / cam[15];
+}
+
+// shader0 for one pixel (direct.cpp:519-555; compute_hit :434-517)
+void restir_initial_px(const Ctx &c, const FisTable &fis, const RParams &rp, const RState &st, uint32_t x,
+                       uint32_t y) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u;
+    Sampler sp;
+    sp.start(x, y, rp.frame, 0u);
+    Swl w = sample_wavelengths(sd, x, y, rp.frame);
+    V3 o, d;
+    float fw, pf[2];
+    camera_ray(s, fis, x, y, sp, o, d, fw, pf);
+    SRec cs;
+    SExt ext;
+    ext.view = o;
+    V3 ro = o, rd = d;
+    Hit h = c.trace({ro, rd, 0.f, FLT_MAX}, false);
+    Surf it{};
+    uint32_t counter = 0;
+    while (true) {
+        cs.tri = h.tri, cs.b1 = h.b1, cs.b2 = h.b2;
+        if (h.tri == kInv) break;
+        it = surface(s, h.tri, h.b1, h.b2, true);
+        it.wo = unit(neg(rd));
+        ext.t_max += len(sub(it.pos, ro)) / len(rd);
+        V3 vpos = add(o, mul(d, ext.t_max));
+        uint32_t m = s.inst[4 * it.inst];
+        cs.flag = mat_flag(s, m);
+        cs.diff = mat_diffuse_factor(s, m);
+        V3 wnext = v3(0.f, 0.f, 0.f);
+        if (cs.flag == kFlagNearSpec) {
+            Bsdf b = bsdf_at(sd, s, it, w, false);
+            V3 t;
+            delta_sample(sd, b, sp, w.n, wnext, t);
+            ext.thr = v3(ext.thr.x * t.x, ext.thr.y * t.y, ext.thr.z * t.z);
+        }
+        if (counter == 0) {
+            cs.depth = affine(rp.w2c, vpos).z;
+            cs.n = it.fr.z;
+            cs.pos = it.pos;
+        }
+        ++counter;
+        if (counter >= rp.max_recursion) break;
+        if (cs.flag != kFlagNearSpec) break;
+        ext.view = it.pos;
+        V3 nn = dotp(it.ng, wnext) > 0.f ? it.ng : neg(it.ng);
+        ro = offset_origin(it.pos, nn);
+        rd = wnext;
+        h = c.trace({ro, rd, 0.f, FLT_MAX}, false);
+        cs.replaced = 1;
+    }
+    bool hit = h.tri != kInv;
+    srec_store(st, cslot, pix, cs);
+    if (cs.replaced) sext_store(st, cslot, pix, ext);
+    Rsv r;
+    if (hit) r = ris(c, rp, it, sp, w, ext.thr);
+    r.update_W(r.s.p_hat);
+    r.truncation(1.f);
+    float mx = 0.f, my = 0.f;
+    if (hit) {// FrameBuffer::compute_motion_vec (frame_buffer.cpp:325-333)
+        float rx, ry;
+        prev_raster(s.cam, rp.prev_w2c, add(o, mul(d, ext.t_max)), rx, ry);
+        mx = pf[0] - rx, my = pf[1] - ry;
+    }
+    if (hit) r.occlude(occluded_to(c, it, r.s.pos));
+    r = temporal_reuse(c, rp, st, r, cs, pix, mx, my, pf, sp, w);
+    rsv_store(st, 2, pix, r);
+}
+
+uint32_t spatial_pick(const RParams &rp, uint32_t x, uint32_t y, Sampler &sp) {
+    float ux = sp.next1d(), uy = sp.next1d();
+    float dx, dy;
+    square_to_disk(ux, uy, dx, dy);
+    float ox = dx * rp.s_radius, oy = dy * rp.s_radius;
+    int ax = std::min(std::max(int(x) + int(std::round(ox)), 0), int(rp.W) - 1);
+    int ay = std::min(std::max(int(y) + int(std::round(oy)), 0), int(rp.H) - 1);
+    return uint32_t(ay) * rp.W + uint32_t(ax);
+}
+
+// ReSTIRDI::spatial_reuse with constant_combine / pairwise_combine
+// (direct.cpp:559-581, 287-309, 264-285, 235-262)
+Rsv spatial_reuse(const Ctx &c, const RParams &rp, const RState &st, Rsv r, const SRec &cs, uint32_t x, uint32_t y,
+                  uint32_t buf, Sampler &sp, Swl &w) {
+    if (!rp.spatial) return r;
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    const uint32_t cslot = (rp.frame + 1u) & 1u, pix = y * rp.W + x;
+    std::vector<uint32_t> idx;
+    for (uint32_t i = 0; i < rp.s_num; ++i) {
+        uint32_t j = spatial_pick(rp, x, y, sp);
+        if (valid_neighbor(cs, srec_load(st, cslot, j), rp.s_dot, rp.s_depth, rp.diff)) idx.push_back(j);
+    }
+    if (!cs.hit()) return r;
+    V3 view = cs.replaced ? sext_load(st, cslot, pix).view : rp.cam_pos;
+    Surf cit = surface(s, cs.tri, cs.b1, cs.b2, true);
+    cit.wo = unit(sub(view, cit.pos));
+    Bsdf cb;
+    if (!idx.empty()) cb = bsdf_at(sd, s, cit, w, true);
+    Rsv ret;
+    if (!rp.pairwise) {
+        float inv = 1.f / float(uint32_t(idx.size()) + 1u);
+        ret.update(0.5f, r.s, weight_of(inv, r.s.p_hat, r.W), r.C);
+        for (uint32_t j : idx) {
+            Rsv nr = rsv_load(st, buf, j);
+            nr.s.p_hat = p_hat_of(sd, s, lt, cit, cb, nr.s, w);
+            ret.update(sp.next1d(), nr.s, weight_of(inv, nr.s.p_hat, nr.W), nr.C);
+        }
+    } else {
+        float num = float(idx.size()), num_i = float(int(num));
+        float canonical = 0.f;
+        for (uint32_t j : idx) {
+            Rsv nr = rsv_load(st, buf, j);
+            SRec ns = srec_load(st, cslot, j);
+            Surf nit = surface(s, ns.tri, ns.b1, ns.b2, true);
+            nit.wo = unit(sub(view, nit.pos));
+            Bsdf nb = bsdf_at(sd, s, nit, w, true);
+            float p_cc = p_hat_of(sd, s, lt, cit, cb, r.s, w);
+            float p_cn = p_hat_of(sd, s, lt, nit, nb, r.s, w);
+            float p_nn = p_hat_of(sd, s, lt, nit, nb, nr.s, w);
+            float p_nc = p_hat_of(sd, s, lt, cit, cb, nr.s, w);
+            float mi = (1.f * p_nn) / (1.f * p_nn + num_i * p_nc);
+            ret.update(sp.next1d(), nr.s, safe_weight_of(mi, nr.s.p_hat, nr.W), nr.C);
+            float cw = ((1.f * p_cc) / (1.f * p_cc + num_i * p_cn)) / num;
+            canonical += std::isnan(cw) ? 0.f : cw;
+        }
+        canonical = canonical == 0.f ? 1.f : canonical;
+        ret.update(sp.next1d(), r.s, safe_weight_of(canonical, r.s.p_hat, r.W), r.C);
+    }
+    ret.update_W(ret.s.p_hat);
+    return ret;
+}
+
+// ReSTIRDI::shading (direct.cpp:583-636); the dead BSDF-sampled emitter term
+// draws after every other use of the sampler and is left out
+V3 restir_shading(const Ctx &c, const RParams &rp, const RState &st, Rsv r, const SRec &cs, uint32_t pix, Swl &w) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    V3 view = rp.cam_pos, thr = v3(1.f, 1.f, 1.f);
+    if (cs.replaced) {
+        SExt e = sext_load(st, (rp.frame + 1u) & 1u, pix);
+        thr = e.thr, view = e.view;
+    }
+    Surf it = surface(s, cs.tri, cs.b1, cs.b2, true);
+    it.wo = unit(sub(view, it.pos));
+    Spec value = sfill(0.f), Le = sfill(0.f);
+    uint32_t l = s.inst[4 * it.inst + 1];
+    if (l != kInv) {
+        if (s.lightu(l, 0) == 0) {
+            V3 wv = sub(rp.cam_pos, it.pos);
+            Le = light_color(sd, s, l, w) * ((dotp(wv, it.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+        }
+    } else {
+        Bsdf b = bsdf_at(sd, s, it, w, true);
+        Spec t;
+        for (int i = 0; i < w.n; ++i) t.v[i] = i == 0 ? thr.x : (i == 1 ? thr.y : thr.z);
+        value = (light_contrib(sd, s, lt, it, b, r.s, w) * t) / luminance(thr);
+        r.occlude(occluded_to(c, it, r.s.pos));
+        value = value * r.W;
+    }
+    return linear_srgb(sd, value + Le, w);
+}
+
+// shader1 for one pixel (direct.cpp:638-671); returns the direct radiance
+V3 restir_final_px(const Ctx &c, const FisTable &fis, const RParams &rp, const RState &st, uint32_t x, uint32_t y,
+                   uint32_t buf) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const LightTables &lt = *c.lt;
+    const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u;
+    Sampler sp;
+    sp.start(x, y, rp.frame, 0u);
+    Swl w = sample_wavelengths(sd, x, y, rp.frame);
+    V3 o, d;
+    float fw;
+    camera_ray(s, fis, x, y, sp, o, d, fw);
+    sp.start(x, y, rp.frame, 1u);
+    SRec cs = srec_load(st, cslot, pix);
+    Rsv r = rsv_load(st, buf, pix);
+    r = spatial_reuse(c, rp, st, r, cs, x, y, buf, sp, w);
+    V3 L = v3(0.f, 0.f, 0.f);
+    if (cs.hit()) {
+        L = restir_shading(c, rp, st, r, cs, pix, w);
+    } else if (lt.env.present) {
+        Spec Lv;
+        float pdf;
+        env_eval(sd, lt.env, unit(sub(add(o, d), o)), w, Lv, pdf);
+        L = linear_srgb(sd, Lv, w);
+    }
+    rsv_store(st, cslot, pix, r);
+    return L;
+}
+
+void restir_spatial_px(const Ctx &c, const RParams &rp, const RState &st, uint32_t x, uint32_t y, uint32_t in,
+                       uint32_t out, uint32_t dim) {
+    const uint32_t pix = y * rp.W + x;
+    Sampler sp;
+    sp.start(x, y, rp.frame, dim);
+    Swl w = sample_wavelengths(*c.sd, x, y, rp.frame);
+    SRec cs = srec_load(st, (rp.frame + 1u) & 1u, pix);
+    Rsv r = rsv_load(st, in, pix);
+    r = spatial_reuse(c, rp, st, r, cs, x, y, in, sp, w);
+    rsv_store(st, out, pix, r);
+}
 
 }// namespace
 
@@ -1568,6 +2313,28 @@ void or_filter_sample(const float *cam, const float *u, uint32_t n, float *p, fl
         }
         fis_sample(t, cam[22], u[2 * i], u[2 * i + 1], p[2 * i], p[2 * i + 1], weight[i]);
     }
+}
+
+// reservoir streaming (DIReservoir::update, reservoir.h:80-86): n weights
+// with their uniforms; returns the kept index (-1: none), weight_sum, C
+int32_t or_reservoir_stream(const float *w, const float *u, uint32_t n, float *weight_sum, float *C) {
+    Rsv r;
+    int32_t kept = -1;
+    for (uint32_t i = 0; i < n; ++i) {
+        DSample d;
+        d.light = i;
+        r.update(u[i], d, w[i], 1.f);
+    }
+    if (r.s.light != kInv) kept = int32_t(r.s.light);
+    *weight_sum = r.ws;
+    *C = r.C;
+    return kept;
+}
+
+// raster coordinates of world points in a camera given as the 32-float
+// camera record (screen window, tan(fov/2)) and a world-to-camera 3x4
+void or_prev_raster(const float *cam, const float *w2c, const float *p, uint32_t n, float *out) {
+    for (uint32_t i = 0; i < n; ++i) prev_raster(cam, w2c, v3(p[3 * i], p[3 * i + 1], p[3 * i + 2]), out[2 * i], out[2 * i + 1]);
 }
 
 void or_offset_ray_origin(const float *p, const float *n, uint32_t count, float *out) {
@@ -1751,6 +2518,91 @@ void or_trace(const float *P, const uint32_t *tri, uint32_t nv, uint32_t nt, con
 // NormalRadianceCollector::add_sample, normal.cpp:87-101).  mode 0 brute
 // force, 1 BVH2.  threads = 0 uses all hardware threads.  spec = null
 // renders sRGB mode without an environment light.
+// One ReSTIR DI frame: shader0 over all pixels, the extra spatial passes,
+// shader1 with the film's running mean.  restir: the exported settings
+// (32 floats: flag, M_light, ..., extra passes, world-to-camera 3x4);
+// prev_w2c / prev_pos: the camera of the previous frame; state buffers as
+// described above (rsv [4][3][P][4], surf [2][3][P][4], ext [2][2][P][4]);
+// rt [P][4] the film.
+uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const uint32_t *tri,
+                         const uint32_t *tri_inst, const uint32_t *inst, const uint32_t *mat, const uint32_t *light,
+                         const float *cam, const float *render, uint32_t nv, uint32_t nt, uint32_t ni, uint32_t nm,
+                         uint32_t nl, const or_spectral *spec, const float *restir, const float *prev_w2c,
+                         const float *prev_pos, uint32_t frame, uint32_t threads, float *rsv, float *surf, float *ext,
+                         float *rt) {
+    Scene s{P, N, UV, tri, tri_inst, inst, mat, light, cam, render, nv, nt, ni, nm, nl};
+    Spectral sd = to_spectral(spec);
+    Bvh2 b;
+    bvh2_build(s, b);
+    LightTables lt = spec ? build_lights(s, sd, spec->env_light, spec->env_w, spec->env_h, spec->env_rgb, spec->env_w2o,
+                                         spec->env_o2w, spec->env_scale, spec->world_radius)
+                          : build_lights(s, sd, kInv, 0, 0, nullptr, nullptr, nullptr, 1.f, 1.f);
+    Ctx c{&s, &b, &lt, &sd, false};
+    const FisTable fis = fis_build(cam);
+    RParams rp{};
+    rp.W = uint32_t(cam[19]);
+    rp.H = uint32_t(cam[20]);
+    rp.P = rp.W * rp.H;
+    rp.frame = frame;
+    rp.M_light = uint32_t(restir[1]);
+    rp.M_bsdf = uint32_t(restir[2]);
+    rp.max_age = uint32_t(restir[3]);
+    rp.max_recursion = uint32_t(restir[4]);
+    rp.diff = restir[5];
+    rp.pairwise = restir[6] != 0.f;
+    rp.spatial = restir[7] != 0.f;
+    rp.s_num = uint32_t(restir[8]);
+    rp.s_dot = restir[9];
+    rp.s_depth = restir[10];
+    rp.s_radius = restir[11];
+    rp.temporal = restir[12] != 0.f;
+    rp.t_limit = uint32_t(restir[13]);
+    rp.t_N = uint32_t(restir[14]);
+    rp.t_radius = restir[15];
+    rp.t_dot = restir[16];
+    rp.t_depth = restir[17];
+    rp.t_mis = restir[18] != 0.f;
+    rp.extra = uint32_t(restir[19]);
+    std::memcpy(rp.w2c, restir + 20, sizeof(rp.w2c));
+    std::memcpy(rp.prev_w2c, prev_w2c, sizeof(rp.prev_w2c));
+    rp.cam_pos = v3(cam[3], cam[7], cam[11]);
+    rp.prev_cam_pos = v3(prev_pos[0], prev_pos[1], prev_pos[2]);
+    RState st{rsv, surf, ext, rp.P};
+    const bool accumulate = render[6] != 0.f;
+    uint32_t nth = threads ? threads : std::max(1u, std::thread::hardware_concurrency());
+    auto rows = [&](auto &&per_pixel) {
+        std::atomic<uint32_t> next{0};
+        auto work = [&]() {
+            while (true) {
+                uint32_t y = next.fetch_add(1);
+                if (y >= rp.H) break;
+                for (uint32_t x = 0; x < rp.W; ++x) per_pixel(x, y);
+            }
+        };
+        std::vector<std::thread> th;
+        for (uint32_t t = 0; t < nth; ++t) th.emplace_back(work);
+        for (auto &t : th) t.join();
+    };
+    rows([&](uint32_t x, uint32_t y) { restir_initial_px(c, fis, rp, st, x, y); });
+    uint32_t in = 2;
+    for (uint32_t k = 0; k < rp.extra; ++k) {
+        rows([&](uint32_t x, uint32_t y) { restir_spatial_px(c, rp, st, x, y, in, in ^ 1u, 2u + k); });
+        in ^= 1u;
+    }
+    rows([&](uint32_t x, uint32_t y) {
+        V3 L = restir_final_px(c, fis, rp, st, x, y, in);
+        float v[4] = {zero_nan_inf(L.x), zero_nan_inf(L.y), zero_nan_inf(L.z), 1.f};
+        float *acc = rt + 4 * (size_t(y) * rp.W + x);
+        if (accumulate) {
+            float a = 1.f / float(frame + 1u);
+            for (int k = 0; k < 4; ++k) acc[k] = acc[k] + a * (v[k] - acc[k]);
+        } else {
+            for (int k = 0; k < 4; ++k) acc[k] = v[k];
+        }
+    });
+    return 0;
+}
+
 uint64_t or_render(const float *P, const float *N, const float *UV, const uint32_t *tri, const uint32_t *tri_inst,
                    const uint32_t *inst, const uint32_t *mat, const uint32_t *light, const float *cam,
                    const float *render, uint32_t nv, uint32_t nt, uint32_t ni, uint32_t nm, uint32_t nl,

@@ -56,6 +56,12 @@ def lib():
         h.or_render.restype = C.c_uint64
         h.or_render.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
                                 [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, C.c_void_p, _f32p])
+        h.or_restir_frame.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
+                                      [C.c_uint32] * 5 + [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_uint32,
+                                                          _f32p, _f32p, _f32p, _f32p])
+        h.or_reservoir_stream.restype = C.c_int32
+        h.or_reservoir_stream.argtypes = [_f32p, _f32p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        h.or_prev_raster.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p]
         h.or_cie_tables.argtypes = [_f32p, _f32p, C.POINTER(C.c_float)]
         h.or_spd_eval.argtypes = [_f32p, C.c_float, _f32p, C.c_uint32, _f32p]
         h.or_wavelengths.argtypes = [_f32p, C.c_uint32, _f32p, _f32p]
@@ -221,6 +227,69 @@ def render(scene: dict, frame_start: int = 0, spp: int = 1, window=None, bvh: bo
                     len(a["lights"]), x0, y0, x1, y1, frame_start, spp, 1 if bvh else 0, threads,
                     None if spec is None else spec.ptr, film)
     return film
+
+
+def reservoir_stream(w: np.ndarray, u: np.ndarray):
+    """Stream weights through one reservoir: (kept index or -1, weight_sum, C)."""
+    w = np.ascontiguousarray(w, np.float32)
+    u = np.ascontiguousarray(u, np.float32)
+    ws, c = C.c_float(), C.c_float()
+    k = lib().or_reservoir_stream(w, u, len(w), C.byref(ws), C.byref(c))
+    return k, ws.value, c.value
+
+
+def prev_raster(cam: np.ndarray, w2c: np.ndarray, p: np.ndarray) -> np.ndarray:
+    """Raster coordinates [n, 2] of world points in the camera (cam record, w2c 3x4)."""
+    p = np.ascontiguousarray(p, np.float32).reshape(-1, 3)
+    out = np.zeros((len(p), 2), np.float32)
+    lib().or_prev_raster(np.ascontiguousarray(cam, np.float32), np.ascontiguousarray(w2c, np.float32), p, len(p), out)
+    return out
+
+
+class Restir:
+    """ReSTIR DI frames on the CPU (or_restir_frame) with the state carried
+    between frames: reservoirs float32[4, 3, P, 4], surfaces [2, 3, P, 4],
+    surface extensions [2, 2, P, 4] and the running-mean film [H, W, 4].
+    The camera of the previous frame is the one of the last rendered scene
+    export; a moved camera is passed as a new export."""
+
+    def __init__(self, scene: dict, threads: int = 0):
+        cam = scene["camera"]
+        self.W, self.H = int(cam[19]), int(cam[20])
+        P = self.W * self.H
+        self.rsv = np.zeros((4, 3, P, 4), np.float32)
+        self.rsv[:, 1, :, 0] = np.frombuffer(np.uint32(0xFFFFFFFF).tobytes(), np.float32)[0]
+        self.surf = np.zeros((2, 3, P, 4), np.float32)
+        self.surf[:, 0, :, 0] = np.frombuffer(np.uint32(0xFFFFFFFF).tobytes(), np.float32)[0]
+        self.surf[:, 0, :, 3] = np.frombuffer(np.uint32(1).tobytes(), np.float32)[0]
+        self.ext = np.zeros((2, 2, P, 4), np.float32)
+        self.ext[:, 0, :, :3] = 1.0
+        self.film = np.zeros((self.H, self.W, 4), np.float32)
+        self.frame = 0
+        self.threads = threads
+        self.scene = scene
+        self.prev_w2c = np.ascontiguousarray(scene["restir"][20:32], np.float32)
+        self.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)
+
+    def render(self, frames: int = 1, scene: dict | None = None) -> np.ndarray:
+        if scene is not None:
+            self.scene = scene
+        sc = self.scene
+        spec = Spectral(sc)
+        cam = np.ascontiguousarray(sc["camera"], np.float32)
+        a = {k: np.ascontiguousarray(v) for k, v in sc.items() if isinstance(v, np.ndarray)}
+        for _ in range(frames):
+            lib().or_restir_frame(a["positions"].reshape(-1), a["normals"].reshape(-1), a["uvs"].reshape(-1),
+                                  a["triangles"].reshape(-1), a["tri_instance"], a["instances"].reshape(-1),
+                                  a["materials"].reshape(-1), a["lights"].reshape(-1), cam, a["render"],
+                                  len(a["positions"]), len(a["triangles"]), len(a["instances"]),
+                                  len(a["materials"]), len(a["lights"]), spec.ptr,
+                                  np.ascontiguousarray(a["restir"], np.float32), self.prev_w2c, self.prev_pos,
+                                  self.frame, self.threads, self.rsv, self.surf, self.ext, self.film)
+            self.frame += 1
+            self.prev_w2c = np.ascontiguousarray(a["restir"][20:32], np.float32)
+            self.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)
+        return self.film
 
 
 # ---------------------------------------------------------------- spectral inputs

@@ -2,6 +2,7 @@
 // the HIP kernels use, compiled by g++ against tests/devcpu/hip/hip_runtime.h,
 // driven like the kernels drive them, for bit-level comparison with the oracle.
 #include "dev_pt.h"
+#include "dev_restir.h"
 #include "data_tables.h"
 #include <vector>
 using namespace vs;
@@ -159,4 +160,64 @@ extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *fil
             else if (rv.spectrum_dim == 4) render_px<4, false>(s, rv, x, y, o);
             else render_px<3, false>(s, rv, x, y, o);
         }
+}
+
+// ---- ReSTIR DI frame of the device code (restir.hip's three kernels, in order)
+template <int NS, bool RGB>
+static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, const float *prev_pos, uint32_t frame,
+                         float *rsv, float *surf, float *ext, float *film, const float *render) {
+    RestirView v{};
+    v.rsv = reinterpret_cast<float4 *>(rsv);
+    v.surf = reinterpret_cast<float4 *>(surf);
+    v.ext = reinterpret_cast<float4 *>(ext);
+    v.width = s->cam.width;
+    v.height = s->cam.height;
+    v.pixels = v.width * v.height;
+    v.frame = frame;
+    v.M_light = uint32_t(e[1]);
+    v.M_bsdf = uint32_t(e[2]);
+    v.max_age = uint32_t(e[3]);
+    v.max_recursion = uint32_t(e[4]);
+    v.diff_factor = e[5];
+    v.pairwise = e[6] != 0.f;
+    v.spatial = e[7] != 0.f;
+    v.spatial_num = uint32_t(e[8]);
+    v.s_dot = e[9];
+    v.s_depth = e[10];
+    v.s_radius = e[11];
+    v.temporal = e[12] != 0.f;
+    v.history_limit = uint32_t(e[13]);
+    v.temporal_N = uint32_t(e[14]);
+    v.t_radius = e[15];
+    v.t_dot = e[16];
+    v.t_depth = e[17];
+    v.temporal_mis = e[18] != 0.f;
+    for (int i = 0; i < 12; ++i) v.w2c[i] = e[20 + i], v.prev_w2c[i] = prev_w2c[i];
+    for (int i = 0; i < 3; ++i) v.cam_pos[i] = s->cam.c2w[4 * i + 3], v.prev_cam_pos[i] = prev_pos[i];
+    std::vector<uint2> stack(64);
+    PathCounters pc{0, 0, 0};
+    for (uint32_t y = 0; y < v.height; ++y)
+        for (uint32_t x = 0; x < v.width; ++x) restir_initial<NS, RGB>(s->v, s->cam, v, x, y, stack.data(), 1, pc);
+    uint32_t in = 2, extra = uint32_t(e[19]);
+    for (uint32_t k = 0; k < extra; ++k) {
+        for (uint32_t y = 0; y < v.height; ++y)
+            for (uint32_t x = 0; x < v.width; ++x) restir_spatial_pass<NS, RGB>(s->v, v, x, y, in, in ^ 1u, 2u + k);
+        in ^= 1u;
+    }
+    RenderView rv{};
+    rv.accumulation = render[6] != 0.f;
+    float4 *f4 = reinterpret_cast<float4 *>(film);
+    for (uint32_t y = 0; y < v.height; ++y)
+        for (uint32_t x = 0; x < v.width; ++x) {
+            f3 L = restir_final<NS, RGB>(s->v, s->cam, v, x, y, in, stack.data(), 1, pc);
+            f4[y * v.width + x] = film_add(rv, f4[y * v.width + x], L, frame);
+        }
+}
+
+extern "C" void dc_restir_frame(void *p, const float *restir, const float *prev_w2c, const float *prev_pos,
+                                uint32_t frame, float *rsv, float *surf, float *ext, float *film, const float *render) {
+    auto *s = static_cast<DcScene *>(p);
+    if (s->pk.rgb) restir_frame<3, true>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+    else if (s->h.spectrum_dim == 4) restir_frame<4, false>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+    else restir_frame<3, false>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
 }

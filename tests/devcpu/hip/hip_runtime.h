@@ -12,6 +12,8 @@
 
 #define __device__
 #define __host__
+typedef int hipError_t;
+typedef void *hipStream_t;
 #define __global__
 #define __forceinline__ inline
 

@@ -145,3 +145,82 @@ def test_device_path_source_equals_oracle_direct_only(dc, tmp_path, max_depth):
     film, ref = _dev_render(dc, scene, 2, str(tmp_path))
     assert _bits_equal(film, ref)
     assert (film[..., :3].max() > 0) == (max_depth > 0)
+
+
+def _restir_pair(dc, scene, frames, base=".", cameras=None, fov=19.5):
+    """The device ReSTIR source and the oracle over the same frames; returns
+    both (film, reservoirs, surfaces)."""
+    dc.dc_restir_frame.argtypes = [C.c_void_p] + [_f32] * 3 + [C.c_uint32] + [_f32] * 5
+    ctx = va.Context(-1)
+    sc = ctx.load_text(scene, base).build_accel()
+    ex = sc.export()
+    p = dc.dc_load(json.dumps(scene).encode(), base.encode())
+    orc = oracle.Restir(ex)
+    dev = oracle.Restir(ex)  # the same initial state buffers, advanced by the device source
+    for f in range(frames):
+        if cameras is not None:
+            sc.update_camera(cameras[f], fov)
+            ex = sc.export()
+        cam = np.ascontiguousarray(ex["camera"], np.float32)
+        dc.dc_set_camera(p, cam)
+        dc.dc_restir_frame(p, np.ascontiguousarray(ex["restir"], np.float32), dev.prev_w2c, dev.prev_pos, f,
+                           dev.rsv, dev.surf, dev.ext, dev.film.reshape(-1),
+                           np.ascontiguousarray(ex["render"], np.float32))
+        dev.prev_w2c = np.ascontiguousarray(ex["restir"][20:32], np.float32)
+        dev.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)
+        orc.render(1, scene=ex)
+    ctx.close()
+    return dev, orc
+
+
+def _restir_equal(dev, orc):
+    assert _bits_equal(dev.surf, orc.surf)
+    assert _bits_equal(dev.rsv, orc.rsv)
+    assert _bits_equal(dev.film, orc.film)
+
+
+@pytest.mark.parametrize("cfg", [dict(sample_num=3, radius=5),
+                                 dict(sample_num=4, radius=6, pairwise=True, temporal_mis=True,
+                                      extra_spatial_passes=1),
+                                 dict(M_light=0, M_bsdf=2, spatial=False)], ids=["default", "pairwise", "bsdf"])
+def test_device_restir_source_equals_oracle_cbox(dc, cfg):
+    scene = scenegen.cornell_box(24, 20, light_sampler="power")
+    scene["integrator"] = scenegen.restir_integrator(**{"M_light": 4, **cfg})
+    dev, orc = _restir_pair(dc, scene, 4)
+    _restir_equal(dev, orc)
+    assert orc.film[..., :3].mean() > 0
+
+
+def test_device_restir_source_equals_oracle_many_lights_moving_camera(dc):
+    scene = scenegen.many_lights_box(28, 20, n_quads=40, n_points=16, seed=2, M_light=8, sample_num=3, radius=6,
+                                     history_limit=8, extra_spatial_passes=1)
+    cams = scenegen.jitter_path(4, amplitude=0.03, seed=5)
+    dev, orc = _restir_pair(dc, scene, 4, cameras=cams)
+    _restir_equal(dev, orc)
+    # the camera moved: reprojection kept some temporal history
+    assert orc.rsv[2, 0, :, 1].max() > 1
+
+
+def test_device_restir_source_equals_oracle_hero_env(dc, tmp_path):
+    scene = scenegen.procedural_scene(tmp_path, seed=6, n_instances=9, level=3, width=28, height=16, spp=1,
+                                      max_depth=2, spectrum="hero", hdri=(128, 64))
+    scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4)
+    dev, orc = _restir_pair(dc, scene, 3, str(tmp_path))
+    _restir_equal(dev, orc)
+
+
+def test_device_restir_source_equals_oracle_near_specular(dc):
+    scene = scenegen.cornell_box(24, 24, light_sampler="power")
+    scene["materials"] += [{"type": "metal", "name": "Mirror", "param": {"material_name": "Cu", "roughness": 0.01}},
+                           {"type": "glass", "name": "Clear", "param": {"ior": 1.5, "roughness": 0.01}}]
+    for sh in scene["shapes"]:
+        if sh["name"] == "short_box":
+            sh["param"]["material"] = "Mirror"
+        if sh["name"] == "tall_box":
+            sh["param"]["material"] = "Clear"
+    scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4)
+    dev, orc = _restir_pair(dc, scene, 3)
+    _restir_equal(dev, orc)
+    # near-specular surfaces were replaced by what they reflect
+    flags = orc.surf[orc.frame & 1, 0, :, 3].view(np.uint32)
+    assert np.any((flags >> 16) == 1)

@@ -1,0 +1,193 @@
+"""The oracle's ReSTIR DI (src/render_core/integrators/ReSTIR/direct.cpp)
+pinned by what the estimator must satisfy: reservoir streaming picks each
+candidate with probability weight / sum, the camera reprojection inverts the
+pinhole projection, RIS without reuse and the full pipeline on an unoccluded
+scene converge to the direct illumination the path tracer computes, and the
+temporal history is capped by history_limit."""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+import vision_amd as va
+from oracle import oracle
+from vision_amd import scenegen
+
+
+def test_reservoir_stream_picks_proportionally():
+    rng = np.random.default_rng(3)
+    w = np.array([0.5, 2.0, 0.0, 1.25, 4.0, 0.25], np.float32)
+    n = 20000
+    counts = np.zeros(len(w))
+    for _ in range(n):
+        k, ws, c = oracle.reservoir_stream(w, rng.random(len(w), dtype=np.float32))
+        counts[k] += 1
+        assert c == len(w)
+        assert ws == pytest.approx(w.sum(), rel=1e-6)
+    assert counts[2] == 0
+    keep = w > 0
+    expect = w[keep] / w.sum() * n
+    chi = ((counts[keep] - expect) ** 2 / expect).sum()
+    assert stats.chi2.sf(chi, keep.sum() - 1) > 1e-4
+    # nothing to keep: the reservoir stays empty
+    assert oracle.reservoir_stream(np.zeros(3, np.float32), np.full(3, 0.5, np.float32))[0] == -1
+    # a single positive weight is always kept, whatever its uniform
+    assert oracle.reservoir_stream(np.array([0, 3, 0], np.float32), np.array([0.9, 0.999, 0.1], np.float32))[0] == 1
+
+
+def _export(scene):
+    with va.Context(-1) as ctx:
+        return ctx.load_text(scene).build_accel().export()
+
+
+def _pinhole_points(cam, fx, fy, t):
+    """World points at distance t along the camera rays through raster
+    (fx, fy) of a pinhole camera record (screen window and tan(fov / 2))."""
+    c2w = cam[:12].reshape(3, 4).astype(np.float64)
+    sx = cam[12] + fx * cam[14]
+    sy = cam[13] - fy * cam[15]
+    d = np.stack([sx * cam[16], sy * cam[16], np.ones_like(sx)], 1)
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return c2w[:, 3] + (d @ c2w[:, :3].T) * t[:, None]
+
+
+def test_prev_raster_inverts_the_projection():
+    ex = _export(scenegen.cornell_box(64, 48))
+    cam, w2c = ex["camera"], ex["restir"][20:32]
+    assert np.all(w2c == 0)  # "pt" integrator: no ReSTIR settings
+    scene = scenegen.cornell_box(64, 48)
+    scene["integrator"] = scenegen.restir_integrator()
+    ex = _export(scene)
+    cam, w2c = ex["camera"], ex["restir"][20:32]
+    rng = np.random.default_rng(1)
+    fx, fy = rng.uniform(0, 64, 500), rng.uniform(0, 48, 500)
+    p = _pinhole_points(cam, fx, fy, rng.uniform(1, 10, 500))
+    r = oracle.prev_raster(cam, w2c, p)
+    np.testing.assert_allclose(r[:, 0], fx, atol=2e-3)
+    np.testing.assert_allclose(r[:, 1], fy, atol=2e-3)
+    # a camera moved by dx along its right axis sees a point at depth z
+    # shifted by -dx / (z tan(fov / 2)) / raster_sx pixels
+    dx = 0.05
+    c2w = cam[:12].reshape(3, 4).astype(np.float64)
+    moved = c2w.copy()
+    moved[:, 3] += dx * c2w[:, 0]
+    R, t = moved[:, :3], moved[:, 3]
+    w2c_moved = np.concatenate([R.T, (-R.T @ t)[:, None]], 1).astype(np.float32).reshape(-1)
+    z = 5.0
+    p = _pinhole_points(cam, fx, fy, np.full(500, z))
+    depth = (p - c2w[:, 3]) @ c2w[:, 2]
+    r2 = oracle.prev_raster(cam, w2c_moved, p)
+    np.testing.assert_allclose(r2[:, 0] - fx, -dx / (depth * cam[16]) / cam[14], rtol=1e-3, atol=2e-3)
+    np.testing.assert_allclose(r2[:, 1], fy, atol=2e-3)
+
+
+def _restir_mean(scene, frames):
+    ex = _export(scene)
+    r = oracle.Restir(ex)
+    return r.render(frames)[..., :3].astype(np.float64), r
+
+
+def _pt_direct(scene, spp):
+    s = dict(scene)
+    s["integrator"] = {"type": "pt", "param": {"max_depth": 1, "min_depth": 5, "rr_threshold": 1}}
+    return oracle.render(_export(s), 0, spp)[..., :3].astype(np.float64)
+
+
+def _block_means(img, b):
+    H, W = img.shape[:2]
+    return img[:H // b * b, :W // b * b].reshape(H // b, b, W // b, b, -1).mean(axis=(1, 3))
+
+
+def test_ris_without_reuse_converges_to_direct_light():
+    """RIS over light and BSDF candidates with visibility at shading is an
+    unbiased direct-light estimator: with both reuse passes off its mean
+    over frames is the path tracer's depth-1 image (NEE + MIS-weighted
+    emitter hits, integrator.cpp:276-281)."""
+    scene = scenegen.cornell_box(24, 24, light_sampler="power")
+    scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, spatial=False, temporal=False)
+    # equal sample counts: both draw the same sub-pixel positions per frame
+    # (the sensor sample opens each frame's stream), so emitter coverage
+    # matches sample by sample and only the direct-light estimates differ
+    got, _ = _restir_mean(scene, 512)
+    ref = _pt_direct(scene, 512)
+    assert got.mean() == pytest.approx(ref.mean(), rel=0.01)
+    gb, rb = _block_means(got, 6), _block_means(ref, 6)
+    assert np.max(np.abs(gb - rb) / (rb + 0.02)) < 0.08
+
+
+def _lamp():
+    return {"type": "quad", "name": "lamp",
+            "param": {"width": 1, "height": 1, "material": "Black",
+                      "transform": scenegen.xform(scenegen.translate(0.3, 1.5, 0.0) @ scenegen.rot_x(180)
+                                                  @ scenegen.scale(0.6, 1, 0.4)),
+                      "emission": {"type": "area", "param": {"color": [5, 4, 3], "two_sided": False, "scale": 1}}}}
+
+
+def test_reuse_converges_to_direct_light_where_every_sample_reaches_every_pixel():
+    """Temporal and spatial reuse with the reference's default weights
+    (constant 1/M spatial combine, confidence-ratio temporal MIS) on a plane
+    under an area and a point light: every sample has non-zero target at
+    every pixel and is visible from it, the condition under which those
+    weights stay unbiased."""
+    point = {"type": "point", "param": {"position": [-0.4, 1.2, 0.2], "color": [1, 1, 1], "scale": 2.0}}
+    scene = scenegen.lit_plane(point, width=24, height=24, fov=60, cam_height=2.0, extra_shapes=[_lamp()])
+    scene["light_sampler"]["type"] = "power"
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, M_bsdf=1, sample_num=3, radius=4,
+                                                     history_limit=4)
+    got, _ = _restir_mean(scene, 512)
+    ref = _pt_direct(scene, 512)
+    assert got.mean() == pytest.approx(ref.mean(), rel=0.01)
+    gb, rb = _block_means(got, 6), _block_means(ref, 6)
+    assert np.max(np.abs(gb - rb) / (rb + 0.02)) < 0.05
+
+
+def test_pairwise_spatial_reuse_handles_partial_support():
+    """A spot light reaches only part of the image: the constant 1/M combine
+    darkens pixels whose neighbours lie outside its cone, while pairwise MIS
+    (course-notes eq. 7.3, direct.cpp:235-285) re-weights by the targets and
+    stays unbiased."""
+    spot = {"type": "spot", "param": {"position": [-0.4, 1.2, 0.2], "direction": [0.1, -1, 0], "angle": 40,
+                                      "falloff": 10, "color": [1, 1, 1], "scale": 2.0}}
+    scene = scenegen.lit_plane(spot, width=24, height=24, fov=60, cam_height=2.0)
+    scene["light_sampler"]["type"] = "power"
+    ref = _pt_direct(scene, 512).mean()
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, M_bsdf=1, temporal=False, sample_num=3, radius=4,
+                                                     pairwise=True)
+    pairwise = _restir_mean(scene, 512)[0].mean()
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, M_bsdf=1, temporal=False, sample_num=3, radius=4)
+    constant = _restir_mean(scene, 512)[0].mean()
+    assert pairwise == pytest.approx(ref, rel=0.01)
+    assert constant < ref * 0.98
+
+
+def test_temporal_history_is_capped():
+    """A static camera reprojects each pixel onto itself: the shader0
+    reservoir holds its own candidate (C = 1) plus the previous frame's,
+    truncated to history_limit (reservoir.h:88-90, direct.cpp:385-395)."""
+    limit = 5
+    scene = scenegen.cornell_box(20, 20)
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, spatial=False, history_limit=limit)
+    _, r = _restir_mean(scene, 8)
+    C = r.rsv[2, 0, :, 1]
+    hit = r.surf[r.frame & 1, 0, :, 0].view(np.uint32) != 0xFFFFFFFF
+    assert np.all(C[~hit] == 0)
+    assert np.median(C[hit]) == limit + 1
+    assert C.max() <= limit + 1
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, spatial=False, temporal=False)
+    _, r = _restir_mean(scene, 3)
+    assert np.all(r.rsv[2, 0, hit, 1] == 1)
+
+
+def test_spatial_combine_sums_confidence():
+    """constant_combine (direct.cpp:287-309) adds the neighbours'
+    confidences to the canonical one: with temporal reuse off every
+    reservoir of the current frame has C = 1 + (valid neighbours)."""
+    scene = scenegen.cornell_box(20, 20)
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, temporal=False, sample_num=4, radius=3)
+    _, r = _restir_mean(scene, 2)
+    cur = r.frame & 1  # the current-frame buffers of the last frame rendered
+    C = r.rsv[cur, 0, :, 1]
+    hit = r.surf[cur, 0, :, 0].view(np.uint32) != 0xFFFFFFFF
+    assert set(np.unique(C[hit])) <= {1.0, 2.0, 3.0, 4.0, 5.0}
+    assert np.mean(C[hit] == 5.0) > 0.25

@@ -60,7 +60,9 @@ class SceneInfo(C.Structure):
 
 class RenderStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("rays", C.c_uint64), ("node_fetches", C.c_uint64),
-                ("tri_fetches", C.c_uint64), ("launches", C.c_uint32), ("samples_per_pixel", C.c_uint32)]
+                ("tri_fetches", C.c_uint64), ("launches", C.c_uint32), ("samples_per_pixel", C.c_uint32),
+                ("stage_ms", C.c_double * 2), ("stage_launches", C.c_uint32 * 2), ("stage_rays", C.c_uint64 * 2),
+                ("stage_node_fetches", C.c_uint64 * 2), ("stage_tri_fetches", C.c_uint64 * 2)]
 
 
 class SceneArrays(C.Structure):
@@ -74,7 +76,8 @@ class SceneArrays(C.Structure):
                 ("spd", C.POINTER(C.c_float)), ("num_spd", C.c_uint32), ("spectrum_rgb", C.c_uint32),
                 ("spectrum_dim", C.c_uint32), ("env_light", C.c_uint32), ("env_w", C.c_uint32),
                 ("env_h", C.c_uint32), ("env_rgb", C.POINTER(C.c_float)), ("env_w2o", C.c_float * 9),
-                ("env_o2w", C.c_float * 9), ("env_scale", C.c_float), ("world_radius", C.c_float)]
+                ("env_o2w", C.c_float * 9), ("env_scale", C.c_float), ("world_radius", C.c_float),
+                ("restir", C.POINTER(C.c_float)), ("num_restir", C.c_uint32)]
 
 
 # every entry point of include/vision_amd.h: name -> (restype, argtypes)
@@ -95,6 +98,8 @@ SIGNATURES = {
                                         C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32)]),
     "vs_scene_set_resolution": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     "vs_scene_set_integrator": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_float]),
+    "vs_scene_update_camera": (C.c_int32, [_P, _P, C.c_float]),
+    "vs_scene_restir_state": (C.c_int32, [_P, _P, _P]),
     "vs_trace_closest": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
@@ -239,6 +244,7 @@ class Scene:
             "env_o2w": np.array(a.env_o2w[:], np.float32).reshape(3, 3),
             "env_scale": float(a.env_scale),
             "world_radius": float(a.world_radius),
+            "restir": arr(a.restir, a.num_restir, np.float32),
         }
 
     def bvh(self) -> tuple[np.ndarray, np.ndarray]:
@@ -283,6 +289,22 @@ class Scene:
                                             wi_out.ctypes.data, f.ctypes.data, pdf.ctypes.data))
         return (wi_out if sample else wi_a), f, pdf
 
+    def update_camera(self, c2w: np.ndarray, fov_y: float):
+        """Move the camera: `c2w` a 4x4 camera-to-world matrix (acting on
+        column vectors), `fov_y` in degrees (the SDK's update_camera)."""
+        m = np.ascontiguousarray(np.asarray(c2w, np.float32).reshape(4, 4).T)
+        self.ctx._check(lib().vs_scene_update_camera(self._h, m.ctypes.data, float(fov_y)))
+
+    def restir_state(self) -> tuple[np.ndarray, np.ndarray]:
+        """ReSTIR DI buffers after the last frame: reservoirs float32[4, 3, H*W, 4]
+        and surfaces float32[2, 3, H*W, 4] (integer fields as bit patterns)."""
+        info = self.info()
+        n = info["width"] * info["height"]
+        rsv = np.zeros((4, 3, n, 4), np.float32)
+        surf = np.zeros((2, 3, n, 4), np.float32)
+        self.ctx._check(lib().vs_scene_restir_state(self._h, rsv.ctypes.data, surf.ctypes.data))
+        return rsv, surf
+
     def set_kernel(self, mode: str = "megakernel", chunk_frames: int = 8, trace_block: int = 128):
         """Kernel form of render(): "megakernel" or "wavefront"."""
         m = {"megakernel": 0, "wavefront": 1}[mode]
@@ -298,7 +320,11 @@ class Scene:
     def stats(self) -> dict:
         s = RenderStats()
         self.ctx._check(lib().vs_last_render_stats(self._h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in RenderStats._fields_}
+        out = {}
+        for f, _ in RenderStats._fields_:
+            v = getattr(s, f)
+            out[f] = list(v) if f.startswith("stage_") else v
+        return out
 
     def reset_film(self):
         self.ctx._check(lib().vs_reset_film(self._h))

@@ -18,6 +18,7 @@
 #include "launch.h"
 #include "scene.h"
 #include "data_tables.h"
+#include "restir.h"
 #include "tables.h"
 #include "wavefront.h"
 
@@ -25,6 +26,7 @@ struct vs_context {
     int device{0};// -1: host-only context (scene parsing, BVH build, export)
     hipStream_t stream{nullptr};
     hipEvent_t ev0{nullptr}, ev1{nullptr};
+    hipEvent_t tr0{nullptr}, tr1{nullptr}, tr2{nullptr};// around wavefront traversal kernels
     std::string last_error;
 };
 
@@ -93,6 +95,11 @@ struct vs_scene {
     DevBuf wf_o, wf_d, wf_thr, wf_L, wf_meta, wf_lam, wf_lpdf, wf_hit, wf_sh_o, wf_sh_d, wf_sh_c, wf_q0, wf_q1,
         wf_counts;
     uint32_t *wf_host_counts{nullptr};// pinned
+    // ReSTIR DI ("rt" integrator): reservoirs, surfaces, surface extensions,
+    // and the camera of the previous frame (3x4 rows of camera-to-world)
+    DevBuf rs_rsv, rs_surf, rs_ext;
+    uint32_t rs_pixels{0};
+    float prev_c2w[12]{};
     vs::SceneView view{};
     vs::CameraView cam{};
     uint32_t frame_index{0};
@@ -100,7 +107,7 @@ struct vs_scene {
     vs_render_stats stats{};
     // exported flat arrays
     std::vector<uint32_t> exp_inst;
-    std::vector<float> exp_camera, exp_render;
+    std::vector<float> exp_camera, exp_render, exp_restir;
 };
 
 namespace {
@@ -172,6 +179,63 @@ void fill_camera(vs_scene *s) {
     s->exp_camera.insert(s->exp_camera.end(), rest, rest + 20);
 }
 
+// inverse of an affine 3x4 camera-to-world (rows), in double precision
+void affine_inverse(const float *m, float *out) {
+    double a[3][3], t[3];
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) a[r][c] = m[r * 4 + c];
+        t[r] = m[r * 4 + 3];
+    }
+    double det = a[0][0] * (a[1][1] * a[2][2] - a[1][2] * a[2][1]) - a[0][1] * (a[1][0] * a[2][2] - a[1][2] * a[2][0]) +
+                 a[0][2] * (a[1][0] * a[2][1] - a[1][1] * a[2][0]);
+    if (det == 0.0) throw std::invalid_argument("camera matrix is singular");
+    double inv[3][3];
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = (a[0][2] * a[2][1] - a[0][1] * a[2][2]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = (a[1][2] * a[2][0] - a[1][0] * a[2][2]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = (a[0][2] * a[1][0] - a[0][0] * a[1][2]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = (a[0][1] * a[2][0] - a[0][0] * a[2][1]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+    for (int r = 0; r < 3; ++r) {
+        for (int c = 0; c < 3; ++c) out[r * 4 + c] = float(inv[r][c]);
+        out[r * 4 + 3] = float(-(inv[r][0] * t[0] + inv[r][1] * t[1] + inv[r][2] * t[2]));
+    }
+}
+
+float radians_to_cos(float deg) { return std::cos(deg * (3.14159265358979323846f / 180.f)); }
+
+void fill_restir_export(vs_scene *s) {
+    s->exp_restir.assign(32, 0.f);
+    const vs::IntegratorHost &in = s->host.integrator;
+    if (in.type != "rt") return;
+    const vs::RestirHost &r = in.restir;
+    float *e = s->exp_restir.data();
+    e[0] = 1.f;
+    e[1] = float(r.M_light);
+    e[2] = float(r.M_bsdf);
+    e[3] = float(r.max_age);
+    e[4] = float(r.max_recursion);
+    e[5] = r.diff_factor;
+    e[6] = r.pairwise ? 1.f : 0.f;
+    e[7] = r.s_open ? 1.f : 0.f;
+    e[8] = float(r.s_num);
+    e[9] = radians_to_cos(r.s_theta);
+    e[10] = r.s_depth;
+    e[11] = r.s_radius;
+    e[12] = r.t_open ? 1.f : 0.f;
+    e[13] = float(r.t_limit);
+    e[14] = float(r.t_N);
+    e[15] = r.t_radius;
+    e[16] = radians_to_cos(r.t_theta);
+    e[17] = r.t_depth;
+    e[18] = r.t_mis ? 1.f : 0.f;
+    e[19] = float(r.extra_spatial_passes);
+    affine_inverse(s->cam.c2w, e + 20);
+}
+
 void fill_render_export(vs_scene *s) {
     const vs::SceneHost &h = s->host;
     s->exp_render = {float(h.integrator.max_depth), float(h.integrator.min_depth), h.integrator.rr_threshold,
@@ -233,7 +297,9 @@ vs_status vs_create(int32_t device, vs_context **out) {
     if (!ctx) return VS_ERROR_OUT_OF_MEMORY;
     ctx->device = device;
     if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) {
+        hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess ||
+        hipEventCreate(&ctx->tr0) != hipSuccess || hipEventCreate(&ctx->tr1) != hipSuccess ||
+        hipEventCreate(&ctx->tr2) != hipSuccess) {
         vs_destroy(ctx);
         return VS_ERROR_DEVICE;
     }
@@ -248,8 +314,8 @@ void vs_destroy(vs_context *ctx) {
         return;
     }
     (void)hipSetDevice(ctx->device);
-    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
-    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
+    for (hipEvent_t e : {ctx->ev0, ctx->ev1, ctx->tr0, ctx->tr1, ctx->tr2})
+        if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -265,7 +331,9 @@ vs_status vs_scene_load(vs_context *ctx, const char *json_text, const char *base
         s->host = vs::load_scene(json_text, base_dir ? base_dir : "");
         s->packed = vs::pack_scene(s->host);
         fill_camera(s.get());
+        std::memcpy(s->prev_c2w, s->cam.c2w, sizeof(s->prev_c2w));
         fill_render_export(s.get());
+        fill_restir_export(s.get());
         for (const auto &ir : s->host.instRecords) {
             s->exp_inst.push_back(ir.mat);
             s->exp_inst.push_back(ir.light);
@@ -461,6 +529,8 @@ vs_status vs_scene_export(const vs_scene *s, vs_scene_arrays *out) {
     }
     out->env_scale = pk.env_scale;
     out->world_radius = pk.world_radius;
+    out->restir = s->exp_restir.data();
+    out->num_restir = uint32_t(s->exp_restir.size());
     return VS_OK;
 }
 
@@ -487,6 +557,24 @@ vs_status vs_scene_set_resolution(vs_scene *s, uint32_t width, uint32_t height) 
             alloc_film(s);
             hip_check(hipStreamSynchronize(s->ctx->stream), "film sync");
         }
+    });
+}
+
+vs_status vs_scene_update_camera(vs_scene *s, const float *c2w, float fov_y) {
+    if (!s || !c2w || !(fov_y > 0.f && fov_y < 180.f))
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "bad camera");
+    return guarded(s->ctx, [&] {
+        vs::CameraHost &c = s->host.camera;
+        // Sensor::set_mat (sensor.cpp:86-89): the matrix as given
+        for (int col = 0; col < 4; ++col) c.c2w.c[col] = vs::mk4(c2w[4 * col], c2w[4 * col + 1], c2w[4 * col + 2], c2w[4 * col + 3]);
+        c.position = vs::mk3(c2w[12], c2w[13], c2w[14]);
+        c.fov_y = fov_y;
+        float probe[12];
+        for (int r = 0; r < 3; ++r)
+            for (int col = 0; col < 4; ++col) probe[r * 4 + col] = c2w[4 * col + r];
+        affine_inverse(probe, probe);// rejects a singular matrix before anything changes
+        fill_camera(s);
+        fill_restir_export(s);
     });
 }
 
@@ -595,7 +683,8 @@ namespace {
 // Wavefront render of `spp` frames starting at s->frame_index: chunks of
 // `wf_chunk_frames` frames, each a loop of trace / shade / shadow over the
 // live-path queue until it empties, then the chunk folds into the film.
-uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr) {
+uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr,
+                          vs_render_stats &ts) {
     hipStream_t st = s->ctx->stream;
     const uint32_t P = s->cam.width * s->cam.height;
     const uint32_t chunk = std::min<uint32_t>(s->wf_chunk_frames, spp);
@@ -644,12 +733,24 @@ uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, 
         int cur = 0;
         while (count) {
             hip_check(vs::wf_zero(counts, st), "wf zero");
+            hip_check(hipEventRecord(s->ctx->tr0, st), "event");
             hip_check(vs::wf_trace(s->view, w, q[cur], count, ctr, s->wf_trace_block, st), "wf trace");
+            hip_check(hipEventRecord(s->ctx->tr1, st), "event");
             hip_check(vs::wf_shade(s->view, rv0, w, dim, rgb, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
-            hip_check(vs::wf_shadow(s->view, w, counts, count, ctr, s->wf_trace_block, st), "wf shadow");
+            hip_check(hipEventRecord(s->ctx->tr2, st), "event");
+            hip_check(vs::wf_shadow(s->view, w, counts, count, ctr ? ctr + 4 : nullptr, s->wf_trace_block, st),
+                      "wf shadow");
+            hip_check(hipEventRecord(s->ctx->ev1, st), "event");
             launches += 4;
             hip_check(hipMemcpyAsync(s->wf_host_counts, counts, 8, hipMemcpyDeviceToHost, st), "wf counts");
             hip_check(hipStreamSynchronize(st), "wf sync");
+            float t_trace = 0.f, t_shadow = 0.f;
+            hip_check(hipEventElapsedTime(&t_trace, s->ctx->tr0, s->ctx->tr1), "event time");
+            hip_check(hipEventElapsedTime(&t_shadow, s->ctx->tr2, s->ctx->ev1), "event time");
+            ts.stage_ms[0] += t_trace;
+            ts.stage_ms[1] += t_shadow;
+            ts.stage_launches[0] += 1;
+            ts.stage_launches[1] += 1;
             count = s->wf_host_counts[vs::WF_LIVE];
             if (count > n) throw std::logic_error("wavefront queue overflow");
             cur ^= 1;
@@ -661,7 +762,104 @@ uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, 
     return launches;
 }
 
+vs::RestirView restir_view(vs_scene *s, uint32_t frame) {
+    const vs::RestirHost &r = s->host.integrator.restir;
+    const float *e = s->exp_restir.data();
+    vs::RestirView v{};
+    v.rsv = s->rs_rsv.as<float4>();
+    v.surf = s->rs_surf.as<float4>();
+    v.ext = s->rs_ext.as<float4>();
+    v.width = s->cam.width;
+    v.height = s->cam.height;
+    v.pixels = v.width * v.height;
+    v.frame = frame;
+    v.M_light = r.M_light;
+    v.M_bsdf = r.M_bsdf;
+    v.max_age = r.max_age;
+    v.diff_factor = r.diff_factor;
+    v.spatial = r.s_open ? 1u : 0u;
+    v.spatial_num = r.s_num;
+    v.s_dot = e[9];
+    v.s_depth = r.s_depth;
+    v.s_radius = r.s_radius;
+    v.temporal = r.t_open ? 1u : 0u;
+    v.history_limit = r.t_limit;
+    v.t_dot = e[16];
+    v.t_depth = r.t_depth;
+    v.t_radius = r.t_radius;
+    v.temporal_N = r.t_N;
+    v.temporal_mis = r.t_mis ? 1u : 0u;
+    v.pairwise = r.pairwise ? 1u : 0u;
+    v.max_recursion = r.max_recursion;
+    for (int i = 0; i < 3; ++i) {
+        v.cam_pos[i] = s->cam.c2w[i * 4 + 3];
+        v.prev_cam_pos[i] = s->prev_c2w[i * 4 + 3];
+    }
+    for (int i = 0; i < 12; ++i) v.w2c[i] = e[20 + i];
+    affine_inverse(s->prev_c2w, v.prev_w2c);
+    return v;
+}
+
+// RealTimeIntegrator::render (rt.cpp:144-160), direct half: per frame
+// shader0, extra spatial passes, shader1 with the film update; the camera
+// becomes the previous camera after each frame (Sensor::after_render)
+uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsigned long long *ctr,
+                       vs_render_stats &ts) {
+    hipStream_t st = s->ctx->stream;
+    const uint32_t P = s->cam.width * s->cam.height;
+    if (s->rs_pixels != P) {
+        s->rs_rsv.alloc(size_t(P) * vs::kRestirReservoirBuffers * 3u * 16u);
+        s->rs_surf.alloc(size_t(P) * 2u * 3u * 16u);
+        s->rs_ext.alloc(size_t(P) * 2u * 2u * 16u);
+        s->rs_pixels = P;
+        hip_check(vs::restir_clear(restir_view(s, 0), st), "restir clear");
+    }
+    const bool rgb = s->packed.rgb;
+    const uint32_t dim = rv.spectrum_dim;
+    const uint32_t extra = s->host.integrator.restir.extra_spatial_passes;
+    uint32_t launches = 0;
+    for (uint32_t f = 0; f < spp; ++f) {
+        vs::RestirView v = restir_view(s, s->frame_index + f);
+        hip_check(hipEventRecord(s->ctx->tr0, st), "event");
+        hip_check(vs::restir_initial(s->view, s->cam, v, dim, rgb, ctr, st), "restir initial");
+        hip_check(hipEventRecord(s->ctx->tr1, st), "event");
+        uint32_t in = 2u;
+        for (uint32_t k = 0; k < extra; ++k) {
+            hip_check(vs::restir_spatial(s->view, v, dim, rgb, in, in ^ 1u, 2u + k, st), "restir spatial");
+            in ^= 1u;
+        }
+        hip_check(hipEventRecord(s->ctx->tr2, st), "event");
+        hip_check(vs::restir_final(s->view, s->cam, v, rv, dim, rgb, in, s->film_rt.as<float4>(),
+                                   s->film_out.as<float4>(), ctr ? ctr + 4 : nullptr, st),
+                  "restir final");
+        hip_check(hipEventRecord(s->ctx->ev1, st), "event");
+        hip_check(hipEventSynchronize(s->ctx->ev1), "restir sync");
+        float t0 = 0.f, t1 = 0.f;
+        hip_check(hipEventElapsedTime(&t0, s->ctx->tr0, s->ctx->tr1), "event time");
+        hip_check(hipEventElapsedTime(&t1, s->ctx->tr2, s->ctx->ev1), "event time");
+        ts.stage_ms[0] += t0;
+        ts.stage_ms[1] += t1;
+        ts.stage_launches[0] += 1;
+        ts.stage_launches[1] += 1;
+        launches += 2u + extra;
+        std::memcpy(s->prev_c2w, s->cam.c2w, sizeof(s->prev_c2w));
+    }
+    return launches;
+}
+
 }// namespace
+
+vs_status vs_scene_restir_state(vs_scene *s, float *reservoirs, float *surfaces) {
+    if (!s || !reservoirs || !surfaces) return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (s->rs_pixels == 0) throw std::logic_error("no ReSTIR frame rendered");
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(s->ctx->stream), "restir sync");
+        hip_check(hipMemcpy(reservoirs, s->rs_rsv.ptr, s->rs_rsv.bytes, hipMemcpyDeviceToHost), "restir D2H");
+        hip_check(hipMemcpy(surfaces, s->rs_surf.ptr, s->rs_surf.bytes, hipMemcpyDeviceToHost), "restir D2H");
+    });
+}
 
 vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
     if (!s) return VS_ERROR_INVALID_ARGUMENT;
@@ -687,9 +885,15 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
             hip_check(hipMemsetAsync(ctr, 0, 64, st), "counter clear");
         }
         uint32_t launches = 0;
+        vs_render_stats ts{};
+        const bool restir = h.integrator.type == "rt";
+        const bool wavefront = !restir && s->kernel_mode == VS_KERNEL_WAVEFRONT;
+        const bool staged = restir || wavefront;// two counter sets, [0..2] and [4..6]
         hip_check(hipEventRecord(s->ctx->ev0, st), "event");
-        if (s->kernel_mode == VS_KERNEL_WAVEFRONT) {
-            launches = render_wavefront(s, rv, spp, ctr);
+        if (restir) {
+            launches = render_restir(s, rv, spp, ctr, ts);
+        } else if (wavefront) {
+            launches = render_wavefront(s, rv, spp, ctr, ts);
         } else if (fused) {
             rv.frame_start = s->frame_index;
             rv.spp = spp;
@@ -710,16 +914,25 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         float ms = 0.f;
         hip_check(hipEventElapsedTime(&ms, s->ctx->ev0, s->ctx->ev1), "event time");
         s->frame_index += spp;
-        s->stats = vs_render_stats{};
+        s->stats = ts;
         s->stats.kernel_ms = ms;
         s->stats.launches = launches;
         s->stats.samples_per_pixel = spp;
         if (ctr) {
-            unsigned long long c[3];
+            // [0..2] rays / node / triangle fetches; staged renders keep their
+            // second kernel's counts apart in [4..6]
+            unsigned long long c[8];
             hip_check(hipMemcpy(c, ctr, sizeof(c), hipMemcpyDeviceToHost), "counters D2H");
-            s->stats.rays = c[0];
-            s->stats.node_fetches = c[1];
-            s->stats.tri_fetches = c[2];
+            s->stats.rays = c[0] + (staged ? c[4] : 0ull);
+            s->stats.node_fetches = c[1] + (staged ? c[5] : 0ull);
+            s->stats.tri_fetches = c[2] + (staged ? c[6] : 0ull);
+            if (staged) {
+                for (int k = 0; k < 2; ++k) {
+                    s->stats.stage_rays[k] = c[4 * k];
+                    s->stats.stage_node_fetches[k] = c[4 * k + 1];
+                    s->stats.stage_tri_fetches[k] = c[4 * k + 2];
+                }
+            }
         }
     });
 }

@@ -90,6 +90,13 @@ __device__ __forceinline__ float light_select_pmf(const SceneView &sc, uint32_t 
     return alias_pmf(sc, sc.select_offset, sc.num_lights, sc.select_integral, li);
 }
 
+// light selection (LightSampler::select_light, lightsampler.cpp:178-197;
+// uniform.cpp:22-31 / power.cpp:21-26)
+__device__ __forceinline__ uint32_t select_light(const SceneView &sc, float u) {
+    if (sc.light_sampler == 0) return uint32_t(fminf(u * float(sc.num_lights), float(sc.num_lights) - 1.f));
+    return alias_sample(sc, sc.select_offset, sc.num_lights, u, nullptr);
+}
+
 // ---- spectra of lights
 template <int NS, bool RGB>
 __device__ __forceinline__ Spec<NS> illum_spectrum(const SceneView &sc, const float *rgb, const float *coef,
@@ -149,6 +156,23 @@ __device__ __forceinline__ void env_eval(const SceneView &sc, f3 dir, const Swl<
     pdf = sin_t == 0.f ? 0.f : p;
 }
 
+// AliasTable2D::sample_continuous (alias2d.cpp:106-129) over the environment
+// map: (u, v) in [0, 1)^2 and the map pdf
+__device__ __forceinline__ f2 env_sample_uv(const SceneView &sc, f2 u, float &pdf_map) {
+    float uvr;
+    uint32_t iv = alias_pick(sc.env_marg, sc.env_h, u.y, &uvr);
+    float fv = (float(iv) + uvr) / float(sc.env_h);
+    float pdf_v = sc.env_marg_integral > 0.f ? sc.env_marg_func[iv] / sc.env_marg_integral : 0.f;
+    size_t row = size_t(sc.env_w) * iv;
+    float uur;
+    uint32_t iu = alias_pick(sc.env_cond + row, sc.env_w, u.x, &uur);
+    float fu = (float(iu) + uur) / float(sc.env_w);
+    float integral_u = sc.env_marg_func[iv];
+    float pdf_u = integral_u > 0.f ? sc.env_cond_func[row + iu] / integral_u : 0.f;
+    pdf_map = pdf_u * pdf_v;
+    return mk2(fu, fv);
+}
+
 template <int NS>
 struct LightSampleN {
     Spec<NS> L;
@@ -164,15 +188,8 @@ __device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3
     LightSampleN<NS> ls;
     float u_light = sampler.next1d();
     f2 u = sampler.next2d();
-    uint32_t li;
-    float sel_pmf;
-    if (sc.light_sampler == 0) {
-        li = uint32_t(fminf(u_light * float(sc.num_lights), float(sc.num_lights) - 1.f));
-        sel_pmf = 1.f / float(sc.num_lights);
-    } else {
-        li = alias_sample(sc, sc.select_offset, sc.num_lights, u_light, nullptr);
-        sel_pmf = alias_pmf(sc, sc.select_offset, sc.num_lights, sc.select_integral, li);
-    }
+    uint32_t li = select_light(sc, u_light);
+    float sel_pmf = light_select_pmf(sc, li);
     const LightRecord &L = sc.lights[li];
     if (L.type == LIGHT_AREA) {
         // AreaLight::sample_wi (src/render_core/lights/area.cpp:120-149)
@@ -193,18 +210,9 @@ __device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3
     } else if (L.type == LIGHT_ENV) {
         // SphericalMap::sample_wi / evaluate (spherical.cpp:94-124, 157-163);
         // AliasTable2D::sample_continuous (alias2d.cpp:109-129)
-        float uvr;
-        uint32_t iv = alias_pick(sc.env_marg, sc.env_h, u.y, &uvr);
-        float fv = (float(iv) + uvr) / float(sc.env_h);
-        float pdf_v = sc.env_marg_integral > 0.f ? sc.env_marg_func[iv] / sc.env_marg_integral : 0.f;
-        size_t row = size_t(sc.env_w) * iv;
-        float uur;
-        uint32_t iu = alias_pick(sc.env_cond + row, sc.env_w, u.x, &uur);
-        float fu = (float(iu) + uur) / float(sc.env_w);
-        float integral_u = sc.env_marg_func[iv];
-        float pdf_u = integral_u > 0.f ? sc.env_cond_func[row + iu] / integral_u : 0.f;
-        float pdf_map = pdf_u * pdf_v;
-        float theta = fv * kPi, phi = fu * 2.f * kPi;
+        float pdf_map;
+        f2 fuv = env_sample_uv(sc, u, pdf_map);
+        float theta = fuv.y * kPi, phi = fuv.x * 2.f * kPi;
         float sin_t = sinf(theta), cos_t = cosf(theta);
         float sp, cp;
         sincosf(phi, &sp, &cp);
@@ -425,7 +433,7 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
 // Sensor::generate_ray (src/base/sensor/sensor.cpp:44-56) + filter sample
 // (Sampler::sensor_sample, src/base/sampler.h:65-73): draws filter 2, lens 2, time 1
 __device__ __forceinline__ void camera_ray(const CameraView &cam, uint32_t px, uint32_t py, Sampler &sampler,
-                                           f3 &o, f3 &d, float &weight) {
+                                           f3 &o, f3 &d, float &weight, f2 *p_film = nullptr) {
     f2 uf = sampler.next2d();
     f2 fs;
     weight = 1.f;
@@ -458,6 +466,7 @@ __device__ __forceinline__ void camera_ray(const CameraView &cam, uint32_t px, u
     (void)sampler.next1d();// time
     float fx = float(px) + 0.5f + fs.x;
     float fy = float(py) + 0.5f + fs.y;
+    if (p_film) *p_film = mk2(fx, fy);
     float sx = cam.lower_x + fx * cam.raster_sx;
     float sy = cam.upper_y - fy * cam.raster_sy;
     f3 dc = normalize(mk3(sx * cam.tan_half_fov, sy * cam.tan_half_fov, 1.f));

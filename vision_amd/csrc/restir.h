@@ -1,0 +1,47 @@
+// ReSTIR DI state shared by the host driver (capi.cpp) and the kernels
+// (restir.hip): device buffers, the DIParam settings
+// (src/render_core/integrators/ReSTIR/direct.h:10-34) and the camera of the
+// current and the previous frame.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "dev_scene.h"
+
+namespace vs {
+
+struct RestirView {
+    float4 *rsv; // reservoirs: 4 buffers x 3 fields x pixels
+    float4 *surf;// surfaces:   2 buffers x 3 fields x pixels
+    float4 *ext; // surface extensions: 2 buffers x 2 fields x pixels
+    uint32_t width, height, pixels;
+    uint32_t frame;
+    uint32_t M_light, M_bsdf, max_age;
+    float diff_factor;
+    uint32_t spatial, spatial_num;
+    float s_dot, s_depth, s_radius;
+    uint32_t temporal, history_limit;
+    float t_dot, t_depth, t_radius;
+    uint32_t temporal_N, temporal_mis, pairwise, max_recursion;
+    // camera now and one frame earlier (Sensor::store_prev_data,
+    // sensor.cpp:99-103); w2c maps world points to camera space (3x4 rows)
+    float cam_pos[3], prev_cam_pos[3];
+    float w2c[12], prev_w2c[12];
+};
+
+// reservoir buffers: previous / current by frame parity, the pass-through of
+// shader0 -> shader1, and a second pass-through for extra spatial passes
+constexpr uint32_t kRestirReservoirBuffers = 4;
+
+hipError_t restir_clear(const RestirView &v, hipStream_t stream);
+hipError_t restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &v, uint32_t spectrum_dim,
+                          bool rgb, unsigned long long *counters, hipStream_t stream);
+hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb, uint32_t in,
+                          uint32_t out, uint32_t dim, hipStream_t stream);
+hipError_t restir_final(const SceneView &sc, const CameraView &cam, const RestirView &v, const RenderView &rv,
+                        uint32_t spectrum_dim, bool rgb, uint32_t in, float4 *rt, float4 *out,
+                        unsigned long long *counters, hipStream_t stream);
+
+}// namespace vs

@@ -1,0 +1,130 @@
+// ReSTIR DI kernels for gfx950: one thread per pixel in 8x8 tiles (one wave
+// per tile keeps neighbouring rays, reservoirs and surfaces together),
+// traversal stacks in LDS.
+#include <hip/hip_runtime.h>
+
+#include "dev_restir.h"
+
+namespace vs {
+
+using namespace dev;
+
+namespace {
+
+__global__ void __launch_bounds__(256) clear_kernel(RestirView v) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= v.pixels) return;
+    Reservoir r = empty_reservoir();
+    for (uint32_t b = 0; b < kRestirReservoirBuffers; ++b) store_reservoir(v, b, i, r);
+    SurfaceRec s = empty_surface();
+    SurfaceExt e;
+    e.throughput = mk3(1.f, 1.f, 1.f);
+    e.view_pos = mk3(0.f, 0.f, 0.f);
+    e.t_max = 0.f;
+    for (uint32_t b = 0; b < 2u; ++b) {
+        store_surface(v, b, i, s);
+        store_ext(v, b, i, e);
+    }
+}
+
+__device__ __forceinline__ void add_counters(unsigned long long *counters, const PathCounters &pc) {
+    if (!counters) return;
+    atomicAdd(counters + 0, (unsigned long long)pc.rays);
+    atomicAdd(counters + 1, (unsigned long long)pc.nodes);
+    atomicAdd(counters + 2, (unsigned long long)pc.tris);
+}
+
+template <int NS, bool RGB>
+__global__ void __launch_bounds__(64) initial_kernel(SceneView sc, CameraView cam, RestirView v,
+                                                     unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= v.width || py >= v.height) return;
+    PathCounters pc{0, 0, 0};
+    restir_initial<NS, RGB>(sc, cam, v, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
+    add_counters(counters, pc);
+}
+
+template <int NS, bool RGB>
+__global__ void __launch_bounds__(64) spatial_kernel(SceneView sc, RestirView v, uint32_t in, uint32_t out,
+                                                     uint32_t dim) {
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= v.width || py >= v.height) return;
+    restir_spatial_pass<NS, RGB>(sc, v, px, py, in, out, dim);
+}
+
+// shader1 plus the film's running mean (RealTimeIntegrator's combine,
+// rt.cpp:121-130, with the indirect half absent)
+template <int NS, bool RGB>
+__global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam, RestirView v, RenderView rv,
+                                                   uint32_t in, float4 *__restrict__ rt, float4 *__restrict__ out,
+                                                   unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= v.width || py >= v.height) return;
+    PathCounters pc{0, 0, 0};
+    f3 L = restir_final<NS, RGB>(sc, cam, v, px, py, in, lds_stack + threadIdx.x, blockDim.x, pc);
+    uint32_t idx = py * v.width + px;
+    float4 acc = film_add(rv, rt[idx], L, v.frame);
+    rt[idx] = acc;
+    if (rv.write_output) out[idx] = film_output(rv, acc);
+    add_counters(counters, pc);
+}
+
+dim3 tiles(const RestirView &v) { return dim3((v.width + 7u) / 8u, (v.height + 7u) / 8u); }
+
+}// namespace
+
+hipError_t restir_clear(const RestirView &v, hipStream_t stream) {
+    if (v.pixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(clear_kernel, dim3((v.pixels + 255u) / 256u), dim3(256), 0, stream, v);
+    return hipGetLastError();
+}
+
+hipError_t restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &v, uint32_t spectrum_dim,
+                          bool rgb, unsigned long long *counters, hipStream_t stream) {
+    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+    if (rgb) hipLaunchKernelGGL((initial_kernel<3, true>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
+    else if (spectrum_dim == 4)
+        hipLaunchKernelGGL((initial_kernel<4, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
+    else if (spectrum_dim == 3)
+        hipLaunchKernelGGL((initial_kernel<3, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb, uint32_t in,
+                          uint32_t out, uint32_t dim, hipStream_t stream) {
+    if (in >= kRestirReservoirBuffers || out >= kRestirReservoirBuffers || in < 2u || out < 2u || in == out)
+        return hipErrorInvalidValue;
+    if (rgb) hipLaunchKernelGGL((spatial_kernel<3, true>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
+    else if (spectrum_dim == 4)
+        hipLaunchKernelGGL((spatial_kernel<4, false>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
+    else if (spectrum_dim == 3)
+        hipLaunchKernelGGL((spatial_kernel<3, false>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t restir_final(const SceneView &sc, const CameraView &cam, const RestirView &v, const RenderView &rv,
+                        uint32_t spectrum_dim, bool rgb, uint32_t in, float4 *rt, float4 *out,
+                        unsigned long long *counters, hipStream_t stream) {
+    if (in < 2u || in >= kRestirReservoirBuffers) return hipErrorInvalidValue;
+    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+    if (rgb)
+        hipLaunchKernelGGL((final_kernel<3, true>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
+                           counters);
+    else if (spectrum_dim == 4)
+        hipLaunchKernelGGL((final_kernel<4, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
+                           counters);
+    else if (spectrum_dim == 3)
+        hipLaunchKernelGGL((final_kernel<3, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
+                           counters);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+}// namespace vs

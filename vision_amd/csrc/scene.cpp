@@ -445,6 +445,38 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         s.integrator.mis_mode = p["mis_mode"].as_int(0);
         if (s.integrator.type != "pt" && s.integrator.type != "rt")
             throw std::runtime_error("unsupported integrator type: " + s.integrator.type);
+        if (s.integrator.type == "rt") {
+            RestirHost &r = s.integrator.restir;
+            const Json &di = p["direct"];
+            r.open = di["open"].as_bool(true);
+            r.M_light = di["M_light"].as_uint(10);
+            r.M_bsdf = di["M_bsdf"].as_uint(1);
+            r.max_age = di["max_age"].as_uint(30);
+            r.max_recursion = di["max_recursion"].as_uint(5);
+            r.diff_factor = float(di["diffuse_factor"].as_float(0.3));
+            r.pairwise = di["pairwise"].as_bool(false);
+            const Json &sp = di["spatial"];
+            r.s_theta = float(sp["theta"].as_float(35.0));
+            r.s_depth = float(sp["depth"].as_float(0.3));
+            r.s_radius = float(sp["radius"].as_float(30.0));
+            r.s_num = sp["sample_num"].as_uint(1);
+            r.s_open = sp["open"].as_bool(true);
+            if (!r.s_open) r.s_num = 1;
+            r.extra_spatial_passes = sp["extra_passes"].as_uint(0);
+            const Json &tp = di["temporal"];
+            r.t_limit = tp["history_limit"].as_uint(5);
+            r.t_radius = float(tp["radius"].as_float(8.0));
+            r.t_theta = float(tp["theta"].as_float(35.0));
+            r.t_depth = float(tp["depth"].as_float(0.3));
+            r.t_open = tp["open"].as_bool(true);
+            r.t_N = tp["N"].as_uint(9);
+            r.t_mis = tp["mis"].as_bool(false);
+            if (!r.open)
+                throw std::runtime_error("unsupported: rt integrator with ReSTIR DI closed (the GI half is not built)");
+            if (r.s_num > 64 || r.t_N > 64 || r.M_light > 4096 || r.M_bsdf > 64 || r.max_recursion > 64 ||
+                r.extra_spatial_passes > 8)
+                throw std::runtime_error("ReSTIR DI settings out of range");
+        }
     }
     // spectrum (SpectrumDesc::init; hero dimension default 3, hero.cpp:239)
     {

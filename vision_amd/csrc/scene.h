@@ -75,12 +75,31 @@ struct CameraHost {
     std::string tone_mapper{"linear"};
 };
 
+// ReSTIR DI settings of the "rt" integrator's "direct" block: ReSTIRDI
+// (direct.cpp:10-17), ReSTIR (ReSTIR/common.h:84-99), SpatialResamplingParam
+// and TemporalResamplingParam (common.h:19-67)
+struct RestirHost {
+    bool open{true};
+    uint32_t M_light{10}, M_bsdf{1}, max_age{30}, max_recursion{5};
+    float diff_factor{0.3f};
+    bool pairwise{false};
+    bool s_open{true};
+    float s_theta{35.f}, s_depth{0.3f}, s_radius{30.f};
+    uint32_t s_num{1};
+    bool t_open{true}, t_mis{false};
+    uint32_t t_limit{5}, t_N{9};
+    float t_radius{8.f}, t_theta{35.f}, t_depth{0.3f};
+    // spatial passes run before the reference's one (0 = the reference)
+    uint32_t extra_spatial_passes{0};
+};
+
 struct IntegratorHost {
     std::string type{"pt"};
     uint32_t max_depth{16}, min_depth{5};
     float rr_threshold{1.f};
     int mis_mode{0};
     Json param;
+    RestirHost restir;
 };
 
 struct SceneHost {

@@ -352,3 +352,85 @@ def write_scene(scene: dict, path: str | Path) -> Path:
     path.parent.mkdir(parents=True, exist_ok=True)
     path.write_text(json.dumps(scene, indent=1))
     return path
+
+
+def restir_integrator(M_light: int = 10, M_bsdf: int = 1, spatial: bool = True, sample_num: int = 1,
+                      radius: float = 30.0, temporal: bool = True, history_limit: int = 5, pairwise: bool = False,
+                      temporal_mis: bool = False, extra_spatial_passes: int = 0, **extra) -> dict:
+    """The "rt" integrator with its ReSTIR DI ("direct") block
+    (src/render_core/integrators/rt.cpp, ReSTIR/direct.cpp:10-17,
+    ReSTIR/common.h:19-67)."""
+    direct = {"M_light": M_light, "M_bsdf": M_bsdf, "pairwise": pairwise,
+              "spatial": {"open": spatial, "sample_num": sample_num, "radius": radius, "theta": 35, "depth": 0.3,
+                          "extra_passes": extra_spatial_passes},
+              "temporal": {"open": temporal, "history_limit": history_limit, "radius": 8, "theta": 35,
+                           "depth": 0.3, "mis": temporal_mis}}
+    direct.update(extra)
+    return {"type": "rt", "param": {"direct": direct, "max_depth": 2, "min_depth": 5, "rr_threshold": 1}}
+
+
+def many_lights_box(width: int = 1920, height: int = 1080, n_quads: int = 5000, n_points: int = 256,
+                    seed: int = 0, M_light: int = 32, M_bsdf: int = 1, sample_num: int = 5, radius: float = 30.0,
+                    history_limit: int = 20, extra_spatial_passes: int = 1) -> dict:
+    """The Cornell-box geometry lit by many lights for ReSTIR DI: `n_quads`
+    small one-sided emissive quads (2 triangles each, radiance log-uniform in
+    [1, 100], random colour and orientation) and `n_points` point and spot
+    lights, chosen by the power light sampler; the ceiling light is removed.
+    Defaults: 10 000 emissive triangles and 256 point/spot lights, 32 light
+    candidates, temporal history 20, 5 spatial neighbours within 30 px in two
+    passes."""
+    rng = np.random.default_rng(seed)
+    base = cornell_box(width, height)
+    shapes = [s for s in base["shapes"] if s["name"] != "light"]
+    for i in range(n_quads):
+        pos = rng.uniform([-0.9, 0.15, -0.9], [0.9, 1.9, 0.9])
+        size = rng.uniform(0.02, 0.06)
+        radiance = float(np.exp(rng.uniform(np.log(1.0), np.log(100.0))))
+        tint = rng.uniform(0.3, 1.0, 3)
+        tint = tint / tint.max() * radiance
+        m = translate(*pos) @ rot_y(rng.uniform(0, 360)) @ rot_x(rng.uniform(0, 360)) @ scale(size, 1, size)
+        shapes.append({"type": "quad", "name": f"emitter{i}",
+                       "param": {"material": "Light", "transform": xform(m),
+                                 "emission": {"type": "area",
+                                              "param": {"color": [float(c) for c in tint], "two_sided": False,
+                                                        "scale": 1}}}})
+    lights = []
+    for i in range(n_points):
+        pos = [float(v) for v in rng.uniform([-0.85, 0.2, -0.85], [0.85, 1.95, 0.85])]
+        power = float(np.exp(rng.uniform(np.log(0.05), np.log(2.0))))
+        color = [float(c) for c in rng.uniform(0.3, 1.0, 3)]
+        if i % 2 == 0:
+            lights.append({"type": "point", "param": {"position": pos, "color": color, "scale": power}})
+        else:
+            d = rng.normal(size=3)
+            d[1] = -abs(d[1]) - 0.5
+            lights.append({"type": "spot", "param": {"position": pos, "direction": [float(v) for v in d],
+                                                     "angle": float(rng.uniform(20, 60)), "falloff": 10,
+                                                     "color": color, "scale": power * 4}})
+    base["shapes"] = shapes
+    base["light_sampler"] = {"type": "power", "param": {"lights": lights}}
+    base["integrator"] = restir_integrator(M_light=M_light, M_bsdf=M_bsdf, sample_num=sample_num, radius=radius,
+                                           history_limit=history_limit,
+                                           extra_spatial_passes=extra_spatial_passes)
+    return base
+
+
+def jitter_path(frames: int, position=(0, 1, 6.8), target=(0, 1, 0), amplitude: float = 0.01,
+                seed: int = 0) -> list:
+    """Camera-to-world matrices (4x4, column vectors) of a seeded jitter path:
+    the look-at camera of the Cornell box with its position displaced by
+    N(0, amplitude) per axis each frame."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(frames):
+        p = np.asarray(position, np.float64) + rng.normal(scale=amplitude, size=3)
+        f = np.asarray(target, np.float64) - p
+        f /= np.linalg.norm(f)
+        r = np.cross(f, [0.0, 1.0, 0.0])
+        r /= np.linalg.norm(r)
+        u = np.cross(r, f)
+        m = np.eye(4)
+        # the columns the scene loader's look_at builds: right, up, forward, position
+        m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = r, u, f, p
+        out.append(m)
+    return out
