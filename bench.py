@@ -13,14 +13,23 @@ Workloads:
 Scene, BVH and film are resident in HBM before the timed region.  Prints one
 JSON line.
 
-The roofline entry is for the dominant kernel, with algorithmic bytes from a
-counting run of the same job (BVH8 node fetches x 80 B + leaf-triangle
-fetches x 48 B + rays x 48 B ray in / hit out, plus the kernel's own record
-traffic) divided by the kernel's HIP-event time in the timed region:
-  megakernel PT: the PT megakernel (+ 32 B film read / write per pixel);
-  wavefront PT: the closest-hit traversal kernel;
-  ReSTIR: shader0 (+ 48 B surface and 48 B reservoir written, the previous
-  frame's 48 B surface and 48 B reservoir read, per pixel).
+The roofline entry is for the dominant kernel (the one with the most
+HIP-event time in the timed region), with algorithmic bytes from a counting
+run of the same job divided by that kernel's event time.  Traversal bytes
+per ray: BVH8 node fetches x 80 B + leaf-triangle fetches x 48 B + 48 B ray
+in / hit out.  Per-unit record bytes (the records the kernel reads and
+writes, cache hits included):
+  megakernel PT: traversal + 32 B film read / write per pixel;
+  wavefront closest-hit: traversal;
+  wavefront shading: SHADE_BYTES per shaded path (path state in / out,
+      hit, queues, triangle, vertices, instance, material, light sample)
+      + 48 B per shadow ray written;
+  wavefront any-hit: traversal + 32 B per shadow ray (contribution, radiance);
+  ReSTIR shader0: traversal + 4 x 48 B records (surface and reservoir
+      written, previous frame's read) + 124 B surface gathers + 128 B
+      material + CANDIDATE_BYTES per light candidate, per pixel;
+  ReSTIR shader1: traversal + records (pass-through, current reservoir,
+      own and neighbour surfaces and reservoirs) + film, per pixel.
 """
 from __future__ import annotations
 
@@ -37,6 +46,11 @@ sys.path.insert(0, str(ROOT))
 METRIC = "Msamples/s on 1 MI355X at relMSE≤1e-3 (batched vs plain); p50/p99 job latency"
 HBM_PEAK_GBS = 8000.0
 NODE_BYTES, TRI_BYTES, RAY_BYTES, FILM_BYTES = 80, 48, 48, 32
+RECORD_BYTES = 48      # ReSTIR surface / reservoir record
+SURFACE_GATHER = 124   # triangle indices 12 + positions 36 + normals 36 + uvs 24 + instance 16
+MATERIAL_BYTES = 128
+CANDIDATE_BYTES = 184  # light selection alias 8 + func 4, light record 112, triangle alias 12, triangle 48
+SHADE_BYTES = 240 + 16 + 8 + SURFACE_GATHER + MATERIAL_BYTES + CANDIDATE_BYTES
 
 
 SCENE_CACHE = ROOT / "vision_amd" / "_scenes"
@@ -132,7 +146,19 @@ def main():
     ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--chunk-frames", type=int, default=16)
     ap.add_argument("--trace-block", type=int, default=128)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only copy 256 MiB device to device three times (the reference point that scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        x = torch.ones(64 * 1024 * 1024, device="cuda", dtype=torch.float32)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": x.numel() * 4, "copies": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU scaling is not part of it", file=sys.stderr)
         sys.exit(2)
@@ -185,24 +211,40 @@ def main():
     sc.set_counting(True)
     cst = step()
     sc.set_counting(False)
-    trav = lambda rays, nodes, tris: NODE_BYTES * nodes + TRI_BYTES * tris + RAY_BYTES * rays
+    trav = lambda k: (NODE_BYTES * cst["stage_node_fetches"][k] + TRI_BYTES * cst["stage_tri_fetches"][k] +
+                      RAY_BYTES * cst["stage_rays"][k])
     avg_kernel_s = sum(s["kernel_ms"] for s in stats) / len(stats) / 1e3
     staged = restir or args.kernel == "wavefront"
     if staged:
-        stage_s = sum(s["stage_ms"][0] for s in stats) / len(stats) / 1e3
-        launches = cst["stage_launches"][0]
-        alg_bytes = trav(cst["stage_rays"][0], cst["stage_node_fetches"][0], cst["stage_tri_fetches"][0])
+        stage_s = [sum(s["stage_ms"][k] for s in stats) / len(stats) / 1e3 for k in range(3)]
+        k = max(range(3), key=lambda i: stage_s[i])
+        px = W * H * spp
         if restir:
-            alg_bytes += 4 * 48 * W * H * spp
-            kernel = "initial_kernel (shader0: candidates, visibility, temporal reuse)"
+            r = scene_desc["integrator"]["param"]["direct"]
+            n_sp = r["spatial"]["sample_num"]
+            names = ["initial_kernel (shader0: candidates, visibility, temporal reuse)",
+                     "spatial_kernel (extra spatial pass)", "final_kernel (shader1: spatial reuse, shading)"]
+            unit = [4 * RECORD_BYTES + SURFACE_GATHER + MATERIAL_BYTES + r["M_light"] * CANDIDATE_BYTES,
+                    (2 + 2 * n_sp) * RECORD_BYTES + n_sp * RECORD_BYTES,
+                    (3 + 3 * n_sp) * RECORD_BYTES + SURFACE_GATHER + MATERIAL_BYTES + FILM_BYTES]
+            alg_bytes = (trav(k) if k != 1 else 0) + unit[k] * px
         else:
-            kernel = "trace_queue_kernel (wavefront closest-hit traversal)"
+            names = ["trace_queue_kernel (wavefront closest-hit traversal)", "shade_kernel (wavefront shading)",
+                     "shadow_kernel (wavefront any-hit traversal)"]
+            if k == 1:
+                alg_bytes = SHADE_BYTES * cst["stage_rays"][0] + RAY_BYTES * cst["stage_rays"][2]
+            elif k == 0:
+                alg_bytes = trav(0)
+            else:
+                alg_bytes = trav(2) + 32 * cst["stage_rays"][2]
+        kernel, kernel_s, launches = names[k], stage_s[k], cst["stage_launches"][k]
     else:
-        stage_s = avg_kernel_s
+        kernel_s = avg_kernel_s
         launches = cst["launches"]
-        alg_bytes = trav(cst["rays"], cst["node_fetches"], cst["tri_fetches"]) + FILM_BYTES * W * H
+        alg_bytes = (NODE_BYTES * cst["node_fetches"] + TRI_BYTES * cst["tri_fetches"] + RAY_BYTES * cst["rays"] +
+                     FILM_BYTES * W * H)
         kernel = "pt_kernel (megakernel)"
-    achieved = alg_bytes / stage_s / 1e9
+    achieved = alg_bytes / kernel_s / 1e9
     mrays = cst["rays"] / avg_kernel_s / 1e6
 
     line = {
@@ -226,12 +268,12 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
                      "launches_per_step": int(launches),
-                     "avg_launch_ms": round(stage_s / max(launches, 1) * 1e3, 4),
+                     "avg_launch_ms": round(kernel_s / max(launches, 1) * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(alg_bytes / max(launches, 1)),
-                     "kernel_ms_per_step": round(stage_s * 1e3, 3)},
+                     "kernel_ms_per_step": round(kernel_s * 1e3, 3)},
     }
     if staged:
-        line["stages_ms_per_step"] = [round(sum(s["stage_ms"][k] for s in stats) / len(stats), 3) for k in range(2)]
+        line["stages_ms_per_step"] = [round(x * 1e3, 3) for x in stage_s]
     if not args.no_cpu:
         line["cpu_baseline"] = restir_cpu_baseline(sc) if restir else cpu_baseline(sc, spp)
     sc.close()

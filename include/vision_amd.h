@@ -77,14 +77,15 @@ typedef struct vs_render_stats {
     uint64_t tri_fetches;  /* leaf triangle fetches (counting runs) */
     uint32_t launches;     /* kernel launches of the last call */
     uint32_t samples_per_pixel;
-    /* two kernels timed on their own (HIP events), with their launch counts
-     * and (counting runs) their rays / node / triangle fetches: in the
-     * wavefront form [0] closest-hit traversal, [1] any-hit traversal; with
-     * the "rt" integrator [0] shader0 (candidates, visibility, temporal
-     * reuse), [1] shader1 (spatial reuse, shading); zeros otherwise */
-    double stage_ms[2];
-    uint32_t stage_launches[2];
-    uint64_t stage_rays[2], stage_node_fetches[2], stage_tri_fetches[2];
+    /* three kernels timed on their own (HIP events), with their launch
+     * counts and (counting runs) their rays / node / triangle fetches: in
+     * the wavefront form [0] closest-hit traversal, [1] shading, [2] any-hit
+     * traversal; with the "rt" integrator [0] shader0 (candidates,
+     * visibility, temporal reuse), [1] the extra spatial passes, [2] shader1
+     * (spatial reuse, shading); zeros otherwise */
+    double stage_ms[3];
+    uint32_t stage_launches[3];
+    uint64_t stage_rays[3], stage_node_fetches[3], stage_tri_fetches[3];
 } vs_render_stats;
 
 /* Flattened world-space scene handed to tests and the CPU oracle.

@@ -163,7 +163,7 @@ extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *fil
 }
 
 // ---- ReSTIR DI frame of the device code (restir.hip's three kernels, in order)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, const float *prev_pos, uint32_t frame,
                          float *rsv, float *surf, float *ext, float *film, const float *render) {
     RestirView v{};
@@ -197,11 +197,11 @@ static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, cons
     std::vector<uint2> stack(64);
     PathCounters pc{0, 0, 0};
     for (uint32_t y = 0; y < v.height; ++y)
-        for (uint32_t x = 0; x < v.width; ++x) restir_initial<NS, RGB>(s->v, s->cam, v, x, y, stack.data(), 1, pc);
+        for (uint32_t x = 0; x < v.width; ++x) restir_initial<NS, RGB, MS>(s->v, s->cam, v, x, y, stack.data(), 1, pc);
     uint32_t in = 2, extra = uint32_t(e[19]);
     for (uint32_t k = 0; k < extra; ++k) {
         for (uint32_t y = 0; y < v.height; ++y)
-            for (uint32_t x = 0; x < v.width; ++x) restir_spatial_pass<NS, RGB>(s->v, v, x, y, in, in ^ 1u, 2u + k);
+            for (uint32_t x = 0; x < v.width; ++x) restir_spatial_pass<NS, RGB, MS>(s->v, v, x, y, in, in ^ 1u, 2u + k);
         in ^= 1u;
     }
     RenderView rv{};
@@ -209,7 +209,7 @@ static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, cons
     float4 *f4 = reinterpret_cast<float4 *>(film);
     for (uint32_t y = 0; y < v.height; ++y)
         for (uint32_t x = 0; x < v.width; ++x) {
-            f3 L = restir_final<NS, RGB>(s->v, s->cam, v, x, y, in, stack.data(), 1, pc);
+            f3 L = restir_final<NS, RGB, MS>(s->v, s->cam, v, x, y, in, stack.data(), 1, pc);
             f4[y * v.width + x] = film_add(rv, f4[y * v.width + x], L, frame);
         }
 }
@@ -217,7 +217,17 @@ static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, cons
 extern "C" void dc_restir_frame(void *p, const float *restir, const float *prev_w2c, const float *prev_pos,
                                 uint32_t frame, float *rsv, float *surf, float *ext, float *film, const float *render) {
     auto *s = static_cast<DcScene *>(p);
-    if (s->pk.rgb) restir_frame<3, true>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-    else if (s->h.spectrum_dim == 4) restir_frame<4, false>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-    else restir_frame<3, false>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+    // the material set the library picks (diffuse family only, or all)
+    bool diffuse_only = true;
+    for (const MaterialRecord &m : s->pk.mats)
+        if (m.type != MAT_DIFFUSE && m.type != MAT_OREN_NAYAR && m.type != MAT_BLACK) diffuse_only = false;
+    if (diffuse_only) {
+        if (s->pk.rgb) restir_frame<3, true, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        else restir_frame<3, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+    } else {
+        if (s->pk.rgb) restir_frame<3, true, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        else restir_frame<3, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+    }
 }

@@ -61,8 +61,8 @@ class SceneInfo(C.Structure):
 class RenderStats(C.Structure):
     _fields_ = [("kernel_ms", C.c_double), ("rays", C.c_uint64), ("node_fetches", C.c_uint64),
                 ("tri_fetches", C.c_uint64), ("launches", C.c_uint32), ("samples_per_pixel", C.c_uint32),
-                ("stage_ms", C.c_double * 2), ("stage_launches", C.c_uint32 * 2), ("stage_rays", C.c_uint64 * 2),
-                ("stage_node_fetches", C.c_uint64 * 2), ("stage_tri_fetches", C.c_uint64 * 2)]
+                ("stage_ms", C.c_double * 3), ("stage_launches", C.c_uint32 * 3), ("stage_rays", C.c_uint64 * 3),
+                ("stage_node_fetches", C.c_uint64 * 3), ("stage_tri_fetches", C.c_uint64 * 3)]
 
 
 class SceneArrays(C.Structure):

@@ -744,13 +744,14 @@ uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, 
             launches += 4;
             hip_check(hipMemcpyAsync(s->wf_host_counts, counts, 8, hipMemcpyDeviceToHost, st), "wf counts");
             hip_check(hipStreamSynchronize(st), "wf sync");
-            float t_trace = 0.f, t_shadow = 0.f;
+            float t_trace = 0.f, t_shade = 0.f, t_shadow = 0.f;
             hip_check(hipEventElapsedTime(&t_trace, s->ctx->tr0, s->ctx->tr1), "event time");
+            hip_check(hipEventElapsedTime(&t_shade, s->ctx->tr1, s->ctx->tr2), "event time");
             hip_check(hipEventElapsedTime(&t_shadow, s->ctx->tr2, s->ctx->ev1), "event time");
             ts.stage_ms[0] += t_trace;
-            ts.stage_ms[1] += t_shadow;
-            ts.stage_launches[0] += 1;
-            ts.stage_launches[1] += 1;
+            ts.stage_ms[1] += t_shade;
+            ts.stage_ms[2] += t_shadow;
+            for (int k = 0; k < 3; ++k) ts.stage_launches[k] += 1;
             count = s->wf_host_counts[vs::WF_LIVE];
             if (count > n) throw std::logic_error("wavefront queue overflow");
             cur ^= 1;
@@ -817,30 +818,37 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
     const bool rgb = s->packed.rgb;
     const uint32_t dim = rv.spectrum_dim;
     const uint32_t extra = s->host.integrator.restir.extra_spatial_passes;
+    // the diffuse-family kernels when no material needs more
+    int ms = 1;
+    for (const vs::MaterialRecord &m : s->packed.mats)
+        if (m.type != vs::MAT_DIFFUSE && m.type != vs::MAT_OREN_NAYAR && m.type != vs::MAT_BLACK) ms = 0;
     uint32_t launches = 0;
     for (uint32_t f = 0; f < spp; ++f) {
         vs::RestirView v = restir_view(s, s->frame_index + f);
         hip_check(hipEventRecord(s->ctx->tr0, st), "event");
-        hip_check(vs::restir_initial(s->view, s->cam, v, dim, rgb, ctr, st), "restir initial");
+        hip_check(vs::restir_initial(s->view, s->cam, v, dim, rgb, ms, ctr, st), "restir initial");
         hip_check(hipEventRecord(s->ctx->tr1, st), "event");
         uint32_t in = 2u;
         for (uint32_t k = 0; k < extra; ++k) {
-            hip_check(vs::restir_spatial(s->view, v, dim, rgb, in, in ^ 1u, 2u + k, st), "restir spatial");
+            hip_check(vs::restir_spatial(s->view, v, dim, rgb, ms, in, in ^ 1u, 2u + k, st), "restir spatial");
             in ^= 1u;
         }
         hip_check(hipEventRecord(s->ctx->tr2, st), "event");
-        hip_check(vs::restir_final(s->view, s->cam, v, rv, dim, rgb, in, s->film_rt.as<float4>(),
+        hip_check(vs::restir_final(s->view, s->cam, v, rv, dim, rgb, ms, in, s->film_rt.as<float4>(),
                                    s->film_out.as<float4>(), ctr ? ctr + 4 : nullptr, st),
                   "restir final");
         hip_check(hipEventRecord(s->ctx->ev1, st), "event");
         hip_check(hipEventSynchronize(s->ctx->ev1), "restir sync");
-        float t0 = 0.f, t1 = 0.f;
+        float t0 = 0.f, t1 = 0.f, t2 = 0.f;
         hip_check(hipEventElapsedTime(&t0, s->ctx->tr0, s->ctx->tr1), "event time");
-        hip_check(hipEventElapsedTime(&t1, s->ctx->tr2, s->ctx->ev1), "event time");
+        hip_check(hipEventElapsedTime(&t1, s->ctx->tr1, s->ctx->tr2), "event time");
+        hip_check(hipEventElapsedTime(&t2, s->ctx->tr2, s->ctx->ev1), "event time");
         ts.stage_ms[0] += t0;
         ts.stage_ms[1] += t1;
+        ts.stage_ms[2] += t2;
         ts.stage_launches[0] += 1;
-        ts.stage_launches[1] += 1;
+        ts.stage_launches[1] += extra;
+        ts.stage_launches[2] += 1;
         launches += 2u + extra;
         std::memcpy(s->prev_c2w, s->cam.c2w, sizeof(s->prev_c2w));
     }
@@ -927,10 +935,10 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
             s->stats.node_fetches = c[1] + (staged ? c[5] : 0ull);
             s->stats.tri_fetches = c[2] + (staged ? c[6] : 0ull);
             if (staged) {
-                for (int k = 0; k < 2; ++k) {
-                    s->stats.stage_rays[k] = c[4 * k];
-                    s->stats.stage_node_fetches[k] = c[4 * k + 1];
-                    s->stats.stage_tri_fetches[k] = c[4 * k + 2];
+                for (int k = 0; k < 2; ++k) {// the first and the last stage trace rays
+                    s->stats.stage_rays[2 * k] = c[4 * k];
+                    s->stats.stage_node_fetches[2 * k] = c[4 * k + 1];
+                    s->stats.stage_tri_fetches[2 * k] = c[4 * k + 2];
                 }
             }
         }

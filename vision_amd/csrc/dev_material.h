@@ -23,6 +23,17 @@ namespace dev {
 // A colour slot holds linear sRGB in sRGB mode and the sigmoid-polynomial
 // coefficients of the RGB albedo spectrum in hero mode.
 constexpr uint32_t MATF_DISPERSIVE = 1u;
+// material sets a kernel is compiled for: every material, or one family
+// (the diffuse family: Lambert, Oren-Nayar, black body; metal; glass;
+// principled) for scenes or queues that hold nothing else, which keeps the
+// evaluator small.  The black body is handled in every set.
+constexpr int MS_ALL = 0;
+constexpr int MS_DIFFUSE = 1;
+constexpr int MS_METAL = 2;
+constexpr int MS_GLASS = 3;
+constexpr int MS_PRINCIPLED = 4;
+template <int MS>
+__device__ __forceinline__ constexpr bool ms_has(int family) { return MS == MS_ALL || MS == family; }
 constexpr uint32_t FLAG_TRANSMISSION = 4u;// BxDFFlag::Transmission
 
 template <int NS, bool RGB>
@@ -94,7 +105,7 @@ __device__ __forceinline__ float pure_reflection_comp(const float *lut, float ax
 // Build the evaluator for material `m` at `it` (create_lobe_set of each
 // material).  `dispersive` reports a wavelength-dependent ior (glass with an
 // ior curve), on which the caller drops the secondary wavelengths.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS = MS_ALL>
 __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const MaterialRecord &m, const Frame &fr,
                                                     f3 ng, f3 wo_w, const Swl<NS> &swl, bool &dispersive) {
     MatCtx<NS> c;
@@ -105,7 +116,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
     c.wo = fr.to_local(wo_w);
     dispersive = false;
     const float *p = m.p;
-    if (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR) {
+    if (ms_has<MS>(MS_DIFFUSE) && (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR)) {
         c.a0 = eval_albedo<NS, RGB>(p, swl);
         if (m.type == MAT_OREN_NAYAR) {
             float sig = p[4] * (kPi * 0.5f);
@@ -113,7 +124,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
             c.onA = 1.f - (s2 / (2.f * (s2 + 0.33f)));
             c.onB = 0.45f * s2 / (s2 + 0.09f);
         }
-    } else if (m.type == MAT_METAL) {
+    } else if (ms_has<MS>(MS_METAL) && m.type == MAT_METAL) {
         c.ggx = {p[6], p[7]};
         if (RGB) {
 #pragma unroll
@@ -131,7 +142,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
             }
         }
         c.comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
-    } else if (m.type == MAT_GLASS) {
+    } else if (ms_has<MS>(MS_GLASS) && m.type == MAT_GLASS) {
         c.ggx = {p[5], p[6]};
         c.a0 = eval_albedo<NS, RGB>(p, swl);
         bool front = dot(wo_w, ng) > 0.f;// Interaction::correct_eta
@@ -149,7 +160,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
         }
         c.eta0 = c.a1.v[0];
         c.comp = dielectric_comp(sc.lut, c.ggx.ax, c.ggx.ay, fabsf(c.wo.z), c.eta0);
-    } else if (m.type == MAT_PRINCIPLED) {
+    } else if (ms_has<MS>(MS_PRINCIPLED) && m.type == MAT_PRINCIPLED) {
         // PrincipledBSDF::create_lobe_set (principled_bsdf.cpp:352-461)
         Spec<NS> color = eval_albedo<NS, RGB>(p, swl);
         Spec<NS> spec_tint = eval_albedo<NS, RGB>(p + 4, swl);
@@ -341,20 +352,20 @@ __device__ __forceinline__ void eval_lambert(const Spec<NS> &kr, f3 wo, f3 wi, f
 
 // full evaluation in world space: f * |cos wi|, pdf, with the geometric
 // hemisphere check of MaterialEvaluator::evaluate
-template <int NS>
+template <int NS, int MS = MS_ALL>
 __device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, float *eta_out) {
     EvalN<NS> e;
     f3 wo = c.wo, wi = c.fr.to_local(wi_w);
-    if (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR) {
+    if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
         eval_lambert(c.a0, wo, wi, c.onA, c.onB, c.type == MAT_OREN_NAYAR, e);
         e.f = e.f * fabsf(wi.z);
-    } else if (c.type == MAT_METAL) {
+    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
         eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), c.comp, e);
         e.f = e.f * fabsf(wi.z);
-    } else if (c.type == MAT_GLASS) {
+    } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
         eval_dielectric(c.ggx, wo, wi, c.a1, c.eta0, false, c.a1, c.a0, c.comp, e, eta_out);
         e.f = e.f * fabsf(wi.z);
-    } else if (c.type == MAT_PRINCIPLED) {
+    } else if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
         // LobeSet::evaluate_impl (lobe.cpp:668-688): each lobe weighted, its pdf
         // scaled by its sample weight, both masked by the lobe's hemisphere factor
         float acos = fabsf(wi.z);
@@ -426,7 +437,7 @@ __device__ __forceinline__ bool sample_dielectric_dir(const Ggx &g, f3 wo, const
     return ok && !(wo.z * wi.z > 0.f);
 }
 
-template <int NS>
+template <int NS, int MS = MS_ALL>
 __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &sp) {
     SampleN<NS> s;
     s.eta = 1.f;
@@ -442,16 +453,16 @@ __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &
         if (!same_geo) s.e.pdf = 0.f;
         return s;
     }
-    if (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR) {
+    if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
         wi = square_to_cosine_hemisphere(sp.next2d());
         if (wo.z < 0.f) wi.z = -wi.z;
-    } else if (c.type == MAT_METAL) {
+    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
         f3 wh = c.ggx.sample_wh(wo, sp.next2d());
         wi = reflect_dir(wo, wh);
         valid = wo.z * wi.z > 0.f;
-    } else if (c.type == MAT_GLASS) {
+    } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
         valid = sample_dielectric_dir(c.ggx, wo, c.a1, c.eta0, false, c.a1, c.a0, sp, wi);
-    } else {
+    } else if (ms_has<MS>(MS_PRINCIPLED)) {
         // LobeSet::sample_wi_impl (lobe.cpp:597-627): uc 1 + an unused 2D draw,
         // then the chosen lobe's draws
         float uc = sp.next1d();
@@ -480,7 +491,7 @@ __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &
         }
     }
     s.wi = c.fr.to_world(wi);
-    s.e = mat_eval(c, s.wi, &s.eta);
+    s.e = mat_eval<NS, MS>(c, s.wi, &s.eta);
     if (!valid) s.e.pdf = 0.f;
     return s;
 }

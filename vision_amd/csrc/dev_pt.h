@@ -324,7 +324,7 @@ struct ShadowRay {
 
 // one bounce of IlluminationIntegrator::Li (src/base/integrator.cpp:150-273)
 // given the closest hit of ps.o + t ps.d
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS = MS_ALL>
 __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<NS> &ps, const TraceHit &h,
                            ShadowRay &sr) {
     sr.valid = 0u;
@@ -371,7 +371,7 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
     // next-event estimation (direct_lighting, integrator.cpp:22-38)
     LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, ps.sampler, ps.swl);
     bool dispersive;
-    MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, ps.swl, dispersive);
+    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, ps.swl, dispersive);
     if (!RGB && dispersive) {
         // SampledWavelengths::check_dispersive (spectrum.cpp:21-39)
         bool secondary_valid = NS > 1;
@@ -383,8 +383,8 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
         }
     }
     f3 wi_l = normalize(ls.p_light - it.pos);
-    EvalN<NS> se = mat_eval(mc, wi_l, nullptr);
-    SampleN<NS> bs = mat_sample(mc, ps.sampler);
+    EvalN<NS> se = mat_eval<NS, MS>(mc, wi_l, nullptr);
+    SampleN<NS> bs = mat_sample<NS, MS>(mc, ps.sampler);
     bool delta = ls.pdf < 0.f;
     float w = delta ? 1.f : mis_weight(ls.pdf, se.pdf);
     if (rv.mis_mode == 1) w = 1.f;

@@ -251,7 +251,7 @@ __device__ __forceinline__ void collapse_if_dispersive(Swl<NS> &swl, bool disper
     }
 }
 
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ MatCtx<NS> material_at(const SceneView &sc, const Surface &it, Swl<NS> &swl) {
     uint32_t m = sc.inst[it.inst].mat;
     if (m == kInvalid) {
@@ -264,7 +264,7 @@ __device__ __forceinline__ MatCtx<NS> material_at(const SceneView &sc, const Sur
         return c;
     }
     bool dispersive;
-    MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[m], it.frame, it.ng, it.wo, swl, dispersive);
+    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[m], it.frame, it.ng, it.wo, swl, dispersive);
     collapse_if_dispersive<NS, RGB>(swl, dispersive);
     return mc;
 }
@@ -388,7 +388,7 @@ __device__ __forceinline__ float light_pdf_point(const SceneView &sc, f3 ref, co
 // p_hat = luminance of f * L for a sample seen from `it` (ReSTIRDI::Li from
 // the light, direct.cpp:123-159, and compute_p_hat, direct.h:128-134).  An
 // empty sample evaluates with zero radiance from the origin.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Spec<NS> light_contribution(const SceneView &sc, const Surface &it, const MatCtx<NS> &mc,
                                                        const DISample &s, const Swl<NS> &swl, float *eval_pdf) {
     PointEval<NS> le;
@@ -400,14 +400,14 @@ __device__ __forceinline__ Spec<NS> light_contribution(const SceneView &sc, cons
         le.p_light = mk3(0.f, 0.f, 0.f);
     }
     f3 wi = normalize(le.p_light - it.pos);
-    EvalN<NS> e = mat_eval(mc, wi, nullptr);
+    EvalN<NS> e = mat_eval<NS, MS>(mc, wi, nullptr);
     if (eval_pdf) *eval_pdf = e.pdf;
     return e.f * le.L;
 }
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ float p_hat_of(const SceneView &sc, const Surface &it, const MatCtx<NS> &mc,
                                           const DISample &s, const Swl<NS> &swl) {
-    return luminance_of(light_contribution<NS, RGB>(sc, it, mc, s, swl, nullptr));
+    return luminance_of(light_contribution<NS, RGB, MS>(sc, it, mc, s, swl, nullptr));
 }
 
 // ---- validity of a neighbour (is_valid_neighbor, ReSTIR/common.h:72-80)
@@ -479,12 +479,17 @@ __device__ __forceinline__ bool occluded(const SceneView &sc, const Surface &it,
 // candidate; `s` becomes the emitter point it reaches (area emitter or
 // environment), `pdf` the BSDF pdf in the light's point measure and
 // `light_pdf` the light's own point-measure pdf
-template <int NS, bool RGB>
-__device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Surface &it, const MatCtx<NS> &mc,
-                                                   Sampler &sp, const Swl<NS> &swl, DISample &s, float &pdf,
-                                                   float &light_pdf, uint2 *stack, uint32_t stride,
-                                                   PathCounters &pc) {
-    SampleN<NS> bs = mat_sample(mc, sp);
+// The evaluator is rebuilt here (the same arithmetic as the caller's), so
+// that no evaluator stays live across the traversal below.
+template <int NS, bool RGB, int MS>
+__device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Surface &it, Sampler &sp,
+                                                   Swl<NS> &swl, DISample &s, float &pdf, float &light_pdf,
+                                                   uint2 *stack, uint32_t stride, PathCounters &pc) {
+    SampleN<NS> bs;
+    {
+        MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);
+        bs = mat_sample<NS, MS>(mc, sp);
+    }
     f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
     TraceHit h = traverse<false>(sc, offset_ray_origin(it.pos, n), bs.wi, 0.f, 3.402823466e38f, stack, stride,
                                  pc.nodes, pc.tris);
@@ -533,20 +538,21 @@ __device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Su
 // lights 1 / M_light), then M_bsdf BSDF candidates, each weighted by the
 // luminance of the path throughput.  Draws per light candidate: 3 + 1, per
 // BSDF candidate: the BSDF's + 1.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Reservoir ris(const SceneView &sc, const RestirView &rv, const Surface &it, Sampler &sp,
                                          Swl<NS> &swl, f3 throughput, uint2 *stack, uint32_t stride,
                                          PathCounters &pc) {
     Reservoir r = empty_reservoir();
     if (rv.M_light + rv.M_bsdf == 0u) return r;
-    MatCtx<NS> mc = material_at<NS, RGB>(sc, it, swl);
     const float lum = luminance(throughput);
     const float Ml = float(rv.M_light), Mb = float(rv.M_bsdf);
+    if (rv.M_light > 0u) {
+    MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);
     for (uint32_t i = 0; i < rv.M_light; ++i) {
         DISample s = light_sample_only(sc, sp);
         PointEval<NS> le = light_eval_point<NS, RGB>(sc, it.pos, s, swl);
         f3 wi = normalize(le.p_light - it.pos);
-        EvalN<NS> e = mat_eval(mc, wi, nullptr);
+        EvalN<NS> e = mat_eval<NS, MS>(mc, wi, nullptr);
         Spec<NS> f = e.f * le.L;
         float bsdf_point = light_pdf_point(sc, it.pos, s, e.pdf);
         s.p_hat = luminance_of(f);
@@ -557,10 +563,11 @@ __device__ __forceinline__ Reservoir ris(const SceneView &sc, const RestirView &
         float w = safe_weight(mis, s.p_hat, 1.f / lpdf) * lum;
         r.update(sp.next1d(), s, w, 1.f);
     }
+    }
     for (uint32_t i = 0; i < rv.M_bsdf; ++i) {
         DISample s = empty_sample();
         float pdf, lpdf;
-        Spec<NS> f = bsdf_candidate<NS, RGB>(sc, it, mc, sp, swl, s, pdf, lpdf, stack, stride, pc);
+        Spec<NS> f = bsdf_candidate<NS, RGB, MS>(sc, it, sp, swl, s, pdf, lpdf, stack, stride, pc);
         s.p_hat = luminance_of(f);
         float w = safe_weight(pdf / (Ml * lpdf + Mb * pdf), s.p_hat, 1.f / pdf) * lum;
         r.update(sp.next1d(), s, w, 1.f);
@@ -569,7 +576,7 @@ __device__ __forceinline__ Reservoir ris(const SceneView &sc, const RestirView &
 }
 
 // ReSTIRDI::combine_temporal (direct.cpp:327-380)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Reservoir combine_temporal(const SceneView &sc, const RestirView &rv, const Reservoir &cur,
                                                       const SurfaceRec &cs, Reservoir other, f3 view_pos,
                                                       f3 prev_view_pos, Sampler &sp, Swl<NS> &swl) {
@@ -578,21 +585,21 @@ __device__ __forceinline__ Reservoir combine_temporal(const SceneView &sc, const
     float mis_cur, mis_prev, p_nn;
     if (rv.temporal_mis) {
         it.wo = normalize(prev_view_pos - it.pos);
-        MatCtx<NS> mp = material_at<NS, RGB>(sc, it, swl);
-        float p_cn = p_hat_of<NS, RGB>(sc, it, mp, cur.s, swl);
-        p_nn = p_hat_of<NS, RGB>(sc, it, mp, other.s, swl);
+        MatCtx<NS> mp = material_at<NS, RGB, MS>(sc, it, swl);
+        float p_cn = p_hat_of<NS, RGB, MS>(sc, it, mp, cur.s, swl);
+        p_nn = p_hat_of<NS, RGB, MS>(sc, it, mp, other.s, swl);
         it.wo = normalize(view_pos - it.pos);
-        MatCtx<NS> mc = material_at<NS, RGB>(sc, it, swl);
-        float p_cc = p_hat_of<NS, RGB>(sc, it, mc, cur.s, swl);
-        float p_nc = p_hat_of<NS, RGB>(sc, it, mc, other.s, swl);
+        MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);
+        float p_cc = p_hat_of<NS, RGB, MS>(sc, it, mc, cur.s, swl);
+        float p_nc = p_hat_of<NS, RGB, MS>(sc, it, mc, other.s, swl);
         // MIS_weight_n with the confidences as integers (warp.h:186-192)
         float nc = float(int(cur.C)), no = float(int(other.C));
         mis_cur = (nc * p_cc) / (nc * p_cc + no * p_cn);
         mis_prev = (no * p_nn) / (no * p_nn + nc * p_nc);
     } else {
         it.wo = normalize(prev_view_pos - it.pos);
-        MatCtx<NS> mp = material_at<NS, RGB>(sc, it, swl);
-        p_nn = p_hat_of<NS, RGB>(sc, it, mp, other.s, swl);
+        MatCtx<NS> mp = material_at<NS, RGB, MS>(sc, it, swl);
+        p_nn = p_hat_of<NS, RGB, MS>(sc, it, mp, other.s, swl);
         mis_cur = cur.C / (cur.C + other.C);
         mis_prev = other.C / (other.C + cur.C);
     }
@@ -608,7 +615,7 @@ __device__ __forceinline__ Reservoir combine_temporal(const SceneView &sc, const
 // ReSTIRDI::temporal_reuse (direct.cpp:382-432): the reprojected pixel of
 // the previous frame, or failing that up to temporal.N random pixels within
 // temporal.radius of it (2 draws each)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Reservoir temporal_reuse(const SceneView &sc, const RestirView &rv, Reservoir rsv,
                                                     const SurfaceRec &cs, uint32_t pix, f2 motion, f2 p_film,
                                                     Sampler &sp, Swl<NS> &swl) {
@@ -630,7 +637,7 @@ __device__ __forceinline__ Reservoir temporal_reuse(const SceneView &sc, const R
     SurfaceRec ps = load_surface(rv, pslot, idx);
     if (ps.replaced) prev_view_pos = load_ext(rv, pslot, idx).view_pos;
     if (valid_neighbor(cs, ps, rv.t_dot, rv.t_depth, rv.diff_factor) && pr.s.age < rv.max_age)
-        return combine_temporal<NS, RGB>(sc, rv, rsv, cs, pr, view_pos, prev_view_pos, sp, swl);
+        return combine_temporal<NS, RGB, MS>(sc, rv, rsv, cs, pr, view_pos, prev_view_pos, sp, swl);
     for (uint32_t i = 0; i < rv.temporal_N; ++i) {
         f2 dk = square_to_disk(sp.next2d());
         f2 q = mk2(dk.x * rv.t_radius + prev.x, dk.y * rv.t_radius + prev.y);
@@ -640,13 +647,13 @@ __device__ __forceinline__ Reservoir temporal_reuse(const SceneView &sc, const R
         SurfaceRec as = load_surface(rv, pslot, j);
         if (as.replaced) prev_view_pos = load_ext(rv, pslot, j).view_pos;
         if (valid_neighbor(cs, as, rv.t_dot, rv.t_depth, rv.diff_factor) && ar.s.age < rv.max_age)
-            return combine_temporal<NS, RGB>(sc, rv, rsv, cs, ar, view_pos, prev_view_pos, sp, swl);
+            return combine_temporal<NS, RGB, MS>(sc, rv, rsv, cs, ar, view_pos, prev_view_pos, sp, swl);
     }
     return rsv;
 }
 
 // shader0 for one pixel (direct.cpp:519-555 with compute_hit, :434-517)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
                                uint32_t py, uint2 *stack, uint32_t stride, PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;
@@ -683,11 +690,11 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
         cs.flag = material_flag(sc, mat);
         cs.diff = material_diffuse_factor(sc, mat);
         f3 w_next = mk3(0.f, 0.f, 0.f);
-        if (cs.near_specular()) {
+        if (MS == MS_ALL && cs.near_specular()) {
             // metal and glass only: the materials with enable_delta whose
             // flag can be near-specular (material.h:264)
             bool dispersive;
-            MatCtx<NS> mc = make_material<NS, RGB>(sc, sc.mat[mat], it.frame, it.ng, it.wo, swl, dispersive);
+            MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[mat], it.frame, it.ng, it.wo, swl, dispersive);
             f3 t;
             sample_delta(mc, sp, w_next, t);
             ext.throughput = ext.throughput * t;
@@ -712,7 +719,7 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
     store_surface(rv, cslot, pix, cs);
     if (cs.replaced) store_ext(rv, cslot, pix, ext);
     Reservoir r = empty_reservoir();
-    if (hit) r = ris<NS, RGB>(sc, rv, it, sp, swl, ext.throughput, stack, stride, pc);
+    if (hit) r = ris<NS, RGB, MS>(sc, rv, it, sp, swl, ext.throughput, stack, stride, pc);
     r.update_W(r.s.p_hat);
     r.truncation(1.f);
     // FrameBuffer::compute_motion_vec (frame_buffer.cpp:325-333)
@@ -722,7 +729,7 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
         motion = mk2(p_film.x - pr.x, p_film.y - pr.y);
     }
     if (hit) r.process_occluded(occluded(sc, it, r.s.pos, stack, stride, pc));
-    r = temporal_reuse<NS, RGB>(sc, rv, r, cs, pix, motion, p_film, sp, swl);
+    r = temporal_reuse<NS, RGB, MS>(sc, rv, r, cs, pix, motion, p_film, sp, swl);
     store_reservoir(rv, kPassThrough, pix, r);
 }
 
@@ -740,7 +747,7 @@ __device__ __forceinline__ uint32_t spatial_neighbor(const RestirView &rv, uint3
 // draws (2 each) first, then the combination over the valid ones (1 draw
 // per neighbour, pairwise one more for the canonical sample).  Neighbour
 // reservoirs come from buffer `buf`.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Reservoir spatial_reuse(const SceneView &sc, const RestirView &rv, Reservoir r,
                                                    const SurfaceRec &cs, uint32_t px, uint32_t py, uint32_t buf,
                                                    Sampler &sp, Swl<NS> &swl) {
@@ -760,7 +767,7 @@ __device__ __forceinline__ Reservoir spatial_reuse(const SceneView &sc, const Re
     Surface cit = surface_at(sc, cs.tri, cs.b1, cs.b2, true);
     cit.wo = normalize(view_pos - cit.pos);
     MatCtx<NS> mc;
-    if (count > 0u) mc = material_at<NS, RGB>(sc, cit, swl);
+    if (count > 0u) mc = material_at<NS, RGB, MS>(sc, cit, swl);
     Reservoir ret = empty_reservoir();
     if (!rv.pairwise) {
         const float inv = 1.f / float(count + 1u);
@@ -769,7 +776,7 @@ __device__ __forceinline__ Reservoir spatial_reuse(const SceneView &sc, const Re
             uint32_t j = spatial_neighbor(rv, px, py, replay);
             if (!valid_neighbor(cs, load_surface(rv, cslot, j), rv.s_dot, rv.s_depth, rv.diff_factor)) continue;
             Reservoir nr = load_reservoir(rv, buf, j);
-            nr.s.p_hat = p_hat_of<NS, RGB>(sc, cit, mc, nr.s, swl);
+            nr.s.p_hat = p_hat_of<NS, RGB, MS>(sc, cit, mc, nr.s, swl);
             ret.update(sp.next1d(), nr.s, cal_weight(inv, nr.s.p_hat, nr.W), nr.C);
         }
     } else {
@@ -783,12 +790,12 @@ __device__ __forceinline__ Reservoir spatial_reuse(const SceneView &sc, const Re
             Reservoir nr = load_reservoir(rv, buf, j);
             Surface nit = surface_at(sc, ns.tri, ns.b1, ns.b2, true);
             nit.wo = normalize(view_pos - nit.pos);
-            MatCtx<NS> nm = material_at<NS, RGB>(sc, nit, swl);
+            MatCtx<NS> nm = material_at<NS, RGB, MS>(sc, nit, swl);
             // ReSTIRDI::neighbor_pairwise_MIS (direct.cpp:235-256)
-            float p_cc = p_hat_of<NS, RGB>(sc, cit, mc, r.s, swl);
-            float p_cn = p_hat_of<NS, RGB>(sc, nit, nm, r.s, swl);
-            float p_nn = p_hat_of<NS, RGB>(sc, nit, nm, nr.s, swl);
-            float p_nc = p_hat_of<NS, RGB>(sc, cit, mc, nr.s, swl);
+            float p_cc = p_hat_of<NS, RGB, MS>(sc, cit, mc, r.s, swl);
+            float p_cn = p_hat_of<NS, RGB, MS>(sc, nit, nm, r.s, swl);
+            float p_nn = p_hat_of<NS, RGB, MS>(sc, nit, nm, nr.s, swl);
+            float p_nc = p_hat_of<NS, RGB, MS>(sc, cit, mc, nr.s, swl);
             float mi = (1.f * p_nn) / (1.f * p_nn + num_i * p_nc);
             ret.update(sp.next1d(), nr.s, safe_weight(mi, nr.s.p_hat, nr.W), nr.C);
             float cw = ((1.f * p_cc) / (1.f * p_cc + num_i * p_cn)) / num;
@@ -805,7 +812,7 @@ __device__ __forceinline__ Reservoir spatial_reuse(const SceneView &sc, const Re
 // ReSTIRDI::shading (direct.cpp:583-636).  Its BSDF-sampled emitter term is
 // overwritten before use and its draws come after every other use of the
 // sampler, so it is left out.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ __forceinline__ f3 restir_shading(const SceneView &sc, const RestirView &rv, Reservoir r,
                                              const SurfaceRec &cs, uint32_t pix, Swl<NS> &swl, uint2 *stack,
                                              uint32_t stride, PathCounters &pc) {
@@ -828,11 +835,11 @@ __device__ __forceinline__ f3 restir_shading(const SceneView &sc, const RestirVi
             Le = light_color<NS, RGB>(sc, L, swl) * ((dot(w, it.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
         }
     } else {
-        MatCtx<NS> mc = material_at<NS, RGB>(sc, it, swl);
+        MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);
         Spec<NS> t;
 #pragma unroll
         for (int i = 0; i < NS; ++i) t.v[i] = i == 0 ? thr.x : (i == 1 ? thr.y : thr.z);
-        value = (light_contribution<NS, RGB>(sc, it, mc, r.s, swl, nullptr) * t) / luminance(thr);
+        value = (light_contribution<NS, RGB, MS>(sc, it, mc, r.s, swl, nullptr) * t) / luminance(thr);
         r.process_occluded(occluded(sc, it, r.s.pos, stack, stride, pc));
         value = value * r.W;
     }
@@ -842,7 +849,7 @@ __device__ __forceinline__ f3 restir_shading(const SceneView &sc, const RestirVi
 // shader1 for one pixel (direct.cpp:638-671): spatial reuse over buffer
 // `buf`, shading; the reservoir is kept for the next frame.  Returns the
 // pixel's direct radiance for the film.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ f3 restir_final(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
                            uint32_t py, uint32_t buf, uint2 *stack, uint32_t stride, PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;
@@ -855,10 +862,10 @@ __device__ f3 restir_final(const SceneView &sc, const CameraView &cam, const Res
     sp.start(px, py, rv.frame, 1u);
     SurfaceRec cs = load_surface(rv, cur_slot(rv.frame), pix);
     Reservoir r = load_reservoir(rv, buf, pix);
-    r = spatial_reuse<NS, RGB>(sc, rv, r, cs, px, py, buf, sp, swl);
+    r = spatial_reuse<NS, RGB, MS>(sc, rv, r, cs, px, py, buf, sp, swl);
     f3 L = mk3(0.f, 0.f, 0.f);
     if (cs.hit()) {
-        L = restir_shading<NS, RGB>(sc, rv, r, cs, pix, swl, stack, stride, pc);
+        L = restir_shading<NS, RGB, MS>(sc, rv, r, cs, pix, swl, stack, stride, pc);
     } else if (sc.env_light != kInvalid) {
         Spec<NS> Lv;
         float pdf;
@@ -871,7 +878,7 @@ __device__ f3 restir_final(const SceneView &sc, const CameraView &cam, const Res
 
 // an extra spatial pass (beyond the reference's one): buffer `in` to `out`,
 // its own sampler dimension
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __device__ void restir_spatial_pass(const SceneView &sc, const RestirView &rv, uint32_t px, uint32_t py,
                                     uint32_t in, uint32_t out, uint32_t dim) {
     const uint32_t pix = py * rv.width + px;
@@ -880,7 +887,7 @@ __device__ void restir_spatial_pass(const SceneView &sc, const RestirView &rv, u
     Swl<NS> swl = sample_wavelengths<NS, RGB>(px, py, rv.frame);
     SurfaceRec cs = load_surface(rv, cur_slot(rv.frame), pix);
     Reservoir r = load_reservoir(rv, in, pix);
-    r = spatial_reuse<NS, RGB>(sc, rv, r, cs, px, py, in, sp, swl);
+    r = spatial_reuse<NS, RGB, MS>(sc, rv, r, cs, px, py, in, sp, swl);
     store_reservoir(rv, out, pix, r);
 }
 

@@ -35,13 +35,15 @@ struct RestirView {
 // shader0 -> shader1, and a second pass-through for extra spatial passes
 constexpr uint32_t kRestirReservoirBuffers = 4;
 
+// material_set: 0 every material, 1 the diffuse family only (a smaller
+// kernel for scenes without metal, glass or principled materials)
 hipError_t restir_clear(const RestirView &v, hipStream_t stream);
 hipError_t restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &v, uint32_t spectrum_dim,
-                          bool rgb, unsigned long long *counters, hipStream_t stream);
-hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb, uint32_t in,
-                          uint32_t out, uint32_t dim, hipStream_t stream);
+                          bool rgb, int material_set, unsigned long long *counters, hipStream_t stream);
+hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb,
+                          int material_set, uint32_t in, uint32_t out, uint32_t dim, hipStream_t stream);
 hipError_t restir_final(const SceneView &sc, const CameraView &cam, const RestirView &v, const RenderView &rv,
-                        uint32_t spectrum_dim, bool rgb, uint32_t in, float4 *rt, float4 *out,
+                        uint32_t spectrum_dim, bool rgb, int material_set, uint32_t in, float4 *rt, float4 *out,
                         unsigned long long *counters, hipStream_t stream);
 
 }// namespace vs

@@ -34,7 +34,7 @@ __device__ __forceinline__ void add_counters(unsigned long long *counters, const
     atomicAdd(counters + 2, (unsigned long long)pc.tris);
 }
 
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) initial_kernel(SceneView sc, CameraView cam, RestirView v,
                                                      unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
@@ -42,22 +42,22 @@ __global__ void __launch_bounds__(64) initial_kernel(SceneView sc, CameraView ca
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     PathCounters pc{0, 0, 0};
-    restir_initial<NS, RGB>(sc, cam, v, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
+    restir_initial<NS, RGB, MS>(sc, cam, v, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
     add_counters(counters, pc);
 }
 
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) spatial_kernel(SceneView sc, RestirView v, uint32_t in, uint32_t out,
                                                      uint32_t dim) {
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
-    restir_spatial_pass<NS, RGB>(sc, v, px, py, in, out, dim);
+    restir_spatial_pass<NS, RGB, MS>(sc, v, px, py, in, out, dim);
 }
 
 // shader1 plus the film's running mean (RealTimeIntegrator's combine,
 // rt.cpp:121-130, with the indirect half absent)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam, RestirView v, RenderView rv,
                                                    uint32_t in, float4 *__restrict__ rt, float4 *__restrict__ out,
                                                    unsigned long long *__restrict__ counters) {
@@ -66,7 +66,7 @@ __global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam,
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     PathCounters pc{0, 0, 0};
-    f3 L = restir_final<NS, RGB>(sc, cam, v, px, py, in, lds_stack + threadIdx.x, blockDim.x, pc);
+    f3 L = restir_final<NS, RGB, MS>(sc, cam, v, px, py, in, lds_stack + threadIdx.x, blockDim.x, pc);
     uint32_t idx = py * v.width + px;
     float4 acc = film_add(rv, rt[idx], L, v.frame);
     rt[idx] = acc;
@@ -75,6 +75,45 @@ __global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam,
 }
 
 dim3 tiles(const RestirView &v) { return dim3((v.width + 7u) / 8u, (v.height + 7u) / 8u); }
+
+// kernel variant of a launch: spectral layout x material set
+template <template <int, bool, int> class K, typename... Args>
+hipError_t dispatch(uint32_t spectrum_dim, bool rgb, int material_set, Args &&...args) {
+    if (material_set != MS_ALL && material_set != MS_DIFFUSE) return hipErrorInvalidValue;
+    const bool all = material_set == MS_ALL;
+    if (rgb) return all ? K<3, true, MS_ALL>::launch(args...) : K<3, true, MS_DIFFUSE>::launch(args...);
+    if (spectrum_dim == 4) return all ? K<4, false, MS_ALL>::launch(args...) : K<4, false, MS_DIFFUSE>::launch(args...);
+    if (spectrum_dim == 3) return all ? K<3, false, MS_ALL>::launch(args...) : K<3, false, MS_DIFFUSE>::launch(args...);
+    return hipErrorInvalidValue;
+}
+
+template <int NS, bool RGB, int MS>
+struct Initial {
+    static hipError_t launch(const SceneView &sc, const CameraView &cam, const RestirView &v,
+                             unsigned long long *counters, hipStream_t stream) {
+        size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+        hipLaunchKernelGGL((initial_kernel<NS, RGB, MS>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
+        return hipGetLastError();
+    }
+};
+template <int NS, bool RGB, int MS>
+struct Spatial {
+    static hipError_t launch(const SceneView &sc, const RestirView &v, uint32_t in, uint32_t out, uint32_t dim,
+                             hipStream_t stream) {
+        hipLaunchKernelGGL((spatial_kernel<NS, RGB, MS>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
+        return hipGetLastError();
+    }
+};
+template <int NS, bool RGB, int MS>
+struct Final {
+    static hipError_t launch(const SceneView &sc, const CameraView &cam, const RestirView &v, const RenderView &rv,
+                             uint32_t in, float4 *rt, float4 *out, unsigned long long *counters, hipStream_t stream) {
+        size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+        hipLaunchKernelGGL((final_kernel<NS, RGB, MS>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
+                           counters);
+        return hipGetLastError();
+    }
+};
 
 }// namespace
 
@@ -85,46 +124,22 @@ hipError_t restir_clear(const RestirView &v, hipStream_t stream) {
 }
 
 hipError_t restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &v, uint32_t spectrum_dim,
-                          bool rgb, unsigned long long *counters, hipStream_t stream) {
-    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
-    if (rgb) hipLaunchKernelGGL((initial_kernel<3, true>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
-    else if (spectrum_dim == 4)
-        hipLaunchKernelGGL((initial_kernel<4, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
-    else if (spectrum_dim == 3)
-        hipLaunchKernelGGL((initial_kernel<3, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, counters);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+                          bool rgb, int material_set, unsigned long long *counters, hipStream_t stream) {
+    return dispatch<Initial>(spectrum_dim, rgb, material_set, sc, cam, v, counters, stream);
 }
 
-hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb, uint32_t in,
-                          uint32_t out, uint32_t dim, hipStream_t stream) {
+hipError_t restir_spatial(const SceneView &sc, const RestirView &v, uint32_t spectrum_dim, bool rgb,
+                          int material_set, uint32_t in, uint32_t out, uint32_t dim, hipStream_t stream) {
     if (in >= kRestirReservoirBuffers || out >= kRestirReservoirBuffers || in < 2u || out < 2u || in == out)
         return hipErrorInvalidValue;
-    if (rgb) hipLaunchKernelGGL((spatial_kernel<3, true>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
-    else if (spectrum_dim == 4)
-        hipLaunchKernelGGL((spatial_kernel<4, false>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
-    else if (spectrum_dim == 3)
-        hipLaunchKernelGGL((spatial_kernel<3, false>), tiles(v), dim3(64), 0, stream, sc, v, in, out, dim);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    return dispatch<Spatial>(spectrum_dim, rgb, material_set, sc, v, in, out, dim, stream);
 }
 
 hipError_t restir_final(const SceneView &sc, const CameraView &cam, const RestirView &v, const RenderView &rv,
-                        uint32_t spectrum_dim, bool rgb, uint32_t in, float4 *rt, float4 *out,
+                        uint32_t spectrum_dim, bool rgb, int material_set, uint32_t in, float4 *rt, float4 *out,
                         unsigned long long *counters, hipStream_t stream) {
     if (in < 2u || in >= kRestirReservoirBuffers) return hipErrorInvalidValue;
-    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
-    if (rgb)
-        hipLaunchKernelGGL((final_kernel<3, true>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
-                           counters);
-    else if (spectrum_dim == 4)
-        hipLaunchKernelGGL((final_kernel<4, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
-                           counters);
-    else if (spectrum_dim == 3)
-        hipLaunchKernelGGL((final_kernel<3, false>), tiles(v), dim3(64), lds, stream, sc, cam, v, rv, in, rt, out,
-                           counters);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    return dispatch<Final>(spectrum_dim, rgb, material_set, sc, cam, v, rv, in, rt, out, counters, stream);
 }
 
 }// namespace vs
