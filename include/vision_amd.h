@@ -235,6 +235,26 @@ vs_status vs_reset_film(vs_scene *scene);
 /* Download the film: which = 0 running-mean radiance buffer (rt_buffer),
  * 1 exposure/tone-mapped output buffer (normal.cpp:87-101); rgba float[W*H*4]. */
 vs_status vs_film_download(vs_scene *scene, int32_t which, float *rgba);
+/* The picture a save writes (Pipeline::final_picture, src/base/mgr/pipeline.cpp:313-330,
+ * with Postprocessor::tone_mapping, src/base/mgr/postprocessor.cpp:13-30): the output
+ * buffer passed through the collector's tone mapper again, then linear_to_srgb when
+ * gamma != 0, w = 1; rgba float[W*H*4] on the host. */
+vs_status vs_film_final_picture(vs_scene *scene, int32_t gamma, float *rgba);
+/* Write the final picture to path (Pipeline::save_result, pipeline.cpp:174-182); NULL or
+ * "" writes the scene's output.fn (default "output.png") under its base directory.
+ * gamma is off for .exr, .hdr and .pfm and on for .png, as final_picture decides by the
+ * file name.  Formats: .exr (float RGBA, uncompressed), .hdr (RGBE), .png (8-bit RGBA),
+ * .pfm; others are VS_ERROR_UNSUPPORTED. */
+vs_status vs_film_save(vs_scene *scene, const char *path);
+/* Host-only image I/O (ocarina's Image::save_image / Image::load, used by
+ * pipeline.cpp:176 and image_pool.cpp:28).  Rows top to bottom, float RGBA.
+ * vs_image_write: .exr .hdr .png .pfm as above.  vs_image_read: .exr (NONE/ZIPS/ZIP,
+ * HALF/FLOAT/UINT), .hdr (flat or RLE), .png (8/16-bit, any colour type, no interlace),
+ * .pfm; call with rgba = NULL to get the size, then with a W*H*4 buffer.  Values of
+ * 8/16-bit files are code / max with no transfer function.  ctx may be a CPU context
+ * (device -1) and only receives the error message. */
+vs_status vs_image_write(vs_context *ctx, const char *path, uint32_t width, uint32_t height, const float *rgba);
+vs_status vs_image_read(vs_context *ctx, const char *path, uint32_t *width, uint32_t *height, float *rgba);
 
 #ifdef __cplusplus
 }

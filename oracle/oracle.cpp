@@ -2652,4 +2652,41 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
     return 0;
 }
 
+
+// ---- film output and the saved picture ----------------------------------
+// ToneMapper::apply of the linear / aces / reinhard plugins
+// (src/render_core/tonemappers/impl.cpp:10-46); tm 0, 1, 2
+static float tone_map(float v, uint32_t tm) {
+    if (tm == 1) {
+        float r = (v * (2.51f * v + 0.03f)) / (v * (2.43f * v + 0.59f) + 0.14f);
+        return std::min(std::max(r, 0.f), 1.f);
+    }
+    if (tm == 2) return v / (v + 1.f);
+    return v;
+}
+// sRGB encoding (IEC 61966-2-1), the linear_to_srgb of Postprocessor::compile_tone_mapping
+static float encode_srgb(float x) {
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * std::pow(x, 1.f / 2.4f) - 0.055f;
+}
+// the collector's output buffer from its running mean:
+// tone_map(1 - exp(-v * exposure)), w = 1 (radiance_collector.cpp:24-26, normal.cpp:96-99)
+void or_film_output(const float *rt, uint32_t n, float exposure, uint32_t tm, float *out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) out[4 * i + c] = tone_map(1.f - std::exp(-rt[4 * i + c] * exposure), tm);
+        out[4 * i + 3] = 1.f;
+    }
+}
+// Pipeline::final_picture (src/base/mgr/pipeline.cpp:313-330) without a denoiser:
+// the output buffer tone-mapped again, then sRGB-encoded when gamma, w = 1
+// (src/base/mgr/postprocessor.cpp:13-21)
+void or_final_picture(const float *out, uint32_t n, uint32_t tm, int32_t gamma, float *pic) {
+    for (uint32_t i = 0; i < n; ++i) {
+        for (int c = 0; c < 3; ++c) {
+            float v = tone_map(out[4 * i + c], tm);
+            pic[4 * i + c] = gamma ? encode_srgb(v) : v;
+        }
+        pic[4 * i + 3] = 1.f;
+    }
+}
+
 }// extern "C"

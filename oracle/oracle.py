@@ -48,6 +48,8 @@ def lib():
         h.or_alias_sample.argtypes = [_f32p, C.c_uint32, _f32p, C.c_uint32, _u32p, C.c_void_p, C.c_void_p]
         h.or_offset_ray_origin.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
         h.or_filter_eval.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
+        h.or_film_output.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _f32p]
+        h.or_final_picture.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int32, _f32p]
         h.or_filter_table.argtypes = [_f32p, _f32p]
         h.or_filter_sample.argtypes = [_f32p, _f32p, C.c_uint32, _f32p, _f32p]
         h.or_eval_diffuse.argtypes = [C.c_uint32, _f32p, C.c_float, _f32p, _f32p, C.c_uint32, _f32p, _f32p]
@@ -159,6 +161,26 @@ def filter_eval(cam: np.ndarray, xy: np.ndarray) -> np.ndarray:
     xy = np.ascontiguousarray(xy, np.float32).reshape(-1, 2)
     out = np.zeros(len(xy), np.float32)
     lib().or_filter_eval(np.ascontiguousarray(cam, np.float32), xy, len(xy), out)
+    return out
+
+
+TONE_MAPPERS = {"linear": 0, "aces": 1, "reinhard": 2}
+
+
+def film_output(rt: np.ndarray, exposure: float = 1.0, tone_mapper: str = "linear") -> np.ndarray:
+    """The collector's output buffer from its running-mean film [..., 4]."""
+    a = np.ascontiguousarray(rt, np.float32)
+    out = np.zeros_like(a)
+    lib().or_film_output(a.reshape(-1), a.size // 4, exposure, TONE_MAPPERS[tone_mapper], out.reshape(-1))
+    return out
+
+
+def final_picture(output: np.ndarray, tone_mapper: str = "linear", gamma: bool = True) -> np.ndarray:
+    """The picture a save writes, from the output buffer [..., 4]."""
+    a = np.ascontiguousarray(output, np.float32)
+    out = np.zeros_like(a)
+    lib().or_final_picture(a.reshape(-1), a.size // 4, TONE_MAPPERS[tone_mapper], 1 if gamma else 0,
+                           out.reshape(-1))
     return out
 
 

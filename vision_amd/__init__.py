@@ -111,6 +111,10 @@ SIGNATURES = {
     "vs_last_render_stats": (C.c_int32, [_P, C.POINTER(RenderStats)]),
     "vs_reset_film": (C.c_int32, [_P]),
     "vs_film_download": (C.c_int32, [_P, C.c_int32, _P]),
+    "vs_film_final_picture": (C.c_int32, [_P, C.c_int32, _P]),
+    "vs_film_save": (C.c_int32, [_P, C.c_char_p]),
+    "vs_image_write": (C.c_int32, [_P, C.c_char_p, C.c_uint32, C.c_uint32, _P]),
+    "vs_image_read": (C.c_int32, [_P, C.c_char_p, _P, _P, _P]),
 }
 
 _lib_handle: Optional[C.CDLL] = None
@@ -167,6 +171,21 @@ class Context:
         s = _P()
         self._check(lib().vs_scene_load(self._h, text.encode(), str(base_dir).encode(), C.byref(s)))
         return Scene(self, s)
+
+    def write_image(self, path: str | os.PathLike, rgba: np.ndarray):
+        """Write float RGBA rows (top first) as .exr / .hdr / .png / .pfm."""
+        a = np.ascontiguousarray(rgba, dtype=np.float32)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("rgba must be float[H, W, 4]")
+        self._check(lib().vs_image_write(self._h, str(path).encode(), a.shape[1], a.shape[0], a.ctypes.data))
+
+    def read_image(self, path: str | os.PathLike) -> np.ndarray:
+        """float32[H, W, 4] from .exr / .hdr / .png / .pfm (8/16-bit values as code / max)."""
+        w, h = C.c_uint32(), C.c_uint32()
+        self._check(lib().vs_image_read(self._h, str(path).encode(), C.byref(w), C.byref(h), None))
+        out = np.zeros((h.value, w.value, 4), dtype=np.float32)
+        self._check(lib().vs_image_read(self._h, str(path).encode(), C.byref(w), C.byref(h), out.ctypes.data))
+        return out
 
     def close(self):
         if self._h:
@@ -334,3 +353,14 @@ class Scene:
         out = np.zeros((info["height"], info["width"], 4), dtype=np.float32)
         self.ctx._check(lib().vs_film_download(self._h, which, out.ctypes.data))
         return out
+
+    def final_picture(self, gamma: bool = True) -> np.ndarray:
+        """The picture a save writes: float32[H, W, 4]."""
+        info = self.info()
+        out = np.zeros((info["height"], info["width"], 4), dtype=np.float32)
+        self.ctx._check(lib().vs_film_final_picture(self._h, 1 if gamma else 0, out.ctypes.data))
+        return out
+
+    def save(self, path: str | os.PathLike | None = None):
+        """Write the final picture; None writes the scene's output.fn."""
+        self.ctx._check(lib().vs_film_save(self._h, None if path is None else str(path).encode()))

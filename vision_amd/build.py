@@ -73,7 +73,7 @@ def build(verbose: bool = False) -> Path:
         objs = list(ex.map(_compile, sources))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-o", str(tmp), *map(str, objs),
-           "-Wl,--no-undefined", "-lpthread"]
+           "-Wl,--no-undefined", "-lpthread", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")

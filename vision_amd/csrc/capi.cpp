@@ -18,6 +18,7 @@
 #include "launch.h"
 #include "scene.h"
 #include "data_tables.h"
+#include "image_io.h"
 #include "restir.h"
 #include "tables.h"
 #include "wavefront.h"
@@ -85,7 +86,7 @@ struct vs_scene {
     DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, mat, light, alias, func;
     DevBuf spd, lut, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
     DevBuf fis_lut, fis_cond, fis_cond_func, fis_marg, fis_marg_func;
-    DevBuf film_rt, film_out, counters;
+    DevBuf film_rt, film_out, film_pic, counters;
     DevBuf scratch_rays, scratch_hits;
     // wavefront form: per-path state, queues, append counters
     uint32_t kernel_mode{VS_KERNEL_MEGAKERNEL};
@@ -976,6 +977,65 @@ vs_status vs_film_download(vs_scene *s, int32_t which, float *rgba) {
         const DevBuf &b = which == 0 ? s->film_rt : s->film_out;
         hip_check(hipMemcpyAsync(rgba, b.ptr, bytes, hipMemcpyDeviceToHost, s->ctx->stream), "film D2H");
         hip_check(hipStreamSynchronize(s->ctx->stream), "film sync");
+    });
+}
+
+}// extern "C"
+
+namespace {
+
+// Pipeline::final_picture (src/base/mgr/pipeline.cpp:313-330): the output
+// buffer tone-mapped again, sRGB-encoded when gamma, downloaded to rgba
+void final_picture(vs_scene *s, bool gamma, float *rgba) {
+    require_built(s);
+    hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+    uint32_t n = s->host.camera.width * s->host.camera.height;
+    if (s->film_pic.bytes < size_t(n) * 16) s->film_pic.alloc(size_t(n) * 16);
+    uint32_t tm = s->host.camera.tone_mapper == "aces" ? 1u : (s->host.camera.tone_mapper == "reinhard" ? 2u : 0u);
+    hip_check(vs::launch_final_picture(s->film_out.as<float4>(), s->film_pic.as<float4>(), n, tm, gamma,
+                                       s->ctx->stream),
+              "final picture");
+    hip_check(hipMemcpyAsync(rgba, s->film_pic.ptr, size_t(n) * 16, hipMemcpyDeviceToHost, s->ctx->stream),
+              "picture D2H");
+    hip_check(hipStreamSynchronize(s->ctx->stream), "picture sync");
+}
+
+}// namespace
+
+extern "C" {
+
+vs_status vs_film_final_picture(vs_scene *s, int32_t gamma, float *rgba) {
+    if (!s || !rgba) return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    return guarded(s->ctx, [&] { final_picture(s, gamma != 0, rgba); });
+}
+
+vs_status vs_film_save(vs_scene *s, const char *path) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        // Pipeline::save_result (pipeline.cpp:174-182): output.fn under the scene's directory
+        std::string fn = (path && *path) ? std::string(path) : vs::join_path(s->host.base_dir, s->host.output_fn);
+        std::string e = vs::image_extension(fn);
+        if (e != "exr" && e != "hdr" && e != "png" && e != "pfm")
+            throw std::runtime_error("unsupported image format ." + e + " (exr, hdr, png, pfm): " + fn);
+        bool gamma = !(e == "exr" || e == "hdr" || e == "pfm");
+        std::vector<float> pic(size_t(s->host.camera.width) * s->host.camera.height * 4);
+        final_picture(s, gamma, pic.data());
+        vs::write_image(fn, s->host.camera.width, s->host.camera.height, pic.data());
+    });
+}
+
+vs_status vs_image_write(vs_context *ctx, const char *path, uint32_t width, uint32_t height, const float *rgba) {
+    if (!path || !rgba || width == 0 || height == 0) return fail(ctx, VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    return guarded(ctx, [&] { vs::write_image(path, width, height, rgba); });
+}
+
+vs_status vs_image_read(vs_context *ctx, const char *path, uint32_t *width, uint32_t *height, float *rgba) {
+    if (!path || !width || !height) return fail(ctx, VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    return guarded(ctx, [&] {
+        vs::ImageRGBA img = vs::read_image(path);
+        *width = img.width;
+        *height = img.height;
+        if (rgba) std::memcpy(rgba, img.rgba.data(), img.rgba.size() * sizeof(float));
     });
 }
 

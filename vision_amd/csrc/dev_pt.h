@@ -567,6 +567,12 @@ __device__ __forceinline__ float tone_map(float v, uint32_t tm) {
     return v;
 }
 
+// sRGB transfer function (IEC 61966-2-1): linear segment below 0.0031308,
+// 1.055 x^(1/2.4) - 0.055 above
+__device__ __forceinline__ float linear_to_srgb(float x) {
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+
 // one sample into the film's running mean (NormalRadianceCollector::add_sample,
 // src/render_core/radiancecollectors/normal.cpp:87-101)
 __device__ __forceinline__ float4 film_add(const RenderView &rv, float4 acc, f3 L, uint32_t frame) {

@@ -15,5 +15,8 @@ hipError_t launch_bsdf(const SceneView &sc, uint32_t spectrum_dim, uint32_t mat,
                        float *pdf, hipStream_t stream);
 hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderView &rv, float4 *rt, float4 *out,
                      unsigned long long *counters, hipStream_t stream);
+// the saved picture: tone map (again) + optional sRGB gamma over the output buffer (film.hip)
+hipError_t launch_final_picture(const float4 *in, float4 *out, uint32_t n, uint32_t tone_mapper, bool gamma,
+                                hipStream_t stream);
 
 }// namespace vs

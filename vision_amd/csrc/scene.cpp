@@ -1,5 +1,7 @@
 #include "scene.h"
 
+#include "image_io.h"
+
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
@@ -10,9 +12,12 @@
 
 namespace vs {
 
-namespace {
+std::string join_path(const std::string &dir, const std::string &fn) {
+    if (fn.empty() || fn[0] == '/' || dir.empty()) return fn;
+    return dir + "/" + fn;
+}
 
-std::string join_path(const std::string &dir, const std::string &fn);
+namespace {
 
 constexpr float kPi = 3.14159265358979323846f;
 
@@ -328,11 +333,6 @@ MeshHost load_vsmesh(const std::string &path) {
     return m;
 }
 
-std::string join_path(const std::string &dir, const std::string &fn) {
-    if (fn.empty() || fn[0] == '/' || dir.empty()) return fn;
-    return dir + "/" + fn;
-}
-
 LightHost parse_light(const Json &desc, const std::string &base_dir) {
     LightHost l;
     std::string type = desc["type"].as_string("area");
@@ -367,7 +367,7 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
         if (fn.empty() && node.contains("param")) fn = node["param"]["fn"].as_string("");
         if (fn.empty()) throw std::runtime_error("spherical light needs an image (color.node.fn)");
         l.env_fn = join_path(base_dir, fn);
-        load_pfm(l.env_fn, l.env_rgb, l.env_w, l.env_h);
+        load_env_image(l.env_fn, l.env_rgb, l.env_w, l.env_h);
         l.color = mk3(1.f);
         l.scale = float(p["scale"].as_float(1.0));
         l.flip_u = p["flip_u"].as_bool(false);
@@ -380,38 +380,20 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
 
 }// namespace
 
-void load_pfm(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h) {
-    std::ifstream in(path, std::ios::binary);
-    if (!in) throw std::runtime_error("cannot open image " + path);
-    std::string magic;
-    in >> magic;
-    if (magic != "PF" && magic != "Pf") throw std::runtime_error("unsupported image format (PFM expected): " + path);
-    int iw = 0, ih = 0;
-    float sc = 0.f;
-    in >> iw >> ih >> sc;
-    in.get();
-    if (iw <= 0 || ih <= 0 || iw > 65536 || ih > 65536) throw std::runtime_error("bad PFM size in " + path);
-    int ch = magic == "PF" ? 3 : 1;
-    std::vector<float> raw(size_t(iw) * ih * ch);
-    in.read(reinterpret_cast<char *>(raw.data()), std::streamsize(raw.size() * 4));
-    if (!in) throw std::runtime_error("truncated PFM " + path);
-    if (sc > 0.f) {// big-endian
-        for (float &f : raw) {
-            uint32_t u;
-            std::memcpy(&u, &f, 4);
-            u = __builtin_bswap32(u);
-            std::memcpy(&f, &u, 4);
+void load_env_image(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h) {
+    ImageRGBA img = read_image(path);
+    // 8/16-bit files hold sRGB-encoded values unless the scene says otherwise
+    // (ImagePool::load_texture, src/base/mgr/image_pool.cpp:17-22)
+    std::string e = image_extension(path);
+    bool srgb = e == "png";
+    w = img.width;
+    h = img.height;
+    rgb.resize(size_t(w) * h * 3);
+    for (size_t i = 0; i < size_t(w) * h; ++i)
+        for (int c = 0; c < 3; ++c) {
+            float v = img.rgba[i * 4 + c];
+            rgb[i * 3 + c] = srgb ? srgb_to_linear(v) : v;
         }
-    }
-    w = uint32_t(iw);
-    h = uint32_t(ih);
-    rgb.assign(size_t(w) * h * 3, 0.f);
-    for (uint32_t y = 0; y < h; ++y) {
-        uint32_t src_row = h - 1 - y;// PFM rows run bottom to top
-        for (uint32_t x = 0; x < w; ++x)
-            for (int c = 0; c < 3; ++c)
-                rgb[(size_t(y) * w + x) * 3 + c] = raw[(size_t(src_row) * w + x) * ch + (ch == 3 ? c : 0)];
-    }
 }
 
 m4 camera_to_world(f3 position, float yaw, float pitch) {

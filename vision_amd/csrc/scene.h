@@ -137,7 +137,11 @@ struct SceneHost {
 };
 
 // Read a PFM image (linear RGB float) into top-row-first RGB texels.
-void load_pfm(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h);
+// an environment map (EXR, HDR, PFM: linear; PNG: sRGB-encoded) as linear RGB rows, top first
+void load_env_image(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h);
+
+// fn relative to dir unless absolute or dir is empty
+std::string join_path(const std::string &dir, const std::string &fn);
 
 // Parse a scene from JSON text; base_dir resolves relative mesh/texture paths.
 SceneHost load_scene(const std::string &text, const std::string &base_dir);
