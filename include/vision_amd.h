@@ -256,6 +256,100 @@ vs_status vs_film_save(vs_scene *scene, const char *path);
 vs_status vs_image_write(vs_context *ctx, const char *path, uint32_t width, uint32_t height, const float *rgba);
 vs_status vs_image_read(vs_context *ctx, const char *path, uint32_t *width, uint32_t *height, float *rgba);
 
+/* ---- Batched serving -----------------------------------------------------
+ *
+ * No reference counterpart: the reference renders one scene per process, one
+ * frame per launch with a host sync per frame (Pipeline::display,
+ * src/base/mgr/pipeline.cpp:296-311).  A server keeps any number of built
+ * scenes resident and renders path-tracing jobs on them: a worker thread
+ * admits queued jobs while job slots and its HBM budget allow (continuous
+ * batching: jobs that arrive while others run join at the next step), and
+ * each step runs every admitted job forward by some frames inside persistent
+ * kernels whose waves claim 64-pixel tiles of any job from a device counter.
+ * A step is a captured hipGraph replayed with a new step table.  A job's film
+ * is bit-identical to vs_render(scene at the job's resolution, spp) of the
+ * same scene, alone or in any mix.  Submit, wait, result and stats calls are
+ * thread-safe. */
+typedef struct vs_server vs_server;
+
+typedef struct vs_server_config {
+    uint64_t hbm_budget;   /* device bytes for job films (32 B per pixel); 0: a quarter of free memory */
+    uint32_t max_jobs;     /* jobs admitted at once, 1..256; 0: 64 */
+    float step_ms;         /* target duration of a step; 0: 16 */
+    uint32_t tile_w;       /* tile width in pixels (tiles hold 64 pixels): 8, 16, 32 or 64; 0: 8 */
+    uint32_t waves_per_cu; /* persistent waves per CU for each kernel variant; 0: the occupancy limit */
+    int32_t use_graphs;    /* 1: replay captured hipGraphs; 0: launch each step's kernels directly */
+    int32_t count_rays;    /* 1: count rays and BVH fetches (slower) */
+} vs_server_config;
+
+typedef struct vs_job_desc {
+    uint32_t scene;        /* index from vs_server_add_scene */
+    uint32_t width, height;/* film resolution (the scene's camera, re-aspected) */
+    uint32_t spp;          /* frames, 1..65536 */
+    uint32_t max_depth;    /* 0: the scene integrator's */
+    uint32_t spectral;     /* 0: sRGB mode, 1: hero wavelengths */
+    uint32_t spectrum_dim; /* hero wavelengths per path, 3 or 4 (0: 4); ignored in sRGB mode */
+    uint32_t reserved;
+} vs_job_desc;
+
+#define VS_JOB_QUEUED 0
+#define VS_JOB_RUNNING 1
+#define VS_JOB_DONE 2
+#define VS_JOB_FAILED 3
+
+typedef struct vs_job_info {
+    int32_t state;
+    uint32_t frames_done;
+    uint64_t samples;      /* width * height * spp */
+    double queue_ms;       /* submit -> admitted */
+    double latency_ms;     /* submit -> film on the host (done jobs) */
+} vs_job_info;
+
+typedef struct vs_server_stats {
+    uint64_t jobs_submitted, jobs_done, jobs_failed;
+    uint64_t samples_done;
+    uint64_t rays, node_fetches, tri_fetches; /* when count_rays */
+    uint64_t steps, graph_launches;
+    double busy_ms;        /* device time of the steps */
+    double wall_ms;        /* first submit -> last completion */
+    double msamples_per_s; /* samples_done / wall */
+    double latency_p50_ms, latency_p99_ms, latency_mean_ms, latency_max_ms;
+    uint32_t max_active;   /* most jobs admitted at once */
+    uint32_t active;       /* admitted now */
+    uint32_t queued;       /* waiting for admission now */
+    uint32_t pad;
+    uint64_t hbm_budget, hbm_peak;
+} vs_server_stats;
+
+/* Create a server on ctx's device (cfg NULL: defaults).  The worker thread
+ * starts at once and waits for jobs. */
+vs_status vs_server_create(vs_context *ctx, const vs_server_config *cfg, vs_server **out);
+/* Make a built scene resident; it must outlive the server.  Both spectrum
+ * modes become available (the other mode's material, light and environment
+ * tables are packed and uploaded here). */
+vs_status vs_server_add_scene(vs_server *srv, vs_scene *scene, uint32_t *index);
+/* Queue a job; VS_ERROR_OUT_OF_MEMORY if its film can never fit the budget. */
+vs_status vs_server_submit(vs_server *srv, const vs_job_desc *job, uint64_t *job_id);
+/* Wait up to timeout_ms (< 0: forever) for a job; *done = 1 when finished (or failed). */
+vs_status vs_server_wait(vs_server *srv, uint64_t job_id, double timeout_ms, int32_t *done);
+vs_status vs_server_job_info(vs_server *srv, uint64_t job_id, vs_job_info *info);
+/* A finished job's film: which = 0 running mean, 1 output buffer; rgba float[W*H*4]. */
+vs_status vs_server_result(vs_server *srv, uint64_t job_id, int32_t which, float *rgba);
+/* Forget a finished job and free its host copy. */
+vs_status vs_server_release(vs_server *srv, uint64_t job_id);
+/* Wait until no job is queued or running (timeout_ms < 0: forever). */
+vs_status vs_server_drain(vs_server *srv, double timeout_ms);
+vs_status vs_server_stats_get(vs_server *srv, vs_server_stats *out);
+/* Clear latency samples and counters (between benchmark phases). */
+vs_status vs_server_reset_stats(vs_server *srv);
+/* Change tile_w / waves_per_cu / step_ms / use_graphs while idle (the
+ * budget and slot count are fixed at creation). */
+vs_status vs_server_set_config(vs_server *srv, const vs_server_config *cfg);
+vs_status vs_server_get_config(vs_server *srv, vs_server_config *cfg);
+/* The message of the last failed server call on this thread. */
+const char *vs_server_last_error(void);
+void vs_server_destroy(vs_server *srv);
+
 #ifdef __cplusplus
 }
 #endif

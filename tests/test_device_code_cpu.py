@@ -26,7 +26,7 @@ HERE = Path(__file__).resolve().parent
 SRC = HERE / "devcpu" / "devcpu.cpp"
 BUILD = HERE / "devcpu" / "_build"
 LIB = BUILD / "libdevcpu.so"
-OBJS = ["data_tables.cpp.o", "json.cpp.o", "scene.cpp.o", "bvh.cpp.o", "tables.cpp.o"]
+OBJS = ["data_tables.cpp.o", "json.cpp.o", "scene.cpp.o", "bvh.cpp.o", "tables.cpp.o", "image_io.cpp.o"]
 
 _f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32 = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
@@ -44,7 +44,7 @@ def _harness():
             data.symlink_to(vbuild.DATA)
         tmp = LIB.with_suffix(".tmp")
         cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-w", "-I", str(HERE / "devcpu"),
-               "-I", str(vbuild.CSRC), "-o", str(tmp), str(SRC), *map(str, objs), "-lpthread"]
+               "-I", str(vbuild.CSRC), "-o", str(tmp), str(SRC), *map(str, objs), "-lpthread", "-lz"]
         subprocess.run(cmd, check=True, capture_output=True)
         tmp.replace(LIB)
     h = C.CDLL(str(LIB))

@@ -80,6 +80,34 @@ class SceneArrays(C.Structure):
                 ("restir", C.POINTER(C.c_float)), ("num_restir", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_uint32), ("step_ms", C.c_float),
+                ("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("use_graphs", C.c_int32),
+                ("count_rays", C.c_int32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("scene", "width", "height", "spp", "max_depth", "spectral",
+                                           "spectrum_dim", "reserved")]
+
+
+class JobInfo(C.Structure):
+    _fields_ = [("state", C.c_int32), ("frames_done", C.c_uint32), ("samples", C.c_uint64),
+                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = ([(n, C.c_uint64) for n in ("jobs_submitted", "jobs_done", "jobs_failed", "samples_done", "rays",
+                                           "node_fetches", "tri_fetches", "steps", "graph_launches")] +
+                [(n, C.c_double) for n in ("busy_ms", "wall_ms", "msamples_per_s", "latency_p50_ms",
+                                           "latency_p99_ms", "latency_mean_ms", "latency_max_ms")] +
+                [(n, C.c_uint32) for n in ("max_active", "active", "queued", "pad")] +
+                [("hbm_budget", C.c_uint64), ("hbm_peak", C.c_uint64)])
+
+
+JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
+
+
 # every entry point of include/vision_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 SIGNATURES = {
@@ -115,6 +143,20 @@ SIGNATURES = {
     "vs_film_save": (C.c_int32, [_P, C.c_char_p]),
     "vs_image_write": (C.c_int32, [_P, C.c_char_p, C.c_uint32, C.c_uint32, _P]),
     "vs_image_read": (C.c_int32, [_P, C.c_char_p, _P, _P, _P]),
+    "vs_server_create": (C.c_int32, [_P, C.POINTER(ServerConfig), C.POINTER(_P)]),
+    "vs_server_add_scene": (C.c_int32, [_P, _P, C.POINTER(C.c_uint32)]),
+    "vs_server_submit": (C.c_int32, [_P, C.POINTER(JobDesc), C.POINTER(C.c_uint64)]),
+    "vs_server_wait": (C.c_int32, [_P, C.c_uint64, C.c_double, C.POINTER(C.c_int32)]),
+    "vs_server_job_info": (C.c_int32, [_P, C.c_uint64, C.POINTER(JobInfo)]),
+    "vs_server_result": (C.c_int32, [_P, C.c_uint64, C.c_int32, _P]),
+    "vs_server_release": (C.c_int32, [_P, C.c_uint64]),
+    "vs_server_drain": (C.c_int32, [_P, C.c_double]),
+    "vs_server_stats_get": (C.c_int32, [_P, C.POINTER(ServerStats)]),
+    "vs_server_reset_stats": (C.c_int32, [_P]),
+    "vs_server_set_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
+    "vs_server_get_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
+    "vs_server_last_error": (C.c_char_p, []),
+    "vs_server_destroy": (None, [_P]),
 }
 
 _lib_handle: Optional[C.CDLL] = None
@@ -364,3 +406,95 @@ class Scene:
     def save(self, path: str | os.PathLike | None = None):
         """Write the final picture; None writes the scene's output.fn."""
         self.ctx._check(lib().vs_film_save(self._h, None if path is None else str(path).encode()))
+
+
+def _server_check(st: int):
+    if st != VS_OK:
+        raise VisionError(st, lib().vs_server_last_error().decode())
+
+
+class Server:
+    """Batched serving of path-tracing jobs over resident scenes
+    (vs_server_*): continuous batching into persistent kernels, admission by
+    an HBM budget, captured step graphs."""
+
+    _CFG = ("hbm_budget", "max_jobs", "step_ms", "tile_w", "waves_per_cu", "use_graphs", "count_rays")
+
+    def __init__(self, ctx: "Context", hbm_budget: int = 0, max_jobs: int = 0, step_ms: float = 0.0,
+                 tile_w: int = 0, waves_per_cu: int = 0, use_graphs: bool = True, count_rays: bool = False):
+        self.ctx = ctx
+        self._h = _P()
+        cfg = ServerConfig(hbm_budget, max_jobs, step_ms, tile_w, waves_per_cu, 1 if use_graphs else 0,
+                           1 if count_rays else 0)
+        _server_check(lib().vs_server_create(ctx._h, C.byref(cfg), C.byref(self._h)))
+        self.scenes: list = []
+
+    def add_scene(self, scene: "Scene") -> int:
+        i = C.c_uint32()
+        _server_check(lib().vs_server_add_scene(self._h, scene._h, C.byref(i)))
+        self.scenes.append(scene)
+        return i.value
+
+    def submit(self, scene: int, width: int, height: int, spp: int, max_depth: int = 0, spectral: bool = False,
+               spectrum_dim: int = 0) -> int:
+        d = JobDesc(scene, width, height, spp, max_depth, 1 if spectral else 0, spectrum_dim, 0)
+        jid = C.c_uint64()
+        _server_check(lib().vs_server_submit(self._h, C.byref(d), C.byref(jid)))
+        return jid.value
+
+    def wait(self, job: int, timeout_ms: float = -1.0) -> bool:
+        done = C.c_int32()
+        _server_check(lib().vs_server_wait(self._h, job, timeout_ms, C.byref(done)))
+        return bool(done.value)
+
+    def info(self, job: int) -> dict:
+        i = JobInfo()
+        _server_check(lib().vs_server_job_info(self._h, job, C.byref(i)))
+        return {"state": JOB_STATES[i.state], "frames_done": i.frames_done, "samples": i.samples,
+                "queue_ms": i.queue_ms, "latency_ms": i.latency_ms}
+
+    def result(self, job: int, which: int = 0, shape: Optional[tuple] = None) -> np.ndarray:
+        if shape is None:
+            raise ValueError("shape=(height, width) is required")
+        out = np.zeros((shape[0], shape[1], 4), np.float32)
+        _server_check(lib().vs_server_result(self._h, job, which, out.ctypes.data))
+        return out
+
+    def release(self, job: int):
+        _server_check(lib().vs_server_release(self._h, job))
+
+    def drain(self, timeout_ms: float = -1.0):
+        _server_check(lib().vs_server_drain(self._h, timeout_ms))
+
+    def stats(self) -> dict:
+        s = ServerStats()
+        _server_check(lib().vs_server_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_ if f != "pad"}
+
+    def reset_stats(self):
+        _server_check(lib().vs_server_reset_stats(self._h))
+
+    def config(self) -> dict:
+        c = ServerConfig()
+        _server_check(lib().vs_server_get_config(self._h, C.byref(c)))
+        return {f: getattr(c, f) for f in self._CFG}
+
+    def set_config(self, **kw):
+        c = ServerConfig()
+        _server_check(lib().vs_server_get_config(self._h, C.byref(c)))
+        for k, v in kw.items():
+            if k not in self._CFG:
+                raise KeyError(k)
+            setattr(c, k, int(v) if k != "step_ms" else float(v))
+        _server_check(lib().vs_server_set_config(self._h, C.byref(c)))
+
+    def close(self):
+        if self._h:
+            lib().vs_server_destroy(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

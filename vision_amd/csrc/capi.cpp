@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "bvh.h"
+#include "context.h"
 #include "dev_scene.h"
 #include "launch.h"
 #include "scene.h"
@@ -23,134 +24,18 @@
 #include "tables.h"
 #include "wavefront.h"
 
-struct vs_context {
-    int device{0};// -1: host-only context (scene parsing, BVH build, export)
-    hipStream_t stream{nullptr};
-    hipEvent_t ev0{nullptr}, ev1{nullptr};
-    hipEvent_t tr0{nullptr}, tr1{nullptr}, tr2{nullptr};// around wavefront traversal kernels
-    std::string last_error;
-};
-
 namespace {
 
-struct DeviceError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-void hip_check(hipError_t e, const char *what) {
-    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
-}
-
-struct DevBuf {
-    void *ptr{nullptr};
-    size_t bytes{0};
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-    void alloc(size_t n) {
-        release();
-        if (n == 0) n = 16;
-        hipError_t e = hipMalloc(&ptr, n);
-        if (e != hipSuccess) {
-            ptr = nullptr;
-            throw DeviceError(std::string("hipMalloc failed: ") + hipGetErrorString(e));
-        }
-        bytes = n;
-    }
-    template <typename T>
-    void upload(const std::vector<T> &v, hipStream_t s) {
-        alloc(v.size() * sizeof(T));
-        if (!v.empty()) hip_check(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
-    }
-    template <typename T>
-    T *as() const { return static_cast<T *>(ptr); }
-};
+using namespace vs_internal;
 
 }// namespace
 
-struct vs_scene {
-    vs_context *ctx{nullptr};
-    vs::SceneHost host;
-    vs::Bvh8 bvh;
-    bool host_bvh{false};// BVH built on the host
-    bool built{false};   // BVH and scene resident in HBM
-    double build_ms{0.0};
-    vs::PackedScene packed;
-    // device buffers
-    DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, mat, light, alias, func;
-    DevBuf spd, lut, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
-    DevBuf fis_lut, fis_cond, fis_cond_func, fis_marg, fis_marg_func;
-    DevBuf film_rt, film_out, film_pic, counters;
-    DevBuf scratch_rays, scratch_hits;
-    // wavefront form: per-path state, queues, append counters
-    uint32_t kernel_mode{VS_KERNEL_MEGAKERNEL};
-    uint32_t wf_chunk_frames{8};
-    uint32_t wf_trace_block{128};
-    size_t wf_capacity{0};
-    DevBuf wf_o, wf_d, wf_thr, wf_L, wf_meta, wf_lam, wf_lpdf, wf_hit, wf_sh_o, wf_sh_d, wf_sh_c, wf_q0, wf_q1,
-        wf_counts;
-    uint32_t *wf_host_counts{nullptr};// pinned
-    // ReSTIR DI ("rt" integrator): reservoirs, surfaces, surface extensions,
-    // and the camera of the previous frame (3x4 rows of camera-to-world)
-    DevBuf rs_rsv, rs_surf, rs_ext;
-    uint32_t rs_pixels{0};
-    float prev_c2w[12]{};
-    vs::SceneView view{};
-    vs::CameraView cam{};
-    uint32_t frame_index{0};
-    bool counting{false};
-    vs_render_stats stats{};
-    // exported flat arrays
-    std::vector<uint32_t> exp_inst;
-    std::vector<float> exp_camera, exp_render, exp_restir;
-};
+namespace vs_internal {
 
-namespace {
-
-vs_status fail(vs_context *ctx, vs_status code, const std::string &msg) {
-    if (ctx) ctx->last_error = msg;
-    return code;
-}
-
-template <typename F>
-vs_status guarded(vs_context *ctx, F &&f) {
-    try {
-        f();
-        if (ctx) ctx->last_error.clear();
-        return VS_OK;
-    } catch (const vs::JsonError &e) {
-        return fail(ctx, VS_ERROR_PARSE, e.what());
-    } catch (const DeviceError &e) {
-        return fail(ctx, VS_ERROR_DEVICE, e.what());
-    } catch (const std::bad_alloc &) {
-        return fail(ctx, VS_ERROR_OUT_OF_MEMORY, "host allocation failed");
-    } catch (const std::invalid_argument &e) {
-        return fail(ctx, VS_ERROR_INVALID_ARGUMENT, e.what());
-    } catch (const std::logic_error &e) {
-        return fail(ctx, VS_ERROR_STATE, e.what());
-    } catch (const std::exception &e) {
-        std::string m = e.what();
-        vs_status code = (m.find("unsupported") != std::string::npos || m.find("not supported") != std::string::npos)
-                             ? VS_ERROR_UNSUPPORTED
-                             : VS_ERROR_INVALID_ARGUMENT;
-        return fail(ctx, code, m);
-    }
-}
-
-void fill_camera(vs_scene *s) {
-    const vs::CameraHost &c = s->host.camera;
-    vs::CameraView &v = s->cam;
-    // c2w rows [R | t]
-    for (int r = 0; r < 3; ++r)
-        for (int col = 0; col < 4; ++col) v.c2w[r * 4 + col] = vs::m4_at(c.c2w, r, col);
+void camera_set_resolution(vs::CameraView &v, uint32_t width, uint32_t height) {
     // RadianceCollector::update_screen_window + Sensor::update_resolution_
-    float ratio = float(c.width) / float(c.height);
+    // (src/base/sensor/radiance_collector.cpp:33-42, src/base/sensor/sensor.cpp:63-76)
+    float ratio = float(width) / float(height);
     float lower_x = -1.f, upper_x = 1.f, lower_y = -1.f, upper_y = 1.f;
     if (ratio > 1.f) {
         lower_x = -ratio;
@@ -161,13 +46,82 @@ void fill_camera(vs_scene *s) {
     }
     v.lower_x = lower_x;
     v.upper_y = upper_y;
-    v.raster_sx = (upper_x - lower_x) / float(c.width);
-    v.raster_sy = (upper_y - lower_y) / float(c.height);
+    v.raster_sx = (upper_x - lower_x) / float(width);
+    v.raster_sy = (upper_y - lower_y) / float(height);
+    v.width = width;
+    v.height = height;
+}
+
+void upload_mode_tables(const vs::PackedScene &pk, ModeTables &t, hipStream_t st) {
+    t.mat.upload(pk.mats, st);
+    t.light.upload(pk.lights, st);
+    t.alias.upload(pk.alias, st);
+    t.func.upload(pk.func, st);
+    t.spd.upload(pk.spd, st);
+    t.env_tex.upload(pk.env_tex, st);
+    t.env_cond.upload(pk.env_cond, st);
+    t.env_cond_func.upload(pk.env_cond_func, st);
+    t.env_marg.upload(pk.env_marg, st);
+    t.env_marg_func.upload(pk.env_marg_func, st);
+}
+
+void set_mode_view(vs::SceneView &v, const vs::PackedScene &pk, const ModeTables &t) {
+    v.mat = t.mat.as<vs::MaterialRecord>();
+    v.lights = t.light.as<vs::LightRecord>();
+    v.alias = t.alias.as<vs::AliasEntry>();
+    v.alias_func = t.func.as<float>();
+    v.select_offset = pk.select_offset;
+    v.select_integral = pk.select_integral;
+    v.spd = t.spd.as<float>();
+    v.cie_interval = pk.cie_interval;
+    v.cie_y_integral = pk.cie_y_integral;
+    v.spectrum_rgb = pk.rgb ? 1u : 0u;
+    v.env_light = pk.env_light;
+    v.env_tex = t.env_tex.as<float>();
+    v.env_cond = t.env_cond.as<vs::AliasEntry>();
+    v.env_cond_func = t.env_cond_func.as<float>();
+    v.env_marg = t.env_marg.as<vs::AliasEntry>();
+    v.env_marg_func = t.env_marg_func.as<float>();
+    v.env_w = pk.env_w;
+    v.env_h = pk.env_h;
+    v.env_marg_integral = pk.env_marg_integral;
+    v.world_diameter = 2.f * pk.world_radius;
+    for (int i = 0; i < 9; ++i) {
+        v.env_w2o[i] = pk.env_w2o[i];
+        v.env_o2w[i] = pk.env_o2w[i];
+    }
+    v.env_scale = pk.env_scale;
+}
+
+vs::RenderView render_view(const vs_scene *s) {
+    const vs::SceneHost &h = s->host;
+    vs::RenderView rv{};
+    rv.max_depth = h.integrator.max_depth;
+    rv.min_depth = h.integrator.min_depth;
+    rv.rr_threshold = h.integrator.rr_threshold;
+    rv.mis_mode = h.integrator.mis_mode;
+    rv.accumulation = h.camera.accumulation;
+    rv.exposure = h.camera.exposure;
+    rv.tone_mapper = h.camera.tone_mapper == "aces" ? 1u : (h.camera.tone_mapper == "reinhard" ? 2u : 0u);
+    rv.write_output = 1;
+    rv.spectrum_dim = s->packed.rgb ? 3u : h.spectrum_dim;
+    return rv;
+}
+
+}// namespace vs_internal
+
+namespace {
+
+void fill_camera(vs_scene *s) {
+    const vs::CameraHost &c = s->host.camera;
+    vs::CameraView &v = s->cam;
+    // c2w rows [R | t]
+    for (int r = 0; r < 3; ++r)
+        for (int col = 0; col < 4; ++col) v.c2w[r * 4 + col] = vs::m4_at(c.c2w, r, col);
+    camera_set_resolution(v, c.width, c.height);
     v.tan_half_fov = std::tan(c.fov_y * 0.5f * 3.14159265358979323846f / 180.f);
     v.lens_radius = c.lens_radius;
     v.focal_distance = c.focal_distance;
-    v.width = c.width;
-    v.height = c.height;
     const int kind = vs::filter_kind(c);
     v.filter = kind <= 1 ? uint32_t(kind) : 2u;
     v.filter_radius = c.filter_radius;
@@ -401,18 +355,8 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         s->tri.upload(h.wTri, st);
         s->tri_inst.upload(h.triInst, st);
         s->inst.upload(h.instRecords, st);
-        const vs::PackedScene &pk = s->packed;
-        s->mat.upload(pk.mats, st);
-        s->light.upload(pk.lights, st);
-        s->alias.upload(pk.alias, st);
-        s->func.upload(pk.func, st);
-        s->spd.upload(pk.spd, st);
         s->lut.upload(vs::lut_tables(), st);
-        s->env_tex.upload(pk.env_tex, st);
-        s->env_cond.upload(pk.env_cond, st);
-        s->env_cond_func.upload(pk.env_cond_func, st);
-        s->env_marg.upload(pk.env_marg, st);
-        s->env_marg_func.upload(pk.env_marg_func, st);
+        upload_mode_tables(s->packed, s->mt, st);
         vs::FilterTable ft = vs::build_filter_table(h.camera);
         s->fis_lut.upload(ft.lut, st);
         s->fis_cond.upload(ft.cond, st);
@@ -437,34 +381,10 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         v.tri = s->tri.as<vs::u3>();
         v.tri_inst = s->tri_inst.as<uint32_t>();
         v.inst = s->inst.as<vs::InstanceRecord>();
-        v.mat = s->mat.as<vs::MaterialRecord>();
-        v.lights = s->light.as<vs::LightRecord>();
-        v.alias = s->alias.as<vs::AliasEntry>();
-        v.alias_func = s->func.as<float>();
         v.num_lights = uint32_t(h.lights.size());
         v.light_sampler = h.light_sampler_type == "power" ? 1u : 0u;
-        v.select_offset = pk.select_offset;
-        v.select_integral = pk.select_integral;
-        v.spd = s->spd.as<float>();
         v.lut = s->lut.as<float>();
-        v.cie_interval = pk.cie_interval;
-        v.cie_y_integral = pk.cie_y_integral;
-        v.spectrum_rgb = pk.rgb ? 1u : 0u;
-        v.env_light = pk.env_light;
-        v.env_tex = s->env_tex.as<float>();
-        v.env_cond = s->env_cond.as<vs::AliasEntry>();
-        v.env_cond_func = s->env_cond_func.as<float>();
-        v.env_marg = s->env_marg.as<vs::AliasEntry>();
-        v.env_marg_func = s->env_marg_func.as<float>();
-        v.env_w = pk.env_w;
-        v.env_h = pk.env_h;
-        v.env_marg_integral = pk.env_marg_integral;
-        v.world_diameter = 2.f * pk.world_radius;
-        for (int i = 0; i < 9; ++i) {
-            v.env_w2o[i] = pk.env_w2o[i];
-            v.env_o2w[i] = pk.env_o2w[i];
-        }
-        v.env_scale = pk.env_scale;
+        set_mode_view(v, s->packed, s->mt);
         v.stack_size = s->bvh.max_depth;
         v.ray_offset_factor = h.ray_offset_factor;
         v.num_tris = uint32_t(h.wTri.size());
@@ -878,16 +798,7 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
         hipStream_t st = s->ctx->stream;
         const vs::SceneHost &h = s->host;
-        vs::RenderView rv{};
-        rv.max_depth = h.integrator.max_depth;
-        rv.min_depth = h.integrator.min_depth;
-        rv.rr_threshold = h.integrator.rr_threshold;
-        rv.mis_mode = h.integrator.mis_mode;
-        rv.accumulation = h.camera.accumulation;
-        rv.exposure = h.camera.exposure;
-        rv.tone_mapper = h.camera.tone_mapper == "aces" ? 1u : (h.camera.tone_mapper == "reinhard" ? 2u : 0u);
-        rv.write_output = 1;
-        rv.spectrum_dim = s->packed.rgb ? 3u : h.spectrum_dim;
+        vs::RenderView rv = render_view(s);
         unsigned long long *ctr = nullptr;
         if (s->counting) {
             ctr = s->counters.as<unsigned long long>();

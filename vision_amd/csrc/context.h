@@ -1,0 +1,162 @@
+// Internal state behind the C ABI, shared by the translation units that
+// implement it (capi.cpp, serve.cpp).  Not installed.
+#pragma once
+
+#include "../../include/vision_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "bvh.h"
+#include "dev_scene.h"
+#include "json.h"
+#include "scene.h"
+#include "tables.h"
+
+struct vs_context {
+    int device{0};// -1: host-only context (scene parsing, BVH build, export)
+    hipStream_t stream{nullptr};
+    hipEvent_t ev0{nullptr}, ev1{nullptr};
+    hipEvent_t tr0{nullptr}, tr1{nullptr}, tr2{nullptr};// around wavefront traversal kernels
+    std::string last_error;
+};
+
+namespace vs_internal {
+
+struct DeviceError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+inline void hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw DeviceError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct DevBuf {
+    void *ptr{nullptr};
+    size_t bytes{0};
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n) {
+        release();
+        if (n == 0) n = 16;
+        hipError_t e = hipMalloc(&ptr, n);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            throw DeviceError(std::string("hipMalloc failed: ") + hipGetErrorString(e));
+        }
+        bytes = n;
+    }
+    template <typename T>
+    void upload(const std::vector<T> &v, hipStream_t s) {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) hip_check(hipMemcpyAsync(ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+    }
+    template <typename T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+// the device tables that depend on the spectrum mode (sRGB or hero
+// wavelengths): materials, lights and their alias tables, spectral curves,
+// the environment map
+struct ModeTables {
+    DevBuf mat, light, alias, func, spd, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
+};
+
+}// namespace vs_internal
+
+using vs_internal::DevBuf;
+
+struct vs_scene {
+    vs_context *ctx{nullptr};
+    vs::SceneHost host;
+    vs::Bvh8 bvh;
+    bool host_bvh{false};// BVH built on the host
+    bool built{false};   // BVH and scene resident in HBM
+    double build_ms{0.0};
+    vs::PackedScene packed;
+    // device buffers
+    DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, lut;
+    vs_internal::ModeTables mt;// materials, lights and environment in the scene's spectrum mode
+    DevBuf fis_lut, fis_cond, fis_cond_func, fis_marg, fis_marg_func;
+    DevBuf film_rt, film_out, film_pic, counters;
+    DevBuf scratch_rays, scratch_hits;
+    // wavefront form: per-path state, queues, append counters
+    uint32_t kernel_mode{VS_KERNEL_MEGAKERNEL};
+    uint32_t wf_chunk_frames{8};
+    uint32_t wf_trace_block{128};
+    size_t wf_capacity{0};
+    DevBuf wf_o, wf_d, wf_thr, wf_L, wf_meta, wf_lam, wf_lpdf, wf_hit, wf_sh_o, wf_sh_d, wf_sh_c, wf_q0, wf_q1,
+        wf_counts;
+    uint32_t *wf_host_counts{nullptr};// pinned
+    // ReSTIR DI ("rt" integrator): reservoirs, surfaces, surface extensions,
+    // and the camera of the previous frame (3x4 rows of camera-to-world)
+    DevBuf rs_rsv, rs_surf, rs_ext;
+    uint32_t rs_pixels{0};
+    float prev_c2w[12]{};
+    vs::SceneView view{};
+    vs::CameraView cam{};
+    uint32_t frame_index{0};
+    bool counting{false};
+    vs_render_stats stats{};
+    // exported flat arrays
+    std::vector<uint32_t> exp_inst;
+    std::vector<float> exp_camera, exp_render, exp_restir;
+};
+
+namespace vs_internal {
+
+inline vs_status fail(vs_context *ctx, vs_status code, const std::string &msg) {
+    if (ctx) ctx->last_error = msg;
+    return code;
+}
+
+template <typename F>
+vs_status guarded(vs_context *ctx, F &&f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return VS_OK;
+    } catch (const vs::JsonError &e) {
+        return fail(ctx, VS_ERROR_PARSE, e.what());
+    } catch (const DeviceError &e) {
+        return fail(ctx, VS_ERROR_DEVICE, e.what());
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, VS_ERROR_OUT_OF_MEMORY, "host allocation failed");
+    } catch (const std::invalid_argument &e) {
+        return fail(ctx, VS_ERROR_INVALID_ARGUMENT, e.what());
+    } catch (const std::logic_error &e) {
+        return fail(ctx, VS_ERROR_STATE, e.what());
+    } catch (const std::exception &e) {
+        std::string m = e.what();
+        vs_status code = (m.find("unsupported") != std::string::npos || m.find("not supported") != std::string::npos)
+                             ? VS_ERROR_UNSUPPORTED
+                             : VS_ERROR_INVALID_ARGUMENT;
+        return fail(ctx, code, m);
+    }
+}
+
+
+}// namespace vs_internal
+
+namespace vs_internal {
+
+// screen window and raster scale of a resolution (camera fields only)
+void camera_set_resolution(vs::CameraView &v, uint32_t width, uint32_t height);
+// upload the mode-dependent tables of a packed scene / point a view at them
+void upload_mode_tables(const vs::PackedScene &pk, ModeTables &t, hipStream_t st);
+void set_mode_view(vs::SceneView &v, const vs::PackedScene &pk, const ModeTables &t);
+// the render settings of a scene's integrator and film
+vs::RenderView render_view(const vs_scene *s);
+
+}// namespace vs_internal

@@ -534,7 +534,7 @@ __device__ __forceinline__ void path_begin(const CameraView &cam, const RenderVi
 }
 
 // the whole path in one thread (the megakernel form)
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS = MS_ALL>
 __device__ f3 trace_path(const SceneView &sc, const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
                          uint32_t frame, uint2 *stack, uint32_t stride, PathCounters &pc) {
     PathState<NS> ps;
@@ -543,7 +543,7 @@ __device__ f3 trace_path(const SceneView &sc, const CameraView &cam, const Rende
         TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
         ++pc.rays;
         ShadowRay sr;
-        path_shade<NS, RGB>(sc, rv, ps, h, sr);
+        path_shade<NS, RGB, MS>(sc, rv, ps, h, sr);
         if (sr.valid) {
             uint32_t dn = 0, dt = 0;
             TraceHit sh = traverse<true>(sc, sr.o, sr.d, 0.f, 1.f - kShadowEpsilon, stack, stride, dn, dt);
