@@ -136,12 +136,249 @@ def restir_cpu_baseline(scene, frames: int = 2) -> dict:
                       f"per-frame BVH2 / light-table setup included)"}
 
 
+SERVE_RES = ((256, 256), (512, 512), (1024, 1024), (1920, 1080))
+SERVE_SPP = (1, 4, 16, 64)
+
+
+def serve_jobs(n: int, n_scenes: int, seed: int = 0) -> list:
+    """The serving mix: scene uniform over the resident scenes, resolution
+    from SERVE_RES, spp from SERVE_SPP, max depth uniform in [8, 16], sRGB or
+    hero-4 spectral with equal odds."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for _ in range(n):
+        w, h = SERVE_RES[int(rng.integers(0, len(SERVE_RES)))]
+        jobs.append(dict(scene=int(rng.integers(0, n_scenes)), width=w, height=h,
+                         spp=SERVE_SPP[int(rng.integers(0, len(SERVE_SPP)))], max_depth=int(rng.integers(8, 17)),
+                         spectral=bool(rng.integers(0, 2))))
+    return jobs
+
+
+def plain_job(sc, job: dict) -> float:
+    """One job the plain way: switch the scene to the job's spectrum mode,
+    resolution and depth, render all its frames in one megakernel launch,
+    download the output film.  Returns its service time in seconds."""
+    t0 = time.perf_counter()
+    sc.set_spectrum(job["spectral"], 4)
+    sc.set_resolution(job["width"], job["height"])
+    sc.set_integrator(job["max_depth"], 5, 1.0)
+    sc.render(job["spp"], fused=True)
+    sc.film(1)
+    return time.perf_counter() - t0
+
+
+def lindley(arrivals, service) -> list:
+    """Latencies of a first-come first-served single server (one job on the
+    device at a time) from arrival and service times."""
+    free, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        start = max(a, free)
+        free = start + s
+        out.append(free - a)
+    return out
+
+
+def percentile(xs, q: float) -> float:
+    ys = sorted(xs)
+    r = max(1, int(-(-q * len(ys) // 1)))
+    return ys[min(len(ys), r) - 1]
+
+
+def serve_main(args) -> None:
+    """Batched serving against the plain per-job form on the same job list:
+    throughput with every job queued at once, then latency under Poisson
+    arrivals at `load` times the plain form's capacity."""
+    import threading
+
+    import numpy as np
+    import vision_amd as va
+    from vision_amd import scenegen
+
+    out = SCENE_CACHE / "serving"
+    ctx = va.Context(0)
+    scenes, build_s, tris = [], 0.0, []
+    for k, t in enumerate(scenegen.SERVING_TRIANGLES):
+        sd = scenegen.serving_scene(out, t, seed=k)
+        sc = ctx.load_text(sd, str(out))
+        t0 = time.perf_counter()
+        sc.build_accel()
+        build_s += time.perf_counter() - t0
+        tris.append(sc.info()["num_triangles"])
+        scenes.append(sc)
+        print(f"scene {k}: {tris[-1]} triangles", file=sys.stderr, flush=True)
+    jobs = serve_jobs(args.jobs, len(scenes), seed=args.seed)
+    total_samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+
+    # plain form: one job at a time, in order
+    for j in jobs[:args.warmup]:
+        plain_job(scenes[j["scene"]], j)
+    service = []
+    t0 = time.perf_counter()
+    for n, j in enumerate(jobs):
+        service.append(plain_job(scenes[j["scene"]], j))
+        if n % 100 == 99:
+            print(f"plain: {n + 1} jobs", file=sys.stderr, flush=True)
+    plain_wall = time.perf_counter() - t0
+    plain_msps = total_samples / plain_wall / 1e6
+    # plain films of a few jobs for the batched-vs-plain comparison
+    check = list(range(min(args.check_jobs, len(jobs))))
+    plain_films = {}
+    for i in check:
+        plain_job(scenes[jobs[i]["scene"]], jobs[i])
+        plain_films[i] = scenes[jobs[i]["scene"]].film(0)
+
+    srv = va.Server(ctx, max_jobs=args.max_jobs, step_ms=args.step_ms, tile_w=args.tile_w,
+                    waves_per_cu=args.waves_per_cu)
+    for sc in scenes:
+        srv.add_scene(sc)
+
+    def run_batch(js, arrivals=None):
+        """Submit js (at their arrival offsets, or all at once) and wait for
+        all; returns the server statistics."""
+        srv.reset_stats()
+        ids = [None] * len(js)
+
+        def submitter():
+            t_start = time.perf_counter()
+            for i, j in enumerate(js):
+                if arrivals is not None:
+                    d = t_start + arrivals[i] - time.perf_counter()
+                    if d > 0:
+                        time.sleep(d)
+                ids[i] = srv.submit(j["scene"], j["width"], j["height"], j["spp"], max_depth=j["max_depth"],
+                                    spectral=j["spectral"], spectrum_dim=4)
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        done = 0
+        for i in range(len(js)):
+            while ids[i] is None:
+                time.sleep(0.0005)
+            srv.wait(ids[i])
+            done += 1
+            if done % 100 == 0:
+                print(f"batched: {done} jobs", file=sys.stderr, flush=True)
+        th.join()
+        return ids, srv.stats()
+
+    # warm-up (graph capture, first-step rate estimate)
+    wids, _ = run_batch(jobs[:max(args.warmup, 1)])
+    for i in wids:
+        srv.release(i)
+    # throughput: every job queued at once
+    ids, st_sat = run_batch(jobs)
+    relmse = 0.0
+    identical = 0
+    for i in check:
+        j = jobs[i]
+        film = srv.result(ids[i], 0, (j["height"], j["width"]))
+        ref = plain_films[i]
+        identical += int(np.array_equal(film.view(np.uint32), ref.view(np.uint32)))
+        x, r = film[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+        relmse = max(relmse, float(np.mean((x - r) ** 2 / (r * r + 1e-2))))
+    for i in ids:
+        srv.release(i)
+    batched_msps = st_sat["msamples_per_s"]
+    # latency: Poisson arrivals at load x the plain capacity
+    mean_service = sum(service) / len(service)
+    lam = args.load / mean_service
+    rng = np.random.default_rng(args.seed + 1)
+    arrivals = np.cumsum(rng.exponential(1.0 / lam, len(jobs))).tolist()
+    arrivals = [a - arrivals[0] for a in arrivals]
+    ids, st_lat = run_batch(jobs, arrivals)
+    for i in ids:
+        srv.release(i)
+    plain_lat = lindley(arrivals, service)
+    # counting run of one saturated batch prefix for the roofline: rays and BVH fetches
+    srv.set_config(count_rays=True)
+    sub = jobs[:args.count_jobs]
+    ids, st_cnt = run_batch(sub)
+    for i in ids:
+        srv.release(i)
+    srv.set_config(count_rays=False)
+    sub_samples = sum(j["width"] * j["height"] * j["spp"] for j in sub)
+    alg_bytes = (NODE_BYTES * st_cnt["node_fetches"] + TRI_BYTES * st_cnt["tri_fetches"] +
+                 RAY_BYTES * st_cnt["rays"] + FILM_BYTES * sub_samples)
+    # device time of the uncounted saturated run, scaled to the same prefix by samples
+    kernel_s = st_sat["busy_ms"] / 1e3 * sub_samples / total_samples
+    achieved = alg_bytes / kernel_s / 1e9
+    rays_per_sample = st_cnt["rays"] / sub_samples
+    line = {
+        "metric": METRIC,
+        "value": round(batched_msps, 3),
+        "unit": "Msamples/s",
+        "n_gpus": 1,
+        "steps": 1,
+        "warmup": args.warmup,
+        "ms_per_step": round(st_sat["wall_ms"], 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched serving: {len(jobs)} jobs over 8 resident Cornell-box scenes of "
+                        f"{min(tris):,}-{max(tris):,} triangles (displaced level-5 icospheres, diffuse); "
+                        "resolution from {256^2, 512^2, 1024^2, 1920x1080}, spp from {1, 4, 16, 64}, max depth "
+                        "U[8, 16], sRGB or hero-4 spectral; persistent multi-job kernels in captured hipGraphs, "
+                        "HBM-budget admission; a step is the whole job list, queued at once",
+            "jobs": len(jobs), "samples": total_samples, "triangles": tris, "seed": args.seed,
+            "max_jobs": args.max_jobs, "step_ms": args.step_ms, "tile_w": args.tile_w,
+            "waves_per_cu": args.waves_per_cu},
+        "plain_msamples_per_s": round(plain_msps, 3),
+        "batched_over_plain": round(batched_msps / plain_msps, 3),
+        "relmse_batched_vs_plain": relmse,
+        "bit_identical_jobs_checked": f"{identical}/{len(check)}",
+        "latency": {
+            "arrivals": "Poisson", "rate_jobs_per_s": round(lam, 3), "load_vs_plain_capacity": args.load,
+            "batched_p50_ms": round(st_lat["latency_p50_ms"], 2), "batched_p99_ms": round(st_lat["latency_p99_ms"], 2),
+            "plain_p50_ms": round(percentile(plain_lat, 0.5) * 1e3, 2),
+            "plain_p99_ms": round(percentile(plain_lat, 0.99) * 1e3, 2),
+            "plain_from": "measured per-job service times, first-come first-served on one device",
+            "saturated_batched_p50_ms": round(st_sat["latency_p50_ms"], 2),
+            "saturated_batched_p99_ms": round(st_sat["latency_p99_ms"], 2)},
+        "mrays_per_s": round(rays_per_sample * batched_msps, 2),
+        "rays_per_sample": round(rays_per_sample, 3),
+        "bvh_build_s": round(build_s, 3),
+        "steps_run": st_sat["steps"], "graph_launches": st_sat["graph_launches"],
+        "max_active_jobs": st_sat["max_active"], "hbm_peak_bytes": st_sat["hbm_peak"],
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "serve_kernel (persistent multi-job megakernel)",
+                     "kernel_ms_per_step": round(st_sat["busy_ms"], 3),
+                     "algorithmic_bytes_per_sample": round(alg_bytes / sub_samples, 1)},
+    }
+    if not args.no_cpu:
+        j0 = dict(scene=0, width=256, height=256, spp=1, max_depth=8, spectral=False)
+        sc = scenes[0]
+        sc.set_spectrum(False, 4)
+        sc.set_resolution(256, 256)
+        sc.set_integrator(8, 5, 1.0)
+        line["cpu_baseline"] = cpu_baseline(sc, 1)
+        line["cpu_baseline"]["sample"] += f"; scene 0 ({tris[0]} triangles), 256x256, depth 8, sRGB"
+    srv.close()
+    for sc in scenes:
+        sc.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox", "restir"])
+    ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox", "restir", "serve"])
+    ap.add_argument("--jobs", type=int, default=1000, help="serve: jobs in the list")
+    ap.add_argument("--seed", type=int, default=0, help="serve: job-list seed")
+    ap.add_argument("--load", type=float, default=0.8, help="serve: Poisson rate / plain capacity")
+    ap.add_argument("--max-jobs", type=int, default=64, help="serve: jobs admitted at once")
+    ap.add_argument("--step-ms", type=float, default=16.0)
+    ap.add_argument("--tile-w", type=int, default=8)
+    ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--check-jobs", type=int, default=16, help="serve: jobs compared with their plain film")
+    ap.add_argument("--count-jobs", type=int, default=100, help="serve: jobs in the ray-counting run")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--chunk-frames", type=int, default=16)
@@ -162,6 +399,9 @@ def main():
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; multi-GPU scaling is not part of it", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "serve":
+        serve_main(args)
+        return
 
     import vision_amd as va
     from vision_amd import scenegen

@@ -178,6 +178,12 @@ vs_status vs_scene_set_resolution(vs_scene *scene, uint32_t width, uint32_t heig
 vs_status vs_scene_update_camera(vs_scene *scene, const float *c2w, float fov_y);
 /* Override max_depth / min_depth / rr_threshold of the integrator. */
 vs_status vs_scene_set_integrator(vs_scene *scene, uint32_t max_depth, uint32_t min_depth, float rr_threshold);
+/* Switch the spectrum mode of a built scene, as its JSON "spectrum" block
+ * would (SRGBSpectrum / HeroWavelengthSpectrum, src/render_core/spectra/srgb.cpp,
+ * hero.cpp:227-348): spectral = 0 sRGB, 1 hero wavelengths with dimension 3 or 4.
+ * The other mode's material, light and environment tables are packed on first
+ * use and kept, so switching back and forth costs nothing after that. */
+vs_status vs_scene_set_spectrum(vs_scene *scene, int32_t spectral, uint32_t dimension);
 
 /* Closest-hit / any-hit for a batch of rays in host memory (the parity hook
  * for Geometry::trace_closest / trace_occlusion, geometry.cpp:167-184). */

@@ -126,6 +126,7 @@ SIGNATURES = {
                                         C.POINTER(C.POINTER(C.c_float)), C.POINTER(C.c_uint32)]),
     "vs_scene_set_resolution": (C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     "vs_scene_set_integrator": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_float]),
+    "vs_scene_set_spectrum": (C.c_int32, [_P, C.c_int32, C.c_uint32]),
     "vs_scene_update_camera": (C.c_int32, [_P, _P, C.c_float]),
     "vs_scene_restir_state": (C.c_int32, [_P, _P, _P]),
     "vs_trace_closest": (C.c_int32, [_P, _P, C.c_uint32, _P]),
@@ -274,6 +275,9 @@ class Scene:
 
     def set_integrator(self, max_depth: int, min_depth: int, rr_threshold: float):
         self.ctx._check(lib().vs_scene_set_integrator(self._h, max_depth, min_depth, rr_threshold))
+
+    def set_spectrum(self, spectral: bool, dimension: int = 4):
+        self.ctx._check(lib().vs_scene_set_spectrum(self._h, 1 if spectral else 0, dimension))
 
     def export(self) -> dict:
         """Flattened scene arrays (copies) — the input of the CPU oracle."""

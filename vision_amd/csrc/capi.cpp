@@ -108,6 +108,27 @@ vs::RenderView render_view(const vs_scene *s) {
     return rv;
 }
 
+void ensure_alt_mode(vs_scene *s) {
+    if (s->have_alt) return;
+    vs::SceneHost other = s->host;
+    const bool rgb = s->packed.rgb;
+    other.spectrum_type = rgb ? "hero" : "srgb";
+    other.spectrum_dim = rgb ? 4u : 3u;
+    s->packed_alt = vs::pack_scene(other);
+    upload_mode_tables(s->packed_alt, s->mt_alt, s->ctx->stream);
+    hip_check(hipStreamSynchronize(s->ctx->stream), "mode tables");
+    s->have_alt = true;
+}
+
+vs::SceneView view_in_mode(vs_scene *s, bool rgb) {
+    vs::SceneView v = s->view;
+    if (rgb != s->packed.rgb) {
+        ensure_alt_mode(s);
+        set_mode_view(v, s->packed_alt, s->mt_alt);
+    }
+    return v;
+}
+
 }// namespace vs_internal
 
 namespace {
@@ -385,6 +406,10 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         v.light_sampler = h.light_sampler_type == "power" ? 1u : 0u;
         v.lut = s->lut.as<float>();
         set_mode_view(v, s->packed, s->mt);
+        s->diffuse_only = true;
+        for (const vs::MaterialRecord &m : s->packed.mats)
+            if (m.type != vs::MAT_DIFFUSE && m.type != vs::MAT_OREN_NAYAR && m.type != vs::MAT_BLACK)
+                s->diffuse_only = false;
         v.stack_size = s->bvh.max_depth;
         v.ray_offset_factor = h.ray_offset_factor;
         v.num_tris = uint32_t(h.wTri.size());
@@ -496,6 +521,25 @@ vs_status vs_scene_update_camera(vs_scene *s, const float *c2w, float fov_y) {
         affine_inverse(probe, probe);// rejects a singular matrix before anything changes
         fill_camera(s);
         fill_restir_export(s);
+    });
+}
+
+vs_status vs_scene_set_spectrum(vs_scene *s, int32_t spectral, uint32_t dimension) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    if (spectral && dimension != 3 && dimension != 4)
+        return fail(s->ctx, VS_ERROR_INVALID_ARGUMENT, "hero dimension must be 3 or 4");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        const bool rgb = !spectral;
+        if (rgb != s->packed.rgb) {
+            ensure_alt_mode(s);
+            s->mt.swap(s->mt_alt);
+            std::swap(s->packed, s->packed_alt);
+            set_mode_view(s->view, s->packed, s->mt);
+        }
+        s->host.spectrum_type = rgb ? "srgb" : "hero";
+        s->host.spectrum_dim = rgb ? 3u : dimension;
     });
 }
 
@@ -817,14 +861,16 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         } else if (fused) {
             rv.frame_start = s->frame_index;
             rv.spp = spp;
-            hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr, st),
+            hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr,
+                                      s->diffuse_only, st),
                       "pt launch");
             ++launches;
         } else {
             for (uint32_t f = 0; f < spp; ++f) {
                 rv.frame_start = s->frame_index + f;
                 rv.spp = 1;
-                hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr, st),
+                hip_check(vs::launch_pt(s->view, s->cam, rv, s->film_rt.as<float4>(), s->film_out.as<float4>(), ctr,
+                                      s->diffuse_only, st),
                           "pt launch");
                 ++launches;
             }

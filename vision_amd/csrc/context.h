@@ -9,6 +9,7 @@
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "bvh.h"
@@ -64,6 +65,10 @@ struct DevBuf {
     }
     template <typename T>
     T *as() const { return static_cast<T *>(ptr); }
+    void swap(DevBuf &o) noexcept {
+        std::swap(ptr, o.ptr);
+        std::swap(bytes, o.bytes);
+    }
 };
 
 // the device tables that depend on the spectrum mode (sRGB or hero
@@ -71,6 +76,18 @@ struct DevBuf {
 // the environment map
 struct ModeTables {
     DevBuf mat, light, alias, func, spd, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
+    void swap(ModeTables &o) noexcept {
+        mat.swap(o.mat);
+        light.swap(o.light);
+        alias.swap(o.alias);
+        func.swap(o.func);
+        spd.swap(o.spd);
+        env_tex.swap(o.env_tex);
+        env_cond.swap(o.env_cond);
+        env_cond_func.swap(o.env_cond_func);
+        env_marg.swap(o.env_marg);
+        env_marg_func.swap(o.env_marg_func);
+    }
 };
 
 }// namespace vs_internal
@@ -88,6 +105,12 @@ struct vs_scene {
     // device buffers
     DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, lut;
     vs_internal::ModeTables mt;// materials, lights and environment in the scene's spectrum mode
+    // the other spectrum mode, packed and uploaded on first use; kept until
+    // the scene is destroyed, so views that point at it stay valid
+    vs_internal::ModeTables mt_alt;
+    vs::PackedScene packed_alt;
+    bool have_alt{false};
+    bool diffuse_only{false};// only diffuse / Oren-Nayar / black materials
     DevBuf fis_lut, fis_cond, fis_cond_func, fis_marg, fis_marg_func;
     DevBuf film_rt, film_out, film_pic, counters;
     DevBuf scratch_rays, scratch_hits;
@@ -158,5 +181,9 @@ void upload_mode_tables(const vs::PackedScene &pk, ModeTables &t, hipStream_t st
 void set_mode_view(vs::SceneView &v, const vs::PackedScene &pk, const ModeTables &t);
 // the render settings of a scene's integrator and film
 vs::RenderView render_view(const vs_scene *s);
+// pack and upload the tables of the spectrum mode the scene is not in
+void ensure_alt_mode(vs_scene *s);
+// a view of the scene in sRGB (rgb) or hero-wavelength mode
+vs::SceneView view_in_mode(vs_scene *s, bool rgb);
 
 }// namespace vs_internal

@@ -50,7 +50,7 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 // one thread per pixel; all `spp` frames of the launch accumulate in registers
 // with the film's running-mean arithmetic (normal.cpp:87-101).  NS: wavelengths
 // carried per path; RGB: sRGB mode (three fixed channels) vs hero wavelengths.
-template <int NS, bool RGB>
+template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
                                                   float4 *__restrict__ rt, float4 *__restrict__ out,
                                                   unsigned long long *__restrict__ counters) {
@@ -63,7 +63,7 @@ __global__ void __launch_bounds__(64) pt_kernel(SceneView sc, CameraView cam, Re
     PathCounters pc{0, 0, 0};
     for (uint32_t s = 0; s < rv.spp; ++s) {
         uint32_t frame = rv.frame_start + s;
-        f3 L = trace_path<NS, RGB>(sc, cam, rv, px, py, frame, lds_stack + threadIdx.x, blockDim.x, pc);
+        f3 L = trace_path<NS, RGB, MS>(sc, cam, rv, px, py, frame, lds_stack + threadIdx.x, blockDim.x, pc);
         acc = film_add(rv, acc, L, frame);
     }
     rt[idx] = acc;
@@ -150,19 +150,29 @@ hipError_t launch_trace(const SceneView &sc, const RayRec *rays, HitRec *hits, u
 }
 
 hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderView &rv, float4 *rt, float4 *out,
-                     unsigned long long *counters, hipStream_t stream) {
+                     unsigned long long *counters, bool diffuse_only, hipStream_t stream) {
     dim3 block(64);
     dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8);
     size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+#define VS_PT(NS_, RGB_)                                                                                        \
+    do {                                                                                                        \
+        if (diffuse_only)                                                                                       \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_DIFFUSE>), grid, block, lds, stream, sc, cam, rv, rt, out, \
+                               counters);                                                                       \
+        else                                                                                                    \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_ALL>), grid, block, lds, stream, sc, cam, rv, rt, out,     \
+                               counters);                                                                       \
+    } while (0)
     if (sc.spectrum_rgb) {
-        hipLaunchKernelGGL((pt_kernel<3, true>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+        VS_PT(3, true);
     } else if (rv.spectrum_dim == 4) {
-        hipLaunchKernelGGL((pt_kernel<4, false>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+        VS_PT(4, false);
     } else if (rv.spectrum_dim == 3) {
-        hipLaunchKernelGGL((pt_kernel<3, false>), grid, block, lds, stream, sc, cam, rv, rt, out, counters);
+        VS_PT(3, false);
     } else {
         return hipErrorInvalidValue;
     }
+#undef VS_PT
     return hipGetLastError();
 }
 

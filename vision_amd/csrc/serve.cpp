@@ -123,7 +123,6 @@ struct SceneSlot {
     vs::CameraView cam{};   // the scene's camera when it joined (jobs re-aspect it)
     vs::RenderView rv{};    // and its integrator / film settings
     bool diffuse_only{false};
-    std::unique_ptr<ModeTables> other;// tables of the mode the scene was not loaded in
 };
 
 }// namespace
@@ -162,6 +161,7 @@ struct vs_server {
     std::condition_variable cv_work, cv_done;
     std::deque<JobRec *> queue;
     std::unordered_map<uint64_t, std::unique_ptr<JobRec>> jobs;
+    std::multimap<size_t, float *> pinned_pool;// released result buffers by size
     uint64_t next_id{1};
     bool stop{false};
     std::string fatal;
@@ -346,6 +346,25 @@ void admit(vs_server *S) {
             j->t_admit = Clock::now();
         }
         j->off = off;
+        {
+            // pinned host copy of the films, recycled by size
+            float *buf = nullptr;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                auto it = S->pinned_pool.find(j->bytes);
+                if (it != S->pinned_pool.end()) {
+                    buf = it->second;
+                    S->pinned_pool.erase(it);
+                }
+            }
+            if (!buf) {
+                void *hp = nullptr;
+                hip_check(hipHostMalloc(&hp, j->bytes, hipHostMallocDefault), "pinned result");
+                buf = static_cast<float *>(hp);
+            }
+            j->h_rt = buf;
+            j->h_out = buf + size_t(j->d.width) * j->d.height * 4;
+        }
         j->slot = uint32_t(free_slot - S->slot_job.begin());
         *free_slot = j;
         const SceneSlot &sc = S->scenes[j->d.scene];
@@ -615,30 +634,19 @@ vs_status vs_server_add_scene(vs_server *S, vs_scene *scene, uint32_t *index) {
         slot.scene = scene;
         slot.cam = scene->cam;
         slot.rv = render_view(scene);
-        slot.diffuse_only = true;
-        for (const vs::MaterialRecord &m : scene->packed.mats)
-            if (m.type != vs::MAT_DIFFUSE && m.type != vs::MAT_OREN_NAYAR && m.type != vs::MAT_BLACK)
-                slot.diffuse_only = false;
-        // the other spectrum mode shares the geometry and BVH buffers
-        vs::SceneHost other = scene->host;
-        bool loaded_rgb = scene->packed.rgb;
-        other.spectrum_type = loaded_rgb ? "hero" : "srgb";
-        other.spectrum_dim = loaded_rgb ? 4u : 3u;
-        vs::PackedScene pk = vs::pack_scene(other);
-        slot.other.reset(new ModeTables());
-        upload_mode_tables(pk, *slot.other, S->st_copy);
-        hip_check(hipStreamSynchronize(S->st_copy), "mode tables");
-        vs::SceneView views[2];
-        views[0] = views[1] = scene->view;
-        set_mode_view(views[loaded_rgb ? 1 : 0], pk, *slot.other);
+        slot.diffuse_only = scene->diffuse_only;
+        // both spectrum modes share the geometry and BVH buffers
+        vs::SceneView views[2] = {view_in_mode(scene, true), view_in_mode(scene, false)};
+        if (scene->view.stack_size > S->max_stack) {
+            for (auto &kv : S->jobs)
+                if (kv.second->state == VS_JOB_QUEUED || kv.second->state == VS_JOB_RUNNING)
+                    throw std::logic_error("a scene with a deeper BVH cannot join while jobs run");
+        }
         uint32_t i = uint32_t(S->scenes.size());
         hip_check(hipMemcpy(S->d_views.as<vs::SceneView>() + 2 * i, views, sizeof(views), hipMemcpyHostToDevice),
                   "view upload");
         S->scenes.push_back(std::move(slot));
         if (scene->view.stack_size > S->max_stack) {
-            for (auto &kv : S->jobs)
-                if (kv.second->state == VS_JOB_QUEUED || kv.second->state == VS_JOB_RUNNING)
-                    throw std::logic_error("a scene with a deeper BVH cannot join while jobs run");
             S->max_stack = scene->view.stack_size;
             drop_graphs(S);
         }
@@ -673,11 +681,6 @@ vs_status vs_server_submit(vs_server *S, const vs_job_desc *d, uint64_t *job_id)
         j->rv.spp = d->spp;
         uint32_t base = !d->spectral ? vs::SV_RGB : (dim == 4 ? vs::SV_HERO4 : vs::SV_HERO3);
         j->variant = base + (sc.diffuse_only ? 1u : 0u);
-        size_t px = size_t(d->width) * d->height;
-        void *hp = nullptr;
-        hip_check(hipHostMalloc(&hp, px * 32, hipHostMallocDefault), "pinned result");
-        j->h_rt = static_cast<float *>(hp);
-        j->h_out = j->h_rt + px * 4;
         j->off = SIZE_MAX;
         j->id = S->next_id++;
         j->t_submit = Clock::now();
@@ -744,7 +747,7 @@ vs_status vs_server_release(vs_server *S, uint64_t id) {
     if (it == S->jobs.end()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown job");
     JobRec *j = it->second.get();
     if (j->state != VS_JOB_DONE && j->state != VS_JOB_FAILED) return server_fail(VS_ERROR_STATE, "job not finished");
-    if (j->h_rt) (void)hipHostFree(j->h_rt);
+    if (j->h_rt) S->pinned_pool.emplace(j->bytes, j->h_rt);
     S->jobs.erase(it);
     return VS_OK;
 }
@@ -878,6 +881,7 @@ void vs_server_destroy(vs_server *S) {
             if (s) (void)hipStreamSynchronize(s);
         if (S->st_copy) (void)hipStreamSynchronize(S->st_copy);
         drop_graphs(S);
+        for (auto &kv : S->pinned_pool) (void)hipHostFree(kv.second);
         for (auto &kv : S->jobs) {
             if (kv.second->h_rt) (void)hipHostFree(kv.second->h_rt);
             if (kv.second->ev_copy) (void)hipEventDestroy(kv.second->ev_copy);

@@ -347,6 +347,42 @@ def procedural_scene(out_dir: str | Path, seed: int = 0, n_instances: int = 64, 
     }
 
 
+SERVING_TRIANGLES = (100_000, 153_000, 234_000, 358_000, 547_000, 837_000, 1_280_000, 2_000_000)
+
+
+def serving_scene(out_dir: str | Path, triangles: int, seed: int = 0, width: int = 256, height: int = 256,
+                  max_depth: int = 8) -> dict:
+    """The Cornell-box layout with about `triangles` triangles: the box, its
+    two blocks and its light as in cornell_box, plus round(triangles / 20480)
+    displaced level-5 icospheres (20 480 triangles each, value-noise
+    displacement 0.1) of random diffuse albedo scattered through the box.
+    Sphere meshes are shared by all scenes (written once into out_dir)."""
+    out = Path(out_dir)
+    out.mkdir(parents=True, exist_ok=True)
+    scene = cornell_box(width, height, max_depth=max_depth, light_sampler="power")
+    n = max(1, int(round(triangles / 20480)))
+    rng = np.random.default_rng(1000 + seed)
+    P0, F = None, None
+    for i in range(n):
+        mesh = out / f"serve_sphere_{i}.vsmesh"
+        if not mesh.exists():
+            if P0 is None:
+                P0, F = icosphere(5)
+            r2 = np.random.default_rng(i)
+            noise = value_noise3(P0 * 2.5 + r2.uniform(-100, 100, 3), 7000 + i)
+            noise += 0.5 * value_noise3(P0 * 5.0 + r2.uniform(-100, 100, 3), 7500 + i)
+            P = P0 * (1 + 0.1 * noise / 1.5)[:, None]
+            write_vsmesh(mesh, P, F, vertex_normals(P, F))
+        r = rng.uniform(0.06, 0.16)
+        c = rng.uniform([-0.85, 0.15, -0.85], [0.85, 1.75, 0.85])
+        m = translate(*c) @ rot_y(rng.uniform(0, 360)) @ scale(r, r, r)
+        name = f"S{i}"
+        scene["materials"].append(diffuse(name, tuple(float(x) for x in rng.uniform(0.1, 0.9, 3))))
+        scene["shapes"].append({"type": "model", "name": f"sphere{i}",
+                                "param": {"fn": mesh.name, "material": name, "transform": xform(m)}})
+    return scene
+
+
 def write_scene(scene: dict, path: str | Path) -> Path:
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
