@@ -266,6 +266,12 @@ def serve_main(args) -> None:
     wids, _ = run_batch(jobs[:max(args.warmup, 1)])
     for i in wids:
         srv.release(i)
+    tune = None
+    if args.autotune:
+        # launch shape picked on a separate probe list of the same mix
+        probe = serve_jobs(args.autotune, len(scenes), seed=args.seed + 100)
+        tune = srv.autotune(probe)
+        print(f"autotune: {tune}", file=sys.stderr, flush=True)
     # throughput: every job queued at once
     ids, st_sat = run_batch(jobs)
     relmse = 0.0
@@ -324,8 +330,9 @@ def serve_main(args) -> None:
                         "U[8, 16], sRGB or hero-4 spectral; persistent multi-job kernels in captured hipGraphs, "
                         "HBM-budget admission; a step is the whole job list, queued at once",
             "jobs": len(jobs), "samples": total_samples, "triangles": tris, "seed": args.seed,
-            "max_jobs": args.max_jobs, "step_ms": args.step_ms, "tile_w": args.tile_w,
-            "waves_per_cu": args.waves_per_cu},
+            "max_jobs": args.max_jobs, "step_ms": args.step_ms, **{k: srv.config()[k] for k in
+                                                                    ("tile_w", "waves_per_cu")}},
+        "autotune": tune,
         "plain_msamples_per_s": round(plain_msps, 3),
         "batched_over_plain": round(batched_msps / plain_msps, 3),
         "relmse_batched_vs_plain": relmse,
@@ -379,6 +386,8 @@ def main():
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--check-jobs", type=int, default=16, help="serve: jobs compared with their plain film")
     ap.add_argument("--count-jobs", type=int, default=100, help="serve: jobs in the ray-counting run")
+    ap.add_argument("--autotune", type=int, default=48, help="serve: probe jobs for the launch-shape autotuner "
+                                                             "(0: keep --tile-w / --waves-per-cu)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--chunk-frames", type=int, default=16)

@@ -352,6 +352,20 @@ vs_status vs_server_reset_stats(vs_server *srv);
  * budget and slot count are fixed at creation). */
 vs_status vs_server_set_config(vs_server *srv, const vs_server_config *cfg);
 vs_status vs_server_get_config(vs_server *srv, vs_server_config *cfg);
+/* Autotuner: render the probe jobs under each candidate launch shape (tile
+ * width 8 / 16 / 32 x persistent waves per CU at the occupancy limit, half
+ * and a quarter of it) and keep the one with the highest Msamples/s.  The
+ * server must be idle; the probe jobs' results are discarded. */
+#define VS_AUTOTUNE_MAX_TRIALS 16
+typedef struct vs_autotune_result {
+    uint32_t tile_w, waves_per_cu; /* chosen (now the server's config) */
+    double msamples_per_s;         /* of the chosen shape */
+    uint32_t trials;
+    uint32_t trial_tile_w[VS_AUTOTUNE_MAX_TRIALS];
+    uint32_t trial_waves_per_cu[VS_AUTOTUNE_MAX_TRIALS];
+    double trial_msamples_per_s[VS_AUTOTUNE_MAX_TRIALS];
+} vs_autotune_result;
+vs_status vs_server_autotune(vs_server *srv, const vs_job_desc *probe, uint32_t n, vs_autotune_result *out);
 /* The message of the last failed server call on this thread. */
 const char *vs_server_last_error(void);
 void vs_server_destroy(vs_server *srv);

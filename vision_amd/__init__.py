@@ -105,6 +105,12 @@ class ServerStats(C.Structure):
                 [("hbm_budget", C.c_uint64), ("hbm_peak", C.c_uint64)])
 
 
+class AutotuneResult(C.Structure):
+    _fields_ = [("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("msamples_per_s", C.c_double),
+                ("trials", C.c_uint32), ("trial_tile_w", C.c_uint32 * 16), ("trial_waves_per_cu", C.c_uint32 * 16),
+                ("trial_msamples_per_s", C.c_double * 16)]
+
+
 JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 
@@ -156,6 +162,7 @@ SIGNATURES = {
     "vs_server_reset_stats": (C.c_int32, [_P]),
     "vs_server_set_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
     "vs_server_get_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
+    "vs_server_autotune": (C.c_int32, [_P, _P, C.c_uint32, C.POINTER(AutotuneResult)]),
     "vs_server_last_error": (C.c_char_p, []),
     "vs_server_destroy": (None, [_P]),
 }
@@ -491,6 +498,19 @@ class Server:
                 raise KeyError(k)
             setattr(c, k, int(v) if k != "step_ms" else float(v))
         _server_check(lib().vs_server_set_config(self._h, C.byref(c)))
+
+    def autotune(self, probe: list) -> dict:
+        """Pick tile width and persistent waves per CU on the probe jobs
+        (dicts of submit()'s arguments); returns the trials and the choice."""
+        arr = (JobDesc * len(probe))()
+        for i, j in enumerate(probe):
+            arr[i] = JobDesc(j["scene"], j["width"], j["height"], j["spp"], j.get("max_depth", 0),
+                             1 if j.get("spectral") else 0, j.get("spectrum_dim", 0), 0)
+        r = AutotuneResult()
+        _server_check(lib().vs_server_autotune(self._h, arr, len(probe), C.byref(r)))
+        return {"tile_w": r.tile_w, "waves_per_cu": r.waves_per_cu, "msamples_per_s": r.msamples_per_s,
+                "trials": [{"tile_w": r.trial_tile_w[i], "waves_per_cu": r.trial_waves_per_cu[i],
+                            "msamples_per_s": r.trial_msamples_per_s[i]} for i in range(r.trials)]}
 
     def close(self):
         if self._h:

@@ -859,6 +859,64 @@ vs_status vs_server_get_config(vs_server *S, vs_server_config *cfg) {
     return VS_OK;
 }
 
+vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n, vs_autotune_result *out) {
+    if (!S || !probe || !n || !out) return server_fail(VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    if (!idle(S)) return server_fail(VS_ERROR_STATE, "server busy");
+    std::memset(out, 0, sizeof(*out));
+    vs_server_config base{};
+    vs_server_get_config(S, &base);
+    // occupancy limit over the variants the probe uses
+    int occ = 64;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (probe[i].scene >= S->scenes.size()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown scene index");
+        uint32_t dim = probe[i].spectral ? (probe[i].spectrum_dim ? probe[i].spectrum_dim : 4u) : 3u;
+        uint32_t v = !probe[i].spectral ? vs::SV_RGB : (dim == 4 ? vs::SV_HERO4 : vs::SV_HERO3);
+        v += S->scenes[probe[i].scene].diffuse_only ? 1u : 0u;
+        occ = std::min(occ, std::max(1, vs::serve_max_blocks_per_cu(v, S->max_stack)));
+    }
+    std::vector<uint32_t> wpcs = {uint32_t(occ)};
+    if (occ >= 2) wpcs.push_back(uint32_t(occ) / 2);
+    if (occ >= 4) wpcs.push_back(uint32_t(occ) / 4);
+    double best = -1.0;
+    vs_server_config best_cfg = base;
+    for (uint32_t tw : {8u, 16u, 32u}) {
+        for (uint32_t wpc : wpcs) {
+            vs_server_config c = base;
+            c.tile_w = tw;
+            c.waves_per_cu = wpc;
+            vs_status st = vs_server_set_config(S, &c);
+            if (st == VS_OK) st = vs_server_reset_stats(S);
+            std::vector<uint64_t> ids(n);
+            for (uint32_t i = 0; i < n && st == VS_OK; ++i) st = vs_server_submit(S, &probe[i], &ids[i]);
+            if (st == VS_OK) st = vs_server_drain(S, -1.0);
+            vs_server_stats ss{};
+            if (st == VS_OK) st = vs_server_stats_get(S, &ss);
+            for (uint64_t id : ids)
+                if (id) vs_server_release(S, id);
+            if (st != VS_OK) {
+                vs_server_set_config(S, &base);
+                return st;
+            }
+            if (out->trials < VS_AUTOTUNE_MAX_TRIALS) {
+                out->trial_tile_w[out->trials] = tw;
+                out->trial_waves_per_cu[out->trials] = wpc;
+                out->trial_msamples_per_s[out->trials] = ss.msamples_per_s;
+                ++out->trials;
+            }
+            if (ss.msamples_per_s > best) {
+                best = ss.msamples_per_s;
+                best_cfg = c;
+            }
+        }
+    }
+    vs_status st = vs_server_set_config(S, &best_cfg);
+    if (st == VS_OK) st = vs_server_reset_stats(S);
+    out->tile_w = best_cfg.tile_w;
+    out->waves_per_cu = best_cfg.waves_per_cu;
+    out->msamples_per_s = best;
+    return st;
+}
+
 void vs_server_destroy(vs_server *S) {
     if (!S) return;
     {
