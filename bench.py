@@ -331,7 +331,8 @@ def serve_main(args) -> None:
                         "HBM-budget admission; a step is the whole job list, queued at once",
             "jobs": len(jobs), "samples": total_samples, "triangles": tris, "seed": args.seed,
             "max_jobs": args.max_jobs, "step_ms": args.step_ms, **{k: srv.config()[k] for k in
-                                                                    ("tile_w", "waves_per_cu")}},
+                                                                    ("tile_w", "waves_per_cu",
+                                                                              "min_waves_per_eu")}},
         "autotune": tune,
         "plain_msamples_per_s": round(plain_msps, 3),
         "batched_over_plain": round(batched_msps / plain_msps, 3),

@@ -286,6 +286,8 @@ typedef struct vs_server_config {
     uint32_t waves_per_cu; /* persistent waves per CU for each kernel variant; 0: the occupancy limit */
     int32_t use_graphs;    /* 1: replay captured hipGraphs; 0: launch each step's kernels directly */
     int32_t count_rays;    /* 1: count rays and BVH fetches (slower) */
+    uint32_t min_waves_per_eu; /* register budget of the kernels: 0 the compiler's choice, 6 at least
+                                  six resident waves per SIMD (spilling what does not fit) */
 } vs_server_config;
 
 typedef struct vs_job_desc {
@@ -352,17 +354,19 @@ vs_status vs_server_reset_stats(vs_server *srv);
  * budget and slot count are fixed at creation). */
 vs_status vs_server_set_config(vs_server *srv, const vs_server_config *cfg);
 vs_status vs_server_get_config(vs_server *srv, vs_server_config *cfg);
-/* Autotuner: render the probe jobs under each candidate launch shape (tile
- * width 8 / 16 / 32 x persistent waves per CU at the occupancy limit, half
- * and a quarter of it) and keep the one with the highest Msamples/s.  The
- * server must be idle; the probe jobs' results are discarded. */
+/* Autotuner: render the probe jobs under each candidate launch shape
+ * (register budget 0 / 6 x tile width 8 / 16 x persistent waves per CU at
+ * the occupancy limit and half of it) and keep the one with the highest
+ * Msamples/s.  The server must be idle; the probe jobs' results are
+ * discarded. */
 #define VS_AUTOTUNE_MAX_TRIALS 16
 typedef struct vs_autotune_result {
-    uint32_t tile_w, waves_per_cu; /* chosen (now the server's config) */
+    uint32_t tile_w, waves_per_cu, min_waves_per_eu; /* chosen (now the server's config) */
     double msamples_per_s;         /* of the chosen shape */
     uint32_t trials;
     uint32_t trial_tile_w[VS_AUTOTUNE_MAX_TRIALS];
     uint32_t trial_waves_per_cu[VS_AUTOTUNE_MAX_TRIALS];
+    uint32_t trial_min_waves_per_eu[VS_AUTOTUNE_MAX_TRIALS];
     double trial_msamples_per_s[VS_AUTOTUNE_MAX_TRIALS];
 } vs_autotune_result;
 vs_status vs_server_autotune(vs_server *srv, const vs_job_desc *probe, uint32_t n, vs_autotune_result *out);

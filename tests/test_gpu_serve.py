@@ -98,8 +98,9 @@ def test_each_job_matches_its_plain_render(gpu_ctx, served):
 
 
 @pytest.mark.parametrize("cfg", [dict(use_graphs=False), dict(tile_w=16, waves_per_cu=1),
-                                 dict(tile_w=64, step_ms=1.0), dict(tile_w=32, max_jobs=2)],
-                         ids=["direct", "tile16-wpc1", "tile64-short-steps", "tile32-two-slots"])
+                                 dict(tile_w=64, step_ms=1.0), dict(tile_w=32, max_jobs=2),
+                                 dict(min_waves_per_eu=6)],
+                         ids=["direct", "tile16-wpc1", "tile64-short-steps", "tile32-two-slots", "regs6"])
 def test_same_films_under_every_launch_shape(gpu_ctx, served, cfg):
     base, _ = _run(gpu_ctx, served, JOBS[:5])
     got, stats = _run(gpu_ctx, served, JOBS[:5], **cfg)
@@ -137,3 +138,23 @@ def test_rejects_bad_jobs(gpu_ctx, served):
                 srv.submit(**kw)
         with pytest.raises(va.VisionError):
             srv.submit(7, 16, 16, 1)  # unknown scene
+
+
+def test_autotuner_picks_a_measured_shape(gpu_ctx, served):
+    with va.Server(gpu_ctx) as srv:
+        i = srv.add_scene(served[1]["cbox"])
+        probe = [dict(scene=i, width=64, height=64, spp=2), dict(scene=i, width=128, height=32, spp=1)]
+        r = srv.autotune(probe)
+        assert len(r["trials"]) >= 4
+        best = max(r["trials"], key=lambda t: t["msamples_per_s"])
+        assert (r["tile_w"], r["waves_per_cu"], r["min_waves_per_eu"]) == (best["tile_w"], best["waves_per_cu"],
+                                                                           best["min_waves_per_eu"])
+        cfg = srv.config()
+        assert (cfg["tile_w"], cfg["waves_per_cu"], cfg["min_waves_per_eu"]) == (r["tile_w"], r["waves_per_cu"],
+                                                                                 r["min_waves_per_eu"])
+        # the server still renders correctly under the chosen shape
+        j = srv.submit(i, 64, 64, 2)
+        srv.wait(j)
+        film = srv.result(j, 0, (64, 64))
+    rt, _ = _plain(gpu_ctx, served[0]["cbox"], 64, 64, 2)
+    np.testing.assert_array_equal(film.view(np.uint32), rt.view(np.uint32))

@@ -83,7 +83,7 @@ class SceneArrays(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_uint32), ("step_ms", C.c_float),
                 ("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("use_graphs", C.c_int32),
-                ("count_rays", C.c_int32)]
+                ("count_rays", C.c_int32), ("min_waves_per_eu", C.c_uint32)]
 
 
 class JobDesc(C.Structure):
@@ -106,8 +106,9 @@ class ServerStats(C.Structure):
 
 
 class AutotuneResult(C.Structure):
-    _fields_ = [("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("msamples_per_s", C.c_double),
-                ("trials", C.c_uint32), ("trial_tile_w", C.c_uint32 * 16), ("trial_waves_per_cu", C.c_uint32 * 16),
+    _fields_ = [("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("min_waves_per_eu", C.c_uint32),
+                ("msamples_per_s", C.c_double), ("trials", C.c_uint32), ("trial_tile_w", C.c_uint32 * 16),
+                ("trial_waves_per_cu", C.c_uint32 * 16), ("trial_min_waves_per_eu", C.c_uint32 * 16),
                 ("trial_msamples_per_s", C.c_double * 16)]
 
 
@@ -429,14 +430,16 @@ class Server:
     (vs_server_*): continuous batching into persistent kernels, admission by
     an HBM budget, captured step graphs."""
 
-    _CFG = ("hbm_budget", "max_jobs", "step_ms", "tile_w", "waves_per_cu", "use_graphs", "count_rays")
+    _CFG = ("hbm_budget", "max_jobs", "step_ms", "tile_w", "waves_per_cu", "use_graphs", "count_rays",
+            "min_waves_per_eu")
 
     def __init__(self, ctx: "Context", hbm_budget: int = 0, max_jobs: int = 0, step_ms: float = 0.0,
-                 tile_w: int = 0, waves_per_cu: int = 0, use_graphs: bool = True, count_rays: bool = False):
+                 tile_w: int = 0, waves_per_cu: int = 0, use_graphs: bool = True, count_rays: bool = False,
+                 min_waves_per_eu: int = 0):
         self.ctx = ctx
         self._h = _P()
         cfg = ServerConfig(hbm_budget, max_jobs, step_ms, tile_w, waves_per_cu, 1 if use_graphs else 0,
-                           1 if count_rays else 0)
+                           1 if count_rays else 0, min_waves_per_eu)
         _server_check(lib().vs_server_create(ctx._h, C.byref(cfg), C.byref(self._h)))
         self.scenes: list = []
 
@@ -508,8 +511,10 @@ class Server:
                              1 if j.get("spectral") else 0, j.get("spectrum_dim", 0), 0)
         r = AutotuneResult()
         _server_check(lib().vs_server_autotune(self._h, arr, len(probe), C.byref(r)))
-        return {"tile_w": r.tile_w, "waves_per_cu": r.waves_per_cu, "msamples_per_s": r.msamples_per_s,
+        return {"tile_w": r.tile_w, "waves_per_cu": r.waves_per_cu, "min_waves_per_eu": r.min_waves_per_eu,
+                "msamples_per_s": r.msamples_per_s,
                 "trials": [{"tile_w": r.trial_tile_w[i], "waves_per_cu": r.trial_waves_per_cu[i],
+                            "min_waves_per_eu": r.trial_min_waves_per_eu[i],
                             "msamples_per_s": r.trial_msamples_per_s[i]} for i in range(r.trials)]}
 
     def close(self):

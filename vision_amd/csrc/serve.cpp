@@ -222,7 +222,7 @@ void drop_graphs(vs_server *S) {
 }
 
 uint32_t variant_blocks(vs_server *S, uint32_t v) {
-    int occ = vs::serve_max_blocks_per_cu(v, S->max_stack);
+    int occ = vs::serve_max_blocks_per_cu(v, S->cfg.min_waves_per_eu, S->max_stack);
     if (occ <= 0) occ = 1;
     uint32_t wpc = S->cfg.waves_per_cu ? std::min<uint32_t>(S->cfg.waves_per_cu, uint32_t(occ)) : uint32_t(occ);
     return S->cu_count * wpc;
@@ -247,7 +247,7 @@ void enqueue_step(vs_server *S, uint32_t mask, uint32_t buf) {
         hipStream_t s = S->st[si];
         if (si && !used[si]) hip_check(hipStreamWaitEvent(s, S->ev_fork, 0), "fork wait");
         used[si] = 1;
-        hip_check(vs::serve_launch(v, S->d_views.as<vs::SceneView>(), S->d_jobs.as<vs::ServeJob>(), tables + v,
+        hip_check(vs::serve_launch(v, S->cfg.min_waves_per_eu, S->d_views.as<vs::SceneView>(), S->d_jobs.as<vs::ServeJob>(), tables + v,
                                    S->d_next.as<uint32_t>() + v, variant_blocks(S, v), S->cfg.tile_w, S->max_stack,
                                    ctr, s),
                   "serve launch");
@@ -546,6 +546,7 @@ void validate_config(const vs_server_config &c) {
         throw std::invalid_argument("tile_w must be 8, 16, 32 or 64");
     if (!(c.step_ms > 0.f && c.step_ms <= 10000.f)) throw std::invalid_argument("step_ms must be in (0, 10000]");
     if (c.waves_per_cu > 64) throw std::invalid_argument("waves_per_cu must be at most 64");
+    if (!vs::serve_occ_supported(c.min_waves_per_eu)) throw std::invalid_argument("min_waves_per_eu must be 0 or 6");
 }
 
 vs_server_config with_defaults(const vs_server_config *in) {
@@ -846,7 +847,7 @@ vs_status vs_server_set_config(vs_server *S, const vs_server_config *cfg) {
         validate_config(c);
         // the worker reads the config only while it has jobs, and it has none
         bool relaunch = c.tile_w != S->cfg.tile_w || c.waves_per_cu != S->cfg.waves_per_cu ||
-                        c.count_rays != S->cfg.count_rays;
+                        c.count_rays != S->cfg.count_rays || c.min_waves_per_eu != S->cfg.min_waves_per_eu;
         S->cfg = c;
         if (relaunch) drop_graphs(S);
     });
@@ -865,25 +866,34 @@ vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n,
     std::memset(out, 0, sizeof(*out));
     vs_server_config base{};
     vs_server_get_config(S, &base);
-    // occupancy limit over the variants the probe uses
-    int occ = 64;
-    for (uint32_t i = 0; i < n; ++i) {
+    for (uint32_t i = 0; i < n; ++i)
         if (probe[i].scene >= S->scenes.size()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown scene index");
-        uint32_t dim = probe[i].spectral ? (probe[i].spectrum_dim ? probe[i].spectrum_dim : 4u) : 3u;
-        uint32_t v = !probe[i].spectral ? vs::SV_RGB : (dim == 4 ? vs::SV_HERO4 : vs::SV_HERO3);
-        v += S->scenes[probe[i].scene].diffuse_only ? 1u : 0u;
-        occ = std::min(occ, std::max(1, vs::serve_max_blocks_per_cu(v, S->max_stack)));
+    // candidate shapes: register budget x tile width x persistent waves per CU
+    // (the occupancy limit of the probe's variants, half and a quarter of it)
+    struct Shape {
+        uint32_t regs, tw, wpc;
+    };
+    std::vector<Shape> shapes;
+    for (uint32_t regs : {0u, 6u}) {
+        int occ = 64;
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t dim = probe[i].spectral ? (probe[i].spectrum_dim ? probe[i].spectrum_dim : 4u) : 3u;
+            uint32_t v = !probe[i].spectral ? vs::SV_RGB : (dim == 4 ? vs::SV_HERO4 : vs::SV_HERO3);
+            v += S->scenes[probe[i].scene].diffuse_only ? 1u : 0u;
+            occ = std::min(occ, std::max(1, vs::serve_max_blocks_per_cu(v, regs, S->max_stack)));
+        }
+        for (uint32_t tw : {8u, 16u})
+            for (uint32_t wpc : {uint32_t(occ), uint32_t(std::max(1, occ / 2))}) shapes.push_back({regs, tw, wpc});
     }
-    std::vector<uint32_t> wpcs = {uint32_t(occ)};
-    if (occ >= 2) wpcs.push_back(uint32_t(occ) / 2);
-    if (occ >= 4) wpcs.push_back(uint32_t(occ) / 4);
     double best = -1.0;
     vs_server_config best_cfg = base;
-    for (uint32_t tw : {8u, 16u, 32u}) {
-        for (uint32_t wpc : wpcs) {
+    for (const Shape &sh : shapes) {
+        {
+            const uint32_t tw = sh.tw, wpc = sh.wpc;
             vs_server_config c = base;
             c.tile_w = tw;
             c.waves_per_cu = wpc;
+            c.min_waves_per_eu = sh.regs;
             vs_status st = vs_server_set_config(S, &c);
             if (st == VS_OK) st = vs_server_reset_stats(S);
             std::vector<uint64_t> ids(n);
@@ -900,6 +910,7 @@ vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n,
             if (out->trials < VS_AUTOTUNE_MAX_TRIALS) {
                 out->trial_tile_w[out->trials] = tw;
                 out->trial_waves_per_cu[out->trials] = wpc;
+                out->trial_min_waves_per_eu[out->trials] = sh.regs;
                 out->trial_msamples_per_s[out->trials] = ss.msamples_per_s;
                 ++out->trials;
             }
@@ -913,6 +924,7 @@ vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n,
     if (st == VS_OK) st = vs_server_reset_stats(S);
     out->tile_w = best_cfg.tile_w;
     out->waves_per_cu = best_cfg.waves_per_cu;
+    out->min_waves_per_eu = best_cfg.min_waves_per_eu;
     out->msamples_per_s = best;
     return st;
 }

@@ -59,11 +59,15 @@ enum ServeVariant : uint32_t {
     SV_COUNT
 };
 
+// register budgets the kernels are compiled for: 0 = the compiler's choice,
+// 6 = at least 6 resident waves per SIMD
+inline bool serve_occ_supported(uint32_t occ) { return occ == 0 || occ == 6; }
+
 // the persistent kernel of one variant: `blocks` one-wave workgroups claim
 // items through *next (zeroed by serve_reset before the step)
-hipError_t serve_launch(uint32_t variant, const SceneView *scenes, const ServeJob *jobs, const StepTable *table,
-                        uint32_t *next, uint32_t blocks, uint32_t tile_w, uint32_t stack_entries,
-                        unsigned long long *counters, hipStream_t stream);
+hipError_t serve_launch(uint32_t variant, uint32_t occ, const SceneView *scenes, const ServeJob *jobs,
+                        const StepTable *table, uint32_t *next, uint32_t blocks, uint32_t tile_w,
+                        uint32_t stack_entries, unsigned long long *counters, hipStream_t stream);
 // zero n counters (a kernel, so the step can be captured in a graph)
 hipError_t serve_reset(uint32_t *next, uint32_t n, hipStream_t stream);
 // clear a film (n float4)
@@ -71,8 +75,8 @@ hipError_t serve_clear(float4 *film, uint32_t n, hipStream_t stream);
 // the collector's output buffer from a finished job's film
 hipError_t serve_output(const RenderView &rv, const float4 *rt, float4 *out, uint32_t n, hipStream_t stream);
 // resident one-wave workgroups per CU of a variant's kernel with its LDS stack
-int serve_max_blocks_per_cu(uint32_t variant, uint32_t stack_entries);
+int serve_max_blocks_per_cu(uint32_t variant, uint32_t occ, uint32_t stack_entries);
 // registers per lane of a variant's kernel
-int serve_kernel_vgprs(uint32_t variant);
+int serve_kernel_vgprs(uint32_t variant, uint32_t occ);
 
 }// namespace vs

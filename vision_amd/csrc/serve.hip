@@ -16,8 +16,10 @@ namespace vs {
 
 using namespace dev;
 
-template <int NS, bool RGB, int MS>
-__global__ void __launch_bounds__(64) serve_kernel(const SceneView *__restrict__ scenes,
+// OCC > 0 asks the compiler for at least OCC resident waves per SIMD (a
+// register budget of 512 / OCC, spilling the rest); 0 leaves it free
+template <int NS, bool RGB, int MS, int OCC>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) serve_kernel(const SceneView *__restrict__ scenes,
                                                      const ServeJob *__restrict__ jobs,
                                                      const StepTable *__restrict__ table, uint32_t *__restrict__ next,
                                                      uint32_t tile_w, unsigned long long *__restrict__ counters) {
@@ -86,24 +88,31 @@ namespace {
 using ServeFn = void (*)(const SceneView *, const ServeJob *, const StepTable *, uint32_t *, uint32_t,
                          unsigned long long *);
 
-ServeFn serve_fn(uint32_t variant) {
+template <int OCC>
+ServeFn serve_fn_occ(uint32_t variant) {
     switch (variant) {
-    case SV_RGB: return serve_kernel<3, true, MS_ALL>;
-    case SV_RGB_DIFFUSE: return serve_kernel<3, true, MS_DIFFUSE>;
-    case SV_HERO4: return serve_kernel<4, false, MS_ALL>;
-    case SV_HERO4_DIFFUSE: return serve_kernel<4, false, MS_DIFFUSE>;
-    case SV_HERO3: return serve_kernel<3, false, MS_ALL>;
-    case SV_HERO3_DIFFUSE: return serve_kernel<3, false, MS_DIFFUSE>;
+    case SV_RGB: return serve_kernel<3, true, MS_ALL, OCC>;
+    case SV_RGB_DIFFUSE: return serve_kernel<3, true, MS_DIFFUSE, OCC>;
+    case SV_HERO4: return serve_kernel<4, false, MS_ALL, OCC>;
+    case SV_HERO4_DIFFUSE: return serve_kernel<4, false, MS_DIFFUSE, OCC>;
+    case SV_HERO3: return serve_kernel<3, false, MS_ALL, OCC>;
+    case SV_HERO3_DIFFUSE: return serve_kernel<3, false, MS_DIFFUSE, OCC>;
     default: return nullptr;
     }
 }
 
+ServeFn serve_fn(uint32_t variant, uint32_t occ) {
+    if (occ == 0) return serve_fn_occ<0>(variant);
+    if (occ == 6) return serve_fn_occ<6>(variant);
+    return nullptr;
+}
+
 }// namespace
 
-hipError_t serve_launch(uint32_t variant, const SceneView *scenes, const ServeJob *jobs, const StepTable *table,
-                        uint32_t *next, uint32_t blocks, uint32_t tile_w, uint32_t stack_entries,
-                        unsigned long long *counters, hipStream_t stream) {
-    ServeFn fn = serve_fn(variant);
+hipError_t serve_launch(uint32_t variant, uint32_t occ, const SceneView *scenes, const ServeJob *jobs,
+                        const StepTable *table, uint32_t *next, uint32_t blocks, uint32_t tile_w,
+                        uint32_t stack_entries, unsigned long long *counters, hipStream_t stream) {
+    ServeFn fn = serve_fn(variant, occ);
     if (!fn || blocks == 0 || !(tile_w == 8 || tile_w == 16 || tile_w == 32 || tile_w == 64) || stack_entries == 0 ||
         stack_entries > 64)
         return hipErrorInvalidValue;
@@ -130,8 +139,8 @@ hipError_t serve_output(const RenderView &rv, const float4 *rt, float4 *out, uin
     return hipGetLastError();
 }
 
-int serve_max_blocks_per_cu(uint32_t variant, uint32_t stack_entries) {
-    ServeFn fn = serve_fn(variant);
+int serve_max_blocks_per_cu(uint32_t variant, uint32_t occ, uint32_t stack_entries) {
+    ServeFn fn = serve_fn(variant, occ);
     if (!fn) return -1;
     int n = 0;
     size_t lds = size_t(stack_entries) * 64 * sizeof(uint2);
@@ -140,8 +149,8 @@ int serve_max_blocks_per_cu(uint32_t variant, uint32_t stack_entries) {
     return n;
 }
 
-int serve_kernel_vgprs(uint32_t variant) {
-    ServeFn fn = serve_fn(variant);
+int serve_kernel_vgprs(uint32_t variant, uint32_t occ) {
+    ServeFn fn = serve_fn(variant, occ);
     if (!fn) return -1;
     hipFuncAttributes a{};
     if (hipFuncGetAttributes(&a, reinterpret_cast<const void *>(fn)) != hipSuccess) return -1;
