@@ -358,13 +358,14 @@ def serve_main(args) -> None:
                      "algorithmic_bytes_per_sample": round(alg_bytes / sub_samples, 1)},
     }
     if not args.no_cpu:
-        j0 = dict(scene=0, width=256, height=256, spp=1, max_depth=8, spectral=False)
+        # a 1920x1080 x 4 spp, depth 12, sRGB job on the smallest scene; the
+        # oracle renders a band of its rows
         sc = scenes[0]
         sc.set_spectrum(False, 4)
-        sc.set_resolution(256, 256)
-        sc.set_integrator(8, 5, 1.0)
-        line["cpu_baseline"] = cpu_baseline(sc, 1)
-        line["cpu_baseline"]["sample"] += f"; scene 0 ({tris[0]} triangles), 256x256, depth 8, sRGB"
+        sc.set_resolution(1920, 1080)
+        sc.set_integrator(12, 5, 1.0)
+        line["cpu_baseline"] = cpu_baseline(sc, 4, budget_s=40.0)
+        line["cpu_baseline"]["sample"] += f"; scene 0 ({tris[0]} triangles), 1920x1080 x 4 spp job, depth 12, sRGB"
     srv.close()
     for sc in scenes:
         sc.close()
