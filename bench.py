@@ -2,7 +2,14 @@
 """Benchmark: rendering throughput on one MI355X.
 
 Workloads:
-  procedural (default)  plain path tracing, 1920x1080, 64 spp, depth 16,
+  serve (default)       batched serving against the plain per-job form on
+                        the same seeded list of 1000 jobs over 8 resident
+                        Cornell-box scenes of 0.1-2M triangles (resolutions
+                        256^2..1920x1080, 1-64 spp, depth 8-16, sRGB or
+                        hero-4): throughput with every job queued at once
+                        (the timed step), then p50/p99 latency under Poisson
+                        arrivals at 0.8 of the plain form's capacity.
+  procedural            plain path tracing, 1920x1080, 64 spp, depth 16,
                         hero-4 spectral, 1.31M triangles + HDRI; a step is
                         one render job (film reset + all 64 frames).
   cbox                  the 256x256 Cornell box, 16 spp, depth 8, sRGB.
@@ -20,6 +27,9 @@ per ray: BVH8 node fetches x 80 B + leaf-triangle fetches x 48 B + 48 B ray
 in / hit out.  Per-unit record bytes (the records the kernel reads and
 writes, cache hits included):
   megakernel PT: traversal + 32 B film read / write per pixel;
+  persistent serving kernel: traversal + 32 B film per sample (the counting
+      run covers the first --count-jobs jobs; the device time is the steps'
+      event time scaled to those jobs' share of the samples);
   wavefront closest-hit: traversal;
   wavefront shading: SHADE_BYTES per shaded path (path state in / out,
       hit, queues, triangle, vertices, instance, material, light sample)
@@ -229,7 +239,7 @@ def serve_main(args) -> None:
         plain_films[i] = scenes[jobs[i]["scene"]].film(0)
 
     srv = va.Server(ctx, max_jobs=args.max_jobs, step_ms=args.step_ms, tile_w=args.tile_w,
-                    waves_per_cu=args.waves_per_cu)
+                    waves_per_cu=args.waves_per_cu, min_waves_per_eu=args.min_waves_per_eu)
     for sc in scenes:
         srv.add_scene(sc)
 
@@ -378,7 +388,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="procedural", choices=["procedural", "cbox", "restir", "serve"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "procedural", "cbox", "restir"])
     ap.add_argument("--jobs", type=int, default=1000, help="serve: jobs in the list")
     ap.add_argument("--seed", type=int, default=0, help="serve: job-list seed")
     ap.add_argument("--load", type=float, default=0.8, help="serve: Poisson rate / plain capacity")
@@ -386,6 +396,7 @@ def main():
     ap.add_argument("--step-ms", type=float, default=16.0)
     ap.add_argument("--tile-w", type=int, default=8)
     ap.add_argument("--waves-per-cu", type=int, default=0)
+    ap.add_argument("--min-waves-per-eu", type=int, default=0, help="serve: kernel register budget (0 or 6)")
     ap.add_argument("--check-jobs", type=int, default=16, help="serve: jobs compared with their plain film")
     ap.add_argument("--count-jobs", type=int, default=100, help="serve: jobs in the ray-counting run")
     ap.add_argument("--autotune", type=int, default=48, help="serve: probe jobs for the launch-shape autotuner "
