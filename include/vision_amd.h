@@ -374,6 +374,67 @@ vs_status vs_server_autotune(vs_server *srv, const vs_job_desc *probe, uint32_t 
 const char *vs_server_last_error(void);
 void vs_server_destroy(vs_server *srv);
 
+/* ---- Radiance cache ---------------------------------------------------------
+ *
+ * No reference counterpart (the reference's integrators/sharc is a stub).  An
+ * online-trained cache of scattered radiance for sRGB path-tracing scenes: a
+ * multiresolution hash-grid encoding of position (16 levels x 2 FP16
+ * features, 2^19 entries per level) and an MLP 32 -> 64 -> 64 -> 64 -> 3
+ * (ReLU, no biases) whose hidden layers run on BF16 MFMA tiles with FP32
+ * accumulation.  Paths stop at bounce term_bounce and take the cache's
+ * answer there; training paths of every stride-th pixel run to full length
+ * and teach the cache the radiance scattered at each of their vertices
+ * (relative L2 loss, Adam).  Without VS_NRC_TERMINATE the same call renders
+ * the plain path tracer (the fallback), bit for bit as vs_render. */
+typedef struct vs_nrc vs_nrc;
+typedef struct vs_nrc_config {
+    uint32_t term_bounce;  /* paths stop into the cache at this bounce (2) */
+    uint32_t train_paths;  /* training paths per frame (65536) */
+    uint32_t train_steps;  /* Adam steps per frame, each over all the frame's samples (1) */
+    float learning_rate;   /* Adam learning rate (1e-2) */
+    uint32_t min_res;      /* coarsest level: cells per axis of the scene box (16) */
+    uint32_t max_res;      /* finest level (2048) */
+    uint32_t seed;         /* initial weights (1) */
+    uint32_t reserved;
+} vs_nrc_config;
+
+typedef struct vs_nrc_stats {
+    uint32_t frames, adam_steps;
+    uint32_t queries_last_frame, train_samples_last_frame, train_paths_per_frame;
+    float last_loss;
+    /* device time of the last vs_nrc_render call, per stage, ms */
+    float render_ms, train_paths_ms, infer_ms, train_ms, accumulate_ms;
+} vs_nrc_stats;
+
+#define VS_NRC_TERMINATE 1u /* stop paths into the cache */
+#define VS_NRC_TRAIN 2u     /* trace training paths and take Adam steps */
+
+void vs_nrc_default_config(vs_nrc_config *cfg);
+/* A cache for a built sRGB scene (cfg NULL: defaults); the scene outlives it. */
+vs_status vs_nrc_create(vs_scene *scene, const vs_nrc_config *cfg, vs_nrc **out);
+/* Render `frames` frames of `spp` samples into the scene's film (its running
+ * mean continues from the scene's frame index, as vs_render's does). */
+vs_status vs_nrc_render(vs_nrc *nrc, uint32_t frames, uint32_t spp, uint32_t flags);
+vs_status vs_nrc_stats_get(vs_nrc *nrc, vs_nrc_stats *out);
+/* Parity and inspection hooks.  Levels: cells per axis of the 16 levels, the
+ * mask of directly indexed levels, and the map position -> unit cube
+ * (u = (p - lo) * inv_ext, clamped).  Parameters: the FP32 masters, MLP as
+ * W1[32][64], W2[64][64], W3[64][64], W4[64][3] (10432 floats), grid as
+ * [16][2^19][2] (grid may be NULL); set_params also clears the Adam state.
+ * forward: n positions -> n x 3 outputs.  gradients: n samples of (p[3],
+ * t[3]) -> mean loss and its gradients (grid_grad may be NULL).  train:
+ * `steps` Adam steps on the given samples. */
+vs_status vs_nrc_levels(vs_nrc *nrc, uint32_t *res16, uint32_t *dense_mask, float *lo3, float *inv_ext3);
+vs_status vs_nrc_get_params(vs_nrc *nrc, float *mlp, float *grid);
+vs_status vs_nrc_set_params(vs_nrc *nrc, const float *mlp, const float *grid);
+vs_status vs_nrc_forward(vs_nrc *nrc, const float *pos, uint32_t n, float *out);
+vs_status vs_nrc_gradients(vs_nrc *nrc, const float *samples, uint32_t n, float *loss, float *mlp_grad,
+                           float *grid_grad);
+vs_status vs_nrc_train(vs_nrc *nrc, const float *samples, uint32_t n, uint32_t steps, float *last_loss);
+/* The message of the last failed radiance-cache call on this thread. */
+const char *vs_nrc_last_error(void);
+void vs_nrc_destroy(vs_nrc *nrc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -49,6 +49,14 @@ def lib():
         h.or_offset_ray_origin.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
         h.or_filter_eval.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
         h.or_film_output.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _f32p]
+        _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+        h.or_nrc_levels.argtypes = [C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_uint32)]
+        h.or_nrc_encode.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_int32, _f64p]
+        h.or_nrc_forward.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_int32, _f64p]
+        h.or_nrc_backward.argtypes = [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_int32, C.POINTER(C.c_double),
+                                      _f64p, C.c_void_p]
+        h.or_nrc_adam.argtypes = [_f32p, _f32p, _f32p, _f32p, C.c_uint32, C.c_float, C.c_float, C.c_float]
+        h.or_nrc_param_count.restype = C.c_uint32
         h.or_final_picture.argtypes = [_f32p, C.c_uint32, C.c_uint32, C.c_int32, _f32p]
         h.or_filter_table.argtypes = [_f32p, _f32p]
         h.or_filter_sample.argtypes = [_f32p, _f32p, C.c_uint32, _f32p, _f32p]
@@ -495,3 +503,64 @@ def bsdf_sample(spec: Spectral, rec: np.ndarray, wo: np.ndarray, frame: int = 0,
     pdf = np.zeros(n, np.float32)
     lib().or_bsdf_sample(spec.ptr, rec, wo, None if lam_a is None else lam_a.ctypes.data, n, frame, wi, f, pdf)
     return wi, f, pdf
+
+
+# ---- radiance cache (oracle/nrc.cpp) ----------------------------------------------
+
+NRC_GRID = 16 * (1 << 19) * 2
+
+
+class NrcCfg(C.Structure):
+    _fields_ = [("res", C.c_uint32 * 16), ("dense_mask", C.c_uint32), ("lo", C.c_float * 3),
+                ("inv_ext", C.c_float * 3)]
+
+
+def nrc_cfg(res, dense_mask, lo, inv_ext) -> NrcCfg:
+    c = NrcCfg()
+    for i in range(16):
+        c.res[i] = int(res[i])
+    c.dense_mask = int(dense_mask)
+    for i in range(3):
+        c.lo[i] = float(lo[i])
+        c.inv_ext[i] = float(inv_ext[i])
+    return c
+
+
+def nrc_levels(min_res: int, max_res: int):
+    res = np.zeros(16, np.uint32)
+    mask = C.c_uint32()
+    lib().or_nrc_levels(min_res, max_res, res, C.byref(mask))
+    return res, mask.value
+
+
+def nrc_encode(cfg: NrcCfg, grid: np.ndarray, pos: np.ndarray, round: bool = True) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    out = np.zeros((len(pos), 32), np.float64)
+    lib().or_nrc_encode(C.byref(cfg), np.ascontiguousarray(grid, np.float32), pos, len(pos), 1 if round else 0, out)
+    return out
+
+
+def nrc_forward(cfg: NrcCfg, mlp: np.ndarray, grid: np.ndarray, pos: np.ndarray, round: bool = True) -> np.ndarray:
+    pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+    out = np.zeros((len(pos), 3), np.float64)
+    lib().or_nrc_forward(C.byref(cfg), np.ascontiguousarray(mlp, np.float32), np.ascontiguousarray(grid, np.float32),
+                         pos, len(pos), 1 if round else 0, out)
+    return out
+
+
+def nrc_backward(cfg: NrcCfg, mlp: np.ndarray, grid: np.ndarray, samples: np.ndarray, round: bool = True,
+                 with_grid: bool = True):
+    sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 6)
+    loss = C.c_double()
+    gm = np.zeros(len(mlp), np.float64)
+    gg = np.zeros(NRC_GRID, np.float64) if with_grid else None
+    lib().or_nrc_backward(C.byref(cfg), np.ascontiguousarray(mlp, np.float32), np.ascontiguousarray(grid, np.float32),
+                          sm, len(sm), 1 if round else 0, C.byref(loss), gm,
+                          None if gg is None else gg.ctypes.data)
+    return loss.value, gm, gg
+
+
+def nrc_adam(p, m, v, g, lr: float, t: int):
+    c1 = 1.0 / (1.0 - 0.9 ** t)
+    c2 = 1.0 / (1.0 - 0.99 ** t)
+    lib().or_nrc_adam(p, m, v, g, len(p), lr, c1, c2)

@@ -115,6 +115,24 @@ class AutotuneResult(C.Structure):
 JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 
+class NrcConfig(C.Structure):
+    _fields_ = [("term_bounce", C.c_uint32), ("train_paths", C.c_uint32), ("train_steps", C.c_uint32),
+                ("learning_rate", C.c_float), ("min_res", C.c_uint32), ("max_res", C.c_uint32),
+                ("seed", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class NrcStats(C.Structure):
+    _fields_ = ([(n, C.c_uint32) for n in ("frames", "adam_steps", "queries_last_frame", "train_samples_last_frame",
+                                           "train_paths_per_frame")] +
+                [(n, C.c_float) for n in ("last_loss", "render_ms", "train_paths_ms", "infer_ms", "train_ms",
+                                          "accumulate_ms")])
+
+
+NRC_MLP_PARAMS = 10432
+NRC_GRID_PARAMS = 16 * (1 << 19) * 2
+NRC_TERMINATE, NRC_TRAIN = 1, 2
+
+
 # every entry point of include/vision_amd.h: name -> (restype, argtypes)
 _P = C.c_void_p
 SIGNATURES = {
@@ -165,6 +183,18 @@ SIGNATURES = {
     "vs_server_get_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
     "vs_server_autotune": (C.c_int32, [_P, _P, C.c_uint32, C.POINTER(AutotuneResult)]),
     "vs_server_last_error": (C.c_char_p, []),
+    "vs_nrc_default_config": (None, [C.POINTER(NrcConfig)]),
+    "vs_nrc_create": (C.c_int32, [_P, C.POINTER(NrcConfig), C.POINTER(_P)]),
+    "vs_nrc_render": (C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    "vs_nrc_stats_get": (C.c_int32, [_P, C.POINTER(NrcStats)]),
+    "vs_nrc_levels": (C.c_int32, [_P, _P, _P, _P, _P]),
+    "vs_nrc_get_params": (C.c_int32, [_P, _P, _P]),
+    "vs_nrc_set_params": (C.c_int32, [_P, _P, _P]),
+    "vs_nrc_forward": (C.c_int32, [_P, _P, C.c_uint32, _P]),
+    "vs_nrc_gradients": (C.c_int32, [_P, _P, C.c_uint32, _P, _P, _P]),
+    "vs_nrc_train": (C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, _P]),
+    "vs_nrc_last_error": (C.c_char_p, []),
+    "vs_nrc_destroy": (None, [_P]),
     "vs_server_destroy": (None, [_P]),
 }
 
@@ -520,6 +550,87 @@ class Server:
     def close(self):
         if self._h:
             lib().vs_server_destroy(self._h)
+            self._h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _nrc_check(st: int):
+    if st != VS_OK:
+        raise VisionError(st, lib().vs_nrc_last_error().decode())
+
+
+class RadianceCache:
+    """Online-trained radiance cache of an sRGB scene (vs_nrc_*)."""
+
+    def __init__(self, scene: "Scene", **cfg):
+        c = NrcConfig()
+        lib().vs_nrc_default_config(C.byref(c))
+        for k, v in cfg.items():
+            if k not in dict(NrcConfig._fields_):
+                raise KeyError(k)
+            setattr(c, k, float(v) if k == "learning_rate" else int(v))
+        self.scene = scene
+        self._h = _P()
+        _nrc_check(lib().vs_nrc_create(scene._h, C.byref(c), C.byref(self._h)))
+
+    def render(self, frames: int, spp: int, terminate: bool = True, train: bool = True):
+        flags = (NRC_TERMINATE if terminate else 0) | (NRC_TRAIN if train else 0)
+        _nrc_check(lib().vs_nrc_render(self._h, frames, spp, flags))
+        return self.stats()
+
+    def stats(self) -> dict:
+        s = NrcStats()
+        _nrc_check(lib().vs_nrc_stats_get(self._h, C.byref(s)))
+        return {f: getattr(s, f) for f, _ in NrcStats._fields_}
+
+    def levels(self) -> dict:
+        res = np.zeros(16, np.uint32)
+        mask = C.c_uint32()
+        lo = np.zeros(3, np.float32)
+        inv = np.zeros(3, np.float32)
+        _nrc_check(lib().vs_nrc_levels(self._h, res.ctypes.data, C.byref(mask), lo.ctypes.data, inv.ctypes.data))
+        return {"res": res, "dense_mask": mask.value, "lo": lo, "inv_ext": inv}
+
+    def params(self, grid: bool = True):
+        mlp = np.zeros(NRC_MLP_PARAMS, np.float32)
+        g = np.zeros(NRC_GRID_PARAMS, np.float32) if grid else None
+        _nrc_check(lib().vs_nrc_get_params(self._h, mlp.ctypes.data, None if g is None else g.ctypes.data))
+        return mlp, g
+
+    def set_params(self, mlp: np.ndarray, grid: Optional[np.ndarray] = None):
+        mlp = np.ascontiguousarray(mlp, np.float32)
+        g = None if grid is None else np.ascontiguousarray(grid, np.float32)
+        _nrc_check(lib().vs_nrc_set_params(self._h, mlp.ctypes.data, None if g is None else g.ctypes.data))
+
+    def forward(self, pos: np.ndarray) -> np.ndarray:
+        pos = np.ascontiguousarray(pos, np.float32).reshape(-1, 3)
+        out = np.zeros((len(pos), 3), np.float32)
+        _nrc_check(lib().vs_nrc_forward(self._h, pos.ctypes.data, len(pos), out.ctypes.data))
+        return out
+
+    def gradients(self, samples: np.ndarray, grid: bool = True):
+        sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 6)
+        loss = C.c_float()
+        gm = np.zeros(NRC_MLP_PARAMS, np.float32)
+        gg = np.zeros(NRC_GRID_PARAMS, np.float32) if grid else None
+        _nrc_check(lib().vs_nrc_gradients(self._h, sm.ctypes.data, len(sm), C.byref(loss), gm.ctypes.data,
+                                          None if gg is None else gg.ctypes.data))
+        return loss.value, gm, gg
+
+    def train(self, samples: np.ndarray, steps: int = 1) -> float:
+        sm = np.ascontiguousarray(samples, np.float32).reshape(-1, 6)
+        loss = C.c_float()
+        _nrc_check(lib().vs_nrc_train(self._h, sm.ctypes.data, len(sm), steps, C.byref(loss)))
+        return loss.value
+
+    def close(self):
+        if self._h:
+            lib().vs_nrc_destroy(self._h)
             self._h = _P()
 
     def __enter__(self):

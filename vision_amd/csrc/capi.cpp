@@ -228,10 +228,6 @@ void alloc_film(vs_scene *s) {
     s->frame_index = 0;
 }
 
-void require_built(vs_scene *s) {
-    if (s->ctx->device < 0) throw std::logic_error("host-only context: no device work possible");
-    if (!s->built) throw std::logic_error("accel not built: call vs_scene_build_accel first");
-}
 
 }// namespace
 

@@ -186,4 +186,10 @@ void ensure_alt_mode(vs_scene *s);
 // a view of the scene in sRGB (rgb) or hero-wavelength mode
 vs::SceneView view_in_mode(vs_scene *s, bool rgb);
 
+// device work needs a GPU context and a built scene
+inline void require_built(vs_scene *s) {
+    if (s->ctx->device < 0) throw std::logic_error("host-only context: no device work possible");
+    if (!s->built) throw std::logic_error("accel not built: call vs_scene_build_accel first");
+}
+
 }// namespace vs_internal
