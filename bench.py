@@ -383,12 +383,88 @@ def serve_main(args) -> None:
     print(json.dumps(line))
 
 
+def nrc_main(args) -> None:
+    """The radiance cache on the Cornell box at 1920x1080, 4 spp per frame x
+    128 frames, depth 8: paths stop into the cache at bounce 2, 65 536
+    training paths per frame, 4 Adam steps of 16 384 samples.  Against the
+    plain path tracer on the same frames; both judged by relMSE against a
+    4096-spp plain render."""
+    import numpy as np
+    import vision_amd as va
+    from vision_amd import scenegen
+
+    W, H, spp, frames = 1920, 1080, 4, args.nrc_frames
+    scene = json.dumps(scenegen.cornell_box(W, H, max_depth=8))
+    ctx = va.Context(0)
+
+    def relmse(x, r):
+        x, r = x[..., :3].astype(np.float64), r[..., :3].astype(np.float64)
+        return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+    ref = ctx.load_text(scene).build_accel()
+    for _ in range(args.nrc_ref_spp // 512):
+        ref.render(512, fused=True)
+    R = ref.film(0)
+    ref.close()
+    plain = ctx.load_text(scene).build_accel()
+    cached = ctx.load_text(scene).build_accel()
+    with va.RadianceCache(plain) as p0, va.RadianceCache(cached) as nrc:
+        for _ in range(args.warmup):
+            p0.render(1, spp, terminate=False, train=False)
+            nrc.render(1, spp)
+        plain.reset_film()
+        cached.reset_film()
+        t0 = time.perf_counter()
+        sp = p0.render(frames, spp, terminate=False, train=False)
+        plain.film(0)
+        t_plain = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        sn = nrc.render(frames, spp)
+        cached.film(0)
+        t_nrc = time.perf_counter() - t0
+        e_plain, e_nrc = relmse(plain.film(0), R), relmse(cached.film(0), R)
+        samples = W * H * spp * frames
+        # MFMA work of the cache per frame: forward on the queries, forward
+        # + backward (3x) on the training batches
+        f_fwd = 2 * (32 * 64 + 64 * 64 + 64 * 64 + 64 * 3)
+        q = sn["queries_last_frame"]
+        trained = min(sn["train_samples_last_frame"], 4 * 16384)
+        infer_tflops = f_fwd * q * frames / (sn["infer_ms"] / 1e3) / 1e12
+        line = {
+            "metric": METRIC, "value": round(samples / t_nrc / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
+            "steps": 1, "warmup": args.warmup, "ms_per_step": round(t_nrc * 1e3, 3), "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (MLP) / f32", "data": "synthetic",
+            "config": {"workload": f"radiance cache: Cornell box 1920x1080, {spp} spp/frame x {frames} frames, "
+                                   "depth 8, paths terminate into the cache at bounce 2; hash grid 16 levels x 2^19 x "
+                                   "2 fp16, MLP 32-64-64-64-3 ReLU on bf16 MFMA, Adam lr 1e-2; 65 536 training "
+                                   "paths/frame, 4 steps x 16 384 samples",
+                       "reference_spp": args.nrc_ref_spp},
+            "plain_msamples_per_s": round(samples / t_plain / 1e6, 3),
+            "relmse_cache": e_nrc, "relmse_plain": e_plain,
+            "stage_ms": {k: round(sn[k], 2) for k in ("render_ms", "train_paths_ms", "infer_ms", "train_ms",
+                                                      "accumulate_ms")},
+            "plain_render_ms": round(sp["render_ms"], 2),
+            "queries_per_frame": q, "train_samples_per_frame": sn["train_samples_last_frame"],
+            "last_loss": sn["last_loss"],
+            "roofline": {"bound": "mfma", "achieved": round(infer_tflops, 3), "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(infer_tflops / 2500.0, 5), "traffic": None,
+                         "kernel": "nrc_forward_kernel (hash-grid encoding + bf16 MLP inference)",
+                         "flops_per_query": f_fwd, "trained_samples_per_frame": trained},
+        }
+    plain.close()
+    cached.close()
+    ctx.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="serve", choices=["serve", "procedural", "cbox", "restir"])
+    ap.add_argument("--workload", default="serve", choices=["serve", "procedural", "cbox", "restir", "nrc"])
+    ap.add_argument("--nrc-frames", type=int, default=128, help="nrc: frames of 4 spp")
+    ap.add_argument("--nrc-ref-spp", type=int, default=4096, help="nrc: spp of the plain reference render")
     ap.add_argument("--jobs", type=int, default=1000, help="serve: jobs in the list")
     ap.add_argument("--seed", type=int, default=0, help="serve: job-list seed")
     ap.add_argument("--load", type=float, default=0.8, help="serve: Poisson rate / plain capacity")
@@ -423,6 +499,9 @@ def main():
         sys.exit(2)
     if args.workload == "serve":
         serve_main(args)
+        return
+    if args.workload == "nrc":
+        nrc_main(args)
         return
 
     import vision_amd as va

@@ -117,8 +117,8 @@ JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
 
 class NrcConfig(C.Structure):
     _fields_ = [("term_bounce", C.c_uint32), ("train_paths", C.c_uint32), ("train_steps", C.c_uint32),
-                ("learning_rate", C.c_float), ("min_res", C.c_uint32), ("max_res", C.c_uint32),
-                ("seed", C.c_uint32), ("reserved", C.c_uint32)]
+                ("train_batch", C.c_uint32), ("learning_rate", C.c_float), ("min_res", C.c_uint32),
+                ("max_res", C.c_uint32), ("seed", C.c_uint32)]
 
 
 class NrcStats(C.Structure):

@@ -84,7 +84,8 @@ void vs_nrc_default_config(vs_nrc_config *c) {
     std::memset(c, 0, sizeof(*c));
     c->term_bounce = 2;
     c->train_paths = 65536;
-    c->train_steps = 1;
+    c->train_steps = 4;
+    c->train_batch = 16384;
     c->learning_rate = 1e-2f;
     c->min_res = 16;
     c->max_res = 2048;
@@ -238,7 +239,7 @@ vs_status vs_nrc_gradients(vs_nrc *n, const float *samples, uint32_t count, floa
         hip_check(hipMemsetAsync(n->loss.ptr, 0, 4, st), "loss clear");
         hip_check(hipMemsetAsync(n->mlp_g.ptr, 0, sizeof(float) * vs::kNrcMlpParams, st), "clear");
         hip_check(hipMemsetAsync(n->grid_g.ptr, 0, sizeof(float) * vs::kNrcGridParams, st), "clear");
-        hip_check(vs::nrc_backward(n->view, n->io_a.as<vs::NrcSample>(), n->counts.as<uint32_t>(), count,
+        hip_check(vs::nrc_backward(n->view, n->io_a.as<vs::NrcSample>(), n->counts.as<uint32_t>(), count, 0, count,
                                    n->loss.as<float>(), 0, st),
                   "backward");
         hip_check(hipMemcpyAsync(loss, n->loss.ptr, 4, hipMemcpyDeviceToHost, st), "loss D2H");
@@ -264,8 +265,8 @@ vs_status vs_nrc_train(vs_nrc *n, const float *samples, uint32_t count, uint32_t
         hip_check(hipMemcpyAsync(n->counts.ptr, &count, 4, hipMemcpyHostToDevice, st), "count");
         for (uint32_t i = 0; i < steps; ++i) {
             hip_check(hipMemsetAsync(n->loss.ptr, 0, 4, st), "loss clear");
-            hip_check(vs::nrc_backward(n->view, n->io_a.as<vs::NrcSample>(), n->counts.as<uint32_t>(), count,
-                                       n->loss.as<float>(), 0, st),
+            hip_check(vs::nrc_backward(n->view, n->io_a.as<vs::NrcSample>(), n->counts.as<uint32_t>(), count, 0,
+                                       count, n->loss.as<float>(), 0, st),
                       "backward");
             adam_step(n, st);
         }
@@ -329,10 +330,14 @@ vs_status vs_nrc_render(vs_nrc *n, uint32_t frames, uint32_t spp, uint32_t flags
                           "infer");
             hip_check(hipEventRecord(n->ev[3], st), "event");
             if (train) {
+                // steps over consecutive batches of the frame's samples (all of
+                // them in each step when train_batch is 0)
+                const uint32_t batch = n->cfg.train_batch ? n->cfg.train_batch : n->sample_cap;
                 for (uint32_t k = 0; k < n->cfg.train_steps; ++k) {
+                    const uint32_t off = n->cfg.train_batch ? (k * batch) % n->sample_cap : 0u;
                     hip_check(hipMemsetAsync(n->loss.ptr, 0, 4, st), "loss clear");
-                    hip_check(vs::nrc_backward(n->view, n->samples.as<vs::NrcSample>(), tcount, n->sample_cap,
-                                               n->loss.as<float>(), 0, st),
+                    hip_check(vs::nrc_backward(n->view, n->samples.as<vs::NrcSample>(), tcount, n->sample_cap, off,
+                                               batch, n->loss.as<float>(), 0, st),
                               "backward");
                     adam_step(n, st);
                 }

@@ -71,11 +71,11 @@ struct NrcQuery {
 // ---- kernels (nrc.hip) ----
 // parity and utility entry points over n positions / samples
 hipError_t nrc_forward(const NrcView &v, const float *pos, uint32_t n, float *out, hipStream_t s);
-// forward + backward over min(*count, capacity) samples: gradients of the
-// mean loss accumulate into v.mlp_g / v.grid_g, the mean loss into *loss
-// (all must start at zero)
+// forward + backward over samples [offset, offset + batch) of the
+// min(*count, capacity) present: gradients of their mean loss accumulate
+// into v.mlp_g / v.grid_g, the mean loss into *loss (all must start at zero)
 hipError_t nrc_backward(const NrcView &v, const NrcSample *samples, const uint32_t *count, uint32_t capacity,
-                        float *loss, uint32_t blocks, hipStream_t s);
+                        uint32_t offset, uint32_t batch, float *loss, uint32_t blocks, hipStream_t s);
 // Adam step over all parameters with bias corrections c1 = 1 / (1 - 0.9^t),
 // c2 = 1 / (1 - 0.99^t); clears the gradients and refreshes the fp16 /
 // bf16 copies

@@ -18,26 +18,28 @@ namespace {
 constexpr int kBlock = 256;// 4 waves
 constexpr uint32_t kTrainStream = 1u << 30;
 
-// per-wave LDS of the forward-only kernels
+// per-wave LDS of the forward-only kernels: one 16-row tile at a time
+// (features, two hidden buffers used in turn, outputs), so that twelve
+// waves per CU fit beside the block's weights
 struct NrcFwdLds {
-    uint16_t x[64 * kXs];
-    uint16_t h[3][16 * kHs];
+    uint16_t x[16 * kXs];
+    uint16_t h[2][16 * kHs];
     float y[16 * 4];
 };
 
-// forward of one row tile on the forward-only scratch (same arithmetic as
-// nrc_forward_tile)
-__device__ __forceinline__ void fwd_tile(const NrcWeightsLds &w, NrcFwdLds &s, int rt, int lane) {
-    nrc_fwd_tile<kNrcIn, kNrcHidden>(s.x + rt * 16 * kXs, kXs, w.w1t, kXs, s.h[0], lane);
+// forward of one row tile (the arithmetic of nrc_forward_tile with the
+// hidden buffers reused: H3 overwrites H1 once layer 2 has read it)
+__device__ __forceinline__ void fwd_tile(const NrcWeightsLds &w, NrcFwdLds &s, int lane) {
+    nrc_fwd_tile<kNrcIn, kNrcHidden>(s.x, kXs, w.w1t, kXs, s.h[0], lane);
     __builtin_amdgcn_wave_barrier();
     nrc_fwd_tile<kNrcHidden, kNrcHidden>(s.h[0], kHs, w.w2t, kHs, s.h[1], lane);
     __builtin_amdgcn_wave_barrier();
-    nrc_fwd_tile<kNrcHidden, kNrcHidden>(s.h[1], kHs, w.w3t, kHs, s.h[2], lane);
+    nrc_fwd_tile<kNrcHidden, kNrcHidden>(s.h[1], kHs, w.w3t, kHs, s.h[0], lane);
     __builtin_amdgcn_wave_barrier();
     if (lane < 48) {
         int row = lane / 3, c = lane % 3;
         float acc = 0.f;
-        for (int k = 0; k < kNrcHidden; ++k) acc += bf2f(s.h[2][row * kHs + k]) * w.w4[k * kNrcOut + c];
+        for (int k = 0; k < kNrcHidden; ++k) acc += bf2f(s.h[0][row * kHs + k]) * w.w4[k * kNrcOut + c];
         s.y[row * 4 + c] = acc;
     }
     __builtin_amdgcn_wave_barrier();
@@ -69,13 +71,15 @@ __global__ void __launch_bounds__(kBlock) nrc_forward_kernel(NrcView v, const fl
         }
         uint16_t x[kNrcIn];
         nrc_encode(v, p, x);
-#pragma unroll
-        for (int k = 0; k < kNrcIn; k += 4)
-            *reinterpret_cast<bf16x4 *>(s.x + lane * kXs + k) = bf16x4{short(x[k]), short(x[k + 1]),
-                                                                         short(x[k + 2]), short(x[k + 3])};
-        __builtin_amdgcn_wave_barrier();
         for (int rt = 0; rt < 4; ++rt) {
-            fwd_tile(w, s, rt, lane);
+            if ((lane >> 4) == rt) {
+#pragma unroll
+                for (int k = 0; k < kNrcIn; k += 4)
+                    *reinterpret_cast<bf16x4 *>(s.x + (lane & 15) * kXs + k) =
+                        bf16x4{short(x[k]), short(x[k + 1]), short(x[k + 2]), short(x[k + 3])};
+            }
+            __builtin_amdgcn_wave_barrier();
+            fwd_tile(w, s, lane);
             if ((lane >> 4) == rt && valid) {
                 const int i = lane & 15;
                 float y0 = s.y[i * 4], y1 = s.y[i * 4 + 1], y2 = s.y[i * 4 + 2];
@@ -102,6 +106,7 @@ __global__ void __launch_bounds__(kBlock) nrc_forward_kernel(NrcView v, const fl
 // reach global memory once per block
 __global__ void __launch_bounds__(kBlock) nrc_backward_kernel(NrcView v, const NrcSample *__restrict__ samples,
                                                               const uint32_t *__restrict__ count, uint32_t cap,
+                                                              uint32_t offset, uint32_t batch,
                                                               float *__restrict__ loss) {
     __shared__ NrcWeightsLds w;
     __shared__ NrcGradLds gl;
@@ -116,7 +121,10 @@ __global__ void __launch_bounds__(kBlock) nrc_backward_kernel(NrcView v, const N
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     NrcWaveLds &s = ws[wave];
-    const uint32_t n = min(*count, cap);
+    // this step's samples: [offset, offset + batch) of the min(*count, cap) present
+    const uint32_t total = min(*count, cap);
+    const uint32_t n = total > offset ? min(total - offset, batch) : 0u;
+    samples += offset;
     const float inv = n ? 1.f / (3.f * float(n)) : 0.f;
     float loss_acc = 0.f;
     for (uint32_t base = (blockIdx.x * (kBlock / 64) + wave) * 64u; base < n; base += gridDim.x * kBlock) {
@@ -413,10 +421,11 @@ hipError_t nrc_infer(const NrcView &v, const NrcQuery *q, const uint32_t *count,
 }
 
 hipError_t nrc_backward(const NrcView &v, const NrcSample *samples, const uint32_t *count, uint32_t capacity,
-                        float *loss, uint32_t blocks, hipStream_t s) {
-    if (!capacity) return hipSuccess;
+                        uint32_t offset, uint32_t batch, float *loss, uint32_t blocks, hipStream_t s) {
+    if (!capacity || !batch) return hipSuccess;
     if (!blocks) blocks = grid_blocks();
-    hipLaunchKernelGGL(nrc_backward_kernel, dim3(blocks), dim3(kBlock), 0, s, v, samples, count, capacity, loss);
+    hipLaunchKernelGGL(nrc_backward_kernel, dim3(blocks), dim3(kBlock), 0, s, v, samples, count, capacity, offset,
+                       batch, loss);
     return hipGetLastError();
 }
 
