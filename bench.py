@@ -386,7 +386,7 @@ def serve_main(args) -> None:
 def nrc_main(args) -> None:
     """The radiance cache on the Cornell box at 1920x1080, 4 spp per frame x
     128 frames, depth 8: paths stop into the cache at bounce 2, 65 536
-    training paths per frame, 4 Adam steps of 16 384 samples.  Against the
+    training paths per frame, 4 Adam steps of 65 536 samples.  Against the
     plain path tracer on the same frames; both judged by relMSE against a
     4096-spp plain render."""
     import numpy as np
@@ -428,7 +428,7 @@ def nrc_main(args) -> None:
         # + backward (3x) on the training batches
         f_fwd = 2 * (32 * 64 + 64 * 64 + 64 * 64 + 64 * 3)
         q = sn["queries_last_frame"]
-        trained = min(sn["train_samples_last_frame"], 4 * 16384)
+        trained = 4 * min(sn["train_samples_last_frame"], 65536)
         infer_tflops = f_fwd * q * frames / (sn["infer_ms"] / 1e3) / 1e12
         line = {
             "metric": METRIC, "value": round(samples / t_nrc / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
@@ -437,7 +437,7 @@ def nrc_main(args) -> None:
             "config": {"workload": f"radiance cache: Cornell box 1920x1080, {spp} spp/frame x {frames} frames, "
                                    "depth 8, paths terminate into the cache at bounce 2; hash grid 16 levels x 2^19 x "
                                    "2 fp16, MLP 32-64-64-64-3 ReLU on bf16 MFMA, Adam lr 1e-2; 65 536 training "
-                                   "paths/frame, 4 steps x 16 384 samples",
+                                   "paths/frame, 4 steps x 65 536 samples",
                        "reference_spp": args.nrc_ref_spp},
             "plain_msamples_per_s": round(samples / t_plain / 1e6, 3),
             "relmse_cache": e_nrc, "relmse_plain": e_plain,

@@ -391,7 +391,7 @@ typedef struct vs_nrc_config {
     uint32_t term_bounce;  /* paths stop into the cache at this bounce (2) */
     uint32_t train_paths;  /* training paths per frame (65536) */
     uint32_t train_steps;  /* Adam steps per frame (4) */
-    uint32_t train_batch;  /* samples per step, consecutive slices of the frame's samples; 0: all (16384) */
+    uint32_t train_batch;  /* samples per step: consecutive slices of the frame's samples, wrapping; 0: all (65536) */
     float learning_rate;   /* Adam learning rate (1e-2) */
     uint32_t min_res;      /* coarsest level: cells per axis of the scene box (16) */
     uint32_t max_res;      /* finest level (2048) */

@@ -83,6 +83,8 @@ __device__ __forceinline__ float nrc_corner_weight(int k, const float f[3]) {
 __device__ __forceinline__ void nrc_encode(const NrcView &v, const float p[3], uint16_t *x) {
     float u[3];
     nrc_unit(v, p, u);
+    // four levels at a time: their 32 gathers are in flight together
+#pragma unroll 4
     for (int l = 0; l < kNrcLevels; ++l) {
         uint32_t c[3];
         float f[3];

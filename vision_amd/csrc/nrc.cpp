@@ -85,7 +85,7 @@ void vs_nrc_default_config(vs_nrc_config *c) {
     c->term_bounce = 2;
     c->train_paths = 65536;
     c->train_steps = 4;
-    c->train_batch = 16384;
+    c->train_batch = 65536;
     c->learning_rate = 1e-2f;
     c->min_res = 16;
     c->max_res = 2048;
@@ -334,7 +334,7 @@ vs_status vs_nrc_render(vs_nrc *n, uint32_t frames, uint32_t spp, uint32_t flags
                 // them in each step when train_batch is 0)
                 const uint32_t batch = n->cfg.train_batch ? n->cfg.train_batch : n->sample_cap;
                 for (uint32_t k = 0; k < n->cfg.train_steps; ++k) {
-                    const uint32_t off = n->cfg.train_batch ? (k * batch) % n->sample_cap : 0u;
+                    const uint32_t off = n->cfg.train_batch ? k * batch : 0u;// wrapped by the kernel
                     hip_check(hipMemsetAsync(n->loss.ptr, 0, 4, st), "loss clear");
                     hip_check(vs::nrc_backward(n->view, n->samples.as<vs::NrcSample>(), tcount, n->sample_cap, off,
                                                batch, n->loss.as<float>(), 0, st),

@@ -53,7 +53,8 @@ struct NrcView {
     float lo[3], inv_ext[3]; // position -> [0, 1]^3
 };
 
-// one training sample: position and the radiance scattered from it
+// one training sample: position and the radiance scattered from it,
+// divided by the surface's diffuse albedo (the query multiplies it back)
 struct NrcSample {
     float p[3];
     float t[3];

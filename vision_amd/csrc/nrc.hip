@@ -121,9 +121,11 @@ __global__ void __launch_bounds__(kBlock) nrc_backward_kernel(NrcView v, const N
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     NrcWaveLds &s = ws[wave];
-    // this step's samples: [offset, offset + batch) of the min(*count, cap) present
+    // this step's samples: [offset, offset + batch) of the min(*count, cap)
+    // present, the offset taken modulo that count
     const uint32_t total = min(*count, cap);
-    const uint32_t n = total > offset ? min(total - offset, batch) : 0u;
+    if (total) offset %= total;
+    const uint32_t n = min(total - offset, batch);
     samples += offset;
     const float inv = n ? 1.f / (3.f * float(n)) : 0.f;
     float loss_acc = 0.f;
@@ -229,6 +231,18 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool active) 
     return base + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// the cache learns radiance divided by the surface's diffuse albedo (sharp
+// colour edges then need no resolution in the grid): the albedo of diffuse
+// and Oren-Nayar materials, 1 for the others and for black surfaces
+__device__ __forceinline__ f3 nrc_albedo(const SceneView &sc, uint32_t mat) {
+    const MaterialRecord &m = sc.mat[mat];
+    if (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR) {
+        f3 a = mk3(m.p[0], m.p[1], m.p[2]);
+        if (a.x > 1e-4f && a.y > 1e-4f && a.z > 1e-4f) return a;
+    }
+    return mk3(1.f, 1.f, 1.f);
+}
+
 // the emitter term of a hit, as path_shade adds it (integrator.cpp:188-196)
 template <int MS>
 __device__ __forceinline__ void emission_at(const SceneView &sc, const RenderView &rv, PathState<3> &ps,
@@ -259,7 +273,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) nr
         path_begin<3, true>(cam, rv, inside ? px : 0u, inside ? py : 0u, frame0 + s, ps);
         if (!inside) ps.alive = 0u;
         bool has_q = false;
-        f3 qp = mk3(0.f, 0.f, 0.f);
+        f3 qp = mk3(0.f, 0.f, 0.f), qa = mk3(1.f, 1.f, 1.f);
         while (ps.alive) {
             TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, lds_stack + threadIdx.x, 64u, nf, tf);
             if (term && ps.bounce == term && !ps.tail && h.tri != kInvalid) {
@@ -269,6 +283,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) nr
                     emission_at<MS>(sc, rv, ps, h, it, inst);
                     has_q = true;
                     qp = it.pos;
+                    qa = nrc_albedo(sc, inst.mat);
                     ps.alive = 0u;
                     break;
                 }
@@ -293,9 +308,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) nr
             o.p[1] = qp.y;
             o.p[2] = qp.z;
             o.slot = slot;
-            o.beta[0] = ps.thr.v[0] * ps.fw;
-            o.beta[1] = ps.thr.v[1] * ps.fw;
-            o.beta[2] = ps.thr.v[2] * ps.fw;
+            o.beta[0] = ps.thr.v[0] * qa.x * ps.fw;
+            o.beta[1] = ps.thr.v[1] * qa.y * ps.fw;
+            o.beta[2] = ps.thr.v[2] * qa.z * ps.fw;
             o.pad = 0u;
         }
     }
@@ -325,9 +340,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) nr
         f3 beta = mk3(ps.thr.v[0], ps.thr.v[1], ps.thr.v[2]);
         if (h.tri != kInvalid && !ps.tail) {
             Surface it = surface_at(sc, h.tri, h.b1, h.b2, false);
-            if (sc.inst[it.inst].mat != kInvalid) {
+            const uint32_t mat = sc.inst[it.inst].mat;
+            if (mat != kInvalid) {
                 record = true;
                 pos = it.pos;
+                f3 a = nrc_albedo(sc, mat);
+                beta = mk3(beta.x * a.x, beta.y * a.y, beta.z * a.z);
             }
         }
         ShadowRay sr;
