@@ -128,6 +128,14 @@ typedef struct vs_scene_arrays {
      * 3x4 rows of the current camera ([20..31]) */
     const float *restir;
     uint32_t num_restir;
+    /* images of textured colour slots (a material record's flag bit 16 + k
+     * marks slot p[4k..4k+3] as textured, p[4k] holding the texture index):
+     * float4 texels in the scene's spectrum mode (linear sRGB, or albedo
+     * sigmoid coefficients) and per texture {first texel, width, height, 0} */
+    const float *textures;
+    uint32_t num_texels;
+    const uint32_t *texture_info;
+    uint32_t num_textures;
 } vs_scene_arrays;
 
 /* library version (major*10000 + minor*100 + patch) */

@@ -52,6 +52,11 @@ struct Spectral {
     const float *lut_diel{nullptr}, *lut_diel_inv{nullptr}, *lut_spec{nullptr}, *lut_coat{nullptr};
     const float *lut_pure{nullptr}, *lut_sheen{nullptr};
     const float *rgb2spec{nullptr};
+    // images of textured colour slots (library export): float4 texels and
+    // {first texel, width, height, 0} per texture
+    const float *tex{nullptr};
+    const uint32_t *tex_info{nullptr};
+    uint32_t ntex{0};
 };
 
 struct Spec {
@@ -194,6 +199,26 @@ Spec albedo_slot(const Spectral &sd, const float *slot, const Swl &w) {
         for (int i = 0; i < w.n; ++i) r.v[i] = sigmoid_poly(slot, w.lambda[i]);
     }
     return r;
+}
+
+// a colour slot that may name an image (ImageNode::evaluate,
+// src/render_core/shadernodes/image.cpp:84-95): flag bit 16 + off / 4 marks
+// it, p[off] holds the texture index; the texel is the nearest one at uv
+// wrapped to [0, 1) with v = 0 the first row (the filtering is ocarina's,
+// absent here: parity unpinned)
+Spec albedo_at(const Spectral &sd, uint32_t flags, const float *p, int off, float u, float v, const Swl &w) {
+    if (flags & (1u << (16 + off / 4))) {
+        uint32_t t;
+        std::memcpy(&t, p + off, 4);
+        if (t >= sd.ntex) return sfill(0.f);
+        const uint32_t *info = sd.tex_info + 4 * t;
+        float uu = u - std::floor(u), vv = v - std::floor(v);
+        uint32_t x = std::min(uint32_t(uu * float(info[1])), info[1] - 1u);
+        uint32_t y = std::min(uint32_t(vv * float(info[2])), info[2] - 1u);
+        const float *q = sd.tex + 4 * (size_t(info[0]) + size_t(y) * info[1] + x);
+        return albedo_slot(sd, q, w);
+    }
+    return albedo_slot(sd, p + off, w);
 }
 
 // RGBIlluminationSpectrum (hero.cpp:171-186) times a light scale; `coef`
@@ -754,7 +779,7 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
         case M_DIFFUSE:
         case M_OREN: {
             L.kind = type == M_DIFFUSE ? LB_LAMBERT : LB_OREN;
-            L.kr = albedo_slot(sd, p, w);
+            L.kr = albedo_at(sd, flags, p, 0, it.u, it.v, w);
             if (type == M_OREN) {// OrenNayar::OrenNayar (bxdf.cpp:86-92)
                 float sigma = p[4] * (kPi * 0.5f);
                 float sigma2 = (sigma * sigma) * (sigma * sigma);
@@ -785,7 +810,7 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
         }
         case M_GLASS: {
             L.kind = LB_DIELECTRIC;
-            L.kr = albedo_slot(sd, p, w);
+            L.kr = albedo_at(sd, flags, p, 0, it.u, it.v, w);
             L.mf = {p[5], p[6]};
             L.fr.kind = FR_DIELECTRIC;
             bool front = dotp(it.wo, it.ng) > 0.f;// Interaction::correct_eta (interaction.cpp:85-88)
@@ -803,8 +828,8 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
         }
         default: {// principled
             b.set = true;
-            Spec color = albedo_slot(sd, p, w);
-            Spec spec_tint = albedo_slot(sd, p + 4, w);
+            Spec color = albedo_at(sd, flags, p, 0, it.u, it.v, w);
+            Spec spec_tint = albedo_at(sd, flags, p, 4, it.u, it.v, w);
             float ior = p[17];
             float roughness = std::min(std::max(p[18], 0.0001f), 1.f);
             float anisotropic = p[19];
@@ -817,7 +842,7 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
             {// sheen
                 Lobe &S = b.lobes[k++];
                 S.kind = LB_SHEEN;
-                Spec tint = albedo_slot(sd, p + 8, w) * (p[20] * front) * weight;
+                Spec tint = albedo_at(sd, flags, p, 8, it.u, it.v, w) * (p[20] * front) * weight;
                 float ltc[4];
                 lut2(sd.lut_sheen, 32, 4, cos_theta, p[21], ltc);
                 S.ltc_a = ltc[0];
@@ -835,7 +860,7 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
                 C.mf = {cc_rough, cc_rough};
                 C.fr.kind = FR_DIELECTRIC;
                 C.fr.eta = sfill(p[24]);
-                C.kr = cc_weight * weight * albedo_slot(sd, p + 12, w);
+                C.kr = cc_weight * weight * albedo_at(sd, flags, p, 12, it.u, it.v, w);
                 float x = std::sqrt(std::sqrt(C.mf.ax * C.mf.ay));
                 float z = (p[24] - 1.003f) / (4.f - 1.003f);
                 float sv;
@@ -2205,6 +2230,9 @@ typedef struct or_spectral {
     const float *env_rgb;
     float env_w2o[9], env_o2w[9];
     float env_scale, world_radius;
+    const float *tex;
+    const uint32_t *tex_info;
+    uint32_t ntex;
 } or_spectral;
 
 static Spectral to_spectral(const or_spectral *x) {
@@ -2221,6 +2249,9 @@ static Spectral to_spectral(const or_spectral *x) {
     sd.lut_pure = x->lut_pure;
     sd.lut_sheen = x->lut_sheen;
     sd.rgb2spec = x->rgb2spec;
+    sd.tex = x->tex;
+    sd.tex_info = x->tex_info;
+    sd.ntex = x->ntex;
     return sd;
 }
 

@@ -330,7 +330,8 @@ class OrSpectral(C.Structure):
                 ("lut_coat", C.c_void_p), ("lut_pure", C.c_void_p), ("lut_sheen", C.c_void_p),
                 ("rgb2spec", C.c_void_p), ("env_light", C.c_uint32), ("env_w", C.c_uint32), ("env_h", C.c_uint32),
                 ("env_rgb", C.c_void_p), ("env_w2o", C.c_float * 9), ("env_o2w", C.c_float * 9),
-                ("env_scale", C.c_float), ("world_radius", C.c_float)]
+                ("env_scale", C.c_float), ("world_radius", C.c_float), ("tex", C.c_void_p),
+                ("tex_info", C.c_void_p), ("ntex", C.c_uint32)]
 
 
 def data(name: str, count: int | None = None) -> np.ndarray:
@@ -386,6 +387,12 @@ class Spectral:
         s.world_radius = 1.0
         if scene is not None:
             s.world_radius = scene["world_radius"]
+            if len(scene.get("texture_info", [])):
+                self.tex = np.ascontiguousarray(scene["textures"], np.float32)
+                self.tex_info = np.ascontiguousarray(scene["texture_info"], np.uint32)
+                s.tex = self.tex.ctypes.data
+                s.tex_info = self.tex_info.ctypes.data
+                s.ntex = len(self.tex_info)
             if scene["env_light"] != 0xFFFFFFFF:
                 self.env_rgb = np.ascontiguousarray(scene["env_rgb"], np.float32)
                 s.env_light = scene["env_light"]

@@ -361,6 +361,7 @@ struct Surf {
     Frame fr;
     float area;
     uint32_t inst;
+    float u, v;// texture coordinates (barycentric blend of the vertices')
 };
 Surf surface(const Scene &s, uint32_t t, float b1, float b2, bool complete) {
     Surf o{};
@@ -374,6 +375,12 @@ Surf surface(const Scene &s, uint32_t t, float b1, float b2, bool complete) {
     o.area = 0.5f * l;
     o.ng = mul(ngu, 1.f / l);
     o.inst = s.tri_inst[t];
+    {
+        float u0 = s.UV[2 * i0], v0 = s.UV[2 * i0 + 1], u1 = s.UV[2 * i1], v1 = s.UV[2 * i1 + 1];
+        float u2 = s.UV[2 * i2], v2 = s.UV[2 * i2 + 1];
+        o.u = u0 * b0 + u1 * b1 + u2 * b2;
+        o.v = v0 * b0 + v1 * b1 + v2 * b2;
+    }
     if (complete) {
         float u0 = s.UV[2 * i0], v0 = s.UV[2 * i0 + 1], u1 = s.UV[2 * i1], v1 = s.UV[2 * i1 + 1];
         float u2 = s.UV[2 * i2], v2 = s.UV[2 * i2 + 1];

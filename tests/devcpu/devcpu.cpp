@@ -25,7 +25,7 @@ static void run(const uint32_t *mat, const float *spd, const float *wo, const fl
         Frame fr{mk3(1, 0, 0), mk3(0, 1, 0), mk3(0, 0, 1)};
         f3 wo3 = mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]);
         bool disp;
-        MatCtx<NS> c = make_material<NS, RGB>(sc, rec, fr, mk3(0, 0, 1), wo3, swl, disp);
+        MatCtx<NS> c = make_material<NS, RGB>(sc, rec, fr, mk3(0, 0, 1), wo3, swl, disp, mk2(0, 0));
         if (!sample) {
             EvalN<NS> e = mat_eval(c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr);
             for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? e.f.v[k] : 0.f;
@@ -104,6 +104,8 @@ extern "C" void *dc_load(const char *text, const char *base) {
     v.world_diameter = 2.f * pk.world_radius;
     for (int i = 0; i < 9; ++i) v.env_w2o[i] = pk.env_w2o[i], v.env_o2w[i] = pk.env_o2w[i];
     v.env_scale = pk.env_scale;
+    v.tex = reinterpret_cast<const float4 *>(pk.tex.data());
+    v.tex_info = reinterpret_cast<const uint4 *>(pk.tex_info.data());
     v.stack_size = s->bvh.max_depth;
     v.ray_offset_factor = h.ray_offset_factor;
     v.num_tris = uint32_t(h.wTri.size());

@@ -224,3 +224,12 @@ def test_device_restir_source_equals_oracle_near_specular(dc):
     # near-specular surfaces were replaced by what they reflect
     flags = orc.surf[orc.frame & 1, 0, :, 3].view(np.uint32)
     assert np.any((flags >> 16) == 1)
+
+
+@pytest.mark.parametrize("material,spectrum", [("diffuse", "srgb"), ("principled", "srgb"), ("diffuse", "hero4")])
+def test_device_path_source_equals_oracle_textured(dc, tmp_path, material, spectrum):
+    with va.Context(-1) as c:
+        c.write_image(tmp_path / "checker.png", scenegen.checker_rgba(16, 8))
+    scene = scenegen.textured_box("checker.png", width=24, height=24, material=material, spectrum=spectrum)
+    film, ref = _dev_render(dc, scene, 2, str(tmp_path))
+    assert _bits_equal(film, ref), (material, spectrum)

@@ -227,3 +227,19 @@ def test_direct_only_bounce_parity(gpu_ctx, kernel):
     frac, _ = _compare(gpu, ref)
     assert frac >= 0.999
     sc.close()
+
+
+@pytest.mark.parametrize("material,spectrum", [("diffuse", "srgb"), ("principled", "srgb"), ("diffuse", "hero4")])
+def test_textured_colour_parity(gpu_ctx, tmp_path, material, spectrum):
+    """A floor whose colour slot is a PNG checkerboard (ImageNode)."""
+    gpu_ctx.write_image(tmp_path / "checker.png", scenegen.checker_rgba(16, 8))
+    scene = scenegen.textured_box("checker.png", width=64, height=64, material=material, spectrum=spectrum)
+    sc = gpu_ctx.load_text(scene, base_dir=str(tmp_path))
+    sc.build_accel()
+    sc.render(2, fused=True)
+    gpu = sc.film(0)
+    ref = oracle.render(sc.export(), 0, 2)
+    frac, _ = _compare(gpu, ref)
+    print(f"textured {material} {spectrum}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
+    assert frac >= 0.999
+    sc.close()

@@ -77,7 +77,9 @@ class SceneArrays(C.Structure):
                 ("spectrum_dim", C.c_uint32), ("env_light", C.c_uint32), ("env_w", C.c_uint32),
                 ("env_h", C.c_uint32), ("env_rgb", C.POINTER(C.c_float)), ("env_w2o", C.c_float * 9),
                 ("env_o2w", C.c_float * 9), ("env_scale", C.c_float), ("world_radius", C.c_float),
-                ("restir", C.POINTER(C.c_float)), ("num_restir", C.c_uint32)]
+                ("restir", C.POINTER(C.c_float)), ("num_restir", C.c_uint32),
+                ("textures", C.POINTER(C.c_float)), ("num_texels", C.c_uint32),
+                ("texture_info", C.POINTER(C.c_uint32)), ("num_textures", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -348,6 +350,8 @@ class Scene:
             "env_scale": float(a.env_scale),
             "world_radius": float(a.world_radius),
             "restir": arr(a.restir, a.num_restir, np.float32),
+            "textures": arr(a.textures, 4 * a.num_texels, np.float32).reshape(-1, 4),
+            "texture_info": arr(a.texture_info, 4 * a.num_textures, np.uint32).reshape(-1, 4),
         }
 
     def bvh(self) -> tuple[np.ndarray, np.ndarray]:

@@ -63,6 +63,8 @@ void upload_mode_tables(const vs::PackedScene &pk, ModeTables &t, hipStream_t st
     t.env_cond_func.upload(pk.env_cond_func, st);
     t.env_marg.upload(pk.env_marg, st);
     t.env_marg_func.upload(pk.env_marg_func, st);
+    t.tex.upload(pk.tex, st);
+    t.tex_info.upload(pk.tex_info, st);
 }
 
 void set_mode_view(vs::SceneView &v, const vs::PackedScene &pk, const ModeTables &t) {
@@ -91,6 +93,8 @@ void set_mode_view(vs::SceneView &v, const vs::PackedScene &pk, const ModeTables
         v.env_o2w[i] = pk.env_o2w[i];
     }
     v.env_scale = pk.env_scale;
+    v.tex = t.tex.as<float4>();
+    v.tex_info = t.tex_info.as<uint4>();
 }
 
 vs::RenderView render_view(const vs_scene *s) {
@@ -473,6 +477,10 @@ vs_status vs_scene_export(const vs_scene *s, vs_scene_arrays *out) {
     out->world_radius = pk.world_radius;
     out->restir = s->exp_restir.data();
     out->num_restir = uint32_t(s->exp_restir.size());
+    out->textures = s->packed.tex.data();
+    out->num_texels = uint32_t(s->packed.tex.size() / 4);
+    out->texture_info = s->packed.tex_info.data();
+    out->num_textures = uint32_t(s->packed.tex_info.size() / 4);
     return VS_OK;
 }
 

@@ -75,8 +75,10 @@ struct DevBuf {
 // wavelengths): materials, lights and their alias tables, spectral curves,
 // the environment map
 struct ModeTables {
-    DevBuf mat, light, alias, func, spd, env_tex, env_cond, env_cond_func, env_marg, env_marg_func;
+    DevBuf mat, light, alias, func, spd, env_tex, env_cond, env_cond_func, env_marg, env_marg_func, tex, tex_info;
     void swap(ModeTables &o) noexcept {
+        tex.swap(o.tex);
+        tex_info.swap(o.tex_info);
         mat.swap(o.mat);
         light.swap(o.light);
         alias.swap(o.alias);

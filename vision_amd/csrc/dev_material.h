@@ -49,6 +49,24 @@ __device__ __forceinline__ Spec<NS> eval_albedo(const float *slot, const Swl<NS>
     return r;
 }
 
+// a colour slot of material m at p[off]: its constant, or the texel of the
+// image it names at uv (nearest texel, uv wrapped to [0, 1), v = 0 the first
+// row)
+template <int NS, bool RGB>
+__device__ __forceinline__ Spec<NS> slot_albedo(const SceneView &sc, const MaterialRecord &m, int off, f2 uv,
+                                                const Swl<NS> &swl) {
+    if (m.flags & (MATF_TEXTURE_SLOT0 << (off >> 2))) {
+        const uint4 info = sc.tex_info[__float_as_uint(m.p[off])];
+        const float u = uv.x - floorf(uv.x), v = uv.y - floorf(uv.y);
+        const uint32_t x = min(uint32_t(u * float(info.y)), info.y - 1u);
+        const uint32_t y = min(uint32_t(v * float(info.z)), info.z - 1u);
+        const float4 t = sc.tex[size_t(info.x) + size_t(y) * info.y + x];
+        const float q[4] = {t.x, t.y, t.z, t.w};
+        return eval_albedo<NS, RGB>(q, swl);
+    }
+    return eval_albedo<NS, RGB>(m.p + off, swl);
+}
+
 template <int NS>
 struct MatCtx {
     uint32_t type;
@@ -107,7 +125,7 @@ __device__ __forceinline__ float pure_reflection_comp(const float *lut, float ax
 // ior curve), on which the caller drops the secondary wavelengths.
 template <int NS, bool RGB, int MS = MS_ALL>
 __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const MaterialRecord &m, const Frame &fr,
-                                                    f3 ng, f3 wo_w, const Swl<NS> &swl, bool &dispersive) {
+                                                    f3 ng, f3 wo_w, const Swl<NS> &swl, bool &dispersive, f2 uv) {
     MatCtx<NS> c;
     c.type = m.type;
     c.fr = fr;
@@ -117,7 +135,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
     dispersive = false;
     const float *p = m.p;
     if (ms_has<MS>(MS_DIFFUSE) && (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR)) {
-        c.a0 = eval_albedo<NS, RGB>(p, swl);
+        c.a0 = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
         if (m.type == MAT_OREN_NAYAR) {
             float sig = p[4] * (kPi * 0.5f);
             float s2 = (sig * sig) * (sig * sig);
@@ -144,7 +162,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
         c.comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
     } else if (ms_has<MS>(MS_GLASS) && m.type == MAT_GLASS) {
         c.ggx = {p[5], p[6]};
-        c.a0 = eval_albedo<NS, RGB>(p, swl);
+        c.a0 = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
         bool front = dot(wo_w, ng) > 0.f;// Interaction::correct_eta
         if (!RGB && (m.flags & MATF_DISPERSIVE)) {
             dispersive = true;
@@ -162,8 +180,8 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
         c.comp = dielectric_comp(sc.lut, c.ggx.ax, c.ggx.ay, fabsf(c.wo.z), c.eta0);
     } else if (ms_has<MS>(MS_PRINCIPLED) && m.type == MAT_PRINCIPLED) {
         // PrincipledBSDF::create_lobe_set (principled_bsdf.cpp:352-461)
-        Spec<NS> color = eval_albedo<NS, RGB>(p, swl);
-        Spec<NS> spec_tint = eval_albedo<NS, RGB>(p + 4, swl);
+        Spec<NS> color = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
+        Spec<NS> spec_tint = slot_albedo<NS, RGB>(sc, m, 4, uv, swl);
         float ior = p[17];
         float rough = fminf(fmaxf(p[18], 0.0001f), 1.f);
         float aniso = p[19];
@@ -175,7 +193,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
         float front = cos_theta > 0.f ? 1.f : 0.f;
         float sw[6];
         {   // sheen (SheenLTC, approximate mode)
-            Spec<NS> tint = eval_albedo<NS, RGB>(p + 8, swl) * (p[20] * front) * weight;
+            Spec<NS> tint = slot_albedo<NS, RGB>(sc, m, 8, uv, swl) * (p[20] * front) * weight;
             float ltc[4];
             lut2<4>(sc.lut + LUT_SHEEN, 32, cos_theta, p[21], ltc);
             c.sh_a = ltc[0];
@@ -190,7 +208,7 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
             cr = cr * cr;
             c.cc_alpha = cr;
             c.cc_eta = p[24];
-            c.cc_kr = (cw * weight) * eval_albedo<NS, RGB>(p + 12, swl);
+            c.cc_kr = (cw * weight) * slot_albedo<NS, RGB>(sc, m, 12, uv, swl);
             float s;
             lut3<1>(sc.lut + LUT_COAT, 32, sqrtf(sqrtf(cr * cr)), cos_theta, (c.cc_eta - 1.003f) / (4.f - 1.003f), &s);
             Spec<NS> cc_albedo = c.cc_kr * s;

@@ -371,7 +371,8 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
     // next-event estimation (direct_lighting, integrator.cpp:22-38)
     LightSampleN<NS> ls = sample_light<NS, RGB>(sc, it.pos, ps.sampler, ps.swl);
     bool dispersive;
-    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, ps.swl, dispersive);
+    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[inst.mat], it.frame, it.ng, it.wo, ps.swl, dispersive,
+                                                   it.uv);
     if (!RGB && dispersive) {
         // SampledWavelengths::check_dispersive (spectrum.cpp:21-39)
         bool secondary_valid = NS > 1;

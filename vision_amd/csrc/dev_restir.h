@@ -264,7 +264,7 @@ __device__ __forceinline__ MatCtx<NS> material_at(const SceneView &sc, const Sur
         return c;
     }
     bool dispersive;
-    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[m], it.frame, it.ng, it.wo, swl, dispersive);
+    MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[m], it.frame, it.ng, it.wo, swl, dispersive, it.uv);
     collapse_if_dispersive<NS, RGB>(swl, dispersive);
     return mc;
 }
@@ -694,7 +694,8 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
             // metal and glass only: the materials with enable_delta whose
             // flag can be near-specular (material.h:264)
             bool dispersive;
-            MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[mat], it.frame, it.ng, it.wo, swl, dispersive);
+            MatCtx<NS> mc = make_material<NS, RGB, MS>(sc, sc.mat[mat], it.frame, it.ng, it.wo, swl, dispersive,
+                                                           it.uv);
             f3 t;
             sample_delta(mc, sp, w_next, t);
             ext.throughput = ext.throughput * t;

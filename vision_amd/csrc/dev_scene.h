@@ -52,6 +52,10 @@ struct SceneView {
     float env_o2w[9];
     float env_scale;
     float pad2;
+    // images of textured colour slots: float4 texels, per texture {first
+    // texel, width, height, 0}
+    const float4 *tex;
+    const uint4 *tex_info;
 };
 
 struct CameraView {

@@ -97,7 +97,8 @@ __global__ void __launch_bounds__(256) bsdf_kernel(SceneView sc, uint32_t mat, c
     Frame fr{mk3(1.f, 0.f, 0.f), mk3(0.f, 1.f, 0.f), mk3(0.f, 0.f, 1.f)};
     bool dispersive;
     MatCtx<NS> c = make_material<NS, RGB>(sc, sc.mat[mat], fr, mk3(0.f, 0.f, 1.f),
-                                          mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), swl, dispersive);
+                                          mk3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), swl, dispersive,
+                                          mk2(0.f, 0.f));
     Spec<NS> fv;
     float p;
     if (sample) {

@@ -232,11 +232,12 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool active) 
 }
 
 // the cache learns radiance divided by the surface's diffuse albedo (sharp
-// colour edges then need no resolution in the grid): the albedo of diffuse
-// and Oren-Nayar materials, 1 for the others and for black surfaces
+// colour edges then need no resolution in the grid): the constant albedo of
+// diffuse and Oren-Nayar materials, 1 for the others, for textured colours
+// and for black surfaces
 __device__ __forceinline__ f3 nrc_albedo(const SceneView &sc, uint32_t mat) {
     const MaterialRecord &m = sc.mat[mat];
-    if (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR) {
+    if ((m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR) && !(m.flags & MATF_TEXTURE_SLOT0)) {
         f3 a = mk3(m.p[0], m.p[1], m.p[2]);
         if (a.x > 1e-4f && a.y > 1e-4f && a.z > 1e-4f) return a;
     }

@@ -21,7 +21,10 @@ enum LightType : uint32_t {
     LIGHT_ENV = 3,
 };
 
-// 128 B material record; meaning of p[] depends on type (see scene.cpp pack)
+// 128 B material record; meaning of p[] depends on type (see scene.cpp pack).
+// A colour slot of four floats at p[4 k] (k = 0..3) that names an image
+// holds the texture index (as bits) in p[4 k] and sets flag bit 16 + k.
+constexpr uint32_t MATF_TEXTURE_SLOT0 = 1u << 16;
 struct MaterialRecord {
     uint32_t type;
     uint32_t flags;

@@ -380,6 +380,62 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
 
 }// namespace
 
+const Json *slot_image_node(const Json &param, const char *key) {
+    const Json &j = param[key];
+    if (j.is_null()) return nullptr;
+    const Json *node = (j.is_object() && j.contains("channels")) ? &j["node"] : &j;
+    if (!node->is_object()) return nullptr;
+    if ((*node)["type"].as_string("") == "image" || (*node)["param"].contains("fn") || node->contains("fn"))
+        return node;
+    return nullptr;
+}
+
+namespace {
+
+// resolved path, colour space and the registry key of an image node
+void texture_key(const SceneHost &s, const Json &node, std::string &path, std::string &cs, std::string &key) {
+    const Json &p = node.contains("param") ? node["param"] : node;
+    std::string fn = p["fn"].as_string("");
+    if (fn.empty()) throw std::runtime_error("image node without \"fn\"");
+    path = join_path(s.base_dir, fn);
+    cs = p["color_space"].as_string("");
+    std::string e = image_extension(path);
+    if (cs.empty()) cs = (e == "exr" || e == "hdr" || e == "pfm") ? "linear" : "srgb";
+    if (cs != "linear" && cs != "srgb") throw std::runtime_error("unsupported colour space " + cs + " for " + fn);
+    key = path + "|" + cs;
+}
+
+}// namespace
+
+uint32_t find_texture(const SceneHost &s, const Json &node) {
+    std::string path, cs, key;
+    texture_key(s, node, path, cs, key);
+    for (size_t i = 0; i < s.textures.size(); ++i)
+        if (s.textures[i].key == key) return uint32_t(i);
+    throw std::logic_error("texture not loaded: " + path);
+}
+
+uint32_t texture_index(SceneHost &s, const Json &node) {
+    std::string path, cs, key;
+    texture_key(s, node, path, cs, key);
+    for (size_t i = 0; i < s.textures.size(); ++i)
+        if (s.textures[i].key == key) return uint32_t(i);
+    ImageRGBA img = read_image(path);
+    TextureHost t;
+    t.key = key;
+    t.width = img.width;
+    t.height = img.height;
+    t.rgb.resize(size_t(img.width) * img.height * 3);
+    const bool srgb = cs == "srgb";
+    for (size_t i = 0; i < size_t(img.width) * img.height; ++i)
+        for (int c = 0; c < 3; ++c) {
+            float v = img.rgba[i * 4 + c];
+            t.rgb[i * 3 + c] = srgb ? srgb_to_linear(v) : v;
+        }
+    s.textures.push_back(std::move(t));
+    return uint32_t(s.textures.size() - 1);
+}
+
 void load_env_image(const std::string &path, std::vector<float> &rgb, uint32_t &w, uint32_t &h) {
     ImageRGBA img = read_image(path);
     // 8/16-bit files hold sRGB-encoded values unless the scene says otherwise
@@ -510,6 +566,9 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                 mh.type = m["type"].as_string("diffuse");
                 mh.name = m["name"].as_string("");
                 mh.param = m["param"];
+                // colour slots may name images; load them once here
+                for (const char *key : {"color", "spec_tint", "sheen_tint", "coat_tint"})
+                    if (const Json *node = slot_image_node(mh.param, key)) texture_index(s, *node);
                 s.materials.push_back(mh);
             }
         }

@@ -37,6 +37,14 @@ struct MaterialHost {
     Json param;
 };
 
+// an image used as a colour parameter (ImageNode, src/render_core/shadernodes/image.cpp):
+// linear RGB rows, top first
+struct TextureHost {
+    std::string key;// resolved path + "|" + colour space
+    uint32_t width{0}, height{0};
+    std::vector<float> rgb;
+};
+
 struct LightHost {
     uint32_t type{LIGHT_AREA};
     std::string name;
@@ -119,6 +127,7 @@ struct SceneHost {
     float ray_offset_factor{1.f};
     float min_world_radius{10.f};
     f3 aabb_lo{0, 0, 0}, aabb_hi{0, 0, 0};// world bounds of all triangles (after flatten)
+    std::vector<TextureHost> textures;     // images the materials' colour slots name
     uint32_t output_spp{0};
     std::string output_fn{"output.png"};
 
@@ -142,6 +151,16 @@ void load_env_image(const std::string &path, std::vector<float> &rgb, uint32_t &
 
 // fn relative to dir unless absolute or dir is empty
 std::string join_path(const std::string &dir, const std::string &fn);
+
+// the index of the image an image shader node names in s.textures (loading
+// it on first use); colour_space "srgb" decodes 8/16-bit values, "linear"
+// keeps them, unset means linear for .exr / .hdr / .pfm and sRGB otherwise
+// (ImagePool::load_texture, src/base/mgr/image_pool.cpp:13-34)
+uint32_t texture_index(SceneHost &s, const Json &node);
+// the index of an already loaded image node
+uint32_t find_texture(const SceneHost &s, const Json &node);
+// the image node of a parameter slot, or nullptr for constants
+const Json *slot_image_node(const Json &param, const char *key);
 
 // Parse a scene from JSON text; base_dir resolves relative mesh/texture paths.
 SceneHost load_scene(const std::string &text, const std::string &base_dir);

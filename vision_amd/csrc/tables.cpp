@@ -112,13 +112,25 @@ uint32_t append_curve(std::vector<float> &spd, const std::vector<float> &c) {
     return off;
 }
 
-MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float> &spd) {
+// a colour slot at p[off]: a constant, or the index of the image it names
+void put_color_slot(MaterialRecord &r, int off, const SceneHost &s, const Json &p, const char *key, f3 def,
+                    bool is_rgb) {
+    if (const Json *node = slot_image_node(p, key)) {
+        r.p[off] = ubits_f(find_texture(s, *node));
+        r.p[off + 1] = r.p[off + 2] = r.p[off + 3] = 0.f;
+        r.flags |= MATF_TEXTURE_SLOT0 << (off / 4);
+        return;
+    }
+    put_slot(r, off, slot3(p, key, def), is_rgb);
+}
+
+MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float> &spd, const SceneHost &s) {
     MaterialRecord r{};
     const Json &p = m.param;
     if (m.type == "diffuse") {
         // DiffuseMaterial::initialize_slots (diffuse.cpp:38-46)
         r.type = MAT_DIFFUSE;
-        put_slot(r, 0, slot3(p, "color", mk3(0.5f)), rgb);
+        put_color_slot(r, 0, s, p, "color", mk3(0.5f), rgb);
         if (p.contains("sigma")) {
             r.type = MAT_OREN_NAYAR;
             r.p[4] = std::clamp(slot1(p, "sigma", 0.5f), 0.f, 1.f);
@@ -146,7 +158,7 @@ MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float>
     } else if (m.type == "glass") {
         // GlassMaterial (glass.cpp:171-258)
         r.type = MAT_GLASS;
-        put_slot(r, 0, slot3(p, "color", mk3(1.f)), rgb);
+        put_color_slot(r, 0, s, p, "color", mk3(1.f), rgb);
         float rough = std::clamp(slot1(p, "roughness", 0.5f), 0.01f, 1.f);
         float aniso = std::clamp(slot1(p, "anisotropic", 0.f), -0.9f, 0.9f);
         if (p["remapping_roughness"].as_bool(true)) rough = rough * rough;
@@ -167,10 +179,10 @@ MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float>
     } else if (m.type == "principled_bsdf") {
         // PrincipledBSDF::initialize_slots (principled_bsdf.cpp:281-307)
         r.type = MAT_PRINCIPLED;
-        put_slot(r, 0, slot3(p, "color", mk3(1.f)), rgb);
-        put_slot(r, 4, slot3(p, "spec_tint", mk3(1.f)), rgb);
-        put_slot(r, 8, slot3(p, "sheen_tint", mk3(1.f)), rgb);
-        put_slot(r, 12, slot3(p, "coat_tint", mk3(1.f)), rgb);
+        put_color_slot(r, 0, s, p, "color", mk3(1.f), rgb);
+        put_color_slot(r, 4, s, p, "spec_tint", mk3(1.f), rgb);
+        put_color_slot(r, 8, s, p, "sheen_tint", mk3(1.f), rgb);
+        put_color_slot(r, 12, s, p, "coat_tint", mk3(1.f), rgb);
         r.p[16] = slot1(p, "metallic", 0.f);
         r.p[17] = slot1(p, "ior", 1.5f);
         r.p[18] = slot1(p, "roughness", 0.5f);
@@ -256,7 +268,19 @@ PackedScene pack_scene(const SceneHost &s) {
     out.spd.insert(out.spd.end(), cie.d65.begin(), cie.d65.end());
     out.cie_interval = cie.interval;
     out.cie_y_integral = cie.y_integral;
-    for (const MaterialHost &m : s.materials) out.mats.push_back(pack_material(m, out.rgb, out.spd));
+    for (const MaterialHost &m : s.materials) out.mats.push_back(pack_material(m, out.rgb, out.spd, s));
+    for (const TextureHost &t : s.textures) {
+        out.tex_info.insert(out.tex_info.end(), {uint32_t(out.tex.size() / 4), t.width, t.height, 0u});
+        for (size_t i = 0; i < size_t(t.width) * t.height; ++i) {
+            f3 c = mk3(t.rgb[3 * i], t.rgb[3 * i + 1], t.rgb[3 * i + 2]);
+            if (out.rgb) {
+                out.tex.insert(out.tex.end(), {c.x, c.y, c.z, 0.f});
+            } else {
+                f4 q = rgb_to_albedo_coeffs(c);
+                out.tex.insert(out.tex.end(), {q.x, q.y, q.z, q.w});
+            }
+        }
+    }
 
     f3 diag = s.aabb_hi - s.aabb_lo;
     out.world_radius = std::max(0.5f * length(diag), s.min_world_radius);

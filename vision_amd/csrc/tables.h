@@ -43,6 +43,11 @@ struct PackedScene {
     float env_w2o[9]{}, env_o2w[9]{};
     float env_scale{1.f};
     float world_radius{10.f};
+    // textures of the colour slots: float4 texels (linear sRGB, or the
+    // albedo sigmoid coefficients of each texel in hero mode), and per
+    // texture {first texel, width, height, 0}
+    std::vector<float> tex;
+    std::vector<uint32_t> tex_info;
 };
 
 PackedScene pack_scene(const SceneHost &s);

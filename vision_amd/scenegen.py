@@ -383,6 +383,35 @@ def serving_scene(out_dir: str | Path, triangles: int, seed: int = 0, width: int
     return scene
 
 
+def checker_rgba(width: int = 16, height: int = 8, a=(0.9, 0.2, 0.1), b=(0.1, 0.3, 0.8), cells: int = 4) -> np.ndarray:
+    """A checkerboard image float32[H, W, 4] of two colours."""
+    y, x = np.mgrid[0:height, 0:width]
+    sel = ((x * cells // width) + (y * cells // height)) % 2 == 0
+    img = np.where(sel[..., None], np.array(a, np.float32), np.array(b, np.float32))
+    return np.concatenate([img, np.ones((height, width, 1), np.float32)], -1).astype(np.float32)
+
+
+def textured_box(image_fn: str, width: int = 32, height: int = 32, color_space: str | None = None,
+                 material: str = "diffuse", spectrum: str = "srgb", max_depth: int = 4) -> dict:
+    """The Cornell box whose floor's colour slot is the image `image_fn`
+    (an ImageNode, {"channels": "xyz", "node": {"type": "image", ...}})."""
+    scene = cornell_box(width, height, max_depth=max_depth, light_sampler="power")
+    param = {"fn": image_fn}
+    if color_space:
+        param["color_space"] = color_space
+    node = {"channels": "xyz", "node": {"type": "image", "param": param}}
+    if material == "diffuse":
+        mat = {"type": "diffuse", "name": "Tex", "param": {"color": node}}
+    else:
+        mat = {"type": "principled_bsdf", "name": "Tex",
+               "param": {"color": node, "roughness": 0.4, "metallic": 0.0, "spec_tint": node}}
+    scene["materials"].append(mat)
+    scene["shapes"][0]["param"]["material"] = "Tex"
+    if spectrum != "srgb":
+        scene["spectrum"] = {"type": "hero", "param": {"dimension": int(spectrum[-1])}}
+    return scene
+
+
 def write_scene(scene: dict, path: str | Path) -> Path:
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
