@@ -316,6 +316,8 @@ typedef struct vs_job_desc {
 typedef struct vs_job_info {
     int32_t state;
     uint32_t frames_done;
+    uint32_t route;        /* 0 persistent megakernel, 1 wavefront lane */
+    uint32_t pad;
     uint64_t samples;      /* width * height * spp */
     double queue_ms;       /* submit -> admitted */
     double latency_ms;     /* submit -> film on the host (done jobs) */
@@ -362,11 +364,20 @@ vs_status vs_server_reset_stats(vs_server *srv);
  * budget and slot count are fixed at creation). */
 vs_status vs_server_set_config(vs_server *srv, const vs_server_config *cfg);
 vs_status vs_server_get_config(vs_server *srv, vs_server_config *cfg);
+/* Routing: jobs of a scene in one spectrum mode go to the persistent
+ * megakernel steps (route 0, the default) or to the wavefront lane (route 1),
+ * a second worker that renders one job at a time in the wavefront form
+ * (trace / shade / shadow kernels over path queues, as vs_scene_set_kernel's
+ * VS_KERNEL_WAVEFRONT) beside the steps.  Set while idle.  The lane's path
+ * state (up to 16 M paths, about 3 GB) lives outside the HBM budget. */
+vs_status vs_server_set_route(vs_server *srv, uint32_t scene, int32_t spectral, uint32_t route);
+vs_status vs_server_get_route(vs_server *srv, uint32_t scene, int32_t spectral, uint32_t *route);
 /* Autotuner: render the probe jobs under each candidate launch shape
  * (register budget 0 / 6 x tile width 8 / 16 x persistent waves per CU at
  * the occupancy limit and half of it) and keep the one with the highest
- * Msamples/s.  The server must be idle; the probe jobs' results are
- * discarded. */
+ * Msamples/s; then, for each (scene, spectrum mode) of the probe list, time
+ * its largest probe job alone on each route and keep the faster.  The
+ * server must be idle; the probe jobs' results are discarded. */
 #define VS_AUTOTUNE_MAX_TRIALS 16
 typedef struct vs_autotune_result {
     uint32_t tile_w, waves_per_cu, min_waves_per_eu; /* chosen (now the server's config) */
@@ -376,6 +387,10 @@ typedef struct vs_autotune_result {
     uint32_t trial_waves_per_cu[VS_AUTOTUNE_MAX_TRIALS];
     uint32_t trial_min_waves_per_eu[VS_AUTOTUNE_MAX_TRIALS];
     double trial_msamples_per_s[VS_AUTOTUNE_MAX_TRIALS];
+    uint32_t routes;               /* (scene, mode) pairs timed */
+    uint32_t route_scene[VS_AUTOTUNE_MAX_TRIALS], route_spectral[VS_AUTOTUNE_MAX_TRIALS];
+    uint32_t route_choice[VS_AUTOTUNE_MAX_TRIALS];
+    double route_megakernel_ms[VS_AUTOTUNE_MAX_TRIALS], route_wavefront_ms[VS_AUTOTUNE_MAX_TRIALS];
 } vs_autotune_result;
 vs_status vs_server_autotune(vs_server *srv, const vs_job_desc *probe, uint32_t n, vs_autotune_result *out);
 /* The message of the last failed server call on this thread. */

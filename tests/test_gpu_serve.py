@@ -64,10 +64,12 @@ JOBS = [  # scene, width, height, spp, max_depth, spectral, dim
 ]
 
 
-def _run(gpu_ctx, served, jobs, staggered=False, **cfg):
+def _run(gpu_ctx, served, jobs, staggered=False, routes=(), **cfg):
     js, loaded = served
     with va.Server(gpu_ctx, **cfg) as srv:
         idx = {k: srv.add_scene(v) for k, v in loaded.items()}
+        for name, spectral in routes:
+            srv.set_route(idx[name], spectral, True)
         ids = []
         for n, (name, w, h, spp, md, spectral, dim) in enumerate(jobs):
             ids.append(srv.submit(idx[name], w, h, spp, max_depth=md, spectral=spectral, spectrum_dim=dim))
@@ -75,8 +77,10 @@ def _run(gpu_ctx, served, jobs, staggered=False, **cfg):
                 time.sleep(0.01)  # later jobs arrive while earlier ones run
         srv.drain(60000)
         res = []
-        for jid, (name, w, h, *_rest) in zip(ids, jobs):
-            assert srv.info(jid)["state"] == "done"
+        for jid, (name, w, h, spp, md, spectral, dim) in zip(ids, jobs):
+            info = srv.info(jid)
+            assert info["state"] == "done"
+            assert info["route"] == ("wavefront" if (name, spectral) in routes else "megakernel")
             res.append((srv.result(jid, 0, (h, w)), srv.result(jid, 1, (h, w))))
             srv.release(jid)
         stats = srv.stats()
@@ -146,6 +150,9 @@ def test_autotuner_picks_a_measured_shape(gpu_ctx, served):
         probe = [dict(scene=i, width=64, height=64, spp=2), dict(scene=i, width=128, height=32, spp=1)]
         r = srv.autotune(probe)
         assert len(r["trials"]) >= 4
+        assert [(x["scene"], x["spectral"]) for x in r["routes"]] == [(i, False)]
+        rt = r["routes"][0]
+        assert srv.route(i, False) == ("wavefront" if rt["wavefront_ms"] < rt["megakernel_ms"] else "megakernel")
         best = max(r["trials"], key=lambda t: t["msamples_per_s"])
         assert (r["tile_w"], r["waves_per_cu"], r["min_waves_per_eu"]) == (best["tile_w"], best["waves_per_cu"],
                                                                            best["min_waves_per_eu"])
@@ -158,3 +165,14 @@ def test_autotuner_picks_a_measured_shape(gpu_ctx, served):
         film = srv.result(j, 0, (64, 64))
     rt, _ = _plain(gpu_ctx, served[0]["cbox"], 64, 64, 2)
     np.testing.assert_array_equal(film.view(np.uint32), rt.view(np.uint32))
+
+
+def test_wavefront_lane_matches_the_plain_render(gpu_ctx, served):
+    """Jobs routed to the wavefront lane render beside the persistent steps
+    and give the same films."""
+    base, _ = _run(gpu_ctx, served, JOBS)
+    got, stats = _run(gpu_ctx, served, JOBS, staggered=True, routes=(("cbox", False), ("proc", True)))
+    for (a, b), (c, d) in zip(base, got):
+        np.testing.assert_array_equal(a.view(np.uint32), c.view(np.uint32))
+        np.testing.assert_array_equal(b.view(np.uint32), d.view(np.uint32))
+    assert stats["jobs_done"] == len(JOBS)

@@ -94,8 +94,8 @@ class JobDesc(C.Structure):
 
 
 class JobInfo(C.Structure):
-    _fields_ = [("state", C.c_int32), ("frames_done", C.c_uint32), ("samples", C.c_uint64),
-                ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
+    _fields_ = [("state", C.c_int32), ("frames_done", C.c_uint32), ("route", C.c_uint32), ("pad", C.c_uint32),
+                ("samples", C.c_uint64), ("queue_ms", C.c_double), ("latency_ms", C.c_double)]
 
 
 class ServerStats(C.Structure):
@@ -111,7 +111,10 @@ class AutotuneResult(C.Structure):
     _fields_ = [("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("min_waves_per_eu", C.c_uint32),
                 ("msamples_per_s", C.c_double), ("trials", C.c_uint32), ("trial_tile_w", C.c_uint32 * 16),
                 ("trial_waves_per_cu", C.c_uint32 * 16), ("trial_min_waves_per_eu", C.c_uint32 * 16),
-                ("trial_msamples_per_s", C.c_double * 16)]
+                ("trial_msamples_per_s", C.c_double * 16), ("routes", C.c_uint32),
+                ("route_scene", C.c_uint32 * 16), ("route_spectral", C.c_uint32 * 16),
+                ("route_choice", C.c_uint32 * 16), ("route_megakernel_ms", C.c_double * 16),
+                ("route_wavefront_ms", C.c_double * 16)]
 
 
 JOB_STATES = {0: "queued", 1: "running", 2: "done", 3: "failed"}
@@ -184,6 +187,8 @@ SIGNATURES = {
     "vs_server_set_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
     "vs_server_get_config": (C.c_int32, [_P, C.POINTER(ServerConfig)]),
     "vs_server_autotune": (C.c_int32, [_P, _P, C.c_uint32, C.POINTER(AutotuneResult)]),
+    "vs_server_set_route": (C.c_int32, [_P, C.c_uint32, C.c_int32, C.c_uint32]),
+    "vs_server_get_route": (C.c_int32, [_P, C.c_uint32, C.c_int32, C.POINTER(C.c_uint32)]),
     "vs_server_last_error": (C.c_char_p, []),
     "vs_nrc_default_config": (None, [C.POINTER(NrcConfig)]),
     "vs_nrc_create": (C.c_int32, [_P, C.POINTER(NrcConfig), C.POINTER(_P)]),
@@ -499,7 +504,16 @@ class Server:
         i = JobInfo()
         _server_check(lib().vs_server_job_info(self._h, job, C.byref(i)))
         return {"state": JOB_STATES[i.state], "frames_done": i.frames_done, "samples": i.samples,
-                "queue_ms": i.queue_ms, "latency_ms": i.latency_ms}
+                "queue_ms": i.queue_ms, "latency_ms": i.latency_ms,
+                "route": "wavefront" if i.route == 1 else "megakernel"}
+
+    def set_route(self, scene: int, spectral: bool, wavefront: bool):
+        _server_check(lib().vs_server_set_route(self._h, scene, 1 if spectral else 0, 1 if wavefront else 0))
+
+    def route(self, scene: int, spectral: bool) -> str:
+        r = C.c_uint32()
+        _server_check(lib().vs_server_get_route(self._h, scene, 1 if spectral else 0, C.byref(r)))
+        return "wavefront" if r.value == 1 else "megakernel"
 
     def result(self, job: int, which: int = 0, shape: Optional[tuple] = None) -> np.ndarray:
         if shape is None:
@@ -549,7 +563,11 @@ class Server:
                 "msamples_per_s": r.msamples_per_s,
                 "trials": [{"tile_w": r.trial_tile_w[i], "waves_per_cu": r.trial_waves_per_cu[i],
                             "min_waves_per_eu": r.trial_min_waves_per_eu[i],
-                            "msamples_per_s": r.trial_msamples_per_s[i]} for i in range(r.trials)]}
+                            "msamples_per_s": r.trial_msamples_per_s[i]} for i in range(r.trials)],
+                "routes": [{"scene": r.route_scene[i], "spectral": bool(r.route_spectral[i]),
+                            "route": "wavefront" if r.route_choice[i] else "megakernel",
+                            "megakernel_ms": r.route_megakernel_ms[i], "wavefront_ms": r.route_wavefront_ms[i]}
+                           for i in range(r.routes)]}
 
     def close(self):
         if self._h:

@@ -337,7 +337,6 @@ void vs_scene_destroy(vs_scene *scene) {
         (void)hipSetDevice(scene->ctx->device);
         (void)hipStreamSynchronize(scene->ctx->stream);
     }
-    if (scene->wf_host_counts) (void)hipHostFree(scene->wf_host_counts);
     delete scene;
 }
 
@@ -649,87 +648,103 @@ vs_status vs_scene_bsdf(vs_scene *s, uint32_t mat, uint32_t n, const float *wo, 
 
 namespace {
 
-// Wavefront render of `spp` frames starting at s->frame_index: chunks of
-// `wf_chunk_frames` frames, each a loop of trace / shade / shadow over the
-// live-path queue until it empties, then the chunk folds into the film.
-uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr,
-                          vs_render_stats &ts) {
-    hipStream_t st = s->ctx->stream;
-    const uint32_t P = s->cam.width * s->cam.height;
-    const uint32_t chunk = std::min<uint32_t>(s->wf_chunk_frames, spp);
+}// namespace
+
+}// extern "C"
+
+uint32_t vs_internal::wavefront_render(const vs::SceneView &view, const vs::CameraView &cam,
+                                       const vs::RenderView &rv0, bool rgb, float4 *rt, float4 *out,
+                                       uint32_t frame_index, uint32_t spp, uint32_t chunk_frames,
+                                       uint32_t trace_block, WavefrontBuffers &wb, hipStream_t st,
+                                       unsigned long long *ctr, vs_render_stats *ts, hipEvent_t *ev) {
+    const uint32_t P = cam.width * cam.height;
+    const uint32_t chunk = std::min<uint32_t>(chunk_frames, spp);
     const size_t cap = size_t(P) * chunk;
     if (cap > 0xFFFFFFFFull) throw std::invalid_argument("wavefront chunk exceeds 2^32 paths");
-    if (s->wf_capacity < cap) {
-        for (DevBuf *b : {&s->wf_o, &s->wf_d, &s->wf_thr, &s->wf_L, &s->wf_meta, &s->wf_lam, &s->wf_lpdf, &s->wf_hit,
-                          &s->wf_sh_o, &s->wf_sh_d, &s->wf_sh_c})
+    if (wb.capacity < cap) {
+        for (DevBuf *b : {&wb.o, &wb.d, &wb.thr, &wb.L, &wb.meta, &wb.lam, &wb.lpdf, &wb.hit, &wb.sh_o, &wb.sh_d,
+                          &wb.sh_c})
             b->alloc(cap * 16);
-        s->wf_q0.alloc(cap * 4);
-        s->wf_q1.alloc(cap * 4);
-        s->wf_counts.alloc(64);
-        if (!s->wf_host_counts) {
+        wb.q0.alloc(cap * 4);
+        wb.q1.alloc(cap * 4);
+        wb.counts.alloc(64);
+        if (!wb.host_counts) {
             void *hp = nullptr;
             hip_check(hipHostMalloc(&hp, 64, hipHostMallocDefault), "pinned counts");
-            s->wf_host_counts = static_cast<uint32_t *>(hp);
+            wb.host_counts = static_cast<uint32_t *>(hp);
         }
-        s->wf_capacity = cap;
+        wb.capacity = cap;
     }
     vs::WavefrontView w{};
-    w.o = s->wf_o.as<float4>();
-    w.d = s->wf_d.as<float4>();
-    w.thr = s->wf_thr.as<float4>();
-    w.L = s->wf_L.as<float4>();
-    w.meta = s->wf_meta.as<uint4>();
-    w.lam = s->wf_lam.as<float4>();
-    w.lpdf = s->wf_lpdf.as<float4>();
-    w.hit = s->wf_hit.as<float4>();
-    w.sh_o = s->wf_sh_o.as<float4>();
-    w.sh_d = s->wf_sh_d.as<float4>();
-    w.sh_c = s->wf_sh_c.as<float4>();
+    w.o = wb.o.as<float4>();
+    w.d = wb.d.as<float4>();
+    w.thr = wb.thr.as<float4>();
+    w.L = wb.L.as<float4>();
+    w.meta = wb.meta.as<uint4>();
+    w.lam = wb.lam.as<float4>();
+    w.lpdf = wb.lpdf.as<float4>();
+    w.hit = wb.hit.as<float4>();
+    w.sh_o = wb.sh_o.as<float4>();
+    w.sh_d = wb.sh_d.as<float4>();
+    w.sh_c = wb.sh_c.as<float4>();
     w.pixels = P;
-    w.width = s->cam.width;
-    const bool rgb = s->packed.rgb;
+    w.width = cam.width;
     const uint32_t dim = rv0.spectrum_dim;
-    uint32_t *q[2] = {s->wf_q0.as<uint32_t>(), s->wf_q1.as<uint32_t>()};
-    uint32_t *counts = s->wf_counts.as<uint32_t>();
+    uint32_t *q[2] = {wb.q0.as<uint32_t>(), wb.q1.as<uint32_t>()};
+    uint32_t *counts = wb.counts.as<uint32_t>();
+    const bool timed = ts && ev;
     uint32_t launches = 0;
     for (uint32_t f0 = 0; f0 < spp; f0 += chunk) {
         const uint32_t fc = std::min(chunk, spp - f0);
         const uint32_t n = P * fc;
-        const uint32_t frame0 = s->frame_index + f0;
-        hip_check(vs::wf_begin(s->cam, rv0, w, dim, rgb, frame0, n, q[0], st), "wf begin");
+        const uint32_t frame0 = frame_index + f0;
+        hip_check(vs::wf_begin(cam, rv0, w, dim, rgb, frame0, n, q[0], st), "wf begin");
         ++launches;
         uint32_t count = rv0.max_depth > 0 ? n : 0u;
         int cur = 0;
         while (count) {
             hip_check(vs::wf_zero(counts, st), "wf zero");
-            hip_check(hipEventRecord(s->ctx->tr0, st), "event");
-            hip_check(vs::wf_trace(s->view, w, q[cur], count, ctr, s->wf_trace_block, st), "wf trace");
-            hip_check(hipEventRecord(s->ctx->tr1, st), "event");
-            hip_check(vs::wf_shade(s->view, rv0, w, dim, rgb, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
-            hip_check(hipEventRecord(s->ctx->tr2, st), "event");
-            hip_check(vs::wf_shadow(s->view, w, counts, count, ctr ? ctr + 4 : nullptr, s->wf_trace_block, st),
-                      "wf shadow");
-            hip_check(hipEventRecord(s->ctx->ev1, st), "event");
+            if (timed) hip_check(hipEventRecord(ev[0], st), "event");
+            hip_check(vs::wf_trace(view, w, q[cur], count, ctr, trace_block, st), "wf trace");
+            if (timed) hip_check(hipEventRecord(ev[1], st), "event");
+            hip_check(vs::wf_shade(view, rv0, w, dim, rgb, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
+            if (timed) hip_check(hipEventRecord(ev[2], st), "event");
+            hip_check(vs::wf_shadow(view, w, counts, count, ctr ? ctr + 4 : nullptr, trace_block, st), "wf shadow");
+            if (timed) hip_check(hipEventRecord(ev[3], st), "event");
             launches += 4;
-            hip_check(hipMemcpyAsync(s->wf_host_counts, counts, 8, hipMemcpyDeviceToHost, st), "wf counts");
+            hip_check(hipMemcpyAsync(wb.host_counts, counts, 8, hipMemcpyDeviceToHost, st), "wf counts");
             hip_check(hipStreamSynchronize(st), "wf sync");
-            float t_trace = 0.f, t_shade = 0.f, t_shadow = 0.f;
-            hip_check(hipEventElapsedTime(&t_trace, s->ctx->tr0, s->ctx->tr1), "event time");
-            hip_check(hipEventElapsedTime(&t_shade, s->ctx->tr1, s->ctx->tr2), "event time");
-            hip_check(hipEventElapsedTime(&t_shadow, s->ctx->tr2, s->ctx->ev1), "event time");
-            ts.stage_ms[0] += t_trace;
-            ts.stage_ms[1] += t_shade;
-            ts.stage_ms[2] += t_shadow;
-            for (int k = 0; k < 3; ++k) ts.stage_launches[k] += 1;
-            count = s->wf_host_counts[vs::WF_LIVE];
+            if (timed) {
+                float t_trace = 0.f, t_shade = 0.f, t_shadow = 0.f;
+                hip_check(hipEventElapsedTime(&t_trace, ev[0], ev[1]), "event time");
+                hip_check(hipEventElapsedTime(&t_shade, ev[1], ev[2]), "event time");
+                hip_check(hipEventElapsedTime(&t_shadow, ev[2], ev[3]), "event time");
+                ts->stage_ms[0] += t_trace;
+                ts->stage_ms[1] += t_shade;
+                ts->stage_ms[2] += t_shadow;
+                for (int k = 0; k < 3; ++k) ts->stage_launches[k] += 1;
+            }
+            count = wb.host_counts[vs::WF_LIVE];
             if (count > n) throw std::logic_error("wavefront queue overflow");
             cur ^= 1;
         }
-        hip_check(vs::wf_accumulate(rv0, w, frame0, fc, s->film_rt.as<float4>(), s->film_out.as<float4>(), st),
-                  "wf accumulate");
+        hip_check(vs::wf_accumulate(rv0, w, frame0, fc, rt, out, st), "wf accumulate");
         ++launches;
     }
     return launches;
+}
+
+extern "C" {
+
+namespace {
+
+// Wavefront render of `spp` frames of a scene's film from its frame index
+uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr,
+                          vs_render_stats &ts) {
+    hipEvent_t ev[4] = {s->ctx->tr0, s->ctx->tr1, s->ctx->tr2, s->ctx->ev1};
+    return wavefront_render(s->view, s->cam, rv0, s->packed.rgb, s->film_rt.as<float4>(), s->film_out.as<float4>(),
+                            s->frame_index, spp, s->wf_chunk_frames, s->wf_trace_block, s->wf, s->ctx->stream, ctr,
+                            &ts, ev);
 }
 
 vs::RestirView restir_view(vs_scene *s, uint32_t frame) {

@@ -92,6 +92,19 @@ struct ModeTables {
     }
 };
 
+// per-path state and queues of the wavefront form, sized on first use
+struct WavefrontBuffers {
+    size_t capacity{0};
+    DevBuf o, d, thr, L, meta, lam, lpdf, hit, sh_o, sh_d, sh_c, q0, q1, counts;
+    uint32_t *host_counts{nullptr};// pinned
+    WavefrontBuffers() = default;
+    WavefrontBuffers(const WavefrontBuffers &) = delete;
+    WavefrontBuffers &operator=(const WavefrontBuffers &) = delete;
+    ~WavefrontBuffers() {
+        if (host_counts) (void)hipHostFree(host_counts);
+    }
+};
+
 }// namespace vs_internal
 
 using vs_internal::DevBuf;
@@ -120,10 +133,7 @@ struct vs_scene {
     uint32_t kernel_mode{VS_KERNEL_MEGAKERNEL};
     uint32_t wf_chunk_frames{8};
     uint32_t wf_trace_block{128};
-    size_t wf_capacity{0};
-    DevBuf wf_o, wf_d, wf_thr, wf_L, wf_meta, wf_lam, wf_lpdf, wf_hit, wf_sh_o, wf_sh_d, wf_sh_c, wf_q0, wf_q1,
-        wf_counts;
-    uint32_t *wf_host_counts{nullptr};// pinned
+    vs_internal::WavefrontBuffers wf;
     // ReSTIR DI ("rt" integrator): reservoirs, surfaces, surface extensions,
     // and the camera of the previous frame (3x4 rows of camera-to-world)
     DevBuf rs_rsv, rs_surf, rs_ext;
@@ -193,5 +203,15 @@ inline void require_built(vs_scene *s) {
     if (s->ctx->device < 0) throw std::logic_error("host-only context: no device work possible");
     if (!s->built) throw std::logic_error("accel not built: call vs_scene_build_accel first");
 }
+
+// The wavefront form over one film: `spp` frames from frame_index in
+// chunks of `chunk` frames, each a loop of trace / shade / shadow over the
+// live-path queue until it empties, then the chunk folds into the film.
+// Synchronises `st` once per bounce (the live count).  Returns launches;
+// adds per-stage times to *ts when it and the three events are given.
+uint32_t wavefront_render(const vs::SceneView &view, const vs::CameraView &cam, const vs::RenderView &rv0,
+                          bool rgb, float4 *rt, float4 *out, uint32_t frame_index, uint32_t spp, uint32_t chunk,
+                          uint32_t trace_block, WavefrontBuffers &wb, hipStream_t st, unsigned long long *ctr,
+                          vs_render_stats *ts, hipEvent_t *ev4);
 
 }// namespace vs_internal

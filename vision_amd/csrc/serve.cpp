@@ -108,6 +108,8 @@ struct JobRec {
     vs::RenderView rv{};
     float *h_rt{nullptr}, *h_out{nullptr};// pinned host copies of the films
     hipEvent_t ev_copy{nullptr};
+    hipEvent_t ev_ready{nullptr};// film cleared (wavefront lane)
+    uint32_t route{0};           // 0 persistent megakernel, 1 wavefront lane
     std::string error;
 };
 
@@ -122,6 +124,8 @@ struct SceneSlot {
     vs_scene *scene{nullptr};
     vs::CameraView cam{};   // the scene's camera when it joined (jobs re-aspect it)
     vs::RenderView rv{};    // and its integrator / film settings
+    vs::SceneView view[2]{};// sRGB and hero-wavelength views
+    uint32_t route[2]{0, 0};// per mode: 0 persistent megakernel, 1 wavefront lane
     bool diffuse_only{false};
 };
 
@@ -154,6 +158,14 @@ struct vs_server {
     std::deque<StepRec> inflight;
     std::vector<hipEvent_t> free_events;
     std::vector<JobRec *> copying;
+    // wavefront lane: its own thread, stream and path buffers; jobs routed to
+    // it render alone in the wavefront form beside the persistent steps
+    std::thread lane;
+    hipStream_t st_lane{nullptr};
+    WavefrontBuffers lane_wb;
+    std::deque<JobRec *> lane_queue;   // under mu
+    std::vector<JobRec *> lane_done;   // under mu
+    std::condition_variable cv_lane;
     double rate{0.0};// samples per ms, measured
     uint64_t step_count{0};
     // shared with callers (under mu)
@@ -381,6 +393,15 @@ void admit(vs_server *S) {
         hip_check(hipMemcpyAsync(S->d_jobs.as<vs::ServeJob>() + j->slot, &sj, sizeof(sj), hipMemcpyHostToDevice,
                                  S->st[0]),
                   "job upload");
+        j->route = sc.route[j->d.spectral ? 1 : 0];
+        if (j->route == 1) {
+            j->ev_ready = take_event(S);
+            hip_check(hipEventRecord(j->ev_ready, S->st[0]), "ready event");
+            std::lock_guard<std::mutex> lk(S->mu);
+            S->lane_queue.push_back(j);
+            S->cv_lane.notify_one();
+            continue;
+        }
         S->active.push_back(j);
         std::lock_guard<std::mutex> lk(S->mu);
         S->max_active = std::max<uint32_t>(S->max_active, uint32_t(S->active.size()));
@@ -507,10 +528,69 @@ void fail_everything(vs_server *S, const std::string &msg) {
     S->cv_done.notify_all();
 }
 
+// jobs the wavefront lane has finished: release their slot and HBM range
+void retire_lane(vs_server *S) {
+    std::vector<JobRec *> done;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        done.swap(S->lane_done);
+    }
+    for (JobRec *j : done) {
+        if (j->ev_ready) {
+            S->free_events.push_back(j->ev_ready);
+            j->ev_ready = nullptr;
+        }
+        std::string err = j->error;
+        finish_job(S, j, err.empty(), err);
+    }
+}
+
+// the wavefront lane: one job at a time, all its frames, output pass and
+// download on the lane's stream; the worker retires it
+void lane_main(vs_server *S) {
+    (void)hipSetDevice(S->ctx->device);
+    while (true) {
+        JobRec *j = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv_lane.wait(lk, [&] { return S->stop || !S->lane_queue.empty(); });
+            if (S->lane_queue.empty()) return;// stopping
+            j = S->lane_queue.front();
+            S->lane_queue.pop_front();
+        }
+        try {
+            const SceneSlot &sc = S->scenes[j->d.scene];
+            const uint32_t px = j->d.width * j->d.height;
+            float4 *rt = reinterpret_cast<float4 *>(static_cast<char *>(S->arena_mem.ptr) + j->off);
+            float4 *out = rt + px;
+            vs::CameraView cam = sc.cam;
+            camera_set_resolution(cam, j->d.width, j->d.height);
+            // frames per chunk: up to 16 M paths of state at a time
+            const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(16u, (16u << 20) / std::max(px, 1u)));
+            hip_check(hipStreamWaitEvent(S->st_lane, j->ev_ready, 0), "lane wait");
+            wavefront_render(sc.view[j->d.spectral ? 1 : 0], cam, j->rv, !j->d.spectral, rt, out, 0, j->d.spp, chunk,
+                             128, S->lane_wb, S->st_lane, nullptr, nullptr, nullptr);
+            hip_check(vs::serve_output(j->rv, rt, out, px, S->st_lane), "output pass");
+            hip_check(hipMemcpyAsync(j->h_rt, rt, size_t(px) * 16, hipMemcpyDeviceToHost, S->st_lane), "result D2H");
+            hip_check(hipMemcpyAsync(j->h_out, out, size_t(px) * 16, hipMemcpyDeviceToHost, S->st_lane), "result D2H");
+            hip_check(hipStreamSynchronize(S->st_lane), "lane sync");
+            std::lock_guard<std::mutex> lk(S->mu);
+            j->frames_done = j->d.spp;
+        } catch (const std::exception &e) {
+            std::lock_guard<std::mutex> lk(S->mu);
+            j->error = e.what();
+        }
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->lane_done.push_back(j);
+        S->cv_work.notify_all();
+    }
+}
+
 void worker_main(vs_server *S) {
     try {
         hip_check(hipSetDevice(S->ctx->device), "hipSetDevice");
         while (true) {
+            retire_lane(S);
             retire_copies(S, false);
             admit(S);
             if (S->inflight.size() < 2 && has_unscheduled(S)) {
@@ -533,7 +613,8 @@ void worker_main(vs_server *S) {
                 // only passes when admission just raced with a submit
                 continue;
             }
-            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (!S->lane_done.empty()) continue;
+            S->cv_work.wait(lk, [&] { return S->stop || !S->queue.empty() || !S->lane_done.empty(); });
         }
     } catch (const std::exception &e) {
         fail_everything(S, e.what());
@@ -610,7 +691,9 @@ vs_status vs_server_create(vs_context *ctx, const vs_server_config *cfg, vs_serv
         S->d_counters.alloc(8 * sizeof(unsigned long long));
         hip_check(hipMemset(S->d_counters.ptr, 0, 8 * sizeof(unsigned long long)), "counter clear");
         S->slot_job.assign(S->cfg.max_jobs, nullptr);
+        hip_check(hipStreamCreateWithFlags(&S->st_lane, hipStreamNonBlocking), "stream");
         S->worker = std::thread(worker_main, S.get());
+        S->lane = std::thread(lane_main, S.get());
     });
     if (st != VS_OK) {
         vs_server *raw = S.release();
@@ -638,6 +721,8 @@ vs_status vs_server_add_scene(vs_server *S, vs_scene *scene, uint32_t *index) {
         slot.diffuse_only = scene->diffuse_only;
         // both spectrum modes share the geometry and BVH buffers
         vs::SceneView views[2] = {view_in_mode(scene, true), view_in_mode(scene, false)};
+        slot.view[0] = views[0];
+        slot.view[1] = views[1];
         if (scene->view.stack_size > S->max_stack) {
             for (auto &kv : S->jobs)
                 if (kv.second->state == VS_JOB_QUEUED || kv.second->state == VS_JOB_RUNNING)
@@ -723,6 +808,7 @@ vs_status vs_server_job_info(vs_server *S, uint64_t id, vs_job_info *info) {
     std::memset(info, 0, sizeof(*info));
     info->state = j->state;
     info->frames_done = j->frames_done;
+    info->route = j->route;
     info->samples = uint64_t(j->d.width) * j->d.height * j->d.spp;
     if (j->state != VS_JOB_QUEUED) info->queue_ms = ms_between(j->t_submit, j->t_admit);
     if (j->state == VS_JOB_DONE) info->latency_ms = ms_between(j->t_submit, j->t_done);
@@ -860,6 +946,22 @@ vs_status vs_server_get_config(vs_server *S, vs_server_config *cfg) {
     return VS_OK;
 }
 
+vs_status vs_server_set_route(vs_server *S, uint32_t scene, int32_t spectral, uint32_t route) {
+    if (!S || route > 1) return server_fail(VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (scene >= S->scenes.size()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown scene index");
+    S->scenes[scene].route[spectral ? 1 : 0] = route;
+    return VS_OK;
+}
+
+vs_status vs_server_get_route(vs_server *S, uint32_t scene, int32_t spectral, uint32_t *route) {
+    if (!S || !route) return server_fail(VS_ERROR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::mutex> lk(S->mu);
+    if (scene >= S->scenes.size()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown scene index");
+    *route = S->scenes[scene].route[spectral ? 1 : 0];
+    return VS_OK;
+}
+
 vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n, vs_autotune_result *out) {
     if (!S || !probe || !n || !out) return server_fail(VS_ERROR_INVALID_ARGUMENT, "bad argument");
     if (!idle(S)) return server_fail(VS_ERROR_STATE, "server busy");
@@ -921,6 +1023,41 @@ vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n,
         }
     }
     vs_status st = vs_server_set_config(S, &best_cfg);
+    // routes: the largest probe job of each (scene, mode), alone on each route
+    std::vector<std::pair<uint32_t, uint32_t>> pairs;
+    for (uint32_t i = 0; i < n && st == VS_OK; ++i) {
+        std::pair<uint32_t, uint32_t> key{probe[i].scene, probe[i].spectral ? 1u : 0u};
+        if (std::find(pairs.begin(), pairs.end(), key) != pairs.end()) continue;
+        pairs.push_back(key);
+        const vs_job_desc *big = &probe[i];
+        for (uint32_t k = i; k < n; ++k)
+            if (probe[k].scene == key.first && (probe[k].spectral ? 1u : 0u) == key.second &&
+                uint64_t(probe[k].width) * probe[k].height * probe[k].spp > uint64_t(big->width) * big->height * big->spp)
+                big = &probe[k];
+        double ms[2] = {0.0, 0.0};
+        for (uint32_t route = 0; route < 2 && st == VS_OK; ++route) {
+            st = vs_server_set_route(S, key.first, int32_t(key.second), route);
+            uint64_t id = 0;
+            if (st == VS_OK) st = vs_server_submit(S, big, &id);
+            int32_t done = 0;
+            if (st == VS_OK) st = vs_server_wait(S, id, -1.0, &done);
+            vs_job_info info{};
+            if (st == VS_OK) st = vs_server_job_info(S, id, &info);
+            if (id) vs_server_release(S, id);
+            ms[route] = info.latency_ms;
+        }
+        if (st != VS_OK) break;
+        uint32_t choice = ms[1] < ms[0] ? 1u : 0u;
+        st = vs_server_set_route(S, key.first, int32_t(key.second), choice);
+        if (out->routes < VS_AUTOTUNE_MAX_TRIALS) {
+            uint32_t r = out->routes++;
+            out->route_scene[r] = key.first;
+            out->route_spectral[r] = key.second;
+            out->route_choice[r] = choice;
+            out->route_megakernel_ms[r] = ms[0];
+            out->route_wavefront_ms[r] = ms[1];
+        }
+    }
     if (st == VS_OK) st = vs_server_reset_stats(S);
     out->tile_w = best_cfg.tile_w;
     out->waves_per_cu = best_cfg.waves_per_cu;
@@ -945,6 +1082,11 @@ void vs_server_destroy(vs_server *S) {
         S->cv_done.notify_all();
     }
     if (S->worker.joinable()) S->worker.join();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->cv_lane.notify_all();
+    }
+    if (S->lane.joinable()) S->lane.join();
     if (S->ctx && S->ctx->device >= 0) {
         (void)hipSetDevice(S->ctx->device);
         for (auto &s : S->st)
@@ -970,6 +1112,10 @@ void vs_server_destroy(vs_server *S) {
         for (auto &s : S->st)
             if (s) (void)hipStreamDestroy(s);
         if (S->st_copy) (void)hipStreamDestroy(S->st_copy);
+        if (S->st_lane) {
+            (void)hipStreamSynchronize(S->st_lane);
+            (void)hipStreamDestroy(S->st_lane);
+        }
     }
     delete S;
 }
