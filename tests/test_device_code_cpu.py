@@ -26,7 +26,7 @@ HERE = Path(__file__).resolve().parent
 SRC = HERE / "devcpu" / "devcpu.cpp"
 BUILD = HERE / "devcpu" / "_build"
 LIB = BUILD / "libdevcpu.so"
-OBJS = ["data_tables.cpp.o", "json.cpp.o", "scene.cpp.o", "bvh.cpp.o", "tables.cpp.o", "image_io.cpp.o"]
+OBJS = ["data_tables.cpp.o", "json.cpp.o", "scene.cpp.o", "bvh.cpp.o", "tables.cpp.o", "image_io.cpp.o", "jpeg.cpp.o"]
 
 _f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _u32 = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")

@@ -613,9 +613,11 @@ void write_image(const std::string &path, uint32_t width, uint32_t height, const
 
 ImageRGBA read_image(const std::string &path) {
     std::string e = image_extension(path);
-    if (e != "exr" && e != "hdr" && e != "png" && e != "pfm")
-        throw std::runtime_error("unsupported image format ." + e + " (exr, hdr, png, pfm): " + path);
+    const bool jpeg = e == "jpg" || e == "jpeg";
+    if (e != "exr" && e != "hdr" && e != "png" && e != "pfm" && !jpeg)
+        throw std::runtime_error("unsupported image format ." + e + " (exr, hdr, png, pfm, jpg): " + path);
     std::vector<uint8_t> d = read_file(path);
+    if (jpeg) return decode_jpeg(d, path);
     if (e == "exr") return decode_exr(d, path);
     if (e == "hdr") return decode_hdr(d, path);
     if (e == "png") return decode_png(d, path);

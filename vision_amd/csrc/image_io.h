@@ -13,6 +13,8 @@
 //   .hdr  Radiance RGBE, flat or new-style run-length scanlines
 //   .png  8- or 16-bit grey, grey+alpha, RGB, RGBA or palette, no interlace
 //   .pfm  colour or grey PFM, either byte order
+//   .jpg  baseline, extended sequential or progressive Huffman JPEG, 8-bit
+//         grey or YCbCr (jpeg.cpp)
 // Values read from 8/16-bit files are code / max; no transfer function is
 // applied here (the caller decides the colour space).
 #pragma once
@@ -43,5 +45,8 @@ std::string image_extension(const std::string &path);
 // throws std::runtime_error on an unsupported extension or an I/O error
 void write_image(const std::string &path, uint32_t width, uint32_t height, const float *rgba);
 ImageRGBA read_image(const std::string &path);
+
+// the JPEG decoder behind read_image (throws on a corrupt or unsupported file)
+ImageRGBA decode_jpeg(const std::vector<uint8_t> &data, const std::string &path);
 
 }// namespace vs

@@ -441,7 +441,7 @@ void load_env_image(const std::string &path, std::vector<float> &rgb, uint32_t &
     // 8/16-bit files hold sRGB-encoded values unless the scene says otherwise
     // (ImagePool::load_texture, src/base/mgr/image_pool.cpp:17-22)
     std::string e = image_extension(path);
-    bool srgb = e == "png";
+    bool srgb = e == "png" || e == "jpg" || e == "jpeg";
     w = img.width;
     h = img.height;
     rgb.resize(size_t(w) * h * 3);
