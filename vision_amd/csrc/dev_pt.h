@@ -594,5 +594,59 @@ __device__ __forceinline__ float4 film_output(const RenderView &rv, float4 acc) 
     return o4;
 }
 
+// The samples [frame_start, frame_start + frames) of up to n_px pixels of one
+// column (rows y0, y0 + dy, ...) into the running mean rt (and the output
+// buffer when out is given), back to back: a lane whose path ends starts its
+// next sample at once (the next frame of its pixel, then its next pixel)
+// instead of waiting at the end of each frame for the longest path of its
+// wave.  Every pixel's samples still run in frame order in one lane with the
+// same arithmetic, so the film equals the frame-by-frame loop's bit for bit.
+template <int NS, bool RGB, int MS = MS_ALL>
+__device__ void render_column(const SceneView &sc, const CameraView &cam, const RenderView &rv,
+                              float4 *__restrict__ rt, float4 *__restrict__ out, uint32_t px, uint32_t y0,
+                              uint32_t dy, uint32_t n_px, uint32_t frame_start, uint32_t frames, uint2 *stack,
+                              uint32_t stride, PathCounters &pc) {
+    if (px >= cam.width || y0 >= cam.height || frames == 0u || n_px == 0u) return;
+    const uint32_t n = min(n_px, (cam.height - 1u - y0) / dy + 1u);
+    uint32_t j = 0, s = 0, py = y0;
+    uint32_t idx = py * cam.width + px;
+    float4 acc = rt[idx];
+    PathState<NS> ps;
+    bool fresh = true;
+    while (true) {
+        if (fresh) {
+            path_begin<NS, RGB>(cam, rv, px, py, frame_start + s, ps);
+            fresh = false;
+        }
+        if (ps.alive) {
+            TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
+            ++pc.rays;
+            ShadowRay sr;
+            path_shade<NS, RGB, MS>(sc, rv, ps, h, sr);
+            if (sr.valid) {
+                uint32_t dn = 0, dt = 0;
+                TraceHit sh = traverse<true>(sc, sr.o, sr.d, 0.f, 1.f - kShadowEpsilon, stack, stride, dn, dt);
+                pc.nodes += dn;
+                pc.tris += dt;
+                ++pc.rays;
+                if (sh.tri == kInvalid) ps.L += sr.c;
+            }
+        }
+        if (!ps.alive) {
+            acc = film_add(rv, acc, ps.L * ps.fw, frame_start + s);
+            fresh = true;
+            if (++s == frames) {
+                rt[idx] = acc;
+                if (out) out[idx] = film_output(rv, acc);
+                s = 0;
+                if (++j == n) break;
+                py += dy;
+                idx = py * cam.width + px;
+                acc = rt[idx];
+            }
+        }
+    }
+}
+
 }// namespace dev
 }// namespace vs

@@ -127,19 +127,19 @@ __device__ __forceinline__ void process_node(const uint4 *__restrict__ nodes, ui
     float ay = (__uint_as_float(n0.y) - r.o.y) * r.idir.y;
     float az = (__uint_as_float(n0.z) - r.o.z) * r.idir.z;
     bool px = r.idir.x >= 0.f, py = r.idir.y >= 0.f, pz = r.idir.z >= 0.f;
+    // per axis the quantised planes the ray enters and leaves by: the low
+    // ones first where its direction is positive (chosen once per node, not
+    // per child)
+    const uint32_t nx0 = px ? n2.x : n3.z, nx1 = px ? n2.y : n3.w, fx0 = px ? n3.z : n2.x, fx1 = px ? n3.w : n2.y;
+    const uint32_t ny0 = py ? n2.z : n4.x, ny1 = py ? n2.w : n4.y, fy0 = py ? n4.x : n2.z, fy1 = py ? n4.y : n2.w;
+    const uint32_t nz0 = pz ? n3.x : n4.z, nz1 = pz ? n3.y : n4.w, fz0 = pz ? n4.z : n3.x, fz1 = pz ? n4.w : n3.y;
     uint32_t imask = n0.w >> 24;
     uint32_t hitmask = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
-        float tlx = fmaf(qbyte(n2.x, n2.y, s), sx, ax);
-        float tly = fmaf(qbyte(n2.z, n2.w, s), sy, ay);
-        float tlz = fmaf(qbyte(n3.x, n3.y, s), sz, az);
-        float thx = fmaf(qbyte(n3.z, n3.w, s), sx, ax);
-        float thy = fmaf(qbyte(n4.x, n4.y, s), sy, ay);
-        float thz = fmaf(qbyte(n4.z, n4.w, s), sz, az);
-        float nx = px ? tlx : thx, fx = px ? thx : tlx;
-        float ny = py ? tly : thy, fy = py ? thy : tly;
-        float nz = pz ? tlz : thz, fz = pz ? thz : tlz;
+        float nx = fmaf(qbyte(nx0, nx1, s), sx, ax), fx = fmaf(qbyte(fx0, fx1, s), sx, ax);
+        float ny = fmaf(qbyte(ny0, ny1, s), sy, ay), fy = fmaf(qbyte(fy0, fy1, s), sy, ay);
+        float nz = fmaf(qbyte(nz0, nz1, s), sz, az), fz = fmaf(qbyte(fz0, fz1, s), sz, az);
         float tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, r.tmin));
         float tf = fminf(fminf(fx, fy), fminf(fz, tmax));
         hitmask |= (tn <= tf ? 1u : 0u) << s;

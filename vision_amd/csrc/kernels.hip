@@ -48,7 +48,8 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 }
 
 // one thread per pixel; all `spp` frames of the launch accumulate in registers
-// with the film's running-mean arithmetic (normal.cpp:87-101).  NS: wavelengths
+// with the film's running-mean arithmetic (normal.cpp:87-101), each frame's
+// path starting as soon as the previous one ends (render_column).  NS: wavelengths
 // carried per path; RGB: sRGB mode (three fixed channels) vs hero wavelengths.
 // The kernel asks for at least six resident waves per SIMD (a register
 // budget of 80 that spills some state to scratch): traversal latency is
@@ -62,16 +63,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) pt
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= cam.width || py >= cam.height) return;
-    uint32_t idx = py * cam.width + px;
-    float4 acc = rt[idx];
     PathCounters pc{0, 0, 0};
-    for (uint32_t s = 0; s < rv.spp; ++s) {
-        uint32_t frame = rv.frame_start + s;
-        f3 L = trace_path<NS, RGB, MS>(sc, cam, rv, px, py, frame, lds_stack + threadIdx.x, blockDim.x, pc);
-        acc = film_add(rv, acc, L, frame);
-    }
-    rt[idx] = acc;
-    if (rv.write_output) out[idx] = film_output(rv, acc);
+    render_column<NS, RGB, MS>(sc, cam, rv, rt, rv.write_output ? out : nullptr, px, py, 1u, 1u, rv.frame_start,
+                               rv.spp, lds_stack + threadIdx.x, blockDim.x, pc);
     if (counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
         atomicAdd(counters + 1, (unsigned long long)pc.nodes);

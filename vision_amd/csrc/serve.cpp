@@ -260,8 +260,8 @@ void enqueue_step(vs_server *S, uint32_t mask, uint32_t buf) {
         if (si && !used[si]) hip_check(hipStreamWaitEvent(s, S->ev_fork, 0), "fork wait");
         used[si] = 1;
         hip_check(vs::serve_launch(v, S->cfg.min_waves_per_eu, S->d_views.as<vs::SceneView>(), S->d_jobs.as<vs::ServeJob>(), tables + v,
-                                   S->d_next.as<uint32_t>() + v, variant_blocks(S, v), S->cfg.tile_w, S->max_stack,
-                                   ctr, s),
+                                   S->d_next.as<uint32_t>() + v, variant_blocks(S, v), S->cfg.tile_w,
+                                   S->max_stack, ctr, s),
                   "serve launch");
     }
     for (uint32_t si = 1; si < kStreams; ++si) {

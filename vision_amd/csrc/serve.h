@@ -24,12 +24,13 @@ struct ServeJob {
     RenderView rv;
     float4 *rt;      // running-mean film, width * height
     uint32_t scene;  // index into the step's SceneView table
-    uint32_t tiles_x;// tiles per row at the step's tile width
+    uint32_t tiles_x;// items per row at the step's tile width
     uint32_t pad[2];
 };
 
 // one job's share of a step: frames [frame_start, frame_start + frames) over
-// all its tiles, which are items [first_item, first_item + tiles)
+// all its items [first_item, first_item + items); an item is a tile of
+// tile_w x (64 / tile_w) pixels
 struct StepEntry {
     uint32_t job;
     uint32_t frame_start;

@@ -48,18 +48,11 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC > 0
         const ServeJob &job = jobs[e.job];
         const SceneView &sc = scenes[job.scene];
         const uint32_t t = k - e.first_item;
+        // one pixel of the item's tile per lane, its frames back to back
         const uint32_t px = (t % job.tiles_x) * tile_w + lx;
         const uint32_t py = (t / job.tiles_x) * tile_h + ly;
-        if (px < job.cam.width && py < job.cam.height) {
-            const uint32_t idx = py * job.cam.width + px;
-            float4 acc = job.rt[idx];
-            for (uint32_t s = 0; s < e.frames; ++s) {
-                const uint32_t frame = e.frame_start + s;
-                f3 L = trace_path<NS, RGB, MS>(sc, job.cam, job.rv, px, py, frame, lds_stack + threadIdx.x, 64u, pc);
-                acc = film_add(job.rv, acc, L, frame);
-            }
-            job.rt[idx] = acc;
-        }
+        render_column<NS, RGB, MS>(sc, job.cam, job.rv, job.rt, nullptr, px, py, 1u, 1u, e.frame_start, e.frames,
+                                   lds_stack + threadIdx.x, 64u, pc);
     }
     if (counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
