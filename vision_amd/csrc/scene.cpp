@@ -211,6 +211,47 @@ MeshHost make_quad(float width, float height) {
     return m;
 }
 
+// Sphere (src/render_core/shapes/sphere.cpp:19-86): a latitude-longitude
+// mesh of theta_div rings and 2 theta_div segments; the vertex and triangle
+// order is the reference's, so triangle ids and uvs agree
+MeshHost make_sphere(float radius, uint32_t theta_div) {
+    if (theta_div < 2 || theta_div > 4096) throw std::runtime_error("sphere sub_div must be in 2..4096");
+    const uint32_t phi_div = 2 * theta_div;
+    MeshHost m;
+    add_vertex(m, mk3(0, radius, 0), mk3(0, 1, 0), mk2(0, 0));
+    for (uint32_t i = 1; i < theta_div; ++i) {
+        const float v = float(i) / float(theta_div);
+        const float theta = float(kPi) * v;
+        const float y = radius * std::cos(theta), r = radius * std::sin(theta);
+        for (uint32_t j = 0; j < phi_div; ++j) {
+            const float u = float(j) / float(phi_div);
+            const float phi = u * (2.f * float(kPi));
+            const f3 p = j == 0 ? mk3(r, y, 0.f) : mk3(std::cos(phi) * r, y, std::sin(phi) * r);
+            add_vertex(m, p, normalize(p), mk2(u, v));
+        }
+    }
+    add_vertex(m, mk3(0, -radius, 0), mk3(0, -1, 0), mk2(0, 1));
+    for (uint32_t i = 0; i < phi_div; ++i) m.tris.push_back({0u, (i + 1) % phi_div + 1, i + 1});
+    for (uint32_t i = 0; i + 2 < theta_div; ++i) {
+        const uint32_t row = 1 + i * phi_div;
+        for (uint32_t j = 0; j < phi_div; ++j) {
+            const uint32_t a = row + j;
+            // the last segment wraps to the ring's first vertex
+            const uint32_t b = j + 1 < phi_div ? a + 1 : a + 1 - phi_div;
+            if (j + 1 < phi_div) {
+                m.tris.push_back({a, b, a + phi_div});
+                m.tris.push_back({b, a + phi_div + 1, a + phi_div});
+            } else {
+                m.tris.push_back({a, b, a + phi_div});
+                m.tris.push_back({b, a + 1, a + phi_div});
+            }
+        }
+    }
+    const uint32_t last = uint32_t(m.P.size()) - 1;
+    for (uint32_t i = 0; i < phi_div; ++i) m.tris.push_back({last, last - ((1 + i) % phi_div + 1), last - (i + 1)});
+    return m;
+}
+
 // Cube (src/render_core/shapes/cube.cpp:20-71)
 MeshHost make_cube(float x, float y, float z) {
     y = y == 0 ? x : y;
@@ -365,11 +406,21 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
         const Json &node = c.contains("node") ? c["node"] : c;
         std::string fn = node["fn"].as_string("");
         if (fn.empty() && node.contains("param")) fn = node["param"]["fn"].as_string("");
-        if (fn.empty()) throw std::runtime_error("spherical light needs an image (color.node.fn)");
-        l.env_fn = join_path(base_dir, fn);
-        load_env_image(l.env_fn, l.env_rgb, l.env_w, l.env_h);
-        l.color = mk3(1.f);
         l.scale = float(p["scale"].as_float(1.0));
+        if (fn.empty()) {
+            // a constant colour: a 1 x 1 map sampled with weight 1
+            // (SphericalMap::prepare, spherical.cpp:198-211), the colour
+            // normalised into the scale as for every light (light.cpp:10-17)
+            f3 c = slot_f3(p["color"], mk3(1.f));
+            l.scale *= normalize_color(c);
+            l.env_w = l.env_h = 1;
+            l.env_rgb = {c.x, c.y, c.z};
+            l.env_constant = true;
+        } else {
+            l.env_fn = join_path(base_dir, fn);
+            load_env_image(l.env_fn, l.env_rgb, l.env_w, l.env_h);
+        }
+        l.color = mk3(1.f);
         l.flip_u = p["flip_u"].as_bool(false);
         l.env_o2w = parse_transform(p["o2w"]);
     } else {
@@ -544,6 +595,12 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
 
     // light sampler + its light list (LightSamplerDesc::init, LightSampler ctor
     // src/base/illumination/lightsampler.cpp:11-26)
+    struct LightQuad {
+        size_t light;
+        float width, height;
+        m4 o2w;
+    };
+    std::vector<LightQuad> light_quads;
     {
         const Json &d = root["light_sampler"];
         s.light_sampler_type = d["type"].as_string("uniform");
@@ -554,7 +611,18 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         s.env_prob = std::clamp(float(p["env_prob"].as_float(0.5)), 0.01f, 0.99f);
         const Json &lights = p["lights"];
         if (lights.is_array())
-            for (const Json &l : lights.elements()) s.lights.push_back(parse_light(l, base_dir));
+            for (const Json &l : lights.elements()) {
+                LightHost lh = parse_light(l, base_dir);
+                // an area light of its own brings its emitter: a width x
+                // height quad placed by "o2w" (AreaLight::init_geometry,
+                // src/render_core/lights/area.cpp:57-68)
+                if (lh.type == LIGHT_AREA) {
+                    const Json &lp = l["param"];
+                    light_quads.push_back({s.lights.size(), float(lp["width"].as_float(1.0)),
+                                           float(lp["height"].as_float(1.0)), parse_transform(lp["o2w"])});
+                }
+                s.lights.push_back(lh);
+            }
     }
 
     // materials (MaterialDesc::init)
@@ -628,6 +696,8 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                 MeshHost mesh;
                 if (type == "quad") {
                     mesh = make_quad(float(p["width"].as_float(1.0)), float(p["height"].as_float(1.0)));
+                } else if (type == "sphere") {
+                    mesh = make_sphere(float(p["radius"].as_float(1.0)), p["sub_div"].as_uint(60));
                 } else if (type == "cube") {
                     mesh = make_cube(float(p["x"].as_float(1.0)), float(p["y"].as_float(1.0)), float(p["z"].as_float(1.0)));
                 } else if (type == "model") {
@@ -658,6 +728,24 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                 s.meshes.push_back(std::move(mesh));
                 s.instances.push_back(inst);
             }
+        }
+    }
+    // the light list's own area emitters, after the scene's shapes, with the
+    // black-body material (Scene::obtain_black_body: no scattering)
+    if (!light_quads.empty()) {
+        MaterialHost black;
+        black.type = "black_body";
+        const int32_t black_index = int32_t(s.materials.size());
+        s.materials.push_back(black);
+        for (const LightQuad &q : light_quads) {
+            InstanceHost inst;
+            inst.mesh = uint32_t(s.meshes.size());
+            inst.o2w = q.o2w;
+            inst.material = black_index;
+            inst.light = int32_t(q.light);
+            s.lights[q.light].inst = uint32_t(s.instances.size());
+            s.meshes.push_back(make_quad(q.width, q.height));
+            s.instances.push_back(inst);
         }
     }
     s.flatten();

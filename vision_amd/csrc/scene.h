@@ -59,6 +59,7 @@ struct LightHost {
     std::string env_fn;
     std::vector<float> env_rgb;
     uint32_t env_w{0}, env_h{0};
+    bool env_constant{false};// a constant colour, not an image
     m4 env_o2w{m4_identity()};
     bool flip_u{false};
     // per-emitter triangle area weights (area lights) filled at flatten time

@@ -346,7 +346,7 @@ PackedScene pack_scene(const SceneHost &s) {
                 // SphericalMap::calculate_weights (spherical.cpp:170-196)
                 float v = float(int(i) / int(l.env_h)) + 0.5f;
                 float theta = v / float(l.env_w);
-                weights[i] = luminance(c) * std::sin(kPiF * theta);
+                weights[i] = l.env_constant ? 1.f : luminance(c) * std::sin(kPiF * theta);
             }
             // AliasTable2D::build (alias2d.cpp:32-71)
             std::vector<float> marg(l.env_h);
