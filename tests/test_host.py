@@ -74,18 +74,15 @@ def test_cornell_box_resolves_to_expected_geometry(host_ctx):
 
 def test_reference_cbox_json_parses(host_ctx):
     # the reference's cbox.json (res/render_scene/cbox/cbox.json) with its
-    # comment syntax; it uses shapes outside this build's scope ("sphere"),
-    # which must be reported as unsupported rather than misparsed
+    # comment syntax; its "mirror" / "mix" materials are outside this build's
+    # scope and must be reported as unsupported rather than misparsed
     text = (GOLDEN / "cbox_reference.json").read_text()
     with pytest.raises(va.VisionError) as e:
         host_ctx.load_text(text)
-    assert e.value.status == 3 and "sphere" in str(e.value)
-    # with the sphere swapped for its commented-out cube alternative the
-    # scene parses: walls and light match the file's matrices
-    text2 = text.replace('"type" : "sphere",', '"type" : "cube",', 1)
-    with pytest.raises(va.VisionError) as e2:
-        host_ctx.load_text(text2)
-    assert e2.value.status == 3  # the "mirror"/"mix" materials are out of scope
+    assert e.value.status == 3 and ("mirror" in str(e.value) or "mix" in str(e.value))
+    # with those materials replaced the scene (sphere included) parses:
+    # walls and light match the file's matrices
+    text2 = text
     doc = json.loads(re.sub(r"//[^\n]*", "", text2))
     keep = {"diffuse"}
     doc["materials"] = [m for m in doc["materials"] if m["type"] in keep]
