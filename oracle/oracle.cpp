@@ -449,7 +449,7 @@ float schlick_f0_from_ior(float ior) {
 }
 
 // Fresnel kinds of src/base/scattering/fresnel.h and metal.cpp:9-21
-enum FresnelKind { FR_CONDUCTOR, FR_DIELECTRIC, FR_SCHLICK, FR_F82 };
+enum FresnelKind { FR_CONDUCTOR, FR_DIELECTRIC, FR_SCHLICK, FR_F82, FR_ONE };
 struct Fresnel {
     int kind{FR_DIELECTRIC};
     Spec eta, k;// conductor eta / k; dielectric and Schlick: eta per wavelength
@@ -458,6 +458,8 @@ struct Fresnel {
 Spec fresnel_eval(const Fresnel &fr, float c, int n) {
     Spec r;
     switch (fr.kind) {
+        case FR_ONE:// FresnelConstant (fresnel.h:132-137)
+            return sfill(1.f);
         case FR_CONDUCTOR:
             for (int i = 0; i < n; ++i) r.v[i] = fresnel_complex(c, fr.eta.v[i], fr.k.v[i]);
             return r;
@@ -750,7 +752,7 @@ BSample bsdf_sample(const Spectral &sd, const Bsdf &b, Sampler &sp, int n) {
 //         p[19] anisotropic, p[20] sheen_weight, p[21] sheen_roughness,
 //         p[22] coat_weight, p[23] coat_roughness, p[24] coat_ior,
 //         p[25] transmission_weight
-enum MatType { M_DIFFUSE = 0, M_OREN = 1, M_BLACK = 2, M_METAL = 3, M_GLASS = 4, M_PRINCIPLED = 5 };
+enum MatType { M_DIFFUSE = 0, M_OREN = 1, M_BLACK = 2, M_METAL = 3, M_GLASS = 4, M_PRINCIPLED = 5, M_MIRROR = 6 };
 
 float layering_div(float a, float w) { return w != 0.f ? a / w : 0.f; }
 // layering_weight (principled_bsdf.cpp:174-179)
@@ -805,6 +807,14 @@ Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, c
                     L.fr.k.v[i] = spd_eval(sd.spd + ok, p[10], w.lambda[i]);
                 }
             }
+            L.compensate = true;
+            break;
+        }
+        case M_MIRROR: {// MirrorMaterial::create_lobe_set (mirror.cpp:60-72)
+            L.kind = LB_REFLECTION;
+            L.kr = albedo_at(sd, flags, p, 0, it.u, it.v, w);
+            L.mf = {p[6], p[7]};
+            L.fr.kind = FR_ONE;
             L.compensate = true;
             break;
         }
@@ -1625,7 +1635,7 @@ float lum3(const Spec &f, int n) { return luminance(v3(f.v[0], n > 1 ? f.v[1] : 
 uint32_t mat_flag(const Scene &s, uint32_t m) {
     if (m == kInv) return kFlagDiffuse | kFlagRefl;
     uint32_t t = s.mat_type(m);
-    if (t == M_METAL) return std::min(s.matp(m, 6), s.matp(m, 7)) < 0.022f ? kFlagNearSpec : kFlagGlossy;
+    if (t == M_METAL || t == M_MIRROR) return std::min(s.matp(m, 6), s.matp(m, 7)) < 0.022f ? kFlagNearSpec : kFlagGlossy;
     if (t == M_GLASS) return std::min(s.matp(m, 5), s.matp(m, 6)) < 0.022f ? kFlagNearSpec : kFlagGlossy;
     if (t == M_PRINCIPLED) return kFlagDiffuse | kFlagRefl | kFlagGlossy;
     return kFlagDiffuse | kFlagRefl;
@@ -1633,7 +1643,7 @@ uint32_t mat_flag(const Scene &s, uint32_t m) {
 float mat_diffuse_factor(const Scene &s, uint32_t m) {
     if (m == kInv) return 1.f;
     uint32_t t = s.mat_type(m);
-    if (t == M_METAL) return std::sqrt(std::sqrt(s.matp(m, 6) * s.matp(m, 7)));
+    if (t == M_METAL || t == M_MIRROR) return std::sqrt(std::sqrt(s.matp(m, 6) * s.matp(m, 7)));
     if (t == M_GLASS) return std::sqrt(s.matp(m, 5));
     return 1.f;
 }

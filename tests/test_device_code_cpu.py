@@ -209,6 +209,20 @@ def test_device_restir_source_equals_oracle_hero_env(dc, tmp_path):
     _restir_equal(dev, orc)
 
 
+def test_device_restir_source_equals_oracle_mirror(dc):
+    scene = scenegen.cornell_box(24, 24, light_sampler="power")
+    scene["materials"] += [{"type": "mirror", "name": "Mirror", "param": {"color": [0.9, 0.8, 0.6]}},
+                           {"type": "mirror", "name": "Rough", "param": {"roughness": 0.4}}]
+    for sh in scene["shapes"]:
+        if sh["name"] == "short_box":
+            sh["param"]["material"] = "Mirror"
+        if sh["name"] == "tall_box":
+            sh["param"]["material"] = "Rough"
+    scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4)
+    dev, orc = _restir_pair(dc, scene, 3)
+    _restir_equal(dev, orc)
+
+
 def test_device_restir_source_equals_oracle_near_specular(dc):
     scene = scenegen.cornell_box(24, 24, light_sampler="power")
     scene["materials"] += [{"type": "metal", "name": "Mirror", "param": {"material_name": "Cu", "roughness": 0.01}},

@@ -50,6 +50,8 @@ TEST_MATERIALS = {
                              "sheen_weight": 0.5, "coat_weight": 0.5, "coat_roughness": 0.15,
                              "transmission_weight": 0.3, "spec_tint": [1.0, 0.9, 0.8]}},
     "black_body": {"type": "black_body", "param": {}},
+    "mirror_tinted": {"type": "mirror", "param": {"color": [0.9, 0.6, 0.3], "roughness": 0.35, "anisotropic": 0.2}},
+    "mirror": {"type": "mirror", "param": {}},
 }
 
 

@@ -533,3 +533,32 @@ def test_env_map_orientation_and_importance(tmp_path):
     want = 0.5 / np.pi * np.sum(L * world[:, 1:2], 0) * dw
     got = film[..., :3].reshape(-1, 3).astype(np.float64).mean(0)
     assert np.allclose(got, want, rtol=0.03, atol=2e-3), (got, want)
+
+
+def _mat_mirror(rough, color=(1, 1, 1)):
+    a = rough * rough
+    return oracle.material_record(6, [*color, 0.0, 0.0, 0.0, a, a])
+
+
+@pytest.mark.parametrize("rough", [0.55, 0.8, 1.0])
+@pytest.mark.parametrize("mu", [0.3, 0.8])
+def test_mirror_reflects_its_tint_with_compensation(rough, mu):
+    # MirrorLobe: constant Fresnel, f = kr f_ss / E(alpha, cos); the
+    # reflectance is kr E_ss(alpha) / E_ss(alpha^2), linear in the tint
+    alpha = rough * rough
+    spec = _rgb_spec()
+    tint = np.array([0.9, 0.5, 0.2])
+    wo1 = np.array([math.sqrt(1 - mu * mu), 0.0, mu], np.float32)
+    d, dw = _hemisphere_grid(1500, 512)
+    f, pdf = oracle.bsdf_eval(spec, _mat_mirror(rough, tint), np.tile(wo1, (len(d), 1)), d)
+    fw, pdfw = oracle.bsdf_eval(spec, _mat_mirror(rough), np.tile(wo1, (len(d), 1)), d)
+    np.testing.assert_allclose(f[:, :3], fw[:, :3] * tint.astype(np.float32), rtol=1e-6, atol=1e-12)
+    np.testing.assert_array_equal(pdf, pdfw)
+    E = f[:, :3].astype(np.float64).sum(axis=0) * dw
+    want = tint * _ggx_single_scatter_albedo(alpha, mu) / _ggx_single_scatter_albedo(alpha * alpha, mu)
+    np.testing.assert_allclose(E, want, rtol=0.03)
+    # the white mirror equals a conductor whose Fresnel is 1 to within its
+    # residual (eta 1, k 1e4: F = 1 - 4e-8)
+    fc, _ = oracle.bsdf_eval(spec, _mat_metal(rough, eta=(1.0, 1.0, 1.0), k=(1e4, 1e4, 1e4)),
+                             np.tile(wo1, (len(d), 1)), d)
+    np.testing.assert_allclose(fw[:, :3], fc[:, :3], rtol=1e-5, atol=1e-12)

@@ -160,6 +160,11 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
             }
         }
         c.comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
+    } else if (ms_has<MS>(MS_METAL) && m.type == MAT_MIRROR) {
+        // MirrorLobe: the tint as kr, constant Fresnel, compensated
+        c.ggx = {p[6], p[7]};
+        c.a0 = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
+        c.comp = pure_reflection_comp(sc.lut, c.ggx.ax, c.ggx.ay, c.wo.z);
     } else if (ms_has<MS>(MS_GLASS) && m.type == MAT_GLASS) {
         c.ggx = {p[5], p[6]};
         c.a0 = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
@@ -287,6 +292,20 @@ __device__ __forceinline__ void eval_conductor_like(const Ggx &g, f3 wo, f3 wi, 
     e.flags = 0;
 }
 
+// MicrofacetReflection with FresnelConstant (fresnel.h:132-137): f = D G kr
+// / (4 |cos o cos i|), then the pure-reflection compensation
+template <int NS>
+__device__ __forceinline__ void eval_mirror(const Ggx &g, f3 wo, f3 wi, const Spec<NS> &kr, float comp,
+                                            EvalN<NS> &e) {
+    bool s = wo.z * wi.z > 0.f;
+    f3 wh = normalize(wo + wi);
+    f3 whf = face_forward(wh, mk3(0.f, 0.f, 1.f));
+    e.f = s ? kr * g.brdf(wo, whf, wi) : Spec<NS>::fill(0.f);
+    e.pdf = s ? g.pdf_reflection(wo, wh) : 0.f;
+    e.f = e.f * comp;
+    e.flags = 0;
+}
+
 template <int NS>
 __device__ __forceinline__ void eval_reflection_dielectric(const Ggx &g, f3 wo, f3 wi, float eta, bool schlick,
                                                            const Spec<NS> &F0, const Spec<NS> &kr, EvalN<NS> &e) {
@@ -379,6 +398,9 @@ __device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, floa
         e.f = e.f * fabsf(wi.z);
     } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
         eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), c.comp, e);
+        e.f = e.f * fabsf(wi.z);
+    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_MIRROR) {
+        eval_mirror(c.ggx, wo, wi, c.a0, c.comp, e);
         e.f = e.f * fabsf(wi.z);
     } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
         eval_dielectric(c.ggx, wo, wi, c.a1, c.eta0, false, c.a1, c.a0, c.comp, e, eta_out);
@@ -474,7 +496,7 @@ __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &
     if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
         wi = square_to_cosine_hemisphere(sp.next2d());
         if (wo.z < 0.f) wi.z = -wi.z;
-    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
+    } else if (ms_has<MS>(MS_METAL) && (c.type == MAT_METAL || c.type == MAT_MIRROR)) {
         f3 wh = c.ggx.sample_wh(wo, sp.next2d());
         wi = reflect_dir(wo, wh);
         valid = wo.z * wi.z > 0.f;

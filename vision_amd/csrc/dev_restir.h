@@ -221,7 +221,8 @@ __device__ __forceinline__ float luminance_of(const Spec<NS> &f) {
 __device__ __forceinline__ uint32_t material_flag(const SceneView &sc, uint32_t mat) {
     if (mat == kInvalid) return BXDF_DIFFUSE | BXDF_REFLECTION;
     const MaterialRecord &m = sc.mat[mat];
-    if (m.type == MAT_METAL) return fminf(m.p[6], m.p[7]) < kNearSpecAlpha ? BXDF_NEAR_SPEC : BXDF_GLOSSY;
+    if (m.type == MAT_METAL || m.type == MAT_MIRROR)
+        return fminf(m.p[6], m.p[7]) < kNearSpecAlpha ? BXDF_NEAR_SPEC : BXDF_GLOSSY;
     if (m.type == MAT_GLASS) return fminf(m.p[5], m.p[6]) < kNearSpecAlpha ? BXDF_NEAR_SPEC : BXDF_GLOSSY;
     if (m.type == MAT_PRINCIPLED) return BXDF_DIFFUSE | BXDF_REFLECTION | BXDF_GLOSSY;
     return BXDF_DIFFUSE | BXDF_REFLECTION;
@@ -233,7 +234,7 @@ __device__ __forceinline__ uint32_t material_flag(const SceneView &sc, uint32_t 
 __device__ __forceinline__ float material_diffuse_factor(const SceneView &sc, uint32_t mat) {
     if (mat == kInvalid) return 1.f;
     const MaterialRecord &m = sc.mat[mat];
-    if (m.type == MAT_METAL) return sqrtf(sqrtf(m.p[6] * m.p[7]));
+    if (m.type == MAT_METAL || m.type == MAT_MIRROR) return sqrtf(sqrtf(m.p[6] * m.p[7]));
     if (m.type == MAT_GLASS) return sqrtf(m.p[5]);
     return 1.f;
 }
@@ -431,6 +432,8 @@ __device__ __forceinline__ bool sample_delta(const MatCtx<NS> &c, Sampler &sp, f
     f3 wi = mk3(-wo.x, -wo.y, wo.z);
     if (c.type == MAT_METAL) {
         eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), 1.f, e);
+    } else if (c.type == MAT_MIRROR) {
+        eval_mirror(c.ggx, wo, wi, c.a0, 1.f, e);
     } else if (c.type == MAT_GLASS) {
         Spec<NS> F;
 #pragma unroll

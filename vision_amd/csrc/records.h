@@ -12,6 +12,7 @@ enum MaterialType : uint32_t {
     MAT_METAL = 3,     // GGX conductor (metal.cpp:39-157)
     MAT_GLASS = 4,     // rough dielectric (glass.cpp:171-258)
     MAT_PRINCIPLED = 5,// principled_bsdf lobe stack
+    MAT_MIRROR = 6,    // GGX reflection, Fresnel 1, tinted (mirror.cpp:11-74)
 };
 
 enum LightType : uint32_t {

@@ -155,6 +155,15 @@ MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float>
             r.p[9] = ubits_f(append_curve(spd, k));
             r.p[10] = 471.f / float(eta.size());
         }
+    } else if (m.type == "mirror") {
+        // MirrorMaterial (src/render_core/materials/mirror.cpp:26-72): colour
+        // slot, roughness 0.001 remapped to alpha, anisotropy
+        r.type = MAT_MIRROR;
+        put_color_slot(r, 0, s, p, "color", mk3(1.f), rgb);
+        float rough = std::clamp(slot1(p, "roughness", 0.001f), 0.0001f, 1.f);
+        float aniso = std::clamp(slot1(p, "anisotropic", 0.f), -0.9f, 0.9f);
+        if (p["remapping_roughness"].as_bool(true)) rough = rough * rough;
+        calculate_alpha(rough, aniso, r.p[6], r.p[7]);
     } else if (m.type == "glass") {
         // GlassMaterial (glass.cpp:171-258)
         r.type = MAT_GLASS;
