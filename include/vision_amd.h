@@ -377,8 +377,8 @@ vs_status vs_server_get_route(vs_server *srv, uint32_t scene, int32_t spectral, 
  * the occupancy limit and half of it) and keep the one with the highest
  * Msamples/s; then, for each (scene, spectrum mode) of the probe list, time
  * its largest probe job alone on each route and route it to the wavefront
- * lane only when that is more than 1.25x faster (the lane renders one job at
- * a time, so under load a small edge alone does not carry over).  The
+ * lane only when that is more than 2x faster (the lane renders one job at a
+ * time, so under load a smaller edge alone does not carry over).  The
  * server must be idle; the probe jobs' results are discarded. */
 #define VS_AUTOTUNE_MAX_TRIALS 16
 typedef struct vs_autotune_result {

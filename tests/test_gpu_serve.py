@@ -152,7 +152,8 @@ def test_autotuner_picks_a_measured_shape(gpu_ctx, served):
         assert len(r["trials"]) >= 4
         assert [(x["scene"], x["spectral"]) for x in r["routes"]] == [(i, False)]
         rt = r["routes"][0]
-        assert srv.route(i, False) == ("wavefront" if rt["wavefront_ms"] < rt["megakernel_ms"] else "megakernel")
+        # the lane is chosen only when it is more than twice as fast alone
+        assert srv.route(i, False) == ("wavefront" if 2 * rt["wavefront_ms"] < rt["megakernel_ms"] else "megakernel")
         best = max(r["trials"], key=lambda t: t["msamples_per_s"])
         assert (r["tile_w"], r["waves_per_cu"], r["min_waves_per_eu"]) == (best["tile_w"], best["waves_per_cu"],
                                                                            best["min_waves_per_eu"])

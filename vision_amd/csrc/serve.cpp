@@ -1047,9 +1047,10 @@ vs_status vs_server_autotune(vs_server *S, const vs_job_desc *probe, uint32_t n,
             ms[route] = info.latency_ms;
         }
         if (st != VS_OK) break;
-        // the lane renders one job at a time: under load it only pays where
-        // the wavefront form is clearly faster alone
-        uint32_t choice = ms[1] * 1.25 < ms[0] ? 1u : 0u;
+        // the lane renders one job at a time beside the persistent steps:
+        // under load it only pays where the wavefront form is more than
+        // twice as fast alone
+        uint32_t choice = ms[1] * 2.0 < ms[0] ? 1u : 0u;
         st = vs_server_set_route(S, key.first, int32_t(key.second), choice);
         if (out->routes < VS_AUTOTUNE_MAX_TRIALS) {
             uint32_t r = out->routes++;
