@@ -1098,7 +1098,20 @@ struct LSample {
     V3 p;
 };
 
-Spec light_color(const Spectral &sd, const Scene &s, uint32_t l, const Swl &w) {
+// a light's radiance; an area light with an emission image (word 7: its
+// texture) takes the nearest texel at the light surface's (u, v) as its
+// colour, wrapped, v = 0 the first row (AreaLight::L, area.cpp:90-94)
+Spec light_color(const Spectral &sd, const Scene &s, uint32_t l, const Swl &w, float u = 0.f, float v = 0.f) {
+    const uint32_t t = s.lightu(l, 7);
+    if (t != 0xFFFFFFFFu) {
+        if (t >= sd.ntex) return sfill(0.f);
+        const uint32_t *info = sd.tex_info + 4 * t;
+        float uu = u - std::floor(u), vv = v - std::floor(v);
+        uint32_t x = std::min(uint32_t(uu * float(info[1])), info[1] - 1u);
+        uint32_t y = std::min(uint32_t(vv * float(info[2])), info[2] - 1u);
+        const float *q = sd.tex + 4 * (size_t(info[0]) + size_t(y) * info[1] + x);
+        return illumination(sd, q, q, s.lightf(l, 11), w);
+    }
     float rgb[3] = {s.lightf(l, 8), s.lightf(l, 9), s.lightf(l, 10)};
     float coef[4] = {s.lightf(l, 24), s.lightf(l, 25), s.lightf(l, 26), s.lightf(l, 27)};
     return illumination(sd, rgb, coef, s.lightf(l, 11), w);
@@ -1136,7 +1149,7 @@ LSample sample_light(const Spectral &sd, const Scene &s, const LightTables &lt, 
         float pdf_pos = (1.f / p.area) * pmf;
         V3 wv = sub(it.pos, p.pos);
         bool front = dotp(wv, p.ng) > 0.f || s.lightu(l, 1);
-        ls.L = front ? light_color(sd, s, l, w) : sfill(0.f);
+        ls.L = front ? light_color(sd, s, l, w, p.u, p.v) : sfill(0.f);
         float pdf = pdf_pos * dotp(wv, wv) / std::fabs(dotp(p.ng, unit(wv)));
         ls.pdf = (std::isinf(pdf) || std::isnan(pdf)) ? 0.f : pdf;
         float f = dotp(p.ng, wv) > 0.f ? 1.f : -1.f;
@@ -1191,7 +1204,7 @@ void eval_hit(const Spectral &sd, const Scene &s, const LightTables &lt, const S
     float pdf_pos = (1.f / it.area) * alias_pmf(lt.tri_alias[l], prim);
     V3 wv = sub(ref, it.pos);
     bool front = dotp(wv, it.ng) > 0.f || s.lightu(l, 1);
-    L = front ? light_color(sd, s, l, w) : sfill(0.f);
+    L = front ? light_color(sd, s, l, w, it.u, it.v) : sfill(0.f);
     float p = pdf_pos * dotp(wv, wv) / std::fabs(dotp(it.ng, unit(wv)));
     p = (std::isinf(p) || std::isnan(p)) ? 0.f : p;
     pdf = p * select_pmf(s, lt, l);
@@ -1733,7 +1746,7 @@ PEval light_eval_point(const Spectral &sd, const Scene &s, const LightTables &lt
     if (type == 0) {
         Surf p = surface(s, s.lightu(l, 3) + d.prim, d.bx, d.by, false);
         V3 wv = sub(ref, p.pos);
-        Spec Le = light_color(sd, s, l, w) * ((dotp(wv, p.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+        Spec Le = light_color(sd, s, l, w, p.u, p.v) * ((dotp(wv, p.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
         float G = dotp(unit(wv), p.ng) / dotp(wv, wv);
         r.L = Le * G;
         r.pdf = (1.f / p.area) * alias_pmf(lt.tri_alias[l], d.prim);
@@ -1877,7 +1890,7 @@ Spec bsdf_candidate(const Ctx &c, const Surf &it, const Bsdf &b, Sampler &sp, co
         uint32_t l = s.inst[4 * nx.inst + 1];
         if (l != kInv) {// LightSampler::evaluate_hit_point (lightsampler.cpp:269-288)
             V3 wv = sub(it.pos, nx.pos);
-            Spec Le = light_color(sd, s, l, w) * ((dotp(wv, nx.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+            Spec Le = light_color(sd, s, l, w, nx.u, nx.v) * ((dotp(wv, nx.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
             float G = dotp(unit(wv), nx.ng) / dotp(wv, wv);
             float pp = pdf * std::fabs(dotp(nx.ng, unit(wv))) / dotp(wv, wv);
             uint32_t prim = h.tri - s.lightu(l, 3);
@@ -2168,7 +2181,7 @@ V3 restir_shading(const Ctx &c, const RParams &rp, const RState &st, Rsv r, cons
     if (l != kInv) {
         if (s.lightu(l, 0) == 0) {
             V3 wv = sub(rp.cam_pos, it.pos);
-            Le = light_color(sd, s, l, w) * ((dotp(wv, it.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
+            Le = light_color(sd, s, l, w, it.u, it.v) * ((dotp(wv, it.ng) > 0.f || s.lightu(l, 1)) ? 1.f : 0.f);
         }
     } else {
         Bsdf b = bsdf_at(sd, s, it, w, true);

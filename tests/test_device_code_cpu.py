@@ -247,3 +247,19 @@ def test_device_path_source_equals_oracle_textured(dc, tmp_path, material, spect
     scene = scenegen.textured_box("checker.png", width=24, height=24, material=material, spectrum=spectrum)
     film, ref = _dev_render(dc, scene, 2, str(tmp_path))
     assert _bits_equal(film, ref), (material, spectrum)
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero4"])
+def test_device_path_source_equals_oracle_emission_image(dc, tmp_path, spectrum):
+    with va.Context(-1) as c:
+        c.write_image(tmp_path / "lamp.png", scenegen.checker_rgba(8, 4, a=(0.9, 0.5, 0.1), b=(0.2, 0.3, 0.9)))
+    scene = scenegen.cornell_box(24, 24, max_depth=4, light_sampler="power")
+    lamp = [sh for sh in scene["shapes"] if "emission" in sh["param"]][0]
+    lamp["param"]["emission"]["param"]["color"] = {"node": {"type": "image", "param": {"fn": "lamp.png"}}}
+    if spectrum != "srgb":
+        scene["spectrum"] = {"type": "hero", "param": {"dimension": 4}}
+    film, ref = _dev_render(dc, scene, 2, str(tmp_path))
+    assert _bits_equal(film, ref), spectrum
+    scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4)
+    dev, orc = _restir_pair(dc, scene, 3, str(tmp_path))
+    _restir_equal(dev, orc)

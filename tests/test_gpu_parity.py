@@ -243,3 +243,24 @@ def test_textured_colour_parity(gpu_ctx, tmp_path, material, spectrum):
     print(f"textured {material} {spectrum}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
     assert frac >= 0.999
     sc.close()
+
+
+@pytest.mark.parametrize("spectrum", ["srgb", "hero4"])
+def test_emission_image_parity(gpu_ctx, tmp_path, spectrum):
+    """The ceiling lamp emits a PNG checkerboard (an ImageNode in the area
+    light's colour slot)."""
+    gpu_ctx.write_image(tmp_path / "lamp.png", scenegen.checker_rgba(8, 4, a=(0.9, 0.5, 0.1), b=(0.2, 0.3, 0.9)))
+    scene = scenegen.cornell_box(64, 64, max_depth=4, light_sampler="power")
+    lamp = [sh for sh in scene["shapes"] if "emission" in sh["param"]][0]
+    lamp["param"]["emission"]["param"]["color"] = {"node": {"type": "image", "param": {"fn": "lamp.png"}}}
+    if spectrum != "srgb":
+        scene["spectrum"] = {"type": "hero", "param": {"dimension": 4}}
+    sc = gpu_ctx.load_text(scene, base_dir=str(tmp_path))
+    sc.build_accel()
+    sc.render(2, fused=True)
+    gpu = sc.film(0)
+    ref = oracle.render(sc.export(), 0, 2)
+    frac, _ = _compare(gpu, ref)
+    print(f"emission image {spectrum}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
+    assert frac >= 0.999
+    sc.close()

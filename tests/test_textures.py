@@ -79,3 +79,73 @@ def test_textured_render_changes_with_the_image(ctx, tmp_path):
     a = oracle.render(textured, 0, 2)
     b = oracle.render(constant, 0, 2)
     np.testing.assert_array_equal(a, b)
+
+
+def _lamp_box(image_fn, spectrum="srgb", width=24, height=24):
+    """The Cornell box whose ceiling lamp emits the image `image_fn`."""
+    scene = scenegen.cornell_box(width, height, max_depth=4, light_sampler="power")
+    lamps = [sh for sh in scene["shapes"] if "emission" in sh["param"]]
+    assert len(lamps) == 1
+    lamps[0]["param"]["emission"]["param"]["color"] = {"channels": "xyz",
+                                                       "node": {"type": "image", "param": {"fn": image_fn}}}
+    if spectrum != "srgb":
+        scene["spectrum"] = {"type": "hero", "param": {"dimension": int(spectrum[-1])}}
+    return scene
+
+
+def test_emission_image_is_registered_as_illumination(ctx, tmp_path):
+    img = scenegen.checker_rgba(8, 4, a=(0.9, 0.5, 0.1), b=(0.2, 0.3, 0.9))
+    ctx.write_image(tmp_path / "lamp.png", img)
+    ex = _export(ctx, _lamp_box("lamp.png"), tmp_path)
+    np.testing.assert_array_equal(ex["texture_info"], [[0, 8, 4, 1]])
+    L = ex["lights"][0]
+    assert L.view(np.uint32)[7] == 0
+    # the record's colour is the image's linear average (the light's power
+    # for the power sampler); the scale is not normalised
+    with Image.open(tmp_path / "lamp.png") as im:
+        codes = np.asarray(im.convert("RGB")).astype(np.float64) / 255
+    np.testing.assert_allclose(L.view(np.float32)[8:11], _srgb_decode(codes).mean(axis=(0, 1)), rtol=1e-5)
+
+
+def test_uniform_emission_image_equals_constant_colour(ctx, tmp_path):
+    c = np.array([0.8, 0.55, 0.3])
+    ctx.write_image(tmp_path / "flat.exr", np.concatenate([np.broadcast_to(c, (3, 5, 3)), np.ones((3, 5, 1))],
+                                                          -1).astype(np.float32))
+    textured = _lamp_box("flat.exr", width=12, height=12)
+    plain = _lamp_box("flat.exr", width=12, height=12)
+    lamp = [sh for sh in plain["shapes"] if "emission" in sh["param"]][0]
+    lamp["param"]["emission"]["param"]["color"] = c.tolist()
+    films = []
+    for sc in (textured, plain):
+        s = ctx.load_text(json.dumps(sc), str(tmp_path))
+        films.append(oracle.render(s.export(), 0, 4))
+        s.close()
+    # same paths; the constant colour is normalised into the scale, so the
+    # products round differently
+    np.testing.assert_allclose(films[0], films[1], rtol=1e-5, atol=1e-7)
+
+
+def test_checker_lamp_form_factor_closed_form(ctx, tmp_path):
+    """A square lamp emitting a fine two-colour checker above a Lambert
+    floor: the floor reflects rho * mean(L) * F (the checker averages out)."""
+    a, h, rho = 0.5, 0.8, 0.5
+    lo, hi = np.array([1.0, 2.0, 4.0]), np.array([7.0, 4.0, 2.0])
+    n = 64
+    yy, xx = np.mgrid[0:n, 0:n]
+    cell = ((xx // 2 + yy // 2) % 2)[..., None]
+    img = np.where(cell == 0, lo, hi)
+    ctx.write_image(tmp_path / "checker.exr", np.concatenate([img, np.ones((n, n, 1))], -1).astype(np.float32))
+    emitter = {"type": "quad", "name": "lamp",
+               "param": {"width": 1, "height": 1, "material": "Black",
+                         "transform": scenegen.xform(scenegen.translate(0, h, 0) @ scenegen.scale(a, -1, -a)),
+                         "emission": {"type": "area", "param": {"color": {"node": {"fn": "checker.exr"}},
+                                                                 "scale": 1}}}}
+    scene = scenegen.lit_plane(None, albedo=rho, cam_height=0.4, fov=0.5, width=8, height=8, extra_shapes=[emitter])
+    s = ctx.load_text(json.dumps(scene), str(tmp_path))
+    film = oracle.render(s.export(), 0, 256)
+    s.close()
+    A = (a / 2) / h
+    Fc = 2 * (A / np.sqrt(1 + A * A) * np.arctan(A / np.sqrt(1 + A * A))) / (2 * np.pi)
+    want = rho * (lo + hi) / 2 * 4 * Fc
+    got = film[2:6, 2:6, :3].mean(axis=(0, 1))
+    np.testing.assert_allclose(got, want, rtol=0.02)

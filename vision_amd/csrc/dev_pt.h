@@ -115,8 +115,20 @@ __device__ __forceinline__ Spec<NS> illum_spectrum(const SceneView &sc, const fl
     return r;
 }
 
+// a light's radiance spectrum; an area light with an emission image takes
+// the nearest texel at the light surface's uv (wrapped, v = 0 the first row)
 template <int NS, bool RGB>
-__device__ __forceinline__ Spec<NS> light_color(const SceneView &sc, const LightRecord &L, const Swl<NS> &swl) {
+__device__ __forceinline__ Spec<NS> light_color(const SceneView &sc, const LightRecord &L, const Swl<NS> &swl,
+                                                f2 uv) {
+    if (L.tex != kInvalid) {
+        const uint4 info = sc.tex_info[L.tex];
+        const float u = uv.x - floorf(uv.x), v = uv.y - floorf(uv.y);
+        const uint32_t x = min(uint32_t(u * float(info.y)), info.y - 1u);
+        const uint32_t y = min(uint32_t(v * float(info.z)), info.z - 1u);
+        const float4 t = sc.tex[size_t(info.x) + size_t(y) * info.y + x];
+        const float q[4] = {t.x, t.y, t.z, t.w};
+        return illum_spectrum<NS, RGB>(sc, q, q, L.scale, swl);
+    }
     return illum_spectrum<NS, RGB>(sc, L.color, L.illum, L.scale, swl);
 }
 
@@ -201,7 +213,7 @@ __device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3
         float pdf_pos = (1.f / p.prim_area) * pmf;
         f3 w = ref_pos - p.pos;
         bool front = dot(w, p.ng) > 0.f || L.two_sided;
-        ls.L = front ? light_color<NS, RGB>(sc, L, swl) : Spec<NS>::fill(0.f);
+        ls.L = front ? light_color<NS, RGB>(sc, L, swl, p.uv) : Spec<NS>::fill(0.f);
         float cos_t = fabsf(dot(p.ng, normalize(w)));
         float pdf = pdf_pos * length_squared(w) / cos_t;
         ls.pdf = (isinf(pdf) || isnan(pdf)) ? 0.f : pdf;
@@ -227,7 +239,7 @@ __device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3
         // point.cpp:39-43, spot.cpp:51-70
         f3 pos = mk3(L.position[0], L.position[1], L.position[2]);
         f3 w_un = ref_pos - pos;
-        Spec<NS> val = light_color<NS, RGB>(sc, L, swl) / length_squared(w_un);
+        Spec<NS> val = light_color<NS, RGB>(sc, L, swl, mk2(0.f, 0.f)) / length_squared(w_un);
         if (L.type == LIGHT_SPOT) {
             f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
             float ct = fminf(fmaxf(dot(dir, normalize(w_un)), L.cos_angle), L.cos_falloff_start);
@@ -253,7 +265,7 @@ __device__ __forceinline__ void eval_hit_light(const SceneView &sc, const Surfac
     float pdf_pos = (1.f / it.prim_area) * alias_pmf(sc, Lr.alias_offset, Lr.tri_count, Lr.alias_integral, prim);
     f3 w = ref_pos - it.pos;
     bool front = dot(w, it.ng) > 0.f || Lr.two_sided;
-    L = front ? light_color<NS, RGB>(sc, Lr, swl) : Spec<NS>::fill(0.f);
+    L = front ? light_color<NS, RGB>(sc, Lr, swl, it.uv) : Spec<NS>::fill(0.f);
     float cos_t = fabsf(dot(it.ng, normalize(w)));
     float p = pdf_pos * length_squared(w) / cos_t;
     p = (isinf(p) || isnan(p)) ? 0.f : p;

@@ -332,7 +332,7 @@ __device__ __forceinline__ PointEval<NS> light_eval_point(const SceneView &sc, f
     if (L.type == LIGHT_AREA) {
         Surface p = surface_at(sc, L.tri_base + s.prim, s.bary.x, s.bary.y, false);
         f3 w = ref - p.pos;
-        Spec<NS> Le = light_color<NS, RGB>(sc, L, swl) * ((dot(w, p.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
+        Spec<NS> Le = light_color<NS, RGB>(sc, L, swl, p.uv) * ((dot(w, p.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
         float G = dot(normalize(w), p.ng) / length_squared(w);
         r.L = Le * G;
         r.pdf = (1.f / p.prim_area) * alias_pmf(sc, L.alias_offset, L.tri_count, L.alias_integral, s.prim);
@@ -349,7 +349,7 @@ __device__ __forceinline__ PointEval<NS> light_eval_point(const SceneView &sc, f
     } else {
         f3 pos = mk3(L.position[0], L.position[1], L.position[2]);
         f3 w_un = ref - pos;
-        Spec<NS> val = light_color<NS, RGB>(sc, L, swl) / length_squared(w_un);
+        Spec<NS> val = light_color<NS, RGB>(sc, L, swl, mk2(0.f, 0.f)) / length_squared(w_un);
         if (L.type == LIGHT_SPOT) {
             f3 dir = mk3(L.direction[0], L.direction[1], L.direction[2]);
             float ct = fminf(fmaxf(dot(dir, normalize(w_un)), L.cos_angle), L.cos_falloff_start);
@@ -507,7 +507,7 @@ __device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Su
             // LightSampler::evaluate_hit_point (lightsampler.cpp:269-288)
             const LightRecord &L = sc.lights[li];
             f3 w = it.pos - nx.pos;
-            Spec<NS> Le = light_color<NS, RGB>(sc, L, swl) * ((dot(w, nx.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
+            Spec<NS> Le = light_color<NS, RGB>(sc, L, swl, nx.uv) * ((dot(w, nx.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
             float G = dot(normalize(w), nx.ng) / length_squared(w);
             float pp = pdf * fabsf(dot(nx.ng, normalize(w))) / length_squared(w);
             uint32_t prim = h.tri - L.tri_base;
@@ -836,7 +836,7 @@ __device__ __forceinline__ f3 restir_shading(const SceneView &sc, const RestirVi
         const LightRecord &L = sc.lights[li];
         if (L.type == LIGHT_AREA) {
             f3 w = cam_pos - it.pos;
-            Le = light_color<NS, RGB>(sc, L, swl) * ((dot(w, it.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
+            Le = light_color<NS, RGB>(sc, L, swl, it.uv) * ((dot(w, it.ng) > 0.f || L.two_sided) ? 1.f : 0.f);
         }
     } else {
         MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);

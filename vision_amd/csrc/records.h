@@ -41,7 +41,7 @@ struct LightRecord {
     uint32_t tri_count;   // triangles of the emitter (area)
     uint32_t alias_offset;// offset of its per-triangle alias table
     float alias_integral; // sum(area)/n as float (alias.h:86-122)
-    float pad0;
+    uint32_t tex;         // area: emission image (index into the textures), or 0xFFFFFFFF
     float color[3];
     float scale;
     float position[3];

@@ -374,13 +374,25 @@ MeshHost load_vsmesh(const std::string &path) {
     return m;
 }
 
-LightHost parse_light(const Json &desc, const std::string &base_dir) {
+LightHost parse_light(const Json &desc, const std::string &base_dir, SceneHost &s) {
     LightHost l;
     std::string type = desc["type"].as_string("area");
     const Json &p = desc["param"];
     l.name = desc["name"].as_string("");
     l.scale = float(p["scale"].as_float(1.0));
-    if (type != "spherical") {
+    if (type == "area" && slot_image_node(p, "color")) {
+        // an emission image (ImageNode): radiance = texel at the light
+        // surface's uv times scale (AreaLight::L, area.cpp:90-94); an image
+        // node does not normalise (ShaderNode::normalize returns 1) and the
+        // light's power takes the image's average
+        l.tex = int32_t(texture_index(s, *slot_image_node(p, "color"), true));
+        const TextureHost &t = s.textures[size_t(l.tex)];
+        double acc[3] = {0, 0, 0};
+        const size_t n = size_t(t.width) * t.height;
+        for (size_t i = 0; i < n; ++i)
+            for (int c = 0; c < 3; ++c) acc[c] += t.rgb[3 * i + c];
+        l.color = mk3(float(acc[0] / double(n)), float(acc[1] / double(n)), float(acc[2] / double(n)));
+    } else if (type != "spherical") {
         // Light::Light folds the colour normalization factor into scale
         l.color = slot_f3(p["color"], mk3(1.f));
         l.scale *= normalize_color(l.color);
@@ -434,7 +446,7 @@ LightHost parse_light(const Json &desc, const std::string &base_dir) {
 const Json *slot_image_node(const Json &param, const char *key) {
     const Json &j = param[key];
     if (j.is_null()) return nullptr;
-    const Json *node = (j.is_object() && j.contains("channels")) ? &j["node"] : &j;
+    const Json *node = (j.is_object() && (j.contains("channels") || j.contains("node"))) ? &j["node"] : &j;
     if (!node->is_object()) return nullptr;
     if ((*node)["type"].as_string("") == "image" || (*node)["param"].contains("fn") || node->contains("fn"))
         return node;
@@ -444,7 +456,8 @@ const Json *slot_image_node(const Json &param, const char *key) {
 namespace {
 
 // resolved path, colour space and the registry key of an image node
-void texture_key(const SceneHost &s, const Json &node, std::string &path, std::string &cs, std::string &key) {
+void texture_key(const SceneHost &s, const Json &node, std::string &path, std::string &cs, std::string &key,
+                 bool illum = false) {
     const Json &p = node.contains("param") ? node["param"] : node;
     std::string fn = p["fn"].as_string("");
     if (fn.empty()) throw std::runtime_error("image node without \"fn\"");
@@ -453,7 +466,7 @@ void texture_key(const SceneHost &s, const Json &node, std::string &path, std::s
     std::string e = image_extension(path);
     if (cs.empty()) cs = (e == "exr" || e == "hdr" || e == "pfm") ? "linear" : "srgb";
     if (cs != "linear" && cs != "srgb") throw std::runtime_error("unsupported colour space " + cs + " for " + fn);
-    key = path + "|" + cs;
+    key = path + "|" + cs + (illum ? "|illum" : "");
 }
 
 }// namespace
@@ -466,14 +479,15 @@ uint32_t find_texture(const SceneHost &s, const Json &node) {
     throw std::logic_error("texture not loaded: " + path);
 }
 
-uint32_t texture_index(SceneHost &s, const Json &node) {
+uint32_t texture_index(SceneHost &s, const Json &node, bool illum) {
     std::string path, cs, key;
-    texture_key(s, node, path, cs, key);
+    texture_key(s, node, path, cs, key, illum);
     for (size_t i = 0; i < s.textures.size(); ++i)
         if (s.textures[i].key == key) return uint32_t(i);
     ImageRGBA img = read_image(path);
     TextureHost t;
     t.key = key;
+    t.illum = illum;
     t.width = img.width;
     t.height = img.height;
     t.rgb.resize(size_t(img.width) * img.height * 3);
@@ -612,7 +626,7 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         const Json &lights = p["lights"];
         if (lights.is_array())
             for (const Json &l : lights.elements()) {
-                LightHost lh = parse_light(l, base_dir);
+                LightHost lh = parse_light(l, base_dir, s);
                 // an area light of its own brings its emitter: a width x
                 // height quad placed by "o2w" (AreaLight::init_geometry,
                 // src/render_core/lights/area.cpp:57-68)
@@ -719,7 +733,7 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                     }
                 }
                 if (p.contains("emission")) {
-                    LightHost l = parse_light(p["emission"], base_dir);
+                    LightHost l = parse_light(p["emission"], base_dir, s);
                     if (l.type != LIGHT_AREA) throw std::runtime_error("shape emission must be an area light");
                     l.inst = uint32_t(s.instances.size());
                     inst.light = int32_t(s.lights.size());

@@ -40,15 +40,17 @@ struct MaterialHost {
 // an image used as a colour parameter (ImageNode, src/render_core/shadernodes/image.cpp):
 // linear RGB rows, top first
 struct TextureHost {
-    std::string key;// resolved path + "|" + colour space
+    std::string key;// resolved path + "|" + colour space (+ "|illum")
     uint32_t width{0}, height{0};
+    bool illum{false};// an emission image (illumination spectra), not an albedo
     std::vector<float> rgb;
 };
 
 struct LightHost {
     uint32_t type{LIGHT_AREA};
     std::string name;
-    f3 color{1, 1, 1};// normalized by its max component (light.cpp:10-17)
+    f3 color{1, 1, 1};// normalized by its max component (light.cpp:10-17); an image's average
+    int32_t tex{-1};  // area: emission image, index into SceneHost::textures
     float scale{1.f};
     uint32_t two_sided{0};
     uint32_t inst{0xFFFFFFFFu};
@@ -157,7 +159,7 @@ std::string join_path(const std::string &dir, const std::string &fn);
 // it on first use); colour_space "srgb" decodes 8/16-bit values, "linear"
 // keeps them, unset means linear for .exr / .hdr / .pfm and sRGB otherwise
 // (ImagePool::load_texture, src/base/mgr/image_pool.cpp:13-34)
-uint32_t texture_index(SceneHost &s, const Json &node);
+uint32_t texture_index(SceneHost &s, const Json &node, bool illum = false);
 // the index of an already loaded image node
 uint32_t find_texture(const SceneHost &s, const Json &node);
 // the image node of a parameter slot, or nullptr for constants

@@ -279,11 +279,15 @@ PackedScene pack_scene(const SceneHost &s) {
     out.cie_y_integral = cie.y_integral;
     for (const MaterialHost &m : s.materials) out.mats.push_back(pack_material(m, out.rgb, out.spd, s));
     for (const TextureHost &t : s.textures) {
-        out.tex_info.insert(out.tex_info.end(), {uint32_t(out.tex.size() / 4), t.width, t.height, 0u});
+        out.tex_info.insert(out.tex_info.end(), {uint32_t(out.tex.size() / 4), t.width, t.height, t.illum ? 1u : 0u});
         for (size_t i = 0; i < size_t(t.width) * t.height; ++i) {
             f3 c = mk3(t.rgb[3 * i], t.rgb[3 * i + 1], t.rgb[3 * i + 2]);
             if (out.rgb) {
                 out.tex.insert(out.tex.end(), {c.x, c.y, c.z, 0.f});
+            } else if (t.illum) {
+                // emission texels: RGBIlluminationSpectrum coefficients
+                f4 q = rgb_to_unbound_coeffs(c);
+                out.tex.insert(out.tex.end(), {q.x, q.y, q.z, q.w});
             } else {
                 f4 q = rgb_to_albedo_coeffs(c);
                 out.tex.insert(out.tex.end(), {q.x, q.y, q.z, q.w});
@@ -300,6 +304,7 @@ PackedScene pack_scene(const SceneHost &s) {
         r.type = l.type;
         r.two_sided = l.two_sided;
         r.inst = l.inst;
+        r.tex = l.tex >= 0 ? uint32_t(l.tex) : 0xFFFFFFFFu;
         r.color[0] = l.color.x;
         r.color[1] = l.color.y;
         r.color[2] = l.color.z;
