@@ -411,8 +411,10 @@ def nrc_main(args) -> None:
         return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
     ref = ctx.load_text(scene).build_accel()
-    for _ in range(args.nrc_ref_spp // 512):
-        ref.render(512, fused=True)
+    left = args.nrc_ref_spp
+    while left > 0:
+        ref.render(min(512, left), fused=True)
+        left -= min(512, left)
     R = ref.film(0)
     ref.close()
     plain = ctx.load_text(scene).build_accel()
