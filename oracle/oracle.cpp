@@ -132,9 +132,9 @@ float spd_eval(const float *tab, float interval, float lambda) {
 }
 
 // hero.cpp:15-25
-float sample_visible_wavelength(float u) { return 538.f - 138.888889f * std::atanh(0.85691062f - 1.82750197f * u); }
+float sample_visible_wavelength(float u) { return 538.f - 138.888889f * vs::em::atanh(0.85691062f - 1.82750197f * u); }
 float visible_wavelength_pdf(float lambda) {
-    float c = std::cosh(0.0072f * (lambda - 538.f));
+    float c = vs::em::cosh(0.0072f * (lambda - 538.f));
     return 0.0039398042f / (c * c);
 }
 
@@ -1058,8 +1058,8 @@ V3 m3(const float *m, V3 v) {
 
 // SphericalMap::calculate_uv (spherical.cpp:62-65) with spherical_theta/phi
 void env_uv(V3 local, float &u, float &v) {
-    float theta = std::acos(std::min(std::max(local.z, -1.f), 1.f));
-    float phi = std::atan2(local.y, local.x);
+    float theta = vs::em::acos(std::min(std::max(local.z, -1.f), 1.f));
+    float phi = vs::em::atan2(local.y, local.x);
     if (phi < 0.f) phi += 2.f * kPi;
     u = phi * (0.5f * kInvPi);
     v = theta * kInvPi;
@@ -1083,8 +1083,8 @@ float env_map_pdf(const EnvMap &e, float u, float v) {
 // SphericalMap::evaluate_wi (spherical.cpp:76-93) for a world direction
 void env_eval(const Spectral &sd, const EnvMap &e, V3 dir, const Swl &w, Spec &L, float &pdf) {
     V3 local = m3(e.w2o, dir);
-    float theta = std::acos(std::min(std::max(local.z, -1.f), 1.f));
-    float sin_t = std::sin(theta);
+    float theta = vs::em::acos(std::min(std::max(local.z, -1.f), 1.f));
+    float sin_t = vs::em::sin(theta);
     float u, v;
     env_uv(local, u, v);
     L = env_L(sd, e, local, w);
@@ -1168,8 +1168,10 @@ LSample sample_light(const Spectral &sd, const Scene &s, const LightTables &lt, 
         float pdf_u = integral_u > 0.f ? row.func[iu] / integral_u : 0.f;
         float pdf_map = pdf_u * pdf_v;
         float theta = fv * kPi, phi = fu * 2.f * kPi;
-        float sin_t = std::sin(theta), cos_t = std::cos(theta);
-        V3 local = v3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+        float sin_t, cos_t, sin_p, cos_p;
+        vs::em::sincos(theta, &sin_t, &cos_t);
+        vs::em::sincos(phi, &sin_p, &cos_p);
+        V3 local = v3(sin_t * cos_p, sin_t * sin_p, cos_t);
         V3 world = unit(m3(e.o2w, local));
         float pdf_dir = pdf_map / (2.f * kPi * kPi * sin_t);
         ls.pdf = std::isinf(pdf_dir) ? 0.f : pdf_dir;
@@ -1685,9 +1687,10 @@ Bsdf bsdf_at(const Spectral &sd, const Scene &s, const Surf &it, Swl &w, bool ch
 // env map coordinate -> direction (SphericalMap::evaluate, spherical.cpp:104-124)
 void env_dir(const EnvMap &e, float u, float v, V3 &local, V3 &world, float &sin_t) {
     float theta = v * kPi, phi = u * 2.f * kPi;
-    sin_t = std::sin(theta);
-    float cos_t = std::cos(theta);
-    local = v3(sin_t * std::cos(phi), sin_t * std::sin(phi), cos_t);
+    float cos_t, sin_p, cos_p;
+    vs::em::sincos(theta, &sin_t, &cos_t);
+    vs::em::sincos(phi, &sin_p, &cos_p);
+    local = v3(sin_t * cos_p, sin_t * sin_p, cos_t);
     world = unit(m3(e.o2w, local));
 }
 // 2D alias sample of the env map (alias2d.cpp:106-129)
@@ -2720,13 +2723,13 @@ static float tone_map(float v, uint32_t tm) {
 }
 // sRGB encoding (IEC 61966-2-1), the linear_to_srgb of Postprocessor::compile_tone_mapping
 static float encode_srgb(float x) {
-    return x <= 0.0031308f ? 12.92f * x : 1.055f * std::pow(x, 1.f / 2.4f) - 0.055f;
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * vs::em::pow(x, 1.f / 2.4f) - 0.055f;
 }
 // the collector's output buffer from its running mean:
 // tone_map(1 - exp(-v * exposure)), w = 1 (radiance_collector.cpp:24-26, normal.cpp:96-99)
 void or_film_output(const float *rt, uint32_t n, float exposure, uint32_t tm, float *out) {
     for (uint32_t i = 0; i < n; ++i) {
-        for (int c = 0; c < 3; ++c) out[4 * i + c] = tone_map(1.f - std::exp(-rt[4 * i + c] * exposure), tm);
+        for (int c = 0; c < 3; ++c) out[4 * i + c] = tone_map(1.f - vs::em::exp(-rt[4 * i + c] * exposure), tm);
         out[4 * i + 3] = 1.f;
     }
 }
@@ -2740,6 +2743,49 @@ void or_final_picture(const float *out, uint32_t n, uint32_t tm, int32_t gamma, 
             pic[4 * i + c] = gamma ? encode_srgb(v) : v;
         }
         pic[4 * i + 3] = 1.f;
+    }
+}
+
+
+// Camera rays of pixel (px, py) for frames frame0 .. frame0 + n - 1 (the
+// sensor sample drawn first on each frame's stream, as the path tracer does):
+// origins o[3n], directions d[3n] and film positions pf[2n].
+void or_camera_rays(const float *cam, uint32_t px, uint32_t py, uint32_t frame0, uint32_t n, float *o, float *d,
+                    float *pf) {
+    Scene s{};
+    s.cam = cam;
+    FisTable fis = fis_build(cam);
+    for (uint32_t i = 0; i < n; ++i) {
+        Sampler sp;
+        sp.start(px, py, frame0 + i, 0);
+        V3 ro, rd;
+        float w;
+        camera_ray(s, fis, px, py, sp, ro, rd, w, pf + 2 * i);
+        o[3 * i] = ro.x, o[3 * i + 1] = ro.y, o[3 * i + 2] = ro.z;
+        d[3 * i] = rd.x, d[3 * i + 1] = rd.y, d[3 * i + 2] = rd.z;
+    }
+}
+
+// The shared elementary functions over arrays, for their accuracy tests:
+// fn 0 sin, 1 cos, 2 exp, 3 log, 4 log1p, 5 atanh, 6 cosh, 7 acos,
+// 8 atan2(x, y), 9 pow(x, y).
+void or_elementary(int fn, const float *x, const float *y, uint32_t n, float *out) {
+    namespace em = vs::em;
+    for (uint32_t i = 0; i < n; ++i) {
+        const float a = x[i];
+        switch (fn) {
+        case 0: out[i] = em::sin(a); break;
+        case 1: out[i] = em::cos(a); break;
+        case 2: out[i] = em::exp(a); break;
+        case 3: out[i] = em::log(a); break;
+        case 4: out[i] = em::log1p(a); break;
+        case 5: out[i] = em::atanh(a); break;
+        case 6: out[i] = em::cosh(a); break;
+        case 7: out[i] = em::acos(a); break;
+        case 8: out[i] = em::atan2(a, y[i]); break;
+        case 9: out[i] = em::pow(a, y[i]); break;
+        default: out[i] = 0.f;
+        }
     }
 }
 

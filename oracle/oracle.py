@@ -49,6 +49,8 @@ def lib():
         h.or_offset_ray_origin.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
         h.or_filter_eval.argtypes = [_f32p, _f32p, C.c_uint32, _f32p]
         h.or_film_output.argtypes = [_f32p, C.c_uint32, C.c_float, C.c_uint32, _f32p]
+        h.or_elementary.argtypes = [C.c_int, _f32p, _f32p, C.c_uint32, _f32p]
+        h.or_camera_rays.argtypes = [_f32p] + [C.c_uint32] * 4 + [_f32p, _f32p, _f32p]
         _f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
         h.or_nrc_levels.argtypes = [C.c_uint32, C.c_uint32, _u32p, C.POINTER(C.c_uint32)]
         h.or_nrc_encode.argtypes = [C.c_void_p, _f32p, _f32p, C.c_uint32, C.c_int32, _f64p]
@@ -571,3 +573,29 @@ def nrc_adam(p, m, v, g, lr: float, t: int):
     c1 = 1.0 / (1.0 - 0.9 ** t)
     c2 = 1.0 / (1.0 - 0.99 ** t)
     lib().or_nrc_adam(p, m, v, g, len(p), lr, c1, c2)
+
+
+ELEMENTARY = {"sin": 0, "cos": 1, "exp": 2, "log": 3, "log1p": 4, "atanh": 5, "cosh": 6, "acos": 7, "atan2": 8,
+              "pow": 9}
+
+
+def elementary(name: str, x, y=None) -> np.ndarray:
+    """The library's shared single-precision elementary function `name` over
+    float32 arrays (atan2 takes (y, x) as (x, y) here: elementary("atan2",
+    y, x); pow takes (base, exponent))."""
+    x = np.ascontiguousarray(x, np.float32).reshape(-1)
+    yy = np.zeros_like(x) if y is None else np.ascontiguousarray(np.broadcast_to(y, x.shape), np.float32)
+    out = np.empty_like(x)
+    lib().or_elementary(ELEMENTARY[name], x, yy, x.size, out)
+    return out
+
+
+def camera_rays(cam: np.ndarray, px: int, py: int, frame0: int, n: int):
+    """World-space origins, directions (float32[n, 3]) and film positions
+    (float32[n, 2]) of pixel (px, py)'s camera rays on frames frame0..+n."""
+    cam = np.ascontiguousarray(cam, np.float32).reshape(-1)
+    o = np.zeros((n, 3), np.float32)
+    d = np.zeros((n, 3), np.float32)
+    pf = np.zeros((n, 2), np.float32)
+    lib().or_camera_rays(cam, px, py, frame0, n, o, d, pf)
+    return o, d, pf

@@ -12,6 +12,11 @@
 #include <thread>
 #include <vector>
 
+// The elementary functions shared with the HIP kernels: sampling decisions
+// depend on their last bits, so both sides evaluate the same operation
+// sequence (tests/test_elementary.py pins them against mpmath).
+#include "../vision_amd/csrc/elementary.h"
+
 namespace {
 
 constexpr float kPi = 3.14159265358979323846f;
@@ -63,8 +68,10 @@ struct Sampler {
 void square_to_disk(float ux, float uy, float &x, float &y) {
     float r = std::sqrt(ux);
     float th = 2.f * kPi * uy;
-    x = r * std::cos(th);
-    y = r * std::sin(th);
+    float sn, cs;
+    vs::em::sincos(th, &sn, &cs);
+    x = r * cs;
+    y = r * sn;
 }
 V3 cosine_hemisphere(float ux, float uy) {
     float x, y;

@@ -264,3 +264,31 @@ def test_emission_image_parity(gpu_ctx, tmp_path, spectrum):
     print(f"emission image {spectrum}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
     assert frac >= 0.999
     sc.close()
+
+
+@pytest.mark.parametrize("mode", ["megakernel", "wavefront"])
+@pytest.mark.parametrize("radius,focal", [(0.15, 6.0), (0.4, 5.2)])
+def test_thin_lens_parity(gpu_ctx, mode, radius, focal):
+    """ThinLensCamera (thin_lens.cpp:34-42): lens samples on the same sampler
+    streams as the oracle, in both kernel forms."""
+    scene = scenegen.cornell_box(96, 96, max_depth=6)
+    scene["camera"]["type"] = "thin_lens"
+    scene["camera"]["param"]["lens_radius"] = radius
+    scene["camera"]["param"]["focal_distance"] = focal
+    print(f"case: thin lens r {radius} focal {focal} {mode}", flush=True)
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    sc.set_kernel(mode, chunk_frames=2)
+    sc.render(3)
+    gpu, ref = sc.film(0), oracle.render(sc.export(), 0, 3)
+    frac, _ = _compare(gpu, ref)
+    print(f"thin lens {radius}/{focal} {mode}: {frac * 100:.4f}% of pixels within {REL_TOL} relative")
+    assert frac >= 0.9999
+    # defocus changes the picture: the same scene through the pinhole differs
+    scene["camera"]["type"] = "pinhole"
+    pin = gpu_ctx.load_text(scene)
+    pin.build_accel()
+    pin.render(3)
+    assert np.abs(pin.film(0)[..., :3] - gpu[..., :3]).max() > 1e-2
+    sc.close()
+    pin.close()

@@ -177,3 +177,86 @@ def test_direct_only_bounce_needs_both_strategies():
 
 def test_zero_depth_is_black():
     assert np.all(_oracle_film(_direct_only_scene(0), 2)[..., :3] == 0)
+
+
+# ------------------------------------------------------------ thin lens
+# ThinLensCamera::generate_ray_in_local_space (thin_lens.cpp:34-42): the
+# pinhole ray is re-aimed from a point of the lens disk, square_to_disk(u)
+# scaled by lens_radius, at the point where it crosses the plane of focus
+# (z = focal_distance in camera space).
+
+def _lens_box(radius, focal, filter_radius=0.5, w=32, h=32):
+    s = scenegen.cornell_box(w, h, spp=4, max_depth=3)
+    s["camera"]["type"] = "thin_lens"
+    s["camera"]["param"]["lens_radius"] = radius
+    s["camera"]["param"]["focal_distance"] = focal
+    s["camera"]["param"]["filter"]["param"]["radius"] = filter_radius
+    return s
+
+
+def _export(scene):
+    with va.Context(-1) as ctx:
+        return ctx.load_text(scene).build_accel().export()
+
+
+def _cam_local(cam, o, d):
+    """camera-space origins and directions from the world-space rays"""
+    m = np.eye(4)
+    m[:3, :] = np.asarray(cam[:12], np.float64).reshape(3, 4)
+    inv = np.linalg.inv(m)
+    oc = (inv[:3, :3] @ o.astype(np.float64).T).T + inv[:3, 3]
+    dc = (inv[:3, :3] @ d.astype(np.float64).T).T
+    return oc, dc
+
+
+def test_thin_lens_of_zero_radius_is_the_pinhole():
+    pin = scenegen.cornell_box(32, 32, spp=4, max_depth=3)
+    lens = _lens_box(0.0, 4.0)
+    a, b = _export(pin), _export(lens)
+    for px, py in ((3, 5), (16, 16), (31, 0)):
+        oa, da, fa = oracle.camera_rays(a["camera"], px, py, 0, 64)
+        ob, db, fb = oracle.camera_rays(b["camera"], px, py, 0, 64)
+        np.testing.assert_array_equal(oa, ob)
+        np.testing.assert_array_equal(da, db)
+        np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(oracle.render(a, 0, 2), oracle.render(b, 0, 2))
+
+
+@pytest.mark.parametrize("radius,focal", [(0.1, 6.8), (0.3, 4.0)])
+def test_thin_lens_rays_meet_on_the_plane_of_focus(radius, focal):
+    # zero filter radius: every frame's sensor sample is the pixel centre,
+    # so every lens sample's ray passes through the pinhole ray's point at
+    # depth focal_distance, and starts on the lens disk (z = 0, |xy| <= r)
+    lens = _export(_lens_box(radius, focal, filter_radius=0.0))
+    pin = _export(_lens_box(0.0, focal, filter_radius=0.0))
+    for px, py in ((2, 30), (16, 16), (29, 7)):
+        o, d, pf = oracle.camera_rays(lens["camera"], px, py, 0, 512)
+        np.testing.assert_array_equal(pf, np.tile([px + 0.5, py + 0.5], (512, 1)).astype(np.float32))
+        oc, dc = _cam_local(lens["camera"], o, d)
+        assert np.abs(oc[:, 2]).max() < 1e-5
+        rr = np.hypot(oc[:, 0], oc[:, 1])
+        assert rr.max() <= radius * (1 + 1e-5) and rr.max() > 0.9 * radius
+        p = oc + ((focal - oc[:, 2]) / dc[:, 2])[:, None] * dc
+        o0, d0, _ = oracle.camera_rays(pin["camera"], px, py, 0, 1)
+        oc0, dc0 = _cam_local(pin["camera"], o0, d0)
+        p0 = oc0 + ((focal - oc0[:, 2]) / dc0[:, 2])[:, None] * dc0
+        assert np.abs(oc0).max() < 1e-5
+        np.testing.assert_allclose(p, np.tile(p0, (512, 1)), atol=2e-5 * focal)
+
+
+def test_thin_lens_disk_samples_are_uniform():
+    from scipy import stats
+    radius = 0.25
+    lens = _export(_lens_box(radius, 5.0))
+    n = 60_000
+    o, d, _ = oracle.camera_rays(lens["camera"], 11, 19, 0, n)
+    oc, _ = _cam_local(lens["camera"], o, d)
+    r2 = (oc[:, 0] ** 2 + oc[:, 1] ** 2) / radius ** 2
+    phi = np.mod(np.arctan2(oc[:, 1], oc[:, 0]), 2 * np.pi)
+    # equal-area cells: r^2 and phi are uniform for a uniform disk density
+    ir = np.minimum((r2 * 10).astype(int), 9)
+    ip = np.minimum((phi / (2 * np.pi) * 12).astype(int), 11)
+    obs = np.bincount(ir * 12 + ip, minlength=120)
+    chi = ((obs - n / 120) ** 2 / (n / 120)).sum()
+    assert stats.chi2.sf(chi, 119) > 1e-3, chi
+    assert r2.max() <= 1 + 1e-5

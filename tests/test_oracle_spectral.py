@@ -562,3 +562,190 @@ def test_mirror_reflects_its_tint_with_compensation(rough, mu):
     fc, _ = oracle.bsdf_eval(spec, _mat_metal(rough, eta=(1.0, 1.0, 1.0), k=(1e4, 1e4, 1e4)),
                              np.tile(wo1, (len(d), 1)), d)
     np.testing.assert_allclose(fw[:, :3], fc[:, :3], rtol=1e-5, atol=1e-12)
+
+
+# ------------------------------------------------------------ principled stack reductions
+# PrincipledBSDF::create_lobe_set (principled_bsdf.cpp:352-461) stacks sheen,
+# coat, metallic, transmission, specular and diffuse lobes with
+# layering_weight between them.  Each test below switches the stack down to
+# one lobe whose closed form is known independently, so a wrong weight, a
+# transposed table axis or a wrong Fresnel term fails it.
+
+def _f32sq(r):
+    r = np.float32(r)
+    return float(r * r)
+
+
+def _mat_mirror_alpha(alpha, color=(1, 1, 1)):
+    return oracle.material_record(6, [*color, 0.0, 0.0, 0.0, alpha, alpha])
+
+
+def _upper_pairs(n, seed):
+    rng = np.random.default_rng(seed)
+
+    def dirs():
+        z = rng.uniform(0.05, 1.0, n)
+        phi = rng.uniform(0, 2 * np.pi, n)
+        s = np.sqrt(1 - z * z)
+        return np.stack([s * np.cos(phi), s * np.sin(phi), z], 1).astype(np.float32)
+
+    return dirs(), dirs()
+
+
+def _metal_only(color, tint, rough):
+    return _mat_principled(color=color, spec_tint=tint, metallic=1.0, roughness=rough, sheen_weight=0.0,
+                           coat_weight=0.0, transmission_weight=0.0)
+
+
+@pytest.mark.parametrize("rough", [0.3, 0.7])
+def test_principled_metallic_white_is_the_compensated_ggx_mirror(rough):
+    # metallic = 1, everything else 0: one MetallicLobe remains; white colour
+    # and tint give F82 = 1, so the stack equals the white mirror (F = 1,
+    # same GGX, same pure-reflection compensation), pdf included
+    spec = _rgb_spec()
+    wo, wi = _upper_pairs(20000, 1)
+    f, pdf = oracle.bsdf_eval(spec, _metal_only((1, 1, 1), (1, 1, 1), rough), wo, wi)
+    fm, pdfm = oracle.bsdf_eval(spec, _mat_mirror_alpha(_f32sq(rough)), wo, wi)
+    assert fm[:, :3].max() > 0
+    np.testing.assert_allclose(f[:, :3], fm[:, :3], rtol=2e-6, atol=1e-12)
+    np.testing.assert_allclose(pdf, pdfm, rtol=2e-6, atol=1e-12)
+
+
+def _f82_ratio(color, tint, rough, wo, wi):
+    spec = _rgb_spec()
+    f, _ = oracle.bsdf_eval(spec, _metal_only(color, tint, rough), wo, wi)
+    fm, _ = oracle.bsdf_eval(spec, _mat_mirror_alpha(_f32sq(rough)), wo, wi)
+    keep = fm[:, 0] > 1e-6
+    h = wo.astype(np.float64) + wi.astype(np.float64)
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    cos_oh = np.abs(np.sum(wo * h, 1))
+    return (f[keep, :3] / fm[keep, :3]).astype(np.float64), cos_oh[keep]
+
+
+@pytest.mark.parametrize("color", [(0.9, 0.6, 0.2), (0.05, 0.3, 0.8)])
+def test_principled_f82_at_unit_tint_is_schlick(color):
+    # FresnelF82Tint (fresnel.h:98-130) with F82 = 1 has B = 0: Schlick's
+    # F0 + (1 - F0)(1 - cos)^5 with F0 = the base colour
+    wo, wi = _upper_pairs(20000, 2)
+    ratio, c = _f82_ratio(color, (1, 1, 1), 0.5, wo, wi)
+    c0 = np.array(color)
+    want = c0[None, :] + (1 - c0[None, :]) * (1 - c[:, None]) ** 5
+    np.testing.assert_allclose(ratio, want, rtol=3e-5)
+
+
+@pytest.mark.parametrize("color,tint", [((0.9, 0.6, 0.2), (0.5, 0.7, 0.9)), ((0.2, 0.4, 0.95), (1.0, 0.3, 0.6))])
+def test_principled_f82_tint_at_cos_one_seventh(color, tint):
+    # by construction F82(cos = 1/7) = tint * Schlick(1/7); F82(1) = colour
+    c0, t0 = np.array(color), np.array(tint)
+    schlick = lambda c: c0 + (1 - c0) * (1 - c) ** 5
+    for cos_target, want in ((1 / 7, t0 * schlick(1 / 7)), (0.999, None)):
+        s = math.sqrt(1 - cos_target * cos_target)
+        # wh = +z, wo and wi mirror images: |wo . wh| = cos_target
+        wo = np.array([[s, 0.0, cos_target]], np.float32)
+        wi = np.array([[-s, 0.0, cos_target]], np.float32)
+        ratio, c = _f82_ratio(color, tint, 0.5, wo, wi)
+        if want is None:
+            # near normal incidence the tint term vanishes as (1 - cos)^6
+            np.testing.assert_allclose(ratio[0], c0, rtol=1e-4)
+        else:
+            assert abs(c[0] - 1 / 7) < 1e-6
+            np.testing.assert_allclose(ratio[0], want, rtol=2e-5)
+
+
+def _sheen_table():
+    return oracle.data("lut_sheen_approx.f32", 4096).reshape(32, 32, 4).astype(np.float64)
+
+
+@pytest.mark.parametrize("i,j", [(6, 20), (20, 6), (27, 11), (11, 27)])
+def test_principled_sheen_alone_reflects_the_ltc_albedo(i, j):
+    # sheen only (colour 0 turns off diffuse and transmission; ior 1 gives
+    # the specular lobe zero Fresnel): its LTC is normalised over the upper
+    # hemisphere, so f|cos| integrates to tint * R with R the table's third
+    # channel at (cos theta_o, sheen roughness); the table's fast axis is
+    # cos theta (principled_bsdf.cpp:49-51, make_float2(cos_theta, alpha)).
+    # At texel centres the bilinear lookup returns the entry itself.
+    tab = _sheen_table()
+    cos_o, rough = (i + 0.5) / 32, (j + 0.5) / 32
+    R = tab[j, i, 2]
+    assert abs(R - tab[i, j, 2]) > 1e-3 * R  # a transposed lookup would read another value
+    tint = np.array([0.9, 0.6, 0.3])
+    rec = _mat_principled(color=(0, 0, 0), metallic=0.0, ior=1.0, sheen_weight=1.0, sheen_roughness=rough,
+                          sheen_tint=tuple(tint), coat_weight=0.0, transmission_weight=0.0)
+    wo1 = np.array([math.sqrt(1 - cos_o * cos_o), 0.0, cos_o], np.float32)
+    d, dw = _hemisphere_grid(1200, 600)
+    f, _ = oracle.bsdf_eval(_rgb_spec(), rec, np.tile(wo1, (len(d), 1)), d)
+    E = f[:, :3].astype(np.float64).sum(0) * dw
+    np.testing.assert_allclose(E, tint * R, rtol=0.01)
+
+
+def _coat_only(tint, rough, ior):
+    return _mat_principled(color=(0, 0, 0), metallic=0.0, ior=1.0, sheen_weight=0.0, coat_weight=1.0,
+                           coat_roughness=rough, coat_ior=ior, coat_tint=tint, transmission_weight=0.0)
+
+
+@pytest.mark.parametrize("rough,ior", [(0.3, 1.5), (0.6, 2.2)])
+def test_principled_coat_alone_is_a_dielectric_ggx_reflection(rough, ior):
+    # coat only: MicrofacetReflection with the exact dielectric Fresnel of
+    # the coat ior, GGX alpha = roughness^2, tinted, no compensation:
+    # f |cos i| = tint D G2 F / (4 cos o), textbook forms in float64
+    tint = np.array([0.8, 0.5, 0.9])
+    wo, wi = _upper_pairs(20000, 3)
+    f, _ = oracle.bsdf_eval(_rgb_spec(), _coat_only(tuple(tint), rough, ior), wo, wi)
+    a = _f32sq(rough)
+    a2 = a * a
+    o, i = wo.astype(np.float64), wi.astype(np.float64)
+    h = o + i
+    h /= np.linalg.norm(h, axis=1, keepdims=True)
+    D = a2 / (np.pi * ((a2 - 1) * h[:, 2] ** 2 + 1) ** 2)
+    lam = lambda w: 0.5 * (np.sqrt(1 + a2 * (1 - w[:, 2] ** 2) / w[:, 2] ** 2) - 1)
+    G2 = 1 / (1 + lam(o) + lam(i))
+    c = np.sum(o * h, 1)
+    g = np.sqrt(ior * ior - 1 + c * c)
+    F = 0.5 * ((g - c) / (g + c)) ** 2 * (1 + ((c * (g + c) - 1) / (c * (g - c) + 1)) ** 2)
+    want = tint[None, :] * (D * G2 * F / (4 * o[:, 2]))[:, None]
+    np.testing.assert_allclose(f[:, :3], want, rtol=2e-4, atol=1e-7)
+
+
+def test_principled_zero_coat_weight_leaves_the_stack_unchanged():
+    # with coat_weight 0 the coat's roughness, ior and tint have no effect
+    spec = _rgb_spec()
+    wo, wi = _upper_pairs(5000, 4)
+    a = _mat_principled(coat_weight=0.0, coat_roughness=0.2, coat_ior=1.5, coat_tint=(1, 1, 1))
+    b = _mat_principled(coat_weight=0.0, coat_roughness=0.8, coat_ior=2.7, coat_tint=(0.1, 0.9, 0.3))
+    fa, pa = oracle.bsdf_eval(spec, a, wo, wi)
+    fb, pb = oracle.bsdf_eval(spec, b, wo, wi)
+    np.testing.assert_array_equal(fa, fb)
+    np.testing.assert_array_equal(pa, pb)
+    # and a nonzero coat does change it
+    fc, _ = oracle.bsdf_eval(spec, _mat_principled(coat_weight=0.5), wo, wi)
+    assert np.abs(fc - fa).max() > 1e-3
+
+
+@pytest.mark.parametrize("rough,ior", [(0.3, 1.5), (0.6, 1.33)])
+def test_principled_full_transmission_is_the_rough_dielectric(rough, ior):
+    # transmission 1 (metallic, sheen, coat 0): the weight below it is 0, so
+    # only the DielectricLobe remains; with spec_tint 1 its Schlick Fresnel
+    # F0 + t (1 - F0), t = (F - F0) / (1 - F0), is the exact dielectric F,
+    # so the stack equals the glass material of the same colour, ior and
+    # alpha, over reflection and transmission, entering from either side
+    # (inside, the lobe's eta and Schlick F0 use 1 / ior)
+    spec = _rgb_spec()
+    color = (0.9, 0.7, 0.5)
+    rec = _mat_principled(color=color, spec_tint=(1, 1, 1), metallic=0.0, sheen_weight=0.0, coat_weight=0.0,
+                          transmission_weight=1.0, roughness=rough, ior=ior)
+    a = _f32sq(rough)
+    glass = oracle.material_record(4, [*color, 0.0, ior, a, a])
+    rng = np.random.default_rng(5)
+    n = 20000
+    wo, _ = _upper_pairs(n, 6)
+    wo[n // 2:, 2] *= -1
+    z = rng.uniform(-1, 1, n)
+    phi = rng.uniform(0, 2 * np.pi, n)
+    s = np.sqrt(1 - z * z)
+    wi = np.stack([s * np.cos(phi), s * np.sin(phi), z], 1).astype(np.float32)
+    f, pdf = oracle.bsdf_eval(spec, rec, wo, wi)
+    fg, pdfg = oracle.bsdf_eval(spec, glass, wo, wi)
+    assert (fg[:, 0] > 0).sum() > n // 2 and (fg[wi[:, 2] * wo[:, 2] < 0, 0] > 0).any()
+    assert (fg[n // 2:, 0] > 0).sum() > n // 8
+    np.testing.assert_allclose(f[:, :3], fg[:, :3], rtol=2e-5, atol=1e-9)
+    np.testing.assert_allclose(pdf, pdfg, rtol=2e-5, atol=1e-9)

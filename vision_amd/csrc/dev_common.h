@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "records.h"
+#include "elementary.h"
 #include "vmath.h"
 
 namespace vs {
@@ -50,7 +51,7 @@ __device__ __forceinline__ f2 square_to_disk(f2 u) {
     float r = sqrtf(u.x);
     float theta = 2.f * kPi * u.y;
     float s, c;
-    sincosf(theta, &s, &c);
+    em::sincos(theta, &s, &c);
     return mk2(r * c, r * s);
 }
 __device__ __forceinline__ f3 square_to_cosine_hemisphere(f2 u) {

@@ -138,8 +138,8 @@ __device__ __forceinline__ f3 mat3_apply(const float *m, f3 v) {
                m[6] * v.x + m[7] * v.y + m[8] * v.z);
 }
 __device__ __forceinline__ f2 spherical_uv(f3 local) {
-    float theta = acosf(fminf(fmaxf(local.z, -1.f), 1.f));
-    float phi = atan2f(local.y, local.x);
+    float theta = em::acos(fminf(fmaxf(local.z, -1.f), 1.f));
+    float phi = em::atan2(local.y, local.x);
     if (phi < 0.f) phi += 2.f * kPi;
     return mk2(phi * (0.5f * kInvPi), theta * kInvPi);
 }
@@ -160,8 +160,8 @@ __device__ __forceinline__ float env_map_pdf(const SceneView &sc, f2 uv) {
 template <int NS, bool RGB>
 __device__ __forceinline__ void env_eval(const SceneView &sc, f3 dir, const Swl<NS> &swl, Spec<NS> &L, float &pdf) {
     f3 local = mat3_apply(sc.env_w2o, dir);
-    float theta = acosf(fminf(fmaxf(local.z, -1.f), 1.f));
-    float sin_t = sinf(theta);
+    float theta = em::acos(fminf(fmaxf(local.z, -1.f), 1.f));
+    float sin_t = em::sin(theta);
     f2 uv = spherical_uv(local);
     L = env_texel<NS, RGB>(sc, uv, swl);
     float p = env_map_pdf(sc, uv) / (2.f * kPi * kPi * sin_t);
@@ -225,9 +225,10 @@ __device__ __forceinline__ LightSampleN<NS> sample_light(const SceneView &sc, f3
         float pdf_map;
         f2 fuv = env_sample_uv(sc, u, pdf_map);
         float theta = fuv.y * kPi, phi = fuv.x * 2.f * kPi;
-        float sin_t = sinf(theta), cos_t = cosf(theta);
+        float sin_t, cos_t;
+        em::sincos(theta, &sin_t, &cos_t);
         float sp, cp;
-        sincosf(phi, &sp, &cp);
+        em::sincos(phi, &sp, &cp);
         f3 local = mk3(sin_t * cp, sin_t * sp, cos_t);
         f3 world = normalize(mat3_apply(sc.env_o2w, local));
         float pdf_dir = pdf_map / (2.f * kPi * kPi * sin_t);
@@ -583,7 +584,7 @@ __device__ __forceinline__ float tone_map(float v, uint32_t tm) {
 // sRGB transfer function (IEC 61966-2-1): linear segment below 0.0031308,
 // 1.055 x^(1/2.4) - 0.055 above
 __device__ __forceinline__ float linear_to_srgb(float x) {
-    return x <= 0.0031308f ? 12.92f * x : 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+    return x <= 0.0031308f ? 12.92f * x : 1.055f * em::pow(x, 1.f / 2.4f) - 0.055f;
 }
 
 // one sample into the film's running mean (NormalRadianceCollector::add_sample,
@@ -599,9 +600,9 @@ __device__ __forceinline__ float4 film_add(const RenderView &rv, float4 acc, f3 
 // the output buffer: tone_map(1 - exp(-v * exposure)) (radiance_collector.cpp:24-26)
 __device__ __forceinline__ float4 film_output(const RenderView &rv, float4 acc) {
     float4 o4;
-    o4.x = tone_map(1.f - expf(-acc.x * rv.exposure), rv.tone_mapper);
-    o4.y = tone_map(1.f - expf(-acc.y * rv.exposure), rv.tone_mapper);
-    o4.z = tone_map(1.f - expf(-acc.z * rv.exposure), rv.tone_mapper);
+    o4.x = tone_map(1.f - em::exp(-acc.x * rv.exposure), rv.tone_mapper);
+    o4.y = tone_map(1.f - em::exp(-acc.y * rv.exposure), rv.tone_mapper);
+    o4.z = tone_map(1.f - em::exp(-acc.z * rv.exposure), rv.tone_mapper);
     o4.w = 1.f;
     return o4;
 }

@@ -313,10 +313,10 @@ __device__ __forceinline__ f3 robust_point(f3 pos, f3 ng, f3 dir, float factor) 
 // spherical.cpp:104-124)
 __device__ __forceinline__ void env_direction(const SceneView &sc, f2 uv, f3 &local, f3 &world, float &sin_t) {
     float theta = uv.y * kPi, phi = uv.x * 2.f * kPi;
-    sin_t = sinf(theta);
-    float cos_t = cosf(theta);
+    float cos_t;
+    em::sincos(theta, &sin_t, &cos_t);
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    em::sincos(phi, &sp, &cp);
     local = mk3(sin_t * cp, sin_t * sp, cos_t);
     world = normalize(mat3_apply(sc.env_o2w, local));
 }

@@ -90,10 +90,10 @@ __device__ __forceinline__ float spd_eval(const float *tab, float interval, floa
 
 // hero.cpp:15-25 (visible-wavelength importance sampling of pbrt-v4)
 __device__ __forceinline__ float sample_visible_wavelength(float u) {
-    return 538.f - 138.888889f * atanhf(0.85691062f - 1.82750197f * u);
+    return 538.f - 138.888889f * em::atanh(0.85691062f - 1.82750197f * u);
 }
 __device__ __forceinline__ float visible_wavelength_pdf(float lambda) {
-    float c = coshf(0.0072f * (lambda - 538.f));
+    float c = em::cosh(0.0072f * (lambda - 538.f));
     return 0.0039398042f / (c * c);
 }
 

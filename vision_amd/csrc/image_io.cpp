@@ -56,7 +56,7 @@ struct Reader {
     size_t pos{0};
     const std::string &path;
     void need(size_t n) const {
-        if (pos + n > d.size()) throw std::runtime_error("truncated image " + path);
+        if (pos > d.size() || n > d.size() - pos) throw std::runtime_error("truncated image " + path);
     }
     uint32_t u32le() {
         need(4);
@@ -221,10 +221,12 @@ ImageRGBA decode_exr(const std::vector<uint8_t> &d, const std::string &path) {
                 chans.push_back({cn, pt});
             }
         } else if (name == "compression") {
+            if (size < 1) throw std::runtime_error("bad EXR compression attribute in " + path);
             compression = d[r.pos];
         } else if (name == "dataWindow") {
             for (int i = 0; i < 4; ++i) box[i] = r.i32le();
         } else if (name == "lineOrder") {
+            if (size < 1) throw std::runtime_error("bad EXR lineOrder attribute in " + path);
             line_order = d[r.pos];
         }
         r.pos = end;
@@ -251,6 +253,7 @@ ImageRGBA decode_exr(const std::vector<uint8_t> &d, const std::string &path) {
     for (const Chan &c : chans) has_rgb |= (c.name == "R" || c.name == "G" || c.name == "B");
     std::vector<uint8_t> buf;
     for (uint32_t k = 0; k < chunks; ++k) {
+        if (offsets[k] >= d.size()) throw std::runtime_error("bad EXR chunk offset in " + path);
         Reader cr{d, size_t(offsets[k]), path};
         int32_t y0 = cr.i32le() - box[1];
         uint32_t packed = cr.u32le();

@@ -175,6 +175,7 @@ struct vs_server {
     std::unordered_map<uint64_t, std::unique_ptr<JobRec>> jobs;
     std::multimap<size_t, float *> pinned_pool;// released result buffers by size
     uint64_t next_id{1};
+    uint64_t submit_seq{0};// bumped by each submit (under mu): wakes a waiting worker
     bool stop{false};
     std::string fatal;
     // statistics (under mu)
@@ -379,10 +380,19 @@ void admit(vs_server *S) {
         }
         j->slot = uint32_t(free_slot - S->slot_job.begin());
         *free_slot = j;
-        const SceneSlot &sc = S->scenes[j->d.scene];
+        vs::CameraView cam;
+        uint32_t route;
+        {
+            // the scene table never reallocates (reserved at creation), but
+            // a route may change under the lock
+            std::lock_guard<std::mutex> lk(S->mu);
+            const SceneSlot &sc = S->scenes[j->d.scene];
+            cam = sc.cam;
+            route = sc.route[j->d.spectral ? 1 : 0];
+        }
         vs::ServeJob &sj = S->h_jobs[j->slot];
         std::memset(&sj, 0, sizeof(sj));
-        sj.cam = sc.cam;
+        sj.cam = cam;
         camera_set_resolution(sj.cam, j->d.width, j->d.height);
         sj.rv = j->rv;
         sj.rt = reinterpret_cast<float4 *>(static_cast<char *>(S->arena_mem.ptr) + off);
@@ -393,7 +403,7 @@ void admit(vs_server *S) {
         hip_check(hipMemcpyAsync(S->d_jobs.as<vs::ServeJob>() + j->slot, &sj, sizeof(sj), hipMemcpyHostToDevice,
                                  S->st[0]),
                   "job upload");
-        j->route = sc.route[j->d.spectral ? 1 : 0];
+        j->route = route;
         if (j->route == 1) {
             j->ev_ready = take_event(S);
             hip_check(hipEventRecord(j->ev_ready, S->st[0]), "ready event");
@@ -517,13 +527,20 @@ void fail_everything(vs_server *S, const std::string &msg) {
     S->copying.clear();
     S->inflight.clear();
     std::lock_guard<std::mutex> lk(S->mu);
-    while (!S->queue.empty()) {
-        JobRec *j = S->queue.front();
-        S->queue.pop_front();
+    // queued jobs and those waiting for the wavefront lane or for their
+    // retirement by this (ending) worker fail too; a job the lane is
+    // rendering now fails through vs_server_wait's check of `fatal`
+    auto fail = [&](JobRec *j) {
         j->state = VS_JOB_FAILED;
         j->error = msg;
         ++S->n_failed;
-    }
+    };
+    for (JobRec *j : S->queue) fail(j);
+    S->queue.clear();
+    for (JobRec *j : S->lane_queue) fail(j);
+    S->lane_queue.clear();
+    for (JobRec *j : S->lane_done) fail(j);
+    S->lane_done.clear();
     S->fatal = msg;
     S->cv_done.notify_all();
 }
@@ -559,16 +576,22 @@ void lane_main(vs_server *S) {
             S->lane_queue.pop_front();
         }
         try {
-            const SceneSlot &sc = S->scenes[j->d.scene];
+            vs::SceneView view;
+            vs::CameraView cam;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                const SceneSlot &sc = S->scenes[j->d.scene];
+                view = sc.view[j->d.spectral ? 1 : 0];
+                cam = sc.cam;
+            }
             const uint32_t px = j->d.width * j->d.height;
             float4 *rt = reinterpret_cast<float4 *>(static_cast<char *>(S->arena_mem.ptr) + j->off);
             float4 *out = rt + px;
-            vs::CameraView cam = sc.cam;
             camera_set_resolution(cam, j->d.width, j->d.height);
             // frames per chunk: up to 16 M paths of state at a time
             const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(16u, (16u << 20) / std::max(px, 1u)));
             hip_check(hipStreamWaitEvent(S->st_lane, j->ev_ready, 0), "lane wait");
-            wavefront_render(sc.view[j->d.spectral ? 1 : 0], cam, j->rv, !j->d.spectral, rt, out, 0, j->d.spp, chunk,
+            wavefront_render(view, cam, j->rv, !j->d.spectral, rt, out, 0, j->d.spp, chunk,
                              128, S->lane_wb, S->st_lane, nullptr, nullptr, nullptr);
             hip_check(vs::serve_output(j->rv, rt, out, px, S->st_lane), "output pass");
             hip_check(hipMemcpyAsync(j->h_rt, rt, size_t(px) * 16, hipMemcpyDeviceToHost, S->st_lane), "result D2H");
@@ -590,6 +613,11 @@ void worker_main(vs_server *S) {
     try {
         hip_check(hipSetDevice(S->ctx->device), "hipSetDevice");
         while (true) {
+            uint64_t seen;
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                seen = S->submit_seq;
+            }
             retire_lane(S);
             retire_copies(S, false);
             admit(S);
@@ -608,9 +636,11 @@ void worker_main(vs_server *S) {
             std::unique_lock<std::mutex> lk(S->mu);
             if (S->stop) break;
             if (!S->queue.empty()) {
-                // nothing runs and the head job was not admitted: with an
-                // empty arena every job within the budget fits, so this
-                // only passes when admission just raced with a submit
+                // the head job was not admitted and no step or copy is in
+                // flight: the wavefront lane holds the slots or the HBM it
+                // needs (a job within the aligned budget always fits an
+                // empty arena), so wait for the lane or for a new submit
+                S->cv_work.wait(lk, [&] { return S->stop || !S->lane_done.empty() || S->submit_seq != seen; });
                 continue;
             }
             if (!S->lane_done.empty()) continue;
@@ -670,6 +700,11 @@ vs_status vs_server_create(vs_context *ctx, const vs_server_config *cfg, vs_serv
             hip_check(hipMemGetInfo(&fr, &tot), "memory info");
             S->cfg.hbm_budget = fr / 4;
         }
+        // the arena hands out kArenaAlign-byte multiples
+        S->cfg.hbm_budget = S->cfg.hbm_budget / kArenaAlign * kArenaAlign;
+        if (!S->cfg.hbm_budget) throw std::invalid_argument("hbm_budget below the arena alignment (256 B)");
+        // scene slots are referenced while jobs run: never reallocate them
+        S->scenes.reserve(kMaxScenes);
         for (auto &s : S->st) hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
         hip_check(hipStreamCreateWithFlags(&S->st_copy, hipStreamNonBlocking), "stream");
         hip_check(hipEventCreateWithFlags(&S->ev_fork, hipEventDisableTiming), "event");
@@ -757,7 +792,7 @@ vs_status vs_server_submit(vs_server *S, const vs_job_desc *d, uint64_t *job_id)
         j->d = *d;
         j->d.spectrum_dim = dim;
         j->bytes = job_bytes(*d);
-        if (j->bytes > S->cfg.hbm_budget) throw std::bad_alloc();
+        if ((j->bytes + kArenaAlign - 1) / kArenaAlign * kArenaAlign > S->cfg.hbm_budget) throw std::bad_alloc();
         const SceneSlot &sc = S->scenes[d->scene];
         j->rv = sc.rv;
         if (d->max_depth) j->rv.max_depth = d->max_depth;
@@ -778,6 +813,7 @@ vs_status vs_server_submit(vs_server *S, const vs_job_desc *d, uint64_t *job_id)
         *job_id = j->id;
         S->queue.push_back(j.get());
         S->jobs.emplace(j->id, std::move(j));
+        ++S->submit_seq;
         S->cv_work.notify_one();
     });
 }
@@ -788,7 +824,7 @@ vs_status vs_server_wait(vs_server *S, uint64_t id, double timeout_ms, int32_t *
     auto it = S->jobs.find(id);
     if (it == S->jobs.end()) return server_fail(VS_ERROR_INVALID_ARGUMENT, "unknown job");
     JobRec *j = it->second.get();
-    auto finished = [&] { return j->state == VS_JOB_DONE || j->state == VS_JOB_FAILED; };
+    auto finished = [&] { return j->state == VS_JOB_DONE || j->state == VS_JOB_FAILED || !S->fatal.empty(); };
     if (timeout_ms < 0) {
         S->cv_done.wait(lk, finished);
     } else {
@@ -796,6 +832,7 @@ vs_status vs_server_wait(vs_server *S, uint64_t id, double timeout_ms, int32_t *
     }
     *done = finished() ? 1 : 0;
     if (j->state == VS_JOB_FAILED) return server_fail(VS_ERROR_DEVICE, "job failed: " + j->error);
+    if (j->state != VS_JOB_DONE && !S->fatal.empty()) return server_fail(VS_ERROR_DEVICE, "server failed: " + S->fatal);
     return VS_OK;
 }
 
