@@ -100,6 +100,22 @@ def workload(name: str):
     raise SystemExit(f"unknown workload {name}")
 
 
+def host_cpu() -> dict:
+    """Threads the CPU baseline uses and what the host has: the GPU box gives
+    each GPU a share of 16 host threads, whatever nproc shows."""
+    model = "unknown"
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    return {"cores": max(1, min(16, nproc)), "nproc": nproc, "cpu_model": model,
+            "cores_note": "16 threads: the host-CPU share per GPU on the GPU box"}
+
+
 def cpu_baseline(scene, spp: int, budget_s: float = 15.0) -> dict:
     """The oracle (scalar C++ port of the reference path) on the host cores
     (at most 16, this machine's share per GPU), over a bounded band of rows
@@ -108,7 +124,8 @@ def cpu_baseline(scene, spp: int, budget_s: float = 15.0) -> dict:
     from oracle import oracle
     ex = scene.export()
     W, H = int(ex["camera"][19]), int(ex["camera"][20])
-    cores = max(1, min(16, os.cpu_count() or 1))
+    hc = host_cpu()
+    cores = hc["cores"]
     t0 = time.perf_counter()
     oracle.render(ex, 0, spp, window=(0, 0, 0, 0), threads=cores)
     setup = time.perf_counter() - t0
@@ -122,7 +139,7 @@ def cpu_baseline(scene, spp: int, budget_s: float = 15.0) -> dict:
             break
         rows = min(H, rows * 4)
     samples = W * rows * spp
-    return {"value": round(samples / t / 1e6, 4), "unit": "Msamples/s", "cores": cores, "kind": "port",
+    return {"value": round(samples / t / 1e6, 4), "unit": "Msamples/s", **hc, "kind": "port",
             "sample": f"{rows} centre rows x {W} px x {spp} spp of the same job ({samples} samples, {t:.2f} s "
                       f"after {setup:.2f} s setup)"}
 
@@ -132,7 +149,8 @@ def restir_cpu_baseline(scene, frames: int = 2) -> dict:
     full image: the first `frames` frames of the same job."""
     from oracle import oracle
     from vision_amd import scenegen
-    cores = max(1, min(16, os.cpu_count() or 1))
+    hc = host_cpu()
+    cores = hc["cores"]
     r = oracle.Restir(scene.export(), threads=cores)
     cams = scenegen.jitter_path(frames, seed=1)
     t0 = time.perf_counter()
@@ -141,7 +159,7 @@ def restir_cpu_baseline(scene, frames: int = 2) -> dict:
         r.render(1, scene=scene.export())
     t = time.perf_counter() - t0
     samples = r.W * r.H * frames
-    return {"value": round(samples / t / 1e6, 4), "unit": "Msamples/s", "cores": cores, "kind": "port",
+    return {"value": round(samples / t / 1e6, 4), "unit": "Msamples/s", **hc, "kind": "port",
             "sample": f"the first {frames} frames of the job at {r.W}x{r.H} ({samples} samples, {t:.2f} s, "
                       f"per-frame BVH2 / light-table setup included)"}
 
@@ -217,11 +235,15 @@ def serve_main(args) -> None:
         tris.append(sc.info()["num_triangles"])
         scenes.append(sc)
         print(f"scene {k}: {tris[-1]} triangles", file=sys.stderr, flush=True)
-    jobs = serve_jobs(args.jobs, len(scenes), seed=args.seed)
+    # a step is `jobs_per_step` jobs of the seeded mix; the timed region
+    # queues all `steps` steps' jobs at once (continuous batching across step
+    # boundaries) and drains the device at both ends
+    jobs = serve_jobs(args.steps * args.jobs_per_step, len(scenes), seed=args.seed)
+    warm_jobs = serve_jobs(max(args.warmup, 1) * args.jobs_per_step, len(scenes), seed=args.seed + 200)
     total_samples = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
 
-    # plain form: one job at a time, in order
-    for j in jobs[:args.warmup]:
+    # plain form: one job at a time, in order (a few untimed jobs first)
+    for j in warm_jobs[:8]:
         plain_job(scenes[j["scene"]], j)
     service = []
     t0 = time.perf_counter()
@@ -272,8 +294,8 @@ def serve_main(args) -> None:
         th.join()
         return ids, srv.stats()
 
-    # warm-up (graph capture, first-step rate estimate)
-    wids, _ = run_batch(jobs[:max(args.warmup, 1)])
+    # warm-up steps (graph capture, first-step rate estimate)
+    wids, _ = run_batch(warm_jobs)
     for i in wids:
         srv.release(i)
     tune = None
@@ -332,9 +354,9 @@ def serve_main(args) -> None:
         "value": round(batched_msps, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
-        "steps": 1,
+        "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(st_sat["wall_ms"], 3),
+        "ms_per_step": round(st_sat["wall_ms"] / args.steps, 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -345,7 +367,9 @@ def serve_main(args) -> None:
                         f"{min(tris):,}-{max(tris):,} triangles (displaced level-5 icospheres, diffuse); "
                         "resolution from {256^2, 512^2, 1024^2, 1920x1080}, spp from {1, 4, 16, 64}, max depth "
                         "U[8, 16], sRGB or hero-4 spectral; persistent multi-job kernels in captured hipGraphs, "
-                        "HBM-budget admission; a step is the whole job list, queued at once",
+                        f"HBM-budget admission; a step is {args.jobs_per_step} jobs of the mix, the timed "
+                        f"{args.steps} steps' jobs queued at once",
+            "jobs_per_step": args.jobs_per_step,
             "jobs": len(jobs), "samples": total_samples, "triangles": tris, "seed": args.seed,
             "max_jobs": args.max_jobs, "step_ms": args.step_ms, **{k: srv.config()[k] for k in
                                                                     ("tile_w", "waves_per_cu",
@@ -389,7 +413,35 @@ def serve_main(args) -> None:
     for sc in scenes:
         sc.close()
     ctx.close()
+    if not args.no_configs:
+        line["configs"] = other_configs(args)
     print(json.dumps(line))
+
+
+def _compact(l: dict) -> dict:
+    keep = ("value", "unit", "steps", "warmup", "ms_per_step", "mrays_per_s", "rays_per_sample", "bvh_build_s",
+            "relmse_vs_4096spp", "relmse_at", "relmse_reference", "stages_ms_per_step")
+    out = {k: l[k] for k in keep if k in l}
+    out["workload"] = l["config"]["workload"] + (f"; {l['config']['kernel']}" if "kernel" in l["config"] else "")
+    rf = l["roofline"]
+    out["roofline"] = {k: rf[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms")}
+    return out
+
+
+def other_configs(args) -> dict:
+    """The plain workloads beside the serving headline, each a few timed
+    jobs plus relMSE against a 4096-spp plain render: the Cornell box 256^2 x
+    16 spp (megakernel), the procedural 1080p x 64 spp hero-4 scene
+    (wavefront form) and ReSTIR DI over 64 jitter-path frames at 1080p."""
+    out = {}
+    for key, name, steps, form in (("C1_cbox", "cbox", 5, "megakernel"), ("C2_procedural", "procedural", 2, "wavefront"),
+                                   ("C3_restir", "restir", 2, "wavefront")):
+        t0 = time.perf_counter()
+        l = plain_workload(name, steps, 1, form, args.chunk_frames, args.trace_block, cpu=False)
+        out[key] = _compact(l)
+        print(f"config {key}: {out[key]['value']} Msamples/s, relMSE {out[key].get('relmse_vs_4096spp')} "
+              f"({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    return out
 
 
 def nrc_main(args) -> None:
@@ -471,12 +523,13 @@ def nrc_main(args) -> None:
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="serve", choices=["serve", "procedural", "cbox", "restir", "nrc"])
     ap.add_argument("--nrc-frames", type=int, default=128, help="nrc: frames of 4 spp")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096, help="nrc: spp of the plain reference render")
-    ap.add_argument("--jobs", type=int, default=1000, help="serve: jobs in the list")
+    ap.add_argument("--jobs-per-step", type=int, default=50, help="serve: jobs in one step of the list")
+    ap.add_argument("--no-configs", action="store_true", help="serve: skip the cbox / procedural / restir keys")
     ap.add_argument("--seed", type=int, default=0, help="serve: job-list seed")
     ap.add_argument("--load", type=float, default=0.8, help="serve: Poisson rate / plain capacity")
     ap.add_argument("--max-jobs", type=int, default=64, help="serve: jobs admitted at once")
@@ -489,6 +542,7 @@ def main():
     ap.add_argument("--autotune", type=int, default=48, help="serve: probe jobs for the launch-shape autotuner "
                                                              "(0: keep --tile-w / --waves-per-cu)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-quality", action="store_true", help="skip the relMSE against 4096-spp references")
     ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
     ap.add_argument("--chunk-frames", type=int, default=16)
     ap.add_argument("--trace-block", type=int, default=128)
@@ -515,18 +569,58 @@ def main():
         nrc_main(args)
         return
 
+    line = plain_workload(args.workload, args.steps, args.warmup, args.kernel, args.chunk_frames, args.trace_block,
+                          cpu=not args.no_cpu, quality=not args.no_quality)
+    print(json.dumps(line))
+
+
+def relmse(x, ref) -> float:
+    """mean over pixels and channels of (x - ref)^2 / (ref^2 + 1e-2)"""
+    import numpy as np
+    x, r = x[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None):
+    """A plain path-traced film of `spp` samples per pixel (megakernel, in
+    launches of at most 512 frames), optionally at another resolution or as
+    a path tracer of the given max depth (for an "rt" scene: its direct-light
+    estimate is the PT image of max depth 1)."""
+    import copy
+    d = copy.deepcopy(scene_desc)
+    if pt_depth is not None:
+        d["integrator"] = {"type": "pt", "param": {"max_depth": pt_depth, "min_depth": 5, "rr_threshold": 1}}
+    sc = ctx.load_text(d, base_dir)
+    sc.build_accel()
+    if resolution:
+        sc.set_resolution(*resolution)
+    sc.set_kernel("megakernel")
+    left = spp
+    while left > 0:
+        sc.render(min(512, left), fused=True)
+        left -= min(512, left)
+    film = sc.film(0)
+    sc.close()
+    return film
+
+
+def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_frames: int, trace_block: int,
+                   cpu: bool = True, quality: bool = True, ref_spp: int = 4096) -> dict:
+    """One of the plain (non-serving) workloads: `steps` timed jobs after
+    `warmup` untimed ones, a counting run for the roofline's algorithmic
+    bytes, relMSE against a 4096-spp plain render, the CPU baseline."""
     import vision_amd as va
     from vision_amd import scenegen
-    scene_desc, base_dir, spp, config = workload(args.workload)
-    restir = args.workload == "restir"
+    scene_desc, base_dir, spp, config = workload(name)
+    restir = name == "restir"
     ctx = va.Context(0)
     sc = ctx.load_text(scene_desc, base_dir)
     t0 = time.perf_counter()
     sc.build_accel()
     build_s = time.perf_counter() - t0
     if not restir:
-        sc.set_kernel(args.kernel, args.chunk_frames, args.trace_block)
-        config["kernel"] = args.kernel + (f" ({args.chunk_frames} frames per chunk)" if args.kernel == "wavefront"
+        sc.set_kernel(kernel_form, chunk_frames, trace_block)
+        config["kernel"] = kernel_form + (f" ({chunk_frames} frames per chunk)" if kernel_form == "wavefront"
                                           else "")
     info = sc.info()
     W, H = info["width"], info["height"]
@@ -548,16 +642,16 @@ def main():
                     tot[k] = [a + b for a, b in zip(tot[k], v)] if isinstance(v, list) else tot[k] + v
         return tot
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         step()
     stats = []
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for _ in range(steps):
         stats.append(step())
     wall = time.perf_counter() - t0
-    ms_per_step = wall / args.steps * 1e3
+    ms_per_step = wall / steps * 1e3
     samples = W * H * spp
-    msps = samples * args.steps / wall / 1e6
+    msps = samples * steps / wall / 1e6
 
     # counting run of the same job: rays, node and triangle fetches
     sc.set_counting(True)
@@ -566,7 +660,7 @@ def main():
     trav = lambda k: (NODE_BYTES * cst["stage_node_fetches"][k] + TRI_BYTES * cst["stage_tri_fetches"][k] +
                       RAY_BYTES * cst["stage_rays"][k])
     avg_kernel_s = sum(s["kernel_ms"] for s in stats) / len(stats) / 1e3
-    staged = restir or args.kernel == "wavefront"
+    staged = restir or kernel_form == "wavefront"
     if staged:
         stage_s = [sum(s["stage_ms"][k] for s in stats) / len(stats) / 1e3 for k in range(3)]
         k = max(range(3), key=lambda i: stage_s[i])
@@ -598,14 +692,15 @@ def main():
         kernel = "pt_kernel (megakernel)"
     achieved = alg_bytes / kernel_s / 1e9
     mrays = cst["rays"] / avg_kernel_s / 1e6
+    film = sc.film(0)
 
     line = {
         "metric": METRIC,
         "value": round(msps, 3),
         "unit": "Msamples/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": steps,
+        "warmup": warmup,
         "ms_per_step": round(ms_per_step, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -626,11 +721,36 @@ def main():
     }
     if staged:
         line["stages_ms_per_step"] = [round(x * 1e3, 3) for x in stage_s]
-    if not args.no_cpu:
+    if quality:
+        # relMSE of the job's film against a 4096-spp plain render of the same
+        # scene; the 1080p workloads are judged at 480x270 (same scene, spp and
+        # camera path), where the reference costs 0.5 G samples
+        if name == "cbox":
+            line["relmse_vs_4096spp"] = relmse(film, reference_film(ctx, scene_desc, base_dir, ref_spp))
+            line["relmse_at"] = f"{W}x{H}, the timed job's film"
+        else:
+            rw, rh = 480, 270
+            if restir:
+                sc.set_resolution(rw, rh)
+                step()
+                test = sc.film(0)
+                ref = reference_film(ctx, scene_desc, base_dir, ref_spp, (rw, rh), pt_depth=1)
+                line["relmse_reference"] = ("plain PT of max depth 1 (emission + direct light, the quantity ReSTIR DI "
+                                            "estimates) at the unjittered camera")
+            else:
+                sc.set_resolution(rw, rh)
+                sc.reset_film()
+                sc.render(spp, fused=True)
+                test = sc.film(0)
+                ref = reference_film(ctx, scene_desc, base_dir, ref_spp, (rw, rh))
+            line["relmse_vs_4096spp"] = relmse(test, ref)
+            line["relmse_at"] = f"{rw}x{rh}, same scene and {spp} {'frames' if restir else 'spp'}"
+            sc.set_resolution(W, H)
+    if cpu:
         line["cpu_baseline"] = restir_cpu_baseline(sc) if restir else cpu_baseline(sc, spp)
     sc.close()
     ctx.close()
-    print(json.dumps(line))
+    return line
 
 
 if __name__ == "__main__":

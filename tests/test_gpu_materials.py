@@ -2,14 +2,12 @@
 the HDRI environment light (MI355X).
 
 Same seeded sampler streams on both sides: per-pixel radiance within 1e-4
-relative.  The device's sin/cos/acos/atan2 differ from the host libm's in
-the last bit; along a path such differences stay in the last bits until a
-discrete decision flips (reflect vs refract, Russian roulette, an edge hit),
-after which the two paths are independent samples.  Glass makes this
-compound with depth.  So: the material layer alone matches direction by
-direction, shallow (depth 2) renders match pixel for pixel, and full-depth
-renders match on nearly all pixels, with the diverged remainder consistent
-with zero mean difference (a z-test on the per-pixel differences).
+relative.  The kernels and the oracle evaluate the same elementary
+functions (sin/cos, acos, atan2, atanh, ...: vision_amd/csrc/elementary.h),
+so a discrete decision (reflect vs refract, Russian roulette, an edge hit)
+goes the same way on both sides at every depth: the material layer matches
+direction by direction, and depth-2 and depth-8 renders match on (at least)
+99.9% of pixels, their means within a z-test.
 """
 import numpy as np
 import pytest
@@ -90,7 +88,7 @@ def test_material_parity(gpu_ctx, mat, spectrum):
     print(f"{mat}/{spectrum}: depth 2 {sfrac * 100:.3f}%, depth 8 {frac * 100:.3f}% pixels within {REL_TOL}; "
           f"mean gpu {gm:.6f} oracle {rm:.6f} (z {z:.2f})")
     assert sfrac >= 0.999
-    assert frac >= 0.90
+    assert frac >= 0.999
     assert z < 4.5
 
 
@@ -106,7 +104,7 @@ def test_procedural_scene_parity(gpu_ctx, tmp_path, spectrum):
     z = _mean_z(gpu, ref)
     print(f"procedural/{spectrum}: {frac * 100:.3f}% pixels within {REL_TOL}; mean gpu {gm:.6f} oracle {rm:.6f} "
           f"(z {z:.2f})")
-    assert frac >= 0.90
+    assert frac >= 0.999
     assert z < 4.5
 
 

@@ -2,8 +2,8 @@
 
 Traversal: identical rays give bit-identical (triangle, t, barycentrics).
 Rendering: identical sampler streams give per-pixel radiance within 1e-4
-relative (the few paths whose transcendental-function rounding flips a
-discrete decision are counted and bounded).
+relative (the kernels and the oracle share their elementary functions, so
+every sampling decision takes the same branch).
 """
 import struct
 import tempfile
