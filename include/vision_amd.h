@@ -201,6 +201,15 @@ vs_status vs_trace_any(vs_scene *scene, const vs_ray *rays, uint32_t n, uint8_t 
  * `stream` when non-null, a hipStream_t); returns without synchronizing. */
 vs_status vs_trace_closest_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, vs_hit *d_hits, void *stream);
 vs_status vs_trace_any_device(vs_scene *scene, const vs_ray *d_rays, uint32_t n, uint8_t *d_occluded, void *stream);
+/* vs_trace_closest without device allocations.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * around hipGraph capture, serving loops, harnesses that forbid hipMalloc).
+ * Rays and hits stay in host memory; they are staged through a 14 MiB array
+ * that the library holds on each GPU until the process ends, so at most
+ * 262144 rays per call (more is VS_ERROR_INVALID_ARGUMENT).  Results equal
+ * vs_trace_closest's.  Calls that overlap on one GPU (several threads, or
+ * several contexts of the device) race on that array: serialise them. */
+vs_status vs_trace_closest_noalloc(vs_scene *scene, const vs_ray *rays, uint32_t n, vs_hit *hits);
 
 /* Material-level parity hook: MaterialEvaluator::evaluate (sample = 0) or
  * ::sample (sample = 1) of scene material `mat` at a surface whose shading

@@ -163,6 +163,7 @@ SIGNATURES = {
     "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
     "vs_trace_any_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
+    "vs_trace_closest_noalloc": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_scene_bsdf": (C.c_int32, [_P, C.c_uint32, C.c_uint32, _P, _P, _P, C.c_int32, C.c_uint32, _P, _P, _P]),
     "vs_scene_set_kernel": (C.c_int32, [_P, C.c_int32, C.c_uint32, C.c_uint32]),
     "vs_render": (C.c_int32, [_P, C.c_uint32, C.c_int32]),
@@ -375,6 +376,14 @@ class Scene:
         rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
         hits = np.zeros(len(rays), dtype=HIT_DTYPE)
         self.ctx._check(lib().vs_trace_closest(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
+        return hits
+
+    def trace_closest_noalloc(self, rays: np.ndarray) -> np.ndarray:
+        """trace_closest through vs_trace_closest_noalloc (no device
+        allocation; at most 262144 rays per call)."""
+        rays = np.ascontiguousarray(rays, dtype=RAY_DTYPE)
+        hits = np.zeros(len(rays), dtype=HIT_DTYPE)
+        self.ctx._check(lib().vs_trace_closest_noalloc(self._h, rays.ctypes.data, len(rays), hits.ctypes.data))
         return hits
 
     def trace_any(self, rays: np.ndarray) -> np.ndarray:

@@ -279,6 +279,12 @@ vs_status vs_create(int32_t device, vs_context **out) {
         vs_destroy(ctx);
         return VS_ERROR_DEVICE;
     }
+    try {
+        vs_internal::noalloc_setup(device);
+    } catch (const std::exception &) {
+        vs_destroy(ctx);
+        return VS_ERROR_DEVICE;
+    }
     *out = ctx;
     return VS_OK;
 }

@@ -199,6 +199,10 @@ void ensure_alt_mode(vs_scene *s);
 vs::SceneView view_in_mode(vs_scene *s, bool rgb);
 
 // device work needs a GPU context and a built scene
+// resolves this device's address of the staging array behind
+// vs_trace_closest_noalloc (noalloc.hip); `device` must be current
+void noalloc_setup(int device);
+
 inline void require_built(vs_scene *s) {
     if (s->ctx->device < 0) throw std::logic_error("host-only context: no device work possible");
     if (!s->built) throw std::logic_error("accel not built: call vs_scene_build_accel first");
