@@ -581,12 +581,16 @@ def relmse(x, ref) -> float:
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
-def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None):
+def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None, cameras=None,
+                   fov_y: float = 19.5):
     """A plain path-traced film of `spp` samples per pixel (megakernel, in
     launches of at most 512 frames), optionally at another resolution or as
     a path tracer of the given max depth (for an "rt" scene: its direct-light
-    estimate is the PT image of max depth 1)."""
+    estimate is the PT image of max depth 1).  With `cameras` (camera-to-world
+    matrices) the spp are split evenly over them and the films averaged: the
+    reference of a job whose camera follows that path."""
     import copy
+    import numpy as np
     d = copy.deepcopy(scene_desc)
     if pt_depth is not None:
         d["integrator"] = {"type": "pt", "param": {"max_depth": pt_depth, "min_depth": 5, "rr_threshold": 1}}
@@ -595,13 +599,26 @@ def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_dept
     if resolution:
         sc.set_resolution(*resolution)
     sc.set_kernel("megakernel")
-    left = spp
-    while left > 0:
-        sc.render(min(512, left), fused=True)
-        left -= min(512, left)
-    film = sc.film(0)
+
+    def render(n):
+        sc.reset_film()
+        left = n
+        while left > 0:
+            sc.render(min(512, left), fused=True)
+            left -= min(512, left)
+        return sc.film(0).astype(np.float64)
+
+    if cameras is None:
+        film = render(spp)
+    else:
+        per = max(1, spp // len(cameras))
+        film = 0.0
+        for c2w in cameras:
+            sc.update_camera(c2w, fov_y)
+            film = film + render(per)
+        film = film / len(cameras)
     sc.close()
-    return film
+    return np.asarray(film, np.float32)
 
 
 def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_frames: int, trace_block: int,
@@ -734,9 +751,9 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
                 sc.set_resolution(rw, rh)
                 step()
                 test = sc.film(0)
-                ref = reference_film(ctx, scene_desc, base_dir, ref_spp, (rw, rh), pt_depth=1)
+                ref = reference_film(ctx, scene_desc, base_dir, ref_spp, (rw, rh), pt_depth=1, cameras=cams)
                 line["relmse_reference"] = ("plain PT of max depth 1 (emission + direct light, the quantity ReSTIR DI "
-                                            "estimates) at the unjittered camera")
+                                            "estimates), 64 spp at each of the job's 64 jitter-path cameras")
             else:
                 sc.set_resolution(rw, rh)
                 sc.reset_film()

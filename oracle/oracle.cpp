@@ -515,7 +515,7 @@ struct Bsdf {
     bool black{false};
     bool set{false};// a LobeSet (principled) rather than a single lobe
     int nlobes{0};
-    Lobe lobes[6];
+    Lobe lobes[12];// a mix / add of two principled stacks flattens to 12
     Frame fr;
     V3 ng, wo_w;
     bool dispersive{false};
@@ -752,7 +752,12 @@ BSample bsdf_sample(const Spectral &sd, const Bsdf &b, Sampler &sp, int n) {
 //         p[19] anisotropic, p[20] sheen_weight, p[21] sheen_roughness,
 //         p[22] coat_weight, p[23] coat_roughness, p[24] coat_ior,
 //         p[25] transmission_weight
-enum MatType { M_DIFFUSE = 0, M_OREN = 1, M_BLACK = 2, M_METAL = 3, M_GLASS = 4, M_PRINCIPLED = 5, M_MIRROR = 6 };
+//  mix / add: p[0], p[1] indices (uint bits) of the two component
+//         records, p[2] the mix fraction
+enum MatType {
+    M_DIFFUSE = 0, M_OREN = 1, M_BLACK = 2, M_METAL = 3, M_GLASS = 4, M_PRINCIPLED = 5, M_MIRROR = 6, M_MIX = 7,
+    M_ADD = 8
+};
 
 float layering_div(float a, float w) { return w != 0.f ? a / w : 0.f; }
 // layering_weight (principled_bsdf.cpp:174-179)
@@ -762,9 +767,51 @@ Spec layering_weight(const Spec &albedo, const Spec &weight, int n) {
     return weight * std::min(std::max(1.f - m, 0.f), 1.f);
 }
 
+Bsdf make_leaf_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, const Swl &w);
+
+// MixMaterial::create_lobe_set (mix.cpp) / AddMaterial::create_lobe_set
+// (add.cpp): LobeSet::create_mix(frac, A, B) gives A and B the weights
+// 1 - frac and frac (sample and evaluation alike); create_add gives both
+// 1 and normalises the sample weights to 1/2 (lobe.cpp:495-521).  Both then
+// flatten (lobe.cpp:533-562): a component that is itself a lobe set (the
+// principled stack) is replaced by its lobes, each lobe's sample and
+// evaluation weight multiplied by the component's sample weight.
+Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, const Swl &w) {
+    const uint32_t *rec = s.mat + 32 * m;
+    if (rec[0] != M_MIX && rec[0] != M_ADD) return make_leaf_bsdf(sd, s, m, it, w);
+    float frac;
+    std::memcpy(&frac, rec + 2 + 2, 4);
+    const bool add = rec[0] == M_ADD;
+    Bsdf b;
+    b.fr = it.fr;
+    b.ng = it.ng;
+    b.wo_w = it.wo;
+    b.set = true;
+    b.nlobes = 0;
+    for (int k = 0; k < 2; ++k) {
+        Bsdf c = make_leaf_bsdf(sd, s, rec[2 + k], it, w);
+        b.dispersive = b.dispersive || c.dispersive;
+        const float wk = k == 0 ? 1.f - frac : frac;
+        const float parent_sample = add ? 0.5f : wk;     // after normalize_sampled_weight
+        const float parent_eval = add ? 1.f : wk;
+        for (int i = 0; i < c.nlobes; ++i) {
+            Lobe L = c.lobes[i];
+            if (c.set) {// flattened: both weights times the parent's sample weight
+                L.sample_weight *= parent_sample;
+                L.weight *= parent_sample;
+            } else {
+                L.sample_weight = parent_sample;
+                L.weight = parent_eval;
+            }
+            b.lobes[b.nlobes++] = L;
+        }
+    }
+    return b;
+}
+
 // Material::create_lobe_set of diffuse.cpp:16-24, metal.cpp:124-140,
 // glass.cpp:215-228, principled_bsdf.cpp:352-461 and black_body.cpp
-Bsdf make_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, const Swl &w) {
+Bsdf make_leaf_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &it, const Swl &w) {
     Bsdf b;
     b.fr = it.fr;
     b.ng = it.ng;

@@ -599,3 +599,16 @@ def camera_rays(cam: np.ndarray, px: int, py: int, frame0: int, n: int):
     pf = np.zeros((n, 2), np.float32)
     lib().or_camera_rays(cam, px, py, frame0, n, o, d, pf)
     return o, d, pf
+
+
+def material_table(materials: np.ndarray, m: int) -> np.ndarray:
+    """The records material `m` of a scene export needs, as a table whose
+    first record is `m` (the form bsdf_eval / bsdf_sample take): `m` alone,
+    or a mix / add followed by its two components with indices 1 and 2."""
+    mats = np.ascontiguousarray(materials, np.uint32).reshape(-1, 32)
+    rec = mats[m].copy()
+    if rec[0] not in (7, 8):
+        return rec
+    comps = [mats[int(rec[2])], mats[int(rec[3])]]
+    rec[2], rec[3] = 1, 2
+    return np.concatenate([rec, *comps])

@@ -14,8 +14,12 @@ static void run(const uint32_t *mat, const float *spd, const float *wo, const fl
     SceneView sc{};
     sc.spd = spd;
     sc.lut = lut_tables().data();
-    MaterialRecord rec;
-    std::memcpy(&rec, mat, sizeof(rec));
+    // a mix / add brings its two components as records 1 and 2
+    MaterialRecord recs[3];
+    std::memcpy(&recs[0], mat, sizeof(MaterialRecord));
+    if (recs[0].type == MAT_MIX || recs[0].type == MAT_ADD) std::memcpy(&recs[1], mat + 32, 2 * sizeof(MaterialRecord));
+    sc.mat = recs;
+    const MaterialRecord &rec = recs[0];
     for (uint32_t i = 0; i < n; ++i) {
         Swl<NS> swl;
         for (int k = 0; k < NS; ++k) {
@@ -27,13 +31,13 @@ static void run(const uint32_t *mat, const float *spd, const float *wo, const fl
         bool disp;
         MatCtx<NS> c = make_material<NS, RGB>(sc, rec, fr, mk3(0, 0, 1), wo3, swl, disp, mk2(0, 0));
         if (!sample) {
-            EvalN<NS> e = mat_eval(c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr);
+            EvalN<NS> e = mat_eval_any<NS, RGB>(sc, c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr, swl);
             for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? e.f.v[k] : 0.f;
             pdf[i] = e.pdf;
         } else {
             Sampler sp;
             sp.start(0u, i, frame, 0u);
-            SampleN<NS> s = mat_sample(c, sp);
+            SampleN<NS> s = mat_sample_any<NS, RGB>(sc, c, sp, swl);
             wi_out[3 * i] = s.wi.x, wi_out[3 * i + 1] = s.wi.y, wi_out[3 * i + 2] = s.wi.z;
             for (int k = 0; k < 4; ++k) f[4 * i + k] = k < NS ? s.e.f.v[k] : 0.f;
             pdf[i] = s.e.pdf;

@@ -5,9 +5,9 @@ what it pulls in) with g++ against a small host stand-in for the HIP
 runtime header, and drives them as the kernels do.  With identical inputs the
 result must equal the oracle bit for bit: BSDF values, sampled directions
 and whole films.  On the GPU the same source differs from the oracle only
-through the device math library's last-bit rounding of sin/cos/acos/atan2
-(test_gpu_materials.py); this test shows that the arithmetic around those
-calls is the oracle's.
+through code generation (the elementary functions are shared,
+vision_amd/csrc/elementary.h); this test shows that the arithmetic is the
+oracle's.
 """
 import ctypes as C
 import json
@@ -90,7 +90,7 @@ def test_device_bsdf_source_equals_oracle(dc, spectrum):
         with va.Context(-1) as ctx:
             s = ctx.load_text(G._material_box(mat, spectrum)).build_accel()
             ex = s.export()
-        rec = np.ascontiguousarray(ex["materials"][-1])
+        rec = oracle.material_table(ex["materials"], len(G._material_box(mat, spectrum)["materials"]) - 1)
         spec = oracle.Spectral(ex)
         rgb, ns = bool(ex["spectrum_rgb"]), 3 if ex["spectrum_rgb"] else ex["spectrum_dim"]
         _, fd, pd = _dev_bsdf(dc, rec, spec.spd, wo, wi, 0, 0, rgb, ns, lam)

@@ -50,6 +50,14 @@ TEST_MATERIALS = {
     "black_body": {"type": "black_body", "param": {}},
     "mirror_tinted": {"type": "mirror", "param": {"color": [0.9, 0.6, 0.3], "roughness": 0.35, "anisotropic": 0.2}},
     "mirror": {"type": "mirror", "param": {}},
+    "mix_glass_principled": {"type": "mix", "param": {
+        "frac": 0.35,
+        "mat0": {"type": "glass", "param": {"ior": 1.45, "roughness": 0.2, "color": [0.95, 0.9, 1.0]}},
+        "mat1": {"type": "principled_bsdf", "param": {"color": [0.7, 0.5, 0.3], "metallic": 0.3, "roughness": 0.5,
+                                                      "sheen_weight": 0.4, "coat_weight": 0.3}}}},
+    "add_diffuse_metal": {"type": "add", "param": {
+        "mat0": {"type": "diffuse", "param": {"color": [0.3, 0.4, 0.5]}},
+        "mat1": {"type": "metal", "param": {"material_name": "Au", "roughness": 0.35}}}},
 }
 
 
@@ -142,7 +150,7 @@ def test_device_bsdf_matches_oracle(gpu_ctx, mat, spectrum):
     sc = gpu_ctx.load_text(scene)
     sc.build_accel()
     ex = sc.export()
-    m = len(ex["materials"]) - 1  # the test material is appended last
+    m = len(scene["materials"]) - 1  # the test material is the scene's last (mix components follow it)
     spec = oracle.Spectral(ex)
     rng = np.random.default_rng(4)
     n = 20000
@@ -154,12 +162,13 @@ def test_device_bsdf_matches_oracle(gpu_ctx, mat, spectrum):
     if spectrum != "srgb":
         lam = oracle.wavelengths(rng.random(4 * n, dtype=np.float32))[0].reshape(n, 4)
     _, fd, pd = sc.bsdf(m, wo, wi, lam)
-    fo, po = oracle.bsdf_eval(spec, ex["materials"][m], wo, wi, lam)
+    rec = oracle.material_table(ex["materials"], m)
+    fo, po = oracle.bsdf_eval(spec, rec, wo, wi, lam)
     ok = np.isclose(fd, fo, rtol=1e-5, atol=1e-7).all(1) & np.isclose(pd, po, rtol=1e-5, atol=1e-7)
     print(f"bsdf eval {mat}/{spectrum}: {ok.mean() * 100:.3f}% within 1e-5")
     assert ok.mean() >= 0.999
     wsd, fsd, psd = sc.bsdf(m, wo, None, lam, frame=7)
-    wso, fso, pso = oracle.bsdf_sample(spec, ex["materials"][m], wo, frame=7, lam=lam)
+    wso, fso, pso = oracle.bsdf_sample(spec, rec, wo, frame=7, lam=lam)
     ok = np.isclose(wsd, wso, atol=1e-5).all(1) & np.isclose(psd, pso, rtol=1e-4, atol=1e-7)
     print(f"bsdf sample {mat}/{spectrum}: {ok.mean() * 100:.3f}% within 1e-5")
     assert ok.mean() >= 0.999

@@ -73,24 +73,12 @@ def test_cornell_box_resolves_to_expected_geometry(host_ctx):
 
 
 def test_reference_cbox_json_parses(host_ctx):
-    # the reference's cbox.json (res/render_scene/cbox/cbox.json) with its
-    # comment syntax; its "mirror" / "mix" materials are outside this build's
-    # scope and must be reported as unsupported rather than misparsed
+    # the reference's cbox.json (res/render_scene/cbox/cbox.json) loads
+    # unchanged: its comment syntax, the thin-lens camera, the rt integrator,
+    # a mix material and the materials this build does not render (metallic,
+    # plastic, substrate), which no shape uses
     text = (GOLDEN / "cbox_reference.json").read_text()
-    with pytest.raises(va.VisionError) as e:
-        host_ctx.load_text(text)
-    assert e.value.status == 3 and ("mirror" in str(e.value) or "mix" in str(e.value))
-    # with those materials replaced the scene (sphere included) parses:
-    # walls and light match the file's matrices
-    text2 = text
-    doc = json.loads(re.sub(r"//[^\n]*", "", text2))
-    keep = {"diffuse"}
-    doc["materials"] = [m for m in doc["materials"] if m["type"] in keep]
-    for s in doc["shapes"]:
-        if s["param"].get("material") not in {m["name"] for m in doc["materials"]}:
-            s["param"]["material"] = "TallBox"
-    doc["camera"]["type"] = "pinhole"
-    sc = host_ctx.load_text(json.dumps(doc))
+    sc = host_ctx.load_text(text)
     ex = sc.export()
     info = sc.info()
     assert info["num_lights"] == 1 and (info["width"], info["height"]) == (1024, 1024)
@@ -103,6 +91,24 @@ def test_reference_cbox_json_parses(host_ctx):
     cam = ex["camera"]
     assert np.allclose(cam[[3, 7, 11]], [0, 1, 6.8], atol=1e-5)
     sc.close()
+    # with the path tracer, the file's "mix" material is packed: components
+    # after the scene's own materials, the record pointing at them (under
+    # "rt" it is listed but not rendered)
+    doc = json.loads(re.sub(r"//[^\n]*", "", text))
+    doc["integrator"] = {"type": "pt", "param": {"max_depth": 8}}
+    names = [m["name"] for m in doc["materials"]]
+    sc = host_ctx.load_text(json.dumps(doc))
+    ex = sc.export()
+    mix = ex["materials"][names.index("mix")]
+    assert mix[0] == 7 and mix[2] >= len(names) and mix[3] == mix[2] + 1
+    assert np.float32(0.1) == mix[4:5].view(np.float32)[0]
+    assert ex["materials"][mix[2]][0] == 5 and ex["materials"][mix[3]][0] == 0  # principled, diffuse
+    sc.close()
+    # a shape that uses an unsupported material is refused, naming it
+    doc["shapes"][0]["param"]["material"] = "plastic"
+    with pytest.raises(va.VisionError) as e:
+        host_ctx.load_text(json.dumps(doc))
+    assert "plastic" in str(e.value)
 
 
 def test_json_comments_and_errors(host_ctx):

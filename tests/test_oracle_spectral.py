@@ -749,3 +749,88 @@ def test_principled_full_transmission_is_the_rough_dielectric(rough, ior):
     assert (fg[n // 2:, 0] > 0).sum() > n // 8
     np.testing.assert_allclose(f[:, :3], fg[:, :3], rtol=2e-5, atol=1e-9)
     np.testing.assert_allclose(pdf, pdfg, rtol=2e-5, atol=1e-9)
+
+
+# ------------------------------------------------------------ mix / add
+# MixMaterial / AddMaterial (mix.cpp, add.cpp) through LobeSet::create_mix /
+# create_add and flatten (lobe.cpp:495-562).  A mix is the (1 - frac, frac)
+# blend of its two components, in f and in pdf; an add sums the
+# evaluations of single-lobe components, halves those of lobe-set
+# components (flatten scales their lobes by the normalised sample weight
+# 1/2) and halves every pdf.
+
+def _mat_mix(frac, a, b, add=False):
+    rec = np.zeros(32, np.uint32)
+    rec[0] = 8 if add else 7
+    rec[2], rec[3] = 1, 2
+    rec[4] = np.float32(frac).view(np.uint32)
+    return np.concatenate([rec, np.asarray(a, np.uint32).reshape(32), np.asarray(b, np.uint32).reshape(32)])
+
+
+def _sphere_pairs(n, seed):
+    rng = np.random.default_rng(seed)
+    wo, _ = _upper_pairs(n, seed)
+    v = rng.normal(size=(n, 3))
+    wi = (v / np.linalg.norm(v, axis=1, keepdims=True)).astype(np.float32)
+    return wo, wi
+
+
+@pytest.mark.parametrize("a,b", [("lambert", "metal"), ("principled", "oren_nayar"), ("glass", "principled"),
+                                 ("metal_aniso", "glass")])
+@pytest.mark.parametrize("frac", [0.0, 0.3, 1.0])
+def test_mix_is_the_weighted_blend_of_its_components(a, b, frac):
+    spec = _rgb_spec()
+    wo, wi = _sphere_pairs(20000, 11)
+    fa, pa = oracle.bsdf_eval(spec, MATERIALS[a], wo, wi)
+    fb, pb = oracle.bsdf_eval(spec, MATERIALS[b], wo, wi)
+    fm, pm = oracle.bsdf_eval(spec, _mat_mix(frac, MATERIALS[a], MATERIALS[b]), wo, wi)
+    w0, w1 = np.float32(1 - np.float32(frac)), np.float32(frac)
+    if frac == 0.0:
+        np.testing.assert_allclose(fm, fa, rtol=1e-6, atol=1e-12)
+    if frac == 1.0:
+        np.testing.assert_allclose(fm, fb, rtol=1e-6, atol=1e-12)
+    want_f = w0 * fa.astype(np.float64) + w1 * fb.astype(np.float64)
+    want_p = w0 * pa.astype(np.float64) + w1 * pb.astype(np.float64)
+    np.testing.assert_allclose(fm, want_f, rtol=2e-5, atol=1e-9)
+    np.testing.assert_allclose(pm, want_p, rtol=2e-5, atol=1e-9)
+    assert fm[:, :3].max() > 0
+
+
+@pytest.mark.parametrize("a,b", [("lambert", "metal"), ("principled", "oren_nayar"), ("glass", "principled_rough_metal")])
+def test_add_sums_single_lobes_and_halves_lobe_sets(a, b):
+    spec = _rgb_spec()
+    wo, wi = _sphere_pairs(20000, 12)
+    fa, pa = oracle.bsdf_eval(spec, MATERIALS[a], wo, wi)
+    fb, pb = oracle.bsdf_eval(spec, MATERIALS[b], wo, wi)
+    fm, pm = oracle.bsdf_eval(spec, _mat_mix(0.0, MATERIALS[a], MATERIALS[b], add=True), wo, wi)
+    ea = 0.5 if a.startswith("principled") else 1.0
+    eb = 0.5 if b.startswith("principled") else 1.0
+    np.testing.assert_allclose(fm, ea * fa.astype(np.float64) + eb * fb.astype(np.float64), rtol=2e-5, atol=1e-9)
+    np.testing.assert_allclose(pm, 0.5 * pa.astype(np.float64) + 0.5 * pb.astype(np.float64), rtol=2e-5, atol=1e-9)
+
+
+MIXES = {
+    "mix_lambert_metal": _mat_mix(0.35, MATERIALS["lambert"], MATERIALS["metal"]),
+    "mix_principled_oren": _mat_mix(0.6, MATERIALS["principled_rough_metal"], MATERIALS["oren_nayar"]),
+    "add_metal_principled": _mat_mix(0.0, MATERIALS["metal_aniso"], MATERIALS["principled_rough_metal"], add=True),
+}
+
+
+@pytest.mark.parametrize("name", list(MIXES))
+def test_mix_sampling_matches_its_pdf(name):
+    # the flattened lobe set's sampler (one lobe picked by its sample
+    # weight) follows the pdf its evaluator reports
+    spec = _rgb_spec()
+    rec = MIXES[name]
+    n = 300_000
+    theta = 0.8
+    wo1 = np.array([math.sin(theta), 0.0, math.cos(theta)], np.float32)
+    wo = np.tile(wo1, (n, 1))
+    wi, f, pdf = oracle.bsdf_sample(spec, rec, wo, frame=5)
+    valid = pdf > 0
+    p, _ = _chi2_sphere(wi[valid], lambda d: oracle.bsdf_eval(spec, rec, np.tile(wo1, (len(d), 1)), d)[1], n_total=n)
+    assert p > 1e-3, (name, p)
+    k = np.flatnonzero(valid)[:2000]
+    f2, pdf2 = oracle.bsdf_eval(spec, rec, wo[k], wi[k])
+    np.testing.assert_allclose(pdf2, pdf[k], rtol=1e-5, atol=1e-7)
+    np.testing.assert_allclose(f2, f[k], rtol=1e-5, atol=1e-7)

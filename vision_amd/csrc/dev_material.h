@@ -20,6 +20,8 @@ namespace dev {
 //          p[19] anisotropic, p[20] sheen_weight, p[21] sheen_roughness,
 //          p[22] coat_weight, p[23] coat_roughness, p[24] coat_ior,
 //          p[25] transmission_weight
+//  mix / add: p[0], p[1] indices (uint bits) of the two component records
+//          (never mix / add / black body themselves), p[2] the mix fraction
 // A colour slot holds linear sRGB in sRGB mode and the sigmoid-polynomial
 // coefficients of the RGB albedo spectrum in hero mode.
 constexpr uint32_t MATF_DISPERSIVE = 1u;
@@ -84,6 +86,11 @@ struct MatCtx {
     Spec<NS> sh_tint, cc_kr, mt_kr, mt_F0, mt_B, tr_kt, tr_F0, sp_kr, sp_F0, df_kr;
     float cc_eta, cc_alpha, mt_comp, tr_eta, tr_weight, tr_comp, sp_eta;
     float sw[6];// normalized sample weights: sheen, coat, metal, trans, spec, diffuse
+    // mix / add: the component records, their evaluation and sample weights
+    // in the flattened lobe set, the surface uv (components are rebuilt on use)
+    uint32_t sub[2];
+    float sub_e[2], sub_s[2];
+    f2 uv;
 };
 
 template <int NS>
@@ -134,6 +141,25 @@ __device__ __forceinline__ MatCtx<NS> make_material(const SceneView &sc, const M
     c.wo = fr.to_local(wo_w);
     dispersive = false;
     const float *p = m.p;
+    if (MS == MS_ALL && (m.type == MAT_MIX || m.type == MAT_ADD)) {
+        // LobeSet::create_mix / create_add (lobe.cpp:495-521) then flatten
+        // (lobe.cpp:533-562): component k carries sample weight s_k (mix:
+        // 1 - frac or frac; add: 1/2 after normalisation) and evaluation
+        // weight e_k (mix: as s_k; add: 1, or 1/2 when the component is a
+        // lobe set, whose lobes flatten with the parent's sample weight)
+        const bool add = m.type == MAT_ADD;
+        c.uv = uv;
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            c.sub[k] = __float_as_uint(p[k]);
+            const MaterialRecord &cm = sc.mat[c.sub[k]];
+            const float wk = k == 0 ? 1.f - p[2] : p[2];
+            c.sub_s[k] = add ? 0.5f : wk;
+            c.sub_e[k] = add ? (cm.type == MAT_PRINCIPLED ? 0.5f : 1.f) : wk;
+            if (!RGB && cm.type == MAT_GLASS && (cm.flags & MATF_DISPERSIVE)) dispersive = true;
+        }
+        return c;
+    }
     if (ms_has<MS>(MS_DIFFUSE) && (m.type == MAT_DIFFUSE || m.type == MAT_OREN_NAYAR)) {
         c.a0 = slot_albedo<NS, RGB>(sc, m, 0, uv, swl);
         if (m.type == MAT_OREN_NAYAR) {
@@ -387,61 +413,92 @@ __device__ __forceinline__ void eval_lambert(const Spec<NS> &kr, f3 wo, f3 wi, f
     e.flags = 0;
 }
 
-// full evaluation in world space: f * |cos wi|, pdf, with the geometric
-// hemisphere check of MaterialEvaluator::evaluate
+// evaluation of a leaf material's lobe(s), added into e: f * |cos wi| and
+// pdf before the geometric hemisphere check, each lobe's evaluation weight
+// times ew and sample weight times sw (1 and 1 for the material alone; a
+// component's weights inside a mix / add).  Lobe by lobe as
+// LobeSet::evaluate_impl (lobe.cpp:668-688) sums them: f |cos| times (weight
+// x factor), pdf times (sample weight x factor), the factor masking lobes
+// that only reflect to the side of wo.
 template <int NS, int MS = MS_ALL>
-__device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, float *eta_out) {
-    EvalN<NS> e;
+__device__ __forceinline__ void leaf_eval_into(const MatCtx<NS> &c, f3 wi_w, float *eta_out, float ew, float sw,
+                                               EvalN<NS> &e) {
     f3 wo = c.wo, wi = c.fr.to_local(wi_w);
-    if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
-        eval_lambert(c.a0, wo, wi, c.onA, c.onB, c.type == MAT_OREN_NAYAR, e);
-        e.f = e.f * fabsf(wi.z);
-    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
-        eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), c.comp, e);
-        e.f = e.f * fabsf(wi.z);
-    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_MIRROR) {
-        eval_mirror(c.ggx, wo, wi, c.a0, c.comp, e);
-        e.f = e.f * fabsf(wi.z);
-    } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
-        eval_dielectric(c.ggx, wo, wi, c.a1, c.eta0, false, c.a1, c.a0, c.comp, e, eta_out);
-        e.f = e.f * fabsf(wi.z);
-    } else if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
-        // LobeSet::evaluate_impl (lobe.cpp:668-688): each lobe weighted, its pdf
-        // scaled by its sample weight, both masked by the lobe's hemisphere factor
+    EvalN<NS> t;
+    if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
         float acos = fabsf(wi.z);
         float same = wo.z * wi.z > 0.f ? 1.f : 0.f;
-        EvalN<NS> t;
-        e.f = Spec<NS>::fill(0.f);
-        e.pdf = 0.f;
-        e.flags = 0;
         eval_sheen(c.sh_a, c.sh_b, c.sh_tint, wo, wi, t);
-        e.f = e.f + t.f * (acos * same);
-        e.pdf += t.pdf * c.sw[0] * same;
+        e.f = e.f + (t.f * acos) * (ew * same);
+        e.pdf += t.pdf * ((c.sw[0] * sw) * same);
         eval_reflection_dielectric(Ggx{c.cc_alpha, c.cc_alpha}, wo, wi, c.cc_eta, false, c.cc_kr, c.cc_kr, t);
-        e.f = e.f + t.f * (acos * same);
-        e.pdf += t.pdf * c.sw[1] * same;
+        e.f = e.f + (t.f * acos) * (ew * same);
+        e.pdf += t.pdf * ((c.sw[1] * sw) * same);
         eval_conductor_like(c.ggx, wo, wi, true, c.mt_F0, c.mt_B, c.mt_kr, c.mt_comp, t);
-        e.f = e.f + t.f * (acos * same);
-        e.pdf += t.pdf * c.sw[2] * same;
+        e.f = e.f + (t.f * acos) * (ew * same);
+        e.pdf += t.pdf * ((c.sw[2] * sw) * same);
         eval_dielectric(c.ggx, wo, wi, c.tr_F0, c.tr_eta, true, c.tr_F0, c.tr_kt, c.tr_comp, t, eta_out);
-        e.f = e.f + (t.f * acos) * c.tr_weight;
-        e.pdf += t.pdf * c.sw[3];
+        e.f = e.f + (t.f * acos) * (c.tr_weight * ew);
+        e.pdf += t.pdf * (c.sw[3] * sw);
         e.flags |= t.flags;
         eval_reflection_dielectric(c.ggx, wo, wi, c.sp_eta, true, c.sp_F0, c.sp_kr, t);
-        e.f = e.f + t.f * (acos * same);
-        e.pdf += t.pdf * c.sw[4] * same;
+        e.f = e.f + (t.f * acos) * (ew * same);
+        e.pdf += t.pdf * ((c.sw[4] * sw) * same);
         eval_lambert(c.df_kr, wo, wi, 0.f, 0.f, false, t);
-        e.f = e.f + t.f * (acos * same);
-        e.pdf += t.pdf * c.sw[5] * same;
+        e.f = e.f + (t.f * acos) * (ew * same);
+        e.pdf += t.pdf * ((c.sw[5] * sw) * same);
+        return;
+    }
+    if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
+        eval_lambert(c.a0, wo, wi, c.onA, c.onB, c.type == MAT_OREN_NAYAR, t);
+    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
+        eval_conductor_like(c.ggx, wo, wi, false, c.a0, c.a1, Spec<NS>::fill(1.f), c.comp, t);
+    } else if (ms_has<MS>(MS_METAL) && c.type == MAT_MIRROR) {
+        eval_mirror(c.ggx, wo, wi, c.a0, c.comp, t);
+    } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
+        eval_dielectric(c.ggx, wo, wi, c.a1, c.eta0, false, c.a1, c.a0, c.comp, t, eta_out);
     } else {
         // black body: f = 0, pdf = 1 (black_body.cpp:36-43)
-        e.f = Spec<NS>::fill(0.f);
-        e.pdf = 1.f;
-        e.flags = 0;
+        t.f = Spec<NS>::fill(0.f);
+        t.pdf = 1.f;
+        t.flags = 0;
+        e.f = e.f + t.f;
+        e.pdf += t.pdf * sw;
+        return;
     }
+    // one lobe: its hemisphere factor is already in f and pdf (zero where
+    // a reflection lobe sees wi on the other side; a dielectric's is 1)
+    t.f = t.f * fabsf(wi.z);
+    e.f = e.f + t.f * ew;
+    e.pdf += t.pdf * sw;
+    e.flags |= t.flags;
+}
+
+template <int NS, int MS = MS_ALL>
+__device__ __forceinline__ EvalN<NS> leaf_eval(const MatCtx<NS> &c, f3 wi_w, float *eta_out) {
+    EvalN<NS> e;
+    e.f = Spec<NS>::fill(0.f);
+    e.pdf = 0.f;
+    e.flags = 0;
+    leaf_eval_into<NS, MS>(c, wi_w, eta_out, 1.f, 1.f, e);
+    return e;
+}
+
+// the geometric hemisphere check of MaterialEvaluator::evaluate
+// (material.cpp:132-148): no pdf for a reflection across the geometric
+// surface or a transmission that stays on one side
+template <int NS>
+__device__ __forceinline__ void geometric_check(const MatCtx<NS> &c, f3 wi_w, EvalN<NS> &e) {
     bool transmission = (e.flags & FLAG_TRANSMISSION) != 0;
     bool same_geo = dot(c.wo_w, c.ng) * dot(wi_w, c.ng) > 0.f;
     if (same_geo == transmission) e.pdf = 0.f;
+}
+
+// full evaluation of a leaf material in world space: f * |cos wi|, pdf
+template <int NS, int MS = MS_ALL>
+__device__ __forceinline__ EvalN<NS> mat_eval(const MatCtx<NS> &c, f3 wi_w, float *eta_out) {
+    EvalN<NS> e = leaf_eval<NS, MS>(c, wi_w, eta_out);
+    geometric_check(c, wi_w, e);
     return e;
 }
 
@@ -477,22 +534,13 @@ __device__ __forceinline__ bool sample_dielectric_dir(const Ggx &g, f3 wo, const
     return ok && !(wo.z * wi.z > 0.f);
 }
 
+// direction of lobe k of a leaf material (k: the principled stack's lobe,
+// 0 for single-lobe materials), in local coordinates; returns validity
 template <int NS, int MS = MS_ALL>
-__device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &sp) {
-    SampleN<NS> s;
-    s.eta = 1.f;
-    f3 wo = c.wo, wi = mk3(0.f, 0.f, 1.f);
+__device__ __forceinline__ bool leaf_sample_dir(const MatCtx<NS> &c, int k, Sampler &sp, f3 &wi) {
+    f3 wo = c.wo;
     bool valid = true;
-    if (c.type == MAT_BLACK) {
-        // BlackBodyLobe::sample_local: wi = wo, no draws
-        s.wi = c.wo_w;
-        s.e.f = Spec<NS>::fill(0.f);
-        s.e.pdf = 1.f;
-        s.e.flags = 0;
-        bool same_geo = dot(c.wo_w, c.ng) * dot(s.wi, c.ng) > 0.f;
-        if (!same_geo) s.e.pdf = 0.f;
-        return s;
-    }
+    wi = mk3(0.f, 0.f, 1.f);
     if (ms_has<MS>(MS_DIFFUSE) && (c.type == MAT_DIFFUSE || c.type == MAT_OREN_NAYAR)) {
         wi = square_to_cosine_hemisphere(sp.next2d());
         if (wo.z < 0.f) wi.z = -wi.z;
@@ -502,18 +550,7 @@ __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &
         valid = wo.z * wi.z > 0.f;
     } else if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
         valid = sample_dielectric_dir(c.ggx, wo, c.a1, c.eta0, false, c.a1, c.a0, sp, wi);
-    } else if (ms_has<MS>(MS_PRINCIPLED)) {
-        // LobeSet::sample_wi_impl (lobe.cpp:597-627): uc 1 + an unused 2D draw,
-        // then the chosen lobe's draws
-        float uc = sp.next1d();
-        (void)sp.next2d();
-        int k = 0;
-        float sum = 0.f;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) {
-            k = uc > sum ? i : k;
-            sum += c.sw[i];
-        }
+    } else if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
         if (k == 0) {
             f3 w = square_to_cosine_hemisphere(sp.next2d());
             w = mk3(w.x / c.sh_a - w.z * c.sh_b / c.sh_a, w.y / c.sh_a, w.z);
@@ -530,10 +567,133 @@ __device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &
             if (wo.z < 0.f) wi.z = -wi.z;
         }
     }
+    return valid;
+}
+
+template <int NS, int MS = MS_ALL>
+__device__ __forceinline__ SampleN<NS> mat_sample(const MatCtx<NS> &c, Sampler &sp) {
+    SampleN<NS> s;
+    s.eta = 1.f;
+    if (c.type == MAT_BLACK) {
+        // BlackBodyLobe::sample_local: wi = wo, no draws
+        s.wi = c.wo_w;
+        s.e.f = Spec<NS>::fill(0.f);
+        s.e.pdf = 1.f;
+        s.e.flags = 0;
+        bool same_geo = dot(c.wo_w, c.ng) * dot(s.wi, c.ng) > 0.f;
+        if (!same_geo) s.e.pdf = 0.f;
+        return s;
+    }
+    int k = 0;
+    if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
+        // LobeSet::sample_wi_impl (lobe.cpp:597-627): uc 1 + an unused 2D draw,
+        // then the chosen lobe's draws
+        float uc = sp.next1d();
+        (void)sp.next2d();
+        float sum = 0.f;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            k = uc > sum ? i : k;
+            sum += c.sw[i];
+        }
+    }
+    f3 wi;
+    bool valid = leaf_sample_dir<NS, MS>(c, k, sp, wi);
     s.wi = c.fr.to_world(wi);
     s.e = mat_eval<NS, MS>(c, s.wi, &s.eta);
     if (!valid) s.e.pdf = 0.f;
     return s;
+}
+
+// ---- mix / add (MixMaterial, AddMaterial; mix.cpp, add.cpp)
+// A mix holds only its components' record indices and weights; each use
+// rebuilds the component evaluators one at a time, so a kernel's register
+// budget is that of one leaf evaluator.
+template <int NS, bool RGB>
+__device__ __forceinline__ MatCtx<NS> mix_component(const SceneView &sc, const MatCtx<NS> &c, int k,
+                                                    const Swl<NS> &swl) {
+    bool d;
+    return make_material<NS, RGB, MS_ALL>(sc, sc.mat[c.sub[k]], c.fr, c.ng, c.wo_w, swl, d, c.uv);
+}
+
+// LobeSet::evaluate_impl over the flattened lobes (lobe.cpp:668-688): each
+// component's lobes (already masked by their hemisphere factors) weighted by
+// the component's evaluation and sample weights; the dielectric lobes report
+// eta in order, the last one's kept
+template <int NS, bool RGB>
+__device__ __forceinline__ EvalN<NS> mix_eval(const SceneView &sc, const MatCtx<NS> &c, f3 wi_w, float *eta_out,
+                                              const Swl<NS> &swl) {
+    EvalN<NS> e;
+    e.f = Spec<NS>::fill(0.f);
+    e.pdf = 0.f;
+    e.flags = 0;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        MatCtx<NS> m = mix_component<NS, RGB>(sc, c, k, swl);
+        leaf_eval_into<NS, MS_ALL>(m, wi_w, eta_out, c.sub_e[k], c.sub_s[k], e);
+    }
+    geometric_check(c, wi_w, e);
+    return e;
+}
+
+// LobeSet::sample_wi_impl over the flattened lobes: uc 1 + an unused 2D
+// draw pick a lobe by its sample weight (the principled stack's six in
+// order, scaled by its component's weight; a single lobe has the
+// component's weight), then that lobe's draws, then the whole set is
+// evaluated at the sampled direction
+template <int NS, bool RGB>
+__device__ __forceinline__ SampleN<NS> mix_sample(const SceneView &sc, const MatCtx<NS> &c, Sampler &sp,
+                                                  const Swl<NS> &swl) {
+    float uc = sp.next1d();
+    (void)sp.next2d();
+    int pick_c = 0, pick_l = 0;
+    float sum = 0.f;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        if (sc.mat[c.sub[k]].type == MAT_PRINCIPLED) {
+            MatCtx<NS> m = mix_component<NS, RGB>(sc, c, k, swl);
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
+                if (uc > sum) {
+                    pick_c = k;
+                    pick_l = i;
+                }
+                sum += m.sw[i] * c.sub_s[k];
+            }
+        } else {
+            if (uc > sum) {
+                pick_c = k;
+                pick_l = 0;
+            }
+            sum += c.sub_s[k];
+        }
+    }
+    SampleN<NS> s;
+    s.eta = 1.f;
+    f3 wi;
+    bool valid;
+    {
+        MatCtx<NS> m = mix_component<NS, RGB>(sc, c, pick_c, swl);
+        valid = leaf_sample_dir<NS, MS_ALL>(m, pick_l, sp, wi);
+    }
+    s.wi = c.fr.to_world(wi);
+    s.e = mix_eval<NS, RGB>(sc, c, s.wi, &s.eta, swl);
+    if (!valid) s.e.pdf = 0.f;
+    return s;
+}
+
+// MaterialEvaluator::evaluate / ::sample for any material of the set MS
+template <int NS, bool RGB, int MS = MS_ALL>
+__device__ __forceinline__ EvalN<NS> mat_eval_any(const SceneView &sc, const MatCtx<NS> &c, f3 wi_w,
+                                                  float *eta_out, const Swl<NS> &swl) {
+    if (MS == MS_ALL && (c.type == MAT_MIX || c.type == MAT_ADD)) return mix_eval<NS, RGB>(sc, c, wi_w, eta_out, swl);
+    return mat_eval<NS, MS>(c, wi_w, eta_out);
+}
+template <int NS, bool RGB, int MS = MS_ALL>
+__device__ __forceinline__ SampleN<NS> mat_sample_any(const SceneView &sc, const MatCtx<NS> &c, Sampler &sp,
+                                                      const Swl<NS> &swl) {
+    if (MS == MS_ALL && (c.type == MAT_MIX || c.type == MAT_ADD)) return mix_sample<NS, RGB>(sc, c, sp, swl);
+    return mat_sample<NS, MS>(c, sp);
 }
 
 }// namespace dev

@@ -397,8 +397,8 @@ __device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<
         }
     }
     f3 wi_l = normalize(ls.p_light - it.pos);
-    EvalN<NS> se = mat_eval<NS, MS>(mc, wi_l, nullptr);
-    SampleN<NS> bs = mat_sample<NS, MS>(mc, ps.sampler);
+    EvalN<NS> se = mat_eval_any<NS, RGB, MS>(sc, mc, wi_l, nullptr, ps.swl);
+    SampleN<NS> bs = mat_sample_any<NS, RGB, MS>(sc, mc, ps.sampler, ps.swl);
     bool delta = ls.pdf < 0.f;
     float w = delta ? 1.f : mis_weight(ls.pdf, se.pdf);
     if (rv.mis_mode == 1) w = 1.f;

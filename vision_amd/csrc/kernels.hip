@@ -98,14 +98,14 @@ __global__ void __launch_bounds__(256) bsdf_kernel(SceneView sc, uint32_t mat, c
     if (sample) {
         Sampler sp;
         sp.start(0u, i, frame, 0u);
-        SampleN<NS> s = mat_sample(c, sp);
+        SampleN<NS> s = mat_sample_any<NS, RGB>(sc, c, sp, swl);
         wi_out[3 * i] = s.wi.x;
         wi_out[3 * i + 1] = s.wi.y;
         wi_out[3 * i + 2] = s.wi.z;
         fv = s.e.f;
         p = s.e.pdf;
     } else {
-        EvalN<NS> e = mat_eval(c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr);
+        EvalN<NS> e = mat_eval_any<NS, RGB>(sc, c, mk3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]), nullptr, swl);
         fv = e.f;
         p = e.pdf;
     }

@@ -13,6 +13,8 @@ enum MaterialType : uint32_t {
     MAT_GLASS = 4,     // rough dielectric (glass.cpp:171-258)
     MAT_PRINCIPLED = 5,// principled_bsdf lobe stack
     MAT_MIRROR = 6,    // GGX reflection, Fresnel 1, tinted (mirror.cpp:11-74)
+    MAT_MIX = 7,       // two component records blended by a fraction (mix.cpp)
+    MAT_ADD = 8,       // two component records summed (add.cpp)
 };
 
 enum LightType : uint32_t {

@@ -648,9 +648,34 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                 mh.type = m["type"].as_string("diffuse");
                 mh.name = m["name"].as_string("");
                 mh.param = m["param"];
+                // mix / add (MaterialDesc::init, node_desc.cpp:156-161) hold
+                // two component materials, param.mat0 and param.mat1
+                auto leaf = [](const std::string &t) {
+                    return t == "diffuse" || t == "metal" || t == "mirror" || t == "glass" || t == "principled_bsdf";
+                };
+                std::vector<const Json *> with_slots{&mh.param};
+                if (mh.type == "mix" || mh.type == "add") {
+                    for (const char *k : {"mat0", "mat1"}) {
+                        const Json &c = mh.param[k];
+                        std::string ct = c.is_object() ? c["type"].as_string("diffuse") : std::string("(missing)");
+                        if (!c.is_object() || !leaf(ct)) {
+                            mh.unsupported = "unsupported " + mh.type + " component " + k + ": " + ct +
+                                             " (components are diffuse, metal, mirror, glass or principled_bsdf)";
+                            continue;
+                        }
+                        with_slots.push_back(&c["param"]);
+                    }
+                    if (s.integrator.type == "rt")
+                        mh.unsupported = "unsupported: " + mh.type + " materials with the rt integrator (path "
+                                                                     "tracing renders them)";
+                } else if (!leaf(mh.type) && mh.type != "black_body") {
+                    mh.unsupported = "unsupported material type: " + mh.type;
+                }
                 // colour slots may name images; load them once here
-                for (const char *key : {"color", "spec_tint", "sheen_tint", "coat_tint"})
-                    if (const Json *node = slot_image_node(mh.param, key)) texture_index(s, *node);
+                if (mh.unsupported.empty())
+                    for (const Json *sp : with_slots)
+                        for (const char *key : {"color", "spec_tint", "sheen_tint", "coat_tint"})
+                            if (const Json *node = slot_image_node(*sp, key)) texture_index(s, *node);
                 s.materials.push_back(mh);
             }
         }
@@ -728,6 +753,9 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
                 inst.material_name = p["material"].as_string("");
                 for (size_t i = 0; i < s.materials.size(); ++i) {
                     if (s.materials[i].name == inst.material_name) {
+                        if (!s.materials[i].unsupported.empty())
+                            throw std::runtime_error(s.materials[i].unsupported + " (material \"" +
+                                                     inst.material_name + "\" of shape \"" + inst.name + "\")");
                         inst.material = int32_t(i);
                         break;
                     }

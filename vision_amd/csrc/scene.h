@@ -32,9 +32,12 @@ struct InstanceHost {
 
 // A material with its resolved slot values; packed into MaterialRecord.
 struct MaterialHost {
-    std::string type;// diffuse | black_body | metal | glass | principled_bsdf
+    std::string type;// diffuse | black_body | metal | mirror | glass | principled_bsdf | mix | add
     std::string name;
     Json param;
+    // why this material cannot be rendered (empty: it can); reported when a
+    // shape uses it, so a scene file listing materials no shape uses loads
+    std::string unsupported;
 };
 
 // an image used as a colour parameter (ImageNode, src/render_core/shadernodes/image.cpp):

@@ -127,7 +127,11 @@ void put_color_slot(MaterialRecord &r, int off, const SceneHost &s, const Json &
 MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float> &spd, const SceneHost &s) {
     MaterialRecord r{};
     const Json &p = m.param;
-    if (m.type == "diffuse") {
+    if (!m.unsupported.empty()) {
+        // listed but used by no shape (the loader refuses a shape that uses
+        // it): a black body keeps the material indices in place
+        r.type = MAT_BLACK;
+    } else if (m.type == "diffuse") {
         // DiffuseMaterial::initialize_slots (diffuse.cpp:38-46)
         r.type = MAT_DIFFUSE;
         put_color_slot(r, 0, s, p, "color", mk3(0.5f), rgb);
@@ -202,6 +206,12 @@ MaterialRecord pack_material(const MaterialHost &m, bool rgb, std::vector<float>
         r.p[23] = slot1(p, "coat_roughness", 0.2f);
         r.p[24] = slot1(p, "coat_ior", 1.5f);
         r.p[25] = slot1(p, "transmission_weight", 0.f);
+    } else if (m.type == "mix" || m.type == "add") {
+        // MixMaterial / AddMaterial (mix.cpp, add.cpp): the component records
+        // are packed after the scene's own materials (pack_scene fills their
+        // indices into p[0], p[1]); the mix fraction slot defaults to 0.5
+        r.type = m.type == "mix" ? MAT_MIX : MAT_ADD;
+        r.p[2] = m.type == "mix" ? slot1(p, "frac", 0.5f) : 0.f;
     } else {
         throw std::runtime_error("unsupported material type: " + m.type);
     }
@@ -278,6 +288,20 @@ PackedScene pack_scene(const SceneHost &s) {
     out.cie_interval = cie.interval;
     out.cie_y_integral = cie.y_integral;
     for (const MaterialHost &m : s.materials) out.mats.push_back(pack_material(m, out.rgb, out.spd, s));
+    // components of mix / add materials follow the scene's own materials
+    for (size_t i = 0; i < s.materials.size(); ++i) {
+        const MaterialHost &m = s.materials[i];
+        if ((m.type != "mix" && m.type != "add") || !m.unsupported.empty()) continue;
+        for (int k = 0; k < 2; ++k) {
+            const Json &c = m.param[k == 0 ? "mat0" : "mat1"];
+            MaterialHost ch;
+            ch.type = c["type"].as_string("diffuse");
+            ch.name = c["name"].as_string("");
+            ch.param = c["param"];
+            out.mats[i].p[k] = ubits_f(uint32_t(out.mats.size()));
+            out.mats.push_back(pack_material(ch, out.rgb, out.spd, s));
+        }
+    }
     for (const TextureHost &t : s.textures) {
         out.tex_info.insert(out.tex_info.end(), {uint32_t(out.tex.size() / 4), t.width, t.height, t.illum ? 1u : 0u});
         for (size_t i = 0; i < size_t(t.width) * t.height; ++i) {
