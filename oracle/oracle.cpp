@@ -1271,19 +1271,30 @@ struct Ctx {
 
 float mis_w(float a, float b) { return a / (a + b); }
 
+// The surface HitContext::it receives (integrator.cpp:181-183): the one hit
+// at bounce 0 (including the bounce-0 hit of the max_depth < 2 extra ray)
+struct FirstHit {
+    bool set{false};
+    V3 pos{0.f, 0.f, 0.f}, ng{0.f, 0.f, 0.f};
+};
+
 // IlluminationIntegrator::Li (src/base/integrator.cpp:134-283), with
-// evaluate_miss (integrator.cpp:111-132) and direct_lighting (integrator.cpp:22-38)
-V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w) {
+// evaluate_miss (integrator.cpp:111-132) and direct_lighting (integrator.cpp:22-38).
+// thr0 / pdf0 / depth: the starting throughput, scatter pdf and max_depth
+// (1, 1e16 and the scene's for a camera path; ReSTIR GI starts its paths at
+// the primary surface, indirect.cpp:65-83); `first` records the bounce-0 hit.
+V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w, const Spec *thr0 = nullptr, float pdf0 = 1e16f,
+      int depth = -1, FirstHit *first = nullptr) {
     const Scene &s = *c.s;
     const Spectral &sd = *c.sd;
     const LightTables &lt = *c.lt;
     int n = w.n;
-    uint32_t max_depth = uint32_t(s.render[0]), min_depth = uint32_t(s.render[1]);
+    uint32_t max_depth = depth >= 0 ? uint32_t(depth) : uint32_t(s.render[0]), min_depth = uint32_t(s.render[1]);
     float rr_th = s.render[2];
     int mis_mode = int(s.render[3]);
     V3 ret = v3(0, 0, 0);
-    Spec thr = sfill(1.f);
-    float scatter_pdf = 1e16f, eta_scale = 1.f;
+    Spec thr = thr0 ? *thr0 : sfill(1.f);
+    float scatter_pdf = pdf0, eta_scale = 1.f;
     uint32_t skips = 0;
     auto correct = [&](float wgt, uint32_t b) {// correct_bsdf_weight
         if (mis_mode == 2) return 1.f;
@@ -1315,6 +1326,7 @@ V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w) {
             --b;
             continue;
         }
+        if (first && b == 0) *first = FirstHit{true, it.pos, it.ng};
         if (light != kInv) {
             Spec Le;
             float lpdf;
@@ -1388,6 +1400,7 @@ V3 Li(const Ctx &c, V3 ro, V3 rd, Sampler &sp, Swl &w) {
                 if (++skips > 64u) break;
                 continue;
             }
+            if (first) *first = FirstHit{true, it.pos, it.ng};
             if (light != kInv) {
                 Spec Le;
                 float lpdf;
@@ -1925,11 +1938,15 @@ bool delta_sample(const Spectral &sd, const Bsdf &b, Sampler &sp, int n, V3 &wi_
 
 // ReSTIRDI::Li from the BSDF (direct.cpp:57-121)
 Spec bsdf_candidate(const Ctx &c, const Surf &it, const Bsdf &b, Sampler &sp, const Swl &w, DSample &d, float &pdf,
-                    float &light_pdf) {
+                    float &light_pdf, float *hit_bsdf = nullptr) {
     const Scene &s = *c.s;
     const Spectral &sd = *c.sd;
     const LightTables &lt = *c.lt;
     BSample bs = bsdf_sample(sd, b, sp, w.n);
+    if (hit_bsdf) {// HitBSDF {wi, f.vec3(), pdf} (direct.cpp:82-85)
+        hit_bsdf[0] = bs.wi.x, hit_bsdf[1] = bs.wi.y, hit_bsdf[2] = bs.wi.z, hit_bsdf[3] = bs.e.pdf;
+        hit_bsdf[4] = bs.e.f.v[0], hit_bsdf[5] = bs.e.f.v[1], hit_bsdf[6] = bs.e.f.v[2];
+    }
     V3 nn = dotp(it.ng, bs.wi) > 0.f ? it.ng : neg(it.ng);
     Hit h = c.trace({offset_origin(it.pos, nn), bs.wi, 0.f, FLT_MAX}, false);
     pdf = bs.e.pdf;
@@ -1966,7 +1983,7 @@ Spec bsdf_candidate(const Ctx &c, const Surf &it, const Bsdf &b, Sampler &sp, co
 }
 
 // ReSTIRDI::RIS (direct.cpp:161-233)
-Rsv ris(const Ctx &c, const RParams &rp, const Surf &it, Sampler &sp, Swl &w, V3 thr) {
+Rsv ris(const Ctx &c, const RParams &rp, const Surf &it, Sampler &sp, Swl &w, V3 thr, float *hit_bsdf = nullptr) {
     const Scene &s = *c.s;
     const Spectral &sd = *c.sd;
     const LightTables &lt = *c.lt;
@@ -1992,7 +2009,7 @@ Rsv ris(const Ctx &c, const RParams &rp, const Surf &it, Sampler &sp, Swl &w, V3
     for (uint32_t i = 0; i < rp.M_bsdf; ++i) {
         DSample d;
         float pdf, lpdf;
-        Spec f = bsdf_candidate(c, it, b, sp, w, d, pdf, lpdf);
+        Spec f = bsdf_candidate(c, it, b, sp, w, d, pdf, lpdf, hit_bsdf);
         d.p_hat = lum3(f, w.n);
         r.update(sp.next1d(), d, safe_weight_of(pdf / (Ml * lpdf + Mb * pdf), d.p_hat, 1.f / pdf) * lum, 1.f);
     }
@@ -2082,7 +2099,7 @@ void prev_raster(const float *cam, const float *prev_w2c, V3 p, float &rx, float
 
 // shader0 for one pixel (direct.cpp:519-555; compute_hit :434-517)
 void restir_initial_px(const Ctx &c, const FisTable &fis, const RParams &rp, const RState &st, uint32_t x,
-                       uint32_t y) {
+                       uint32_t y, float *aux = nullptr) {
     const Scene &s = *c.s;
     const Spectral &sd = *c.sd;
     const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u;
@@ -2135,7 +2152,10 @@ void restir_initial_px(const Ctx &c, const FisTable &fis, const RParams &rp, con
     srec_store(st, cslot, pix, cs);
     if (cs.replaced) sext_store(st, cslot, pix, ext);
     Rsv r;
-    if (hit) r = ris(c, rp, it, sp, w, ext.thr);
+    // the HitBSDF of the last BSDF candidate (for ReSTIR GI; its default
+    // when there is none: wi 0, f 0, pdf -1, interaction.h:60-66)
+    float hb[7] = {0.f, 0.f, 0.f, -1.f, 0.f, 0.f, 0.f};
+    if (hit) r = ris(c, rp, it, sp, w, ext.thr, hb);
     r.update_W(r.s.p_hat);
     r.truncation(1.f);
     float mx = 0.f, my = 0.f;
@@ -2143,6 +2163,12 @@ void restir_initial_px(const Ctx &c, const FisTable &fis, const RParams &rp, con
         float rx, ry;
         prev_raster(s.cam, rp.prev_w2c, add(o, mul(d, ext.t_max)), rx, ry);
         mx = pf[0] - rx, my = pf[1] - ry;
+    }
+    if (aux) {// [3][P]: wi pdf | f 0 | motion 0 0
+        float *a0 = aux + 4 * size_t(pix), *a1 = aux + 4 * (size_t(rp.P) + pix), *a2 = aux + 4 * (2 * size_t(rp.P) + pix);
+        a0[0] = hb[0], a0[1] = hb[1], a0[2] = hb[2], a0[3] = hb[3];
+        a1[0] = hb[4], a1[1] = hb[5], a1[2] = hb[6], a1[3] = 0.f;
+        a2[0] = mx, a2[1] = my, a2[2] = 0.f, a2[3] = 0.f;
     }
     if (hit) r.occlude(occluded_to(c, it, r.s.pos));
     r = temporal_reuse(c, rp, st, r, cs, pix, mx, my, pf, sp, w);
@@ -2284,6 +2310,290 @@ void restir_spatial_px(const Ctx &c, const RParams &rp, const RState &st, uint32
     Rsv r = rsv_load(st, in, pix);
     r = spatial_reuse(c, rp, st, r, cs, x, y, in, sp, w);
     rsv_store(st, out, pix, r);
+}
+
+// ---------------------------------------------------------------- ReSTIR GI
+// The indirect half of the real-time integrator
+// (src/render_core/integrators/ReSTIR/indirect.cpp): an initial sample per
+// pixel (a path from the primary surface along the direction of ReSTIR DI's
+// BSDF candidate), temporal reuse, spatial reuse with visibility, shading;
+// GIReservoir / GISample (ReSTIR/reservoir.h:157-222).  State as the device
+// buffers: GI reservoirs [3][4][P] (weight_sum C W age | sample position |
+// sample normal | Lo), samples [4][P] (the same four fields), and the
+// per-pixel record ReSTIR DI leaves for it [3][P]: HitBSDF (wi, pdf | f) and
+// the motion vector.
+struct GParams {
+    bool open;
+    uint32_t max_age;
+    float diff;
+    bool spatial;
+    uint32_t s_num;
+    float s_dot, s_depth, s_radius;
+    bool temporal;
+    uint32_t t_limit, t_N;
+    float t_dot, t_depth, t_radius;
+    int depth;// the real-time integrator's max_depth - 1 (rt.cpp:30)
+};
+struct GState {
+    float *rsv, *smp, *aux, *direct, *indirect;
+    uint32_t P;
+    float *at(float *base, uint32_t buf, uint32_t nfield, uint32_t field, uint32_t i) const {
+        return base + 4 * ((size_t(buf) * nfield + field) * P + i);
+    }
+};
+struct GSample {
+    V3 pos{0.f, 0.f, 0.f}, n{0.f, 0.f, 0.f}, Lo{0.f, 0.f, 0.f};
+    float age{0.f};
+};
+struct GRsv {
+    float ws{0.f}, C{0.f}, W{0.f};
+    GSample s;
+    // GIReservoir::update / truncation / cal_W (reservoir.h:196-219)
+    void update(float u, const GSample &v, float w, float c = 1.f) {
+        ws += w;
+        C += c;
+        if (u * ws < w || ws == 0.f) s = v;
+    }
+    void truncation(float limit) {
+        float factor = limit / C;
+        C = std::min(limit, C);
+        ws = factor < 1.f ? ws * factor : ws;
+    }
+    void update_W(float p) { W = p == 0.f ? 0.f : ws / (p * C); }
+};
+GSample gsample_load(const GState &g, float *base, uint32_t buf, uint32_t i, float *age_out = nullptr) {
+    const float *a = g.at(base, buf, 4, 0, i), *b = g.at(base, buf, 4, 1, i), *c = g.at(base, buf, 4, 2, i),
+                *d = g.at(base, buf, 4, 3, i);
+    GSample s;
+    s.age = a[3];
+    s.pos = v3(b[0], b[1], b[2]);
+    s.n = v3(c[0], c[1], c[2]);
+    s.Lo = v3(d[0], d[1], d[2]);
+    if (age_out) *age_out = a[3];
+    return s;
+}
+GRsv grsv_load(const GState &g, uint32_t buf, uint32_t i) {
+    GRsv r;
+    r.s = gsample_load(g, g.rsv, buf, i);
+    const float *a = g.at(g.rsv, buf, 4, 0, i);
+    r.ws = a[0], r.C = a[1], r.W = a[2];
+    return r;
+}
+void gfields_store(const GState &g, float *base, uint32_t buf, uint32_t i, float f0, float f1, float f2,
+                   const GSample &s) {
+    float *a = g.at(base, buf, 4, 0, i), *b = g.at(base, buf, 4, 1, i), *c = g.at(base, buf, 4, 2, i),
+          *d = g.at(base, buf, 4, 3, i);
+    a[0] = f0, a[1] = f1, a[2] = f2, a[3] = s.age;
+    b[0] = s.pos.x, b[1] = s.pos.y, b[2] = s.pos.z, b[3] = 0.f;
+    c[0] = s.n.x, c[1] = s.n.y, c[2] = s.n.z, c[3] = 0.f;
+    d[0] = s.Lo.x, d[1] = s.Lo.y, d[2] = s.Lo.z, d[3] = 0.f;
+}
+void grsv_store(const GState &g, uint32_t buf, uint32_t i, const GRsv &r) { gfields_store(g, g.rsv, buf, i, r.ws, r.C, r.W, r.s); }
+
+// Pipeline::compute_surface_interaction(hit, view_pos): the surface seen
+// from view_pos
+Surf surface_from(const Scene &s, const SRec &cs, V3 view) {
+    Surf it = surface(s, cs.tri, cs.b1, cs.b2, true);
+    it.wo = unit(sub(view, it.pos));
+    return it;
+}
+
+// ReSTIRGI::eval_bsdf / compute_p_hat (indirect.cpp:112-130): f |cos| of the
+// surface's material towards the sample point, luminance of Lo f (sRGB)
+V3 gi_bsdf(const Ctx &c, const Surf &it, const GSample &g, Swl &w) {
+    const Scene &s = *c.s;
+    if (s.inst[4 * it.inst] == kInv) return v3(0.f, 0.f, 0.f);
+    Bsdf b = bsdf_at(*c.sd, s, it, w, true);
+    BEval e = bsdf_eval(*c.sd, b, unit(sub(g.pos, it.pos)), w.n, nullptr);
+    return v3(e.f.v[0], e.f.v[1], e.f.v[2]);
+}
+float gi_p_hat(const Ctx &c, const Surf &it, const GSample &g, Swl &w) {
+    V3 f = gi_bsdf(c, it, g, w);
+    return luminance(v3(g.Lo.x * f.x, g.Lo.y * f.y, g.Lo.z * f.z));
+}
+
+// ReSTIRGI::Jacobian_det (indirect.cpp:15-31)
+float gi_jacobian(V3 cur, V3 nb, const GSample &g) {
+    V3 cv = sub(cur, g.pos), nv = sub(nb, g.pos);
+    float cpc = std::fabs(dotp(unit(cv), g.n)), cpn = std::fabs(dotp(unit(nv), g.n));
+    float r = (cpc * dotp(nv, nv)) / (cpn * dotp(cv, cv));
+    r = zero_nan_inf(r);
+    const float lower = 0.6f;
+    return std::min(std::max(r, lower), 1.f / lower);
+}
+
+// the initial sample (ReSTIRGI::init_sample / initial_samples,
+// indirect.cpp:65-108): the path from the primary surface along the DI BSDF
+// candidate's direction, its radiance divided by that candidate's
+// throughput, the bounce-0 surface as the sample point
+void gi_initial_px(const Ctx &c, const RParams &rp, const GParams &gp, const RState &st, const GState &g,
+                   uint32_t x, uint32_t y) {
+    const Scene &s = *c.s;
+    const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u;
+    float *rad = g.indirect + 4 * size_t(pix);
+    SRec cs = srec_load(st, cslot, pix);
+    if (!cs.hit()) {
+        rad[0] = rad[1] = rad[2] = rad[3] = 0.f;
+        return;
+    }
+    Swl w = sample_wavelengths(*c.sd, x, y, rp.frame);
+    SExt ext;
+    if (cs.replaced) ext = sext_load(st, cslot, pix);
+    V3 view = cs.replaced ? ext.view : rp.cam_pos;
+    Surf it = surface_from(s, cs, view);
+    const float *h0 = g.at(g.aux, 0, 3, 0, pix), *h1 = g.at(g.aux, 0, 3, 1, pix);
+    V3 wi = v3(h0[0], h0[1], h0[2]);
+    const float pdf = h0[3];
+    V3 t = v3(zero_nan_inf(h1[0] / pdf), zero_nan_inf(h1[1] / pdf), zero_nan_inf(h1[2] / pdf));
+    Sampler sp;
+    sp.start(x, y, rp.frame, 3u);
+    V3 nn = dotp(it.ng, wi) > 0.f ? it.ng : neg(it.ng);
+    Spec thr = sfill(0.f);
+    thr.v[0] = t.x, thr.v[1] = t.y, thr.v[2] = t.z;
+    FirstHit fh;
+    V3 L = Li(c, offset_origin(it.pos, nn), wi, sp, w, &thr, pdf, gp.depth, &fh);
+    GSample smp;
+    smp.Lo = v3(zero_nan_inf(L.x / t.x), zero_nan_inf(L.y / t.y), zero_nan_inf(L.z / t.z));
+    if (fh.set) smp.pos = fh.pos, smp.n = fh.ng;
+    if (cs.replaced) smp.Lo = v3(smp.Lo.x * ext.thr.x, smp.Lo.y * ext.thr.y, smp.Lo.z * ext.thr.z);
+    gfields_store(g, g.smp, 0, pix, 0.f, 0.f, 0.f, smp);
+}
+
+// ReSTIRGI::combine_temporal (indirect.cpp:132-152)
+GRsv gi_combine_temporal(const Ctx &c, const GRsv &cur, const SRec &cs, GRsv other, const SRec *nb, V3 view,
+                         V3 prev_view, Sampler &sp, Swl &w) {
+    const Scene &s = *c.s;
+    other.s.age += 1.f;
+    Surf it = surface_from(s, cs, view);
+    GRsv ret;
+    float cur_p = gi_p_hat(c, it, cur.s, w);
+    ret.update(sp.next1d(), cur.s, safe_weight_of(cur.C, cur_p, cur.W));
+    float other_p = gi_p_hat(c, it, other.s, w);
+    ret.update(sp.next1d(), other.s, safe_weight_of(other.C, other_p, other.W), other.C);
+    float p = gi_p_hat(c, it, ret.s, w);
+    if (nb) p = p * gi_jacobian(it.pos, surface_from(s, *nb, prev_view).pos, other.s);
+    ret.update_W(p);
+    return ret;
+}
+
+// the temporal pass (compile_temporal_reuse / temporal_reuse,
+// indirect.cpp:154-238): the initial sample as a reservoir, combined with the
+// reprojected pixel's previous reservoir, or failing that with one of up to
+// temporal.N random pixels within temporal.radius of it (2 draws each)
+void gi_temporal_px(const Ctx &c, const FisTable &fis, const RParams &rp, const GParams &gp, const RState &st,
+                    const GState &g, uint32_t x, uint32_t y) {
+    const Scene &s = *c.s;
+    const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u, pslot = rp.frame & 1u;
+    SRec cs = srec_load(st, cslot, pix);
+    if (!cs.hit()) return;
+    Swl w = sample_wavelengths(*c.sd, x, y, rp.frame);
+    float pf[2];
+    {
+        Sampler tmp;
+        tmp.start(x, y, rp.frame, 0u);
+        V3 o, d;
+        float fw;
+        camera_ray(s, fis, x, y, tmp, o, d, fw, pf);
+    }
+    Sampler sp;
+    sp.start(x, y, rp.frame, 4u);
+    GSample smp = gsample_load(g, g.smp, 0, pix);
+    const float *h0 = g.at(g.aux, 0, 3, 0, pix), *h1 = g.at(g.aux, 0, 3, 1, pix), *mv = g.at(g.aux, 0, 3, 2, pix);
+    GRsv rsv;
+    float p_hat = luminance(v3(smp.Lo.x * h1[0], smp.Lo.y * h1[1], smp.Lo.z * h1[2]));
+    rsv.update(0.5f, smp, safe_weight_of(1.f, p_hat, 1.f / h0[3]));
+    rsv.update_W(p_hat);
+    const float prev_x = pf[0] - mv[0], prev_y = pf[1] - mv[1];
+    const float limit = rsv.C * float(gp.t_limit);
+    V3 view = cs.replaced ? sext_load(st, cslot, pix).view : rp.cam_pos;
+    V3 prev_view = rp.prev_cam_pos;
+    int ix = cast_i32(prev_x), iy = cast_i32(prev_y);
+    if (ix >= 0 && iy >= 0 && ix < int(rp.W) && iy < int(rp.H) && gp.temporal) {
+        auto visit = [&](uint32_t j, GRsv &pr, SRec &ps) {
+            pr = grsv_load(g, pslot, j);
+            pr.truncation(limit);
+            ps = srec_load(st, pslot, j);
+            if (ps.replaced) prev_view = sext_load(st, pslot, j).view;
+        };
+        auto valid = [&](const SRec &ps, const GRsv &pr) {
+            return pr.s.age < float(gp.max_age) && valid_neighbor(cs, ps, gp.t_dot, gp.t_depth, gp.diff);
+        };
+        GRsv pr;
+        SRec ps;
+        visit(std::min(cast_u32(prev_y), rp.H - 1) * rp.W + std::min(cast_u32(prev_x), rp.W - 1), pr, ps);
+        if (valid(ps, pr)) {
+            rsv = gi_combine_temporal(c, rsv, cs, pr, nullptr, view, prev_view, sp, w);
+        } else {
+            for (uint32_t i = 0; i < gp.t_N; ++i) {
+                float ux = sp.next1d(), uy = sp.next1d();
+                float dx, dy;
+                square_to_disk(ux, uy, dx, dy);
+                float qx = dx * gp.t_radius + prev_x, qy = dy * gp.t_radius + prev_y;
+                GRsv ar;
+                SRec as;
+                visit(std::min(cast_u32(qy), rp.H - 1) * rp.W + std::min(cast_u32(qx), rp.W - 1), ar, as);
+                if (valid(as, ar)) {
+                    rsv = gi_combine_temporal(c, rsv, cs, ar, &as, view, prev_view, sp, w);
+                    break;
+                }
+            }
+        }
+    }
+    grsv_store(g, 2, pix, rsv);
+}
+
+// spatial reuse with visibility, then shading (compile_spatial_shading,
+// spatial_reuse, constant_combine, shading; indirect.cpp:240-332)
+void gi_spatial_px(const Ctx &c, const RParams &rp, const GParams &gp, const RState &st, const GState &g, uint32_t x,
+                   uint32_t y) {
+    const Scene &s = *c.s;
+    const uint32_t pix = y * rp.W + x, cslot = (rp.frame + 1u) & 1u;
+    SRec cs = srec_load(st, cslot, pix);
+    if (!cs.hit()) return;
+    Swl w = sample_wavelengths(*c.sd, x, y, rp.frame);
+    Sampler sp;
+    sp.start(x, y, rp.frame, 5u);
+    GRsv rsv = grsv_load(g, 2, pix);
+    V3 view = rp.cam_pos;
+    if (cs.replaced) view = sext_load(st, cslot, pix).view;
+    if (gp.spatial) {
+        uint32_t idx[64];
+        uint32_t cnt = 0;
+        for (uint32_t i = 0; i < gp.s_num; ++i) {
+            float ux = sp.next1d(), uy = sp.next1d();
+            float dx, dy;
+            square_to_disk(ux, uy, dx, dy);
+            float ox = dx * gp.s_radius, oy = dy * gp.s_radius;
+            int ax = std::min(std::max(int(x) + int(std::round(ox)), 0), int(rp.W) - 1);
+            int ay = std::min(std::max(int(y) + int(std::round(oy)), 0), int(rp.H) - 1);
+            uint32_t j = uint32_t(ay) * rp.W + uint32_t(ax);
+            if (valid_neighbor(cs, srec_load(st, cslot, j), gp.s_dot, gp.s_depth, gp.diff)) idx[cnt++] = j;
+        }
+        Surf cit = surface_from(s, cs, view);
+        GRsv ret = rsv;
+        for (uint32_t k = 0; k < cnt; ++k) {
+            GRsv nr = grsv_load(g, 2, idx[k]);
+            if (!(luminance(nr.s.Lo) > 0.f)) continue;
+            SRec ns = srec_load(st, cslot, idx[k]);
+            Surf nit = surface_from(s, ns, view);
+            float p = gi_p_hat(c, cit, nr.s, w) * gi_jacobian(cit.pos, nit.pos, nr.s);
+            float v = occluded_to(c, cit, nr.s.pos) ? 0.f : 1.f;
+            float wt = safe_weight_of(nr.C, p, nr.W);
+            ret.update(sp.next1d(), nr.s, wt * v, nr.C * v);
+        }
+        ret.update_W(gi_p_hat(c, cit, ret.s, w));
+        rsv = ret;
+    }
+    // ReSTIRGI::shading: Lo f W (the near-specular chain's throughput is read
+    // but unused there, indirect.cpp:305-320)
+    Surf it = surface_from(s, cs, view);
+    V3 f = gi_bsdf(c, it, rsv.s, w);
+    float *rad = g.indirect + 4 * size_t(pix);
+    rad[0] = rsv.s.Lo.x * f.x * rsv.W;
+    rad[1] = rsv.s.Lo.y * f.y * rsv.W;
+    rad[2] = rsv.s.Lo.z * f.z * rsv.W;
+    rad[3] = 1.f;
+    grsv_store(g, (rp.frame + 1u) & 1u, pix, rsv);
 }
 
 }// namespace
@@ -2633,7 +2943,7 @@ uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const 
                          const float *cam, const float *render, uint32_t nv, uint32_t nt, uint32_t ni, uint32_t nm,
                          uint32_t nl, const or_spectral *spec, const float *restir, const float *prev_w2c,
                          const float *prev_pos, uint32_t frame, uint32_t threads, float *rsv, float *surf, float *ext,
-                         float *rt) {
+                         float *rt, float *gi_rsv, float *gi_smp, float *gi_aux, float *gi_rad) {
     Scene s{P, N, UV, tri, tri_inst, inst, mat, light, cam, render, nv, nt, ni, nm, nl};
     Spectral sd = to_spectral(spec);
     Bvh2 b;
@@ -2687,14 +2997,33 @@ uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const 
         for (uint32_t t = 0; t < nth; ++t) th.emplace_back(work);
         for (auto &t : th) t.join();
     };
-    rows([&](uint32_t x, uint32_t y) { restir_initial_px(c, fis, rp, st, x, y); });
+    // ReSTIR GI settings (restir[32..47]) and state
+    GParams gp{};
+    gp.open = restir[32] != 0.f && gi_rsv != nullptr;
+    gp.max_age = uint32_t(restir[33]);
+    gp.diff = restir[34];
+    gp.spatial = restir[35] != 0.f;
+    gp.s_num = uint32_t(restir[36]);
+    gp.s_dot = restir[37];
+    gp.s_depth = restir[38];
+    gp.s_radius = restir[39];
+    gp.temporal = restir[40] != 0.f;
+    gp.t_limit = uint32_t(restir[41]);
+    gp.t_N = uint32_t(restir[42]);
+    gp.t_dot = restir[43];
+    gp.t_depth = restir[44];
+    gp.t_radius = restir[45];
+    gp.depth = int(restir[46]);
+    GState g{gi_rsv, gi_smp, gi_aux, nullptr, gi_rad, rp.P};
+    std::vector<float> direct;
+    if (gp.open) direct.assign(4 * size_t(rp.P), 0.f);
+    rows([&](uint32_t x, uint32_t y) { restir_initial_px(c, fis, rp, st, x, y, gp.open ? gi_aux : nullptr); });
     uint32_t in = 2;
     for (uint32_t k = 0; k < rp.extra; ++k) {
         rows([&](uint32_t x, uint32_t y) { restir_spatial_px(c, rp, st, x, y, in, in ^ 1u, 2u + k); });
         in ^= 1u;
     }
-    rows([&](uint32_t x, uint32_t y) {
-        V3 L = restir_final_px(c, fis, rp, st, x, y, in);
+    auto film_add = [&](uint32_t x, uint32_t y, V3 L) {
         float v[4] = {zero_nan_inf(L.x), zero_nan_inf(L.y), zero_nan_inf(L.z), 1.f};
         float *acc = rt + 4 * (size_t(y) * rp.W + x);
         if (accumulate) {
@@ -2703,6 +3032,25 @@ uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const 
         } else {
             for (int k = 0; k < 4; ++k) acc[k] = v[k];
         }
+    };
+    rows([&](uint32_t x, uint32_t y) {
+        V3 L = restir_final_px(c, fis, rp, st, x, y, in);
+        if (!gp.open) {
+            film_add(x, y, L);
+            return;
+        }
+        float *dl = direct.data() + 4 * (size_t(y) * rp.W + x);
+        dl[0] = L.x, dl[1] = L.y, dl[2] = L.z, dl[3] = 1.f;
+    });
+    if (!gp.open) return 0;
+    // ReSTIRGI::dispatch (indirect.cpp:357-365), then the combine of
+    // RealTimeIntegrator::compile (rt.cpp:119-126): direct x 1 + indirect x 1
+    rows([&](uint32_t x, uint32_t y) { gi_initial_px(c, rp, gp, st, g, x, y); });
+    rows([&](uint32_t x, uint32_t y) { gi_temporal_px(c, fis, rp, gp, st, g, x, y); });
+    rows([&](uint32_t x, uint32_t y) { gi_spatial_px(c, rp, gp, st, g, x, y); });
+    rows([&](uint32_t x, uint32_t y) {
+        const float *dl = direct.data() + 4 * (size_t(y) * rp.W + x), *il = gi_rad + 4 * (size_t(y) * rp.W + x);
+        film_add(x, y, v3(dl[0] * 1.f + il[0] * 1.f, dl[1] * 1.f + il[1] * 1.f, dl[2] * 1.f + il[2] * 1.f));
     });
     return 0;
 }

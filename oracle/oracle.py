@@ -70,7 +70,7 @@ def lib():
                                 [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, C.c_void_p, _f32p])
         h.or_restir_frame.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
                                       [C.c_uint32] * 5 + [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_uint32,
-                                                          _f32p, _f32p, _f32p, _f32p])
+                                                          _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p])
         h.or_reservoir_stream.restype = C.c_int32
         h.or_reservoir_stream.argtypes = [_f32p, _f32p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         h.or_prev_raster.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p]
@@ -297,6 +297,12 @@ class Restir:
         self.ext = np.zeros((2, 2, P, 4), np.float32)
         self.ext[:, 0, :, :3] = 1.0
         self.film = np.zeros((self.H, self.W, 4), np.float32)
+        # ReSTIR GI: reservoirs [3][4][P], samples [4][P], DI's per-pixel
+        # record [3][P] (HitBSDF, motion), indirect radiance [P]
+        self.gi_rsv = np.zeros((3, 4, P, 4), np.float32)
+        self.gi_smp = np.zeros((4, P, 4), np.float32)
+        self.gi_aux = np.zeros((3, P, 4), np.float32)
+        self.gi_rad = np.zeros((P, 4), np.float32)
         self.frame = 0
         self.threads = threads
         self.scene = scene
@@ -317,7 +323,8 @@ class Restir:
                                   len(a["positions"]), len(a["triangles"]), len(a["instances"]),
                                   len(a["materials"]), len(a["lights"]), spec.ptr,
                                   np.ascontiguousarray(a["restir"], np.float32), self.prev_w2c, self.prev_pos,
-                                  self.frame, self.threads, self.rsv, self.surf, self.ext, self.film)
+                                  self.frame, self.threads, self.rsv, self.surf, self.ext, self.film, self.gi_rsv,
+                                  self.gi_smp, self.gi_aux, self.gi_rad)
             self.frame += 1
             self.prev_w2c = np.ascontiguousarray(a["restir"][20:32], np.float32)
             self.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)

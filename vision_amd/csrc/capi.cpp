@@ -188,7 +188,7 @@ void affine_inverse(const float *m, float *out) {
 float radians_to_cos(float deg) { return std::cos(deg * (3.14159265358979323846f / 180.f)); }
 
 void fill_restir_export(vs_scene *s) {
-    s->exp_restir.assign(32, 0.f);
+    s->exp_restir.assign(48, 0.f);
     const vs::IntegratorHost &in = s->host.integrator;
     if (in.type != "rt") return;
     const vs::RestirHost &r = in.restir;
@@ -214,6 +214,24 @@ void fill_restir_export(vs_scene *s) {
     e[18] = r.t_mis ? 1.f : 0.f;
     e[19] = float(r.extra_spatial_passes);
     affine_inverse(s->cam.c2w, e + 20);
+    // ReSTIR GI: construct_param (indirect.cpp:334-352) and the path depth
+    // (the real-time integrator's max_depth - 1, rt.cpp:30)
+    const vs::RestirGiHost &g = in.gi;
+    e[32] = g.open ? 1.f : 0.f;
+    e[33] = float(g.max_age);
+    e[34] = g.diff_factor;
+    e[35] = g.s_open ? 1.f : 0.f;
+    e[36] = float(g.s_num);
+    e[37] = radians_to_cos(g.s_theta);
+    e[38] = g.s_depth;
+    e[39] = g.s_radius;
+    e[40] = g.t_open ? 1.f : 0.f;
+    e[41] = float(g.t_limit);
+    e[42] = float(g.t_N);
+    e[43] = radians_to_cos(g.t_theta);
+    e[44] = g.t_depth;
+    e[45] = g.t_radius;
+    e[46] = float(std::max(int(in.max_depth) - 1, 0));
 }
 
 void fill_render_export(vs_scene *s) {

@@ -575,7 +575,28 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
             r.t_N = tp["N"].as_uint(9);
             r.t_mis = tp["mis"].as_bool(false);
             if (!r.open)
-                throw std::runtime_error("unsupported: rt integrator with ReSTIR DI closed (the GI half is not built)");
+                throw std::runtime_error("unsupported: rt integrator with ReSTIR DI closed");
+            // ReSTIR GI (ReSTIR(integrator, desc), ReSTIR/common.h:98-105)
+            RestirGiHost &g = s.integrator.gi;
+            const Json &ii = p["indirect"];
+            g.open = ii["open"].as_bool(true);
+            g.max_age = ii["max_age"].as_uint(30);
+            g.diff_factor = float(ii["diffuse_factor"].as_float(0.3));
+            const Json &gs = ii["spatial"];
+            g.s_theta = float(gs["theta"].as_float(35.0));
+            g.s_depth = float(gs["depth"].as_float(0.3));
+            g.s_radius = float(gs["radius"].as_float(30.0));
+            g.s_num = gs["sample_num"].as_uint(1);
+            g.s_open = gs["open"].as_bool(true);
+            if (!g.s_open) g.s_num = 1;
+            const Json &gt = ii["temporal"];
+            g.t_limit = gt["history_limit"].as_uint(5);
+            g.t_radius = float(gt["radius"].as_float(8.0));
+            g.t_theta = float(gt["theta"].as_float(35.0));
+            g.t_depth = float(gt["depth"].as_float(0.3));
+            g.t_open = gt["open"].as_bool(true);
+            g.t_N = gt["N"].as_uint(9);
+            if (g.s_num > 64 || g.t_N > 64) throw std::runtime_error("ReSTIR GI settings out of range");
             if (r.s_num > 64 || r.t_N > 64 || r.M_light > 4096 || r.M_bsdf > 64 || r.max_recursion > 64 ||
                 r.extra_spatial_passes > 8)
                 throw std::runtime_error("ReSTIR DI settings out of range");
@@ -593,6 +614,9 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         }
         else if (s.spectrum_type == "srgb") s.spectrum_dim = 3;
         else throw std::runtime_error("unsupported spectrum type: " + s.spectrum_type);
+        if (s.integrator.type == "rt" && s.integrator.gi.open && s.spectrum_type != "srgb")
+            throw std::runtime_error("unsupported: ReSTIR GI (the rt integrator's open \"indirect\" block) with "
+                                     "hero-wavelength spectra; close it or render in sRGB mode");
     }
     // sampler
     {

@@ -107,6 +107,20 @@ struct RestirHost {
     uint32_t extra_spatial_passes{0};
 };
 
+// the "indirect" block: ReSTIR GI (ReSTIR/indirect.h:10-28, the ReSTIR base
+// and resampling parameters of ReSTIR/common.h:19-113)
+struct RestirGiHost {
+    bool open{true};
+    uint32_t max_age{30};
+    float diff_factor{0.3f};
+    bool s_open{true};
+    float s_theta{35.f}, s_depth{0.3f}, s_radius{30.f};
+    uint32_t s_num{1};
+    bool t_open{true};
+    uint32_t t_limit{5}, t_N{9};
+    float t_radius{8.f}, t_theta{35.f}, t_depth{0.3f};
+};
+
 struct IntegratorHost {
     std::string type{"pt"};
     uint32_t max_depth{16}, min_depth{5};
@@ -114,6 +128,7 @@ struct IntegratorHost {
     int mis_mode{0};
     Json param;
     RestirHost restir;
+    RestirGiHost gi;
 };
 
 struct SceneHost {

@@ -421,17 +421,22 @@ def write_scene(scene: dict, path: str | Path) -> Path:
 
 def restir_integrator(M_light: int = 10, M_bsdf: int = 1, spatial: bool = True, sample_num: int = 1,
                       radius: float = 30.0, temporal: bool = True, history_limit: int = 5, pairwise: bool = False,
-                      temporal_mis: bool = False, extra_spatial_passes: int = 0, **extra) -> dict:
+                      temporal_mis: bool = False, extra_spatial_passes: int = 0, indirect: dict | None = None,
+                      max_depth: int = 2, **extra) -> dict:
     """The "rt" integrator with its ReSTIR DI ("direct") block
     (src/render_core/integrators/rt.cpp, ReSTIR/direct.cpp:10-17,
-    ReSTIR/common.h:19-67)."""
+    ReSTIR/common.h:19-67) and its ReSTIR GI ("indirect") block, closed
+    unless `indirect` gives it (the reference opens it when the block is
+    absent)."""
     direct = {"M_light": M_light, "M_bsdf": M_bsdf, "pairwise": pairwise,
               "spatial": {"open": spatial, "sample_num": sample_num, "radius": radius, "theta": 35, "depth": 0.3,
                           "extra_passes": extra_spatial_passes},
               "temporal": {"open": temporal, "history_limit": history_limit, "radius": 8, "theta": 35,
                            "depth": 0.3, "mis": temporal_mis}}
     direct.update(extra)
-    return {"type": "rt", "param": {"direct": direct, "max_depth": 2, "min_depth": 5, "rr_threshold": 1}}
+    gi = {"open": False} if indirect is None else indirect
+    return {"type": "rt", "param": {"direct": direct, "indirect": gi, "max_depth": max_depth, "min_depth": 5,
+                                    "rr_threshold": 1}}
 
 
 def many_lights_box(width: int = 1920, height: int = 1080, n_quads: int = 5000, n_points: int = 256,
