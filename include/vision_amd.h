@@ -82,7 +82,8 @@ typedef struct vs_render_stats {
      * the wavefront form [0] closest-hit traversal, [1] shading, [2] any-hit
      * traversal; with the "rt" integrator [0] shader0 (candidates,
      * visibility, temporal reuse), [1] the extra spatial passes, [2] shader1
-     * (spatial reuse, shading); zeros otherwise */
+     * (spatial reuse, shading) and, with ReSTIR GI open, the two GI kernels
+     * and the combine; zeros otherwise */
     double stage_ms[3];
     uint32_t stage_launches[3];
     uint64_t stage_rays[3], stage_node_fetches[3], stage_tri_fetches[3];
@@ -239,7 +240,11 @@ vs_status vs_scene_set_kernel(vs_scene *scene, int32_t mode, uint32_t chunk_fram
  * Both produce the same film.  With the "rt" integrator each frame runs ReSTIR
  * DI (RealTimeIntegrator::render, src/render_core/integrators/rt.cpp:144-160,
  * direct half): initial candidates + temporal reuse, any extra spatial
- * passes, spatial reuse + shading into the film.  Synchronizes before
+ * passes, spatial reuse + shading; with the "indirect" block open (sRGB
+ * mode) then ReSTIR GI (ReSTIRGI::dispatch,
+ * src/render_core/integrators/ReSTIR/indirect.cpp:354-362: initial samples
+ * fused with temporal reuse, spatial reuse + shading) and the combine of
+ * both halves into the film (rt.cpp:119-126).  Synchronizes before
  * returning. */
 vs_status vs_render(vs_scene *scene, uint32_t spp, int32_t fused);
 /* ReSTIR DI state after the last rendered frame (the "rt" integrator): the
@@ -250,6 +255,13 @@ vs_status vs_render(vs_scene *scene, uint32_t spp, int32_t fused);
  * diffuse_factor).  Integer fields are stored as their bit patterns.
  * Test and inspection hook; no reference counterpart. */
 vs_status vs_scene_restir_state(vs_scene *scene, float *reservoirs, float *surfaces);
+/* ReSTIR GI state after the last rendered frame: the GI reservoir buffers
+ * [3][4][pixels][4] (previous / current by frame parity, pass-through;
+ * fields weight_sum C W age | sample position | sample normal | Lo, the
+ * GIReservoir of src/render_core/integrators/ReSTIR/reservoir.h:157-222)
+ * and the indirect radiance [pixels][4].  Test and inspection hook; no
+ * reference counterpart. */
+vs_status vs_scene_restir_gi_state(vs_scene *scene, float *reservoirs, float *radiance);
 /* Enable per-ray counters (rays, node and triangle fetches) for later renders. */
 vs_status vs_set_counting(vs_scene *scene, int32_t enable);
 vs_status vs_last_render_stats(const vs_scene *scene, vs_render_stats *out);

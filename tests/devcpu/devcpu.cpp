@@ -2,7 +2,7 @@
 // the HIP kernels use, compiled by g++ against tests/devcpu/hip/hip_runtime.h,
 // driven like the kernels drive them, for bit-level comparison with the oracle.
 #include "dev_pt.h"
-#include "dev_restir.h"
+#include "dev_restir_gi.h"
 #include "data_tables.h"
 #include <vector>
 using namespace vs;
@@ -171,7 +171,8 @@ extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *fil
 // ---- ReSTIR DI frame of the device code (restir.hip's three kernels, in order)
 template <int NS, bool RGB, int MS>
 static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, const float *prev_pos, uint32_t frame,
-                         float *rsv, float *surf, float *ext, float *film, const float *render) {
+                         float *rsv, float *surf, float *ext, float *film, const float *render, float *gi_rsv,
+                         float *gi_aux, float *gi_rad) {
     RestirView v{};
     v.rsv = reinterpret_cast<float4 *>(rsv);
     v.surf = reinterpret_cast<float4 *>(surf);
@@ -200,6 +201,14 @@ static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, cons
     v.temporal_mis = e[18] != 0.f;
     for (int i = 0; i < 12; ++i) v.w2c[i] = e[20 + i], v.prev_w2c[i] = prev_w2c[i];
     for (int i = 0; i < 3; ++i) v.cam_pos[i] = s->cam.c2w[4 * i + 3], v.prev_cam_pos[i] = prev_pos[i];
+    // ReSTIR GI open (restir[32]): shader0's record and the direct radiance
+    const bool gi = RGB && e[32] != 0.f && gi_rsv;
+    std::vector<float4> direct;
+    if (gi) {
+        direct.resize(v.pixels);
+        v.gi_aux = reinterpret_cast<float4 *>(gi_aux);
+        v.gi_direct = direct.data();
+    }
     std::vector<uint2> stack(64);
     PathCounters pc{0, 0, 0};
     for (uint32_t y = 0; y < v.height; ++y)
@@ -211,29 +220,72 @@ static void restir_frame(DcScene *s, const float *e, const float *prev_w2c, cons
         in ^= 1u;
     }
     RenderView rv{};
+    rv.max_depth = uint32_t(render[0]);
+    rv.min_depth = uint32_t(render[1]);
+    rv.rr_threshold = render[2];
+    rv.mis_mode = int(render[3]);
     rv.accumulation = render[6] != 0.f;
     float4 *f4 = reinterpret_cast<float4 *>(film);
     for (uint32_t y = 0; y < v.height; ++y)
         for (uint32_t x = 0; x < v.width; ++x) {
             f3 L = restir_final<NS, RGB, MS>(s->v, s->cam, v, x, y, in, stack.data(), 1, pc);
-            f4[y * v.width + x] = film_add(rv, f4[y * v.width + x], L, frame);
+            if (gi) direct[y * v.width + x] = make_float4(L.x, L.y, L.z, 1.f);
+            else f4[y * v.width + x] = film_add(rv, f4[y * v.width + x], L, frame);
+        }
+    if (!gi) return;
+    // ReSTIR GI: restir.hip's two GI kernels, the second with the combine
+    RestirGiView g{};
+    g.rsv = reinterpret_cast<float4 *>(gi_rsv);
+    g.aux = reinterpret_cast<const float4 *>(gi_aux);
+    g.direct = direct.data();
+    g.rad = reinterpret_cast<float4 *>(gi_rad);
+    g.max_age = uint32_t(e[33]);
+    g.diff_factor = e[34];
+    g.spatial = e[35] != 0.f;
+    g.s_num = uint32_t(e[36]);
+    g.s_dot = e[37];
+    g.s_depth = e[38];
+    g.s_radius = e[39];
+    g.temporal = e[40] != 0.f;
+    g.t_limit = uint32_t(e[41]);
+    g.t_N = uint32_t(e[42]);
+    g.t_dot = e[43];
+    g.t_depth = e[44];
+    g.t_radius = e[45];
+    g.depth = uint32_t(e[46]);
+    for (uint32_t y = 0; y < v.height; ++y)
+        for (uint32_t x = 0; x < v.width; ++x)
+            restir_gi_initial_temporal<MS>(s->v, s->cam, rv, v, g, x, y, stack.data(), 1, pc);
+    for (uint32_t y = 0; y < v.height; ++y)
+        for (uint32_t x = 0; x < v.width; ++x) {
+            const uint32_t idx = y * v.width + x;
+            const SurfaceRec cs = load_surface(v, cur_slot(v.frame), idx);
+            f3 il = mk3(0.f, 0.f, 0.f);
+            if (cs.hit()) {
+                il = restir_gi_spatial_shade<MS>(s->v, v, g, cs, x, y, stack.data(), 1, pc);
+                g.rad[idx] = make_float4(il.x, il.y, il.z, 1.f);
+            }
+            const float4 dl = direct[idx];
+            f3 L = mk3(dl.x * 1.f + il.x * 1.f, dl.y * 1.f + il.y * 1.f, dl.z * 1.f + il.z * 1.f);
+            f4[idx] = film_add(rv, f4[idx], L, frame);
         }
 }
 
 extern "C" void dc_restir_frame(void *p, const float *restir, const float *prev_w2c, const float *prev_pos,
-                                uint32_t frame, float *rsv, float *surf, float *ext, float *film, const float *render) {
+                                uint32_t frame, float *rsv, float *surf, float *ext, float *film, const float *render,
+                                float *gi_rsv, float *gi_aux, float *gi_rad) {
     auto *s = static_cast<DcScene *>(p);
     // the material set the library picks (diffuse family only, or all)
     bool diffuse_only = true;
     for (const MaterialRecord &m : s->pk.mats)
         if (m.type != MAT_DIFFUSE && m.type != MAT_OREN_NAYAR && m.type != MAT_BLACK) diffuse_only = false;
     if (diffuse_only) {
-        if (s->pk.rgb) restir_frame<3, true, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-        else restir_frame<3, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        if (s->pk.rgb) restir_frame<3, true, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
+        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
+        else restir_frame<3, false, MS_DIFFUSE>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
     } else {
-        if (s->pk.rgb) restir_frame<3, true, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
-        else restir_frame<3, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render);
+        if (s->pk.rgb) restir_frame<3, true, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
+        else if (s->h.spectrum_dim == 4) restir_frame<4, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
+        else restir_frame<3, false, MS_ALL>(s, restir, prev_w2c, prev_pos, frame, rsv, surf, ext, film, render, gi_rsv, gi_aux, gi_rad);
     }
 }

@@ -150,7 +150,7 @@ def test_device_path_source_equals_oracle_direct_only(dc, tmp_path, max_depth):
 def _restir_pair(dc, scene, frames, base=".", cameras=None, fov=19.5):
     """The device ReSTIR source and the oracle over the same frames; returns
     both (film, reservoirs, surfaces)."""
-    dc.dc_restir_frame.argtypes = [C.c_void_p] + [_f32] * 3 + [C.c_uint32] + [_f32] * 5
+    dc.dc_restir_frame.argtypes = [C.c_void_p] + [_f32] * 3 + [C.c_uint32] + [_f32] * 8
     ctx = va.Context(-1)
     sc = ctx.load_text(scene, base).build_accel()
     ex = sc.export()
@@ -165,7 +165,8 @@ def _restir_pair(dc, scene, frames, base=".", cameras=None, fov=19.5):
         dc.dc_set_camera(p, cam)
         dc.dc_restir_frame(p, np.ascontiguousarray(ex["restir"], np.float32), dev.prev_w2c, dev.prev_pos, f,
                            dev.rsv, dev.surf, dev.ext, dev.film.reshape(-1),
-                           np.ascontiguousarray(ex["render"], np.float32))
+                           np.ascontiguousarray(ex["render"], np.float32), dev.gi_rsv.reshape(-1),
+                           dev.gi_aux.reshape(-1), dev.gi_rad.reshape(-1))
         dev.prev_w2c = np.ascontiguousarray(ex["restir"][20:32], np.float32)
         dev.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)
         orc.render(1, scene=ex)
@@ -177,6 +178,10 @@ def _restir_equal(dev, orc):
     assert _bits_equal(dev.surf, orc.surf)
     assert _bits_equal(dev.rsv, orc.rsv)
     assert _bits_equal(dev.film, orc.film)
+    # ReSTIR GI state (zeros when its block is closed)
+    assert _bits_equal(dev.gi_aux, orc.gi_aux)
+    assert _bits_equal(dev.gi_rsv, orc.gi_rsv)
+    assert _bits_equal(dev.gi_rad, orc.gi_rad)
 
 
 @pytest.mark.parametrize("cfg", [dict(sample_num=3, radius=5),
@@ -263,3 +268,46 @@ def test_device_path_source_equals_oracle_emission_image(dc, tmp_path, spectrum)
     scene["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4)
     dev, orc = _restir_pair(dc, scene, 3, str(tmp_path))
     _restir_equal(dev, orc)
+
+
+# ---------------------------------------------------------------- ReSTIR GI
+def _gi_scene(w, h, max_depth=4, temporal=True, spatial=True, point_lights=False, **gi_extra):
+    s = scenegen.cornell_box(w, h, max_depth=max_depth, light_sampler="power")
+    if point_lights:
+        s["shapes"] = [x for x in s["shapes"] if x["name"] != "light"]
+        s["light_sampler"]["param"]["lights"] = [
+            {"type": "point", "param": {"position": [0.2, 1.7, 0.3], "color": [6, 5, 4], "scale": 1.0}},
+            {"type": "spot", "param": {"position": [-0.5, 1.6, -0.4], "color": [3, 4, 5], "scale": 1.0,
+                                       "direction": [0.2, -1, 0.1], "angle": 50, "falloff": 10}}]
+    gi = {"open": True, "max_age": 6, "temporal": {"open": temporal, "history_limit": 8, "radius": 4, "N": 5},
+          "spatial": {"open": spatial, "sample_num": 3, "radius": 4}, **gi_extra}
+    s["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=4, indirect=gi,
+                                                 max_depth=max_depth)
+    return s
+
+
+@pytest.mark.parametrize("cfg", [dict(), dict(point_lights=True), dict(temporal=False, spatial=False),
+                                 dict(max_depth=2), dict(max_depth=1), dict(max_depth=6, spatial=False)],
+                         ids=["area", "point_spot", "no_reuse", "depth2", "depth1", "depth6"])
+def test_device_restir_gi_source_equals_oracle(dc, cfg):
+    scene = _gi_scene(22, 18, **cfg)
+    dev, orc = _restir_pair(dc, scene, 4)
+    _restir_equal(dev, orc)
+    # GI ran: the pass-through reservoirs hold samples with light
+    assert (orc.gi_rsv[2, 3, :, :3].sum(-1) > 0).mean() > (0.2 if cfg.get("max_depth", 4) > 1 else -1)
+
+
+def test_device_restir_gi_source_equals_oracle_moving_camera_near_specular(dc):
+    scene = _gi_scene(24, 20)
+    scene["materials"] += [{"type": "metal", "name": "Mirror", "param": {"material_name": "Cu", "roughness": 0.01}},
+                           {"type": "metal", "name": "Rough", "param": {"material_name": "Al", "roughness": 0.3}}]
+    for sh in scene["shapes"]:
+        if sh["name"] == "short_box":
+            sh["param"]["material"] = "Mirror"
+        if sh["name"] == "tall_box":
+            sh["param"]["material"] = "Rough"
+    cams = scenegen.jitter_path(4, amplitude=0.03, seed=9)
+    dev, orc = _restir_pair(dc, scene, 4, cameras=cams)
+    _restir_equal(dev, orc)
+    # temporal reuse kept history somewhere
+    assert orc.gi_rsv[2, 0, :, 1].max() > 1

@@ -159,6 +159,7 @@ SIGNATURES = {
     "vs_scene_set_spectrum": (C.c_int32, [_P, C.c_int32, C.c_uint32]),
     "vs_scene_update_camera": (C.c_int32, [_P, _P, C.c_float]),
     "vs_scene_restir_state": (C.c_int32, [_P, _P, _P]),
+    "vs_scene_restir_gi_state": (C.c_int32, [_P, _P, _P]),
     "vs_trace_closest": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
@@ -425,6 +426,17 @@ class Scene:
         surf = np.zeros((2, 3, n, 4), np.float32)
         self.ctx._check(lib().vs_scene_restir_state(self._h, rsv.ctypes.data, surf.ctypes.data))
         return rsv, surf
+
+    def restir_gi_state(self) -> tuple[np.ndarray, np.ndarray]:
+        """ReSTIR GI buffers after the last frame: reservoirs float32[3, 4, H*W, 4]
+        (weight_sum C W age | position | normal | Lo) and the indirect
+        radiance float32[H*W, 4]."""
+        info = self.info()
+        n = info["width"] * info["height"]
+        rsv = np.zeros((3, 4, n, 4), np.float32)
+        rad = np.zeros((n, 4), np.float32)
+        self.ctx._check(lib().vs_scene_restir_gi_state(self._h, rsv.ctypes.data, rad.ctypes.data))
+        return rsv, rad
 
     def set_kernel(self, mode: str = "megakernel", chunk_frames: int = 8, trace_block: int = 128):
         """Kernel form of render(): "megakernel" or "wavefront"."""

@@ -806,7 +806,37 @@ vs::RestirView restir_view(vs_scene *s, uint32_t frame) {
     }
     for (int i = 0; i < 12; ++i) v.w2c[i] = e[20 + i];
     affine_inverse(s->prev_c2w, v.prev_w2c);
+    if (s->host.integrator.gi.open && s->gi_pixels == v.pixels) {
+        v.gi_aux = s->gi_aux.as<float4>();
+        v.gi_direct = s->gi_direct.as<float4>();
+    }
     return v;
+}
+
+// ReSTIR GI settings and state (construct_param, indirect.cpp:335-352, as
+// exported in restir[32..46])
+vs::RestirGiView restir_gi_view(vs_scene *s) {
+    const float *e = s->exp_restir.data();
+    vs::RestirGiView g{};
+    g.rsv = s->gi_rsv.as<float4>();
+    g.aux = s->gi_aux.as<float4>();
+    g.direct = s->gi_direct.as<float4>();
+    g.rad = s->gi_rad.as<float4>();
+    g.max_age = uint32_t(e[33]);
+    g.diff_factor = e[34];
+    g.spatial = e[35] != 0.f ? 1u : 0u;
+    g.s_num = uint32_t(e[36]);
+    g.s_dot = e[37];
+    g.s_depth = e[38];
+    g.s_radius = e[39];
+    g.temporal = e[40] != 0.f ? 1u : 0u;
+    g.t_limit = uint32_t(e[41]);
+    g.t_N = uint32_t(e[42]);
+    g.t_dot = e[43];
+    g.t_depth = e[44];
+    g.t_radius = e[45];
+    g.depth = uint32_t(e[46]);
+    return g;
 }
 
 // RealTimeIntegrator::render (rt.cpp:144-160), direct half: per frame
@@ -822,6 +852,15 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
         s->rs_ext.alloc(size_t(P) * 2u * 2u * 16u);
         s->rs_pixels = P;
         hip_check(vs::restir_clear(restir_view(s, 0), st), "restir clear");
+    }
+    const bool gi = s->host.integrator.gi.open;
+    if (gi && s->gi_pixels != P) {
+        s->gi_rsv.alloc(size_t(P) * vs::kGiReservoirBuffers * 4u * 16u);
+        s->gi_aux.alloc(size_t(P) * 3u * 16u);
+        s->gi_direct.alloc(size_t(P) * 16u);
+        s->gi_rad.alloc(size_t(P) * 16u);
+        s->gi_pixels = P;
+        hip_check(vs::restir_gi_clear(restir_gi_view(s), P, st), "restir gi clear");
     }
     const bool rgb = s->packed.rgb;
     const uint32_t dim = rv.spectrum_dim;
@@ -845,6 +884,17 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
         hip_check(vs::restir_final(s->view, s->cam, v, rv, dim, rgb, ms, in, s->film_rt.as<float4>(),
                                    s->film_out.as<float4>(), ctr ? ctr + 4 : nullptr, st),
                   "restir final");
+        if (gi) {
+            // ReSTIR GI and the combine (rt.cpp:153-158), timed with shader1
+            vs::RestirGiView g = restir_gi_view(s);
+            hip_check(vs::restir_gi_initial_temporal(s->view, s->cam, v, g, rv, ms, ctr ? ctr + 4 : nullptr, st),
+                      "restir gi initial / temporal");
+            hip_check(vs::restir_gi_spatial_shade(s->view, v, g, rv, ms, s->film_rt.as<float4>(),
+                                                  s->film_out.as<float4>(), ctr ? ctr + 4 : nullptr, st),
+                      "restir gi spatial / shading");
+            launches += 2u;
+            ts.stage_launches[2] += 2u;
+        }
         hip_check(hipEventRecord(s->ctx->ev1, st), "event");
         hip_check(hipEventSynchronize(s->ctx->ev1), "restir sync");
         float t0 = 0.f, t1 = 0.f, t2 = 0.f;
@@ -874,6 +924,19 @@ vs_status vs_scene_restir_state(vs_scene *s, float *reservoirs, float *surfaces)
         hip_check(hipStreamSynchronize(s->ctx->stream), "restir sync");
         hip_check(hipMemcpy(reservoirs, s->rs_rsv.ptr, s->rs_rsv.bytes, hipMemcpyDeviceToHost), "restir D2H");
         hip_check(hipMemcpy(surfaces, s->rs_surf.ptr, s->rs_surf.bytes, hipMemcpyDeviceToHost), "restir D2H");
+    });
+}
+
+vs_status vs_scene_restir_gi_state(vs_scene *s, float *reservoirs, float *radiance) {
+    if (!s || !reservoirs || !radiance)
+        return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (s->gi_pixels == 0) throw std::logic_error("no ReSTIR GI frame rendered");
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(s->ctx->stream), "restir gi sync");
+        hip_check(hipMemcpy(reservoirs, s->gi_rsv.ptr, s->gi_rsv.bytes, hipMemcpyDeviceToHost), "restir gi D2H");
+        hip_check(hipMemcpy(radiance, s->gi_rad.ptr, s->gi_rad.bytes, hipMemcpyDeviceToHost), "restir gi D2H");
     });
 }
 

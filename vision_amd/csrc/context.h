@@ -138,6 +138,10 @@ struct vs_scene {
     // and the camera of the previous frame (3x4 rows of camera-to-world)
     DevBuf rs_rsv, rs_surf, rs_ext;
     uint32_t rs_pixels{0};
+    // ReSTIR GI (the "indirect" block open): reservoirs, shader0's record,
+    // direct and indirect radiance
+    DevBuf gi_rsv, gi_aux, gi_direct, gi_rad;
+    uint32_t gi_pixels{0};
     float prev_c2w[12]{};
     vs::SceneView view{};
     vs::CameraView cam{};

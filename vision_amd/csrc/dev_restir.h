@@ -484,14 +484,28 @@ __device__ __forceinline__ bool occluded(const SceneView &sc, const Surface &it,
 // `light_pdf` the light's own point-measure pdf
 // The evaluator is rebuilt here (the same arithmetic as the caller's), so
 // that no evaluator stays live across the traversal below.
+// HitBSDF of a BSDF candidate {wi, f, pdf} (ReSTIRDI::Li, direct.cpp:82-85),
+// kept for ReSTIR GI; its default without a candidate is wi 0, f 0, pdf -1
+// (interaction.h:60-66)
+struct HitBsdfRec {
+    f3 wi, f;
+    float pdf;
+};
+
 template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Surface &it, Sampler &sp,
                                                    Swl<NS> &swl, DISample &s, float &pdf, float &light_pdf,
-                                                   uint2 *stack, uint32_t stride, PathCounters &pc) {
+                                                   uint2 *stack, uint32_t stride, PathCounters &pc,
+                                                   HitBsdfRec *hb) {
     SampleN<NS> bs;
     {
         MatCtx<NS> mc = material_at<NS, RGB, MS>(sc, it, swl);
         bs = mat_sample<NS, MS>(mc, sp);
+    }
+    if (hb) {
+        hb->wi = bs.wi;
+        hb->pdf = bs.e.pdf;
+        hb->f = mk3(bs.e.f.v[0], bs.e.f.v[NS > 1 ? 1 : 0], bs.e.f.v[NS > 2 ? 2 : 0]);
     }
     f3 n = dot(it.ng, bs.wi) > 0.f ? it.ng : -it.ng;
     TraceHit h = traverse<false>(sc, offset_ray_origin(it.pos, n), bs.wi, 0.f, 3.402823466e38f, stack, stride,
@@ -544,7 +558,7 @@ __device__ __forceinline__ Spec<NS> bsdf_candidate(const SceneView &sc, const Su
 template <int NS, bool RGB, int MS>
 __device__ __forceinline__ Reservoir ris(const SceneView &sc, const RestirView &rv, const Surface &it, Sampler &sp,
                                          Swl<NS> &swl, f3 throughput, uint2 *stack, uint32_t stride,
-                                         PathCounters &pc) {
+                                         PathCounters &pc, HitBsdfRec *hb) {
     Reservoir r = empty_reservoir();
     if (rv.M_light + rv.M_bsdf == 0u) return r;
     const float lum = luminance(throughput);
@@ -570,7 +584,7 @@ __device__ __forceinline__ Reservoir ris(const SceneView &sc, const RestirView &
     for (uint32_t i = 0; i < rv.M_bsdf; ++i) {
         DISample s = empty_sample();
         float pdf, lpdf;
-        Spec<NS> f = bsdf_candidate<NS, RGB, MS>(sc, it, sp, swl, s, pdf, lpdf, stack, stride, pc);
+        Spec<NS> f = bsdf_candidate<NS, RGB, MS>(sc, it, sp, swl, s, pdf, lpdf, stack, stride, pc, hb);
         s.p_hat = luminance_of(f);
         float w = safe_weight(pdf / (Ml * lpdf + Mb * pdf), s.p_hat, 1.f / pdf) * lum;
         r.update(sp.next1d(), s, w, 1.f);
@@ -723,7 +737,11 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
     store_surface(rv, cslot, pix, cs);
     if (cs.replaced) store_ext(rv, cslot, pix, ext);
     Reservoir r = empty_reservoir();
-    if (hit) r = ris<NS, RGB, MS>(sc, rv, it, sp, swl, ext.throughput, stack, stride, pc);
+    HitBsdfRec hb;
+    hb.wi = mk3(0.f, 0.f, 0.f);
+    hb.f = mk3(0.f, 0.f, 0.f);
+    hb.pdf = -1.f;
+    if (hit) r = ris<NS, RGB, MS>(sc, rv, it, sp, swl, ext.throughput, stack, stride, pc, rv.gi_aux ? &hb : nullptr);
     r.update_W(r.s.p_hat);
     r.truncation(1.f);
     // FrameBuffer::compute_motion_vec (frame_buffer.cpp:325-333)
@@ -731,6 +749,13 @@ __device__ void restir_initial(const SceneView &sc, const CameraView &cam, const
     if (hit) {
         f2 pr = prev_raster(cam, rv, o + d * ext.t_max);
         motion = mk2(p_film.x - pr.x, p_film.y - pr.y);
+    }
+    if (rv.gi_aux) {
+        // the HitBSDF and motion vector ReSTIR GI reads
+        // (frame_buffer.h hit_bsdfs / motion_vectors)
+        rv.gi_aux[pix] = make_float4(hb.wi.x, hb.wi.y, hb.wi.z, hb.pdf);
+        rv.gi_aux[size_t(rv.pixels) + pix] = make_float4(hb.f.x, hb.f.y, hb.f.z, 0.f);
+        rv.gi_aux[2u * size_t(rv.pixels) + pix] = make_float4(motion.x, motion.y, 0.f, 0.f);
     }
     if (hit) r.process_occluded(occluded(sc, it, r.s.pos, stack, stride, pc));
     r = temporal_reuse<NS, RGB, MS>(sc, rv, r, cs, pix, motion, p_film, sp, swl);

@@ -3,7 +3,7 @@
 // traversal stacks in LDS.
 #include <hip/hip_runtime.h>
 
-#include "dev_restir.h"
+#include "dev_restir_gi.h"
 
 namespace vs {
 
@@ -68,6 +68,61 @@ __global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam,
     PathCounters pc{0, 0, 0};
     f3 L = restir_final<NS, RGB, MS>(sc, cam, v, px, py, in, lds_stack + threadIdx.x, blockDim.x, pc);
     uint32_t idx = py * v.width + px;
+    add_counters(counters, pc);
+    if (v.gi_direct) {
+        // ReSTIR GI open: the combine adds this after the indirect half
+        v.gi_direct[idx] = make_float4(L.x, L.y, L.z, 1.f);
+        return;
+    }
+    float4 acc = film_add(rv, rt[idx], L, v.frame);
+    rt[idx] = acc;
+    if (rv.write_output) out[idx] = film_output(rv, acc);
+}
+
+// ---- ReSTIR GI
+
+__global__ void __launch_bounds__(256) gi_clear_kernel(RestirGiView g, uint32_t pixels) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    GiReservoir r = empty_gi_reservoir();
+    for (uint32_t b = 0; b < kGiReservoirBuffers; ++b) store_gi_reservoir(g, pixels, b, i, r);
+    g.rad[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+template <int MS>
+__global__ void __launch_bounds__(64) gi_initial_temporal_kernel(SceneView sc, CameraView cam, RestirView v,
+                                                                 RestirGiView g, RenderView rv,
+                                                                 unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= v.width || py >= v.height) return;
+    PathCounters pc{0, 0, 0};
+    restir_gi_initial_temporal<MS>(sc, cam, rv, v, g, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
+    add_counters(counters, pc);
+}
+
+// spatial reuse and shading, then RealTimeIntegrator's combine (rt.cpp:119-126):
+// direct x 1 + indirect x 1 into the film's running mean
+template <int MS>
+__global__ void __launch_bounds__(64) gi_spatial_shade_kernel(SceneView sc, RestirView v, RestirGiView g,
+                                                              RenderView rv, float4 *__restrict__ rt,
+                                                              float4 *__restrict__ out,
+                                                              unsigned long long *__restrict__ counters) {
+    extern __shared__ uint2 lds_stack[];
+    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    if (px >= v.width || py >= v.height) return;
+    const uint32_t idx = py * v.width + px;
+    PathCounters pc{0, 0, 0};
+    const SurfaceRec cs = load_surface(v, cur_slot(v.frame), idx);
+    f3 il = mk3(0.f, 0.f, 0.f);
+    if (cs.hit()) {
+        il = restir_gi_spatial_shade<MS>(sc, v, g, cs, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
+        g.rad[idx] = make_float4(il.x, il.y, il.z, 1.f);
+    }
+    const float4 dl = g.direct[idx];
+    f3 L = mk3(dl.x * 1.f + il.x * 1.f, dl.y * 1.f + il.y * 1.f, dl.z * 1.f + il.z * 1.f);
     float4 acc = film_add(rv, rt[idx], L, v.frame);
     rt[idx] = acc;
     if (rv.write_output) out[idx] = film_output(rv, acc);
@@ -116,6 +171,40 @@ struct Final {
 };
 
 }// namespace
+
+hipError_t restir_gi_clear(const RestirGiView &g, uint32_t pixels, hipStream_t stream) {
+    if (pixels == 0) return hipSuccess;
+    hipLaunchKernelGGL(gi_clear_kernel, dim3((pixels + 255u) / 256u), dim3(256), 0, stream, g, pixels);
+    return hipGetLastError();
+}
+
+hipError_t restir_gi_initial_temporal(const SceneView &sc, const CameraView &cam, const RestirView &v,
+                                      const RestirGiView &g, const RenderView &rv, int material_set,
+                                      unsigned long long *counters, hipStream_t stream) {
+    if (material_set != MS_ALL && material_set != MS_DIFFUSE) return hipErrorInvalidValue;
+    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+    if (material_set == MS_ALL)
+        hipLaunchKernelGGL((gi_initial_temporal_kernel<MS_ALL>), tiles(v), dim3(64), lds, stream, sc, cam, v, g, rv,
+                           counters);
+    else
+        hipLaunchKernelGGL((gi_initial_temporal_kernel<MS_DIFFUSE>), tiles(v), dim3(64), lds, stream, sc, cam, v, g,
+                           rv, counters);
+    return hipGetLastError();
+}
+
+hipError_t restir_gi_spatial_shade(const SceneView &sc, const RestirView &v, const RestirGiView &g,
+                                   const RenderView &rv, int material_set, float4 *rt, float4 *out,
+                                   unsigned long long *counters, hipStream_t stream) {
+    if (material_set != MS_ALL && material_set != MS_DIFFUSE) return hipErrorInvalidValue;
+    size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
+    if (material_set == MS_ALL)
+        hipLaunchKernelGGL((gi_spatial_shade_kernel<MS_ALL>), tiles(v), dim3(64), lds, stream, sc, v, g, rv, rt, out,
+                           counters);
+    else
+        hipLaunchKernelGGL((gi_spatial_shade_kernel<MS_DIFFUSE>), tiles(v), dim3(64), lds, stream, sc, v, g, rv, rt,
+                           out, counters);
+    return hipGetLastError();
+}
 
 hipError_t restir_clear(const RestirView &v, hipStream_t stream) {
     if (v.pixels == 0) return hipSuccess;
