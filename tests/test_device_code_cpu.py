@@ -11,6 +11,7 @@ oracle's.
 """
 import ctypes as C
 import json
+import os
 import subprocess
 from pathlib import Path
 
@@ -41,8 +42,11 @@ def _harness():
         BUILD.mkdir(parents=True, exist_ok=True)
         data = BUILD / "data"
         if not data.exists():
-            data.symlink_to(vbuild.DATA)
-        tmp = LIB.with_suffix(".tmp")
+            try:
+                data.symlink_to(vbuild.DATA)
+            except FileExistsError:  # another pytest worker made it first
+                pass
+        tmp = LIB.with_suffix(f".tmp{os.getpid()}")
         cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-ffp-contract=off", "-w", "-I", str(HERE / "devcpu"),
                "-I", str(vbuild.CSRC), "-o", str(tmp), str(SRC), *map(str, objs), "-lpthread", "-lz"]
         subprocess.run(cmd, check=True, capture_output=True)
