@@ -107,9 +107,11 @@ typedef struct vs_scene_arrays {
                                    thin_lens flag, filter kind (0 box, 1 triangle,
                                    2 gaussian, 3 mitchell, 4 sinc), gaussian sigma,
                                    mitchell b, c, sinc tau, 3 unused */
-    const float *render;        /* [8]: max_depth, min_depth, rr_threshold, mis_mode,
+    const float *render;        /* [12]: max_depth, min_depth, rr_threshold, mis_mode,
                                    light_sampler(0 uniform,1 power), ray_offset_factor,
-                                   accumulation, exposure */
+                                   accumulation, exposure, tone mapper (0 linear,
+                                   1 aces, 2 reinhard), adaptive sampling on,
+                                   its threshold, its minimum sample number */
     uint32_t num_vertices, num_triangles, num_instances, num_materials, num_lights;
     const float *spd;           /* [num_spd]: CIE X, Y, Z, D65 (4 x 94) then material curves */
     uint32_t num_spd;
@@ -262,6 +264,40 @@ vs_status vs_scene_restir_state(vs_scene *scene, float *reservoirs, float *surfa
  * and the indirect radiance [pixels][4].  Test and inspection hook; no
  * reference counterpart. */
 vs_status vs_scene_restir_gi_state(vs_scene *scene, float *reservoirs, float *radiance);
+/* Adaptive sampling of the "pt" integrator (ConvergenceInspector,
+ * src/render_core/integrators/adaptive/inspector.cpp:9-38, used by
+ * PathTracingIntegrator, src/render_core/integrators/pt.cpp:43-64): the GUI's
+ * switch, threshold and minimum sample number (the reference's file sets only
+ * the latter two, as the integrator's "adaptive" block {threshold,
+ * start_index}, which the loader reads).  When on, a pixel whose statistics
+ * (the running mean, variance and count of the output buffer's x + y + z)
+ * show more than min_samples samples and variance / mean below the threshold
+ * adds no more samples; the statistics reset at frame 0.  Renders with it
+ * on use the megakernel form. */
+vs_status vs_scene_set_adaptive(vs_scene *scene, int32_t on, float threshold, uint32_t min_samples);
+/* The per-pixel VarianceStats (avg, var, N as uint32 bits, 0), float[W*H*4].
+ * Test and inspection hook; no reference counterpart. */
+vs_status vs_scene_variance_stats(vs_scene *scene, float *stats);
+/* Switch the SVGF denoiser of the "rt" integrator on or off, with its
+ * reprojection and moment-filter passes: the check boxes of SVGF::render_sub_UI
+ * (src/render_core/denoisers/SVGF/svgf.cpp:29-47; the reference has no file
+ * setting for them, its GUI sets them).  The other SVGF parameters come from
+ * the integrator's "denoiser" block (SVGF::SVGF, SVGF/svgf.h:44-53).  When
+ * on, each rt frame first computes the G-buffer (FrameBuffer::compute_GBuffer,
+ * src/base/frame_buffer.cpp:221-229) and after the combine runs
+ * SVGF::dispatch (svgf.cpp:89-106), writing the modulated picture to the
+ * output buffer (film which = 1). */
+vs_status vs_scene_set_denoiser(vs_scene *scene, int32_t on, int32_t reproject, int32_t filter_moments);
+/* G-buffer and SVGF state after the last denoised frame; any pointer may be
+ * null: gbuffer [2][2][pixels][4] (slots by frame parity, current =
+ * (frame + 1) & 1; normal.xyz fwidth | p_film.xy depth_gradient
+ * linear_depth, PixelGeometry of frame_buffer.h:15-20), albedo and emission
+ * [pixels][4], records [3][2][pixels][4] (SVGFData of SVGF/utils.h:22-26:
+ * illumination.rgb variance | history moments.xy 0; buffers 0 / 1 by frame
+ * parity, 2 the ping-pong spare) and the motion vectors [pixels][4] (.xy).
+ * Test and inspection hook; no reference counterpart. */
+vs_status vs_scene_svgf_state(vs_scene *scene, float *gbuffer, float *albedo, float *emission, float *records,
+                              float *motion);
 /* Enable per-ray counters (rays, node and triangle fetches) for later renders. */
 vs_status vs_set_counting(vs_scene *scene, int32_t enable);
 vs_status vs_last_render_stats(const vs_scene *scene, vs_render_stats *out);

@@ -503,6 +503,10 @@ struct Lobe {
     Fresnel fr;
     bool compensate{false};
     float ltc_a{1.f}, ltc_b{0.f};
+    // the albedo of the principled coat and specular lobes reads a table at
+    // (lut_x, cos_theta, lut_z) (CoatLobe / SpecularLobe::albedo)
+    int albedo_lut{0};// 0 none, 1 coat, 2 specular
+    float lut_x{0.f}, lut_z{0.f};
 };
 
 struct LEval {
@@ -920,6 +924,7 @@ Bsdf make_leaf_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &
                 C.kr = cc_weight * weight * albedo_at(sd, flags, p, 12, it.u, it.v, w);
                 float x = std::sqrt(std::sqrt(C.mf.ax * C.mf.ay));
                 float z = (p[24] - 1.003f) / (4.f - 1.003f);
+                C.albedo_lut = 1, C.lut_x = x, C.lut_z = z;
                 float sv;
                 lut3(sd.lut_coat, 32, 1, x, cos_theta, z, &sv);
                 Spec alb = sv * C.kr;
@@ -968,6 +973,7 @@ Bsdf make_leaf_bsdf(const Spectral &sd, const Scene &s, uint32_t m, const Surf &
                 P.kr = weight;
                 float x = std::sqrt(std::sqrt(mf.ax * mf.ay));
                 float z = std::sqrt(std::fabs((ior - 1.0f) / (ior + 1.0f)));
+                P.albedo_lut = 2, P.lut_x = x, P.lut_z = z;
                 float sv;
                 lut3(sd.lut_spec, 32, 1, x, cos_theta, z, &sv);
                 Spec alb = (P.fr.F0 + sv * (1.f - P.fr.F0)) * P.kr;
@@ -2596,9 +2602,92 @@ void gi_spatial_px(const Ctx &c, const RParams &rp, const GParams &gp, const RSt
     grsv_store(g, (rp.frame + 1u) & 1u, pix, rsv);
 }
 
+// Lobe::albedo of each lobe kind: MicrofacetLobe (lobe.cpp:208-210: kr x
+// F(cos)), CoatLobe / SpecularLobe (principled_bsdf.cpp:154-160, :198-205:
+// the table value s, then s kr and (F0 + s (1 - F0)) kr), DielectricLobe
+// (lobe.cpp:308-313: kt (1 - F(|cos|)) + F), DiffuseLobe / SheenLTC (their
+// R / tint)
+Spec lobe_albedo(const Spectral &sd, const Lobe &L, float cos_theta, int n) {
+    switch (L.kind) {
+        case LB_REFLECTION: {
+            if (L.albedo_lut != 0) {
+                float sv;
+                lut3(L.albedo_lut == 1 ? sd.lut_coat : sd.lut_spec, 32, 1, L.lut_x, cos_theta, L.lut_z, &sv);
+                if (L.albedo_lut == 1) return sv * L.kr;
+                return (L.fr.F0 + sv * (1.f - L.fr.F0)) * L.kr;
+            }
+            return L.kr * fresnel_eval(L.fr, cos_theta, n);
+        }
+        case LB_DIELECTRIC: {
+            Spec F = fresnel_eval(L.fr, std::fabs(cos_theta), n);
+            return L.kr * (1.f - F) + F;
+        }
+        default: return L.kr;
+    }
+}
+
+// MaterialEvaluator::albedo (material.cpp:91-98): cos_theta against the
+// shading normal; a lobe set sums its lobes' albedos times their evaluation
+// weights (LobeSet::albedo, lobe.cpp:564-570); the black body's is zero
+Spec bsdf_albedo(const Spectral &sd, const Bsdf &b, V3 wo_w, int n) {
+    if (b.black) return sfill(0.f);
+    const float c = dotp(b.fr.z, wo_w);
+    if (!b.set) return lobe_albedo(sd, b.lobes[0], c, n);
+    Spec r = sfill(0.f);
+    for (int i = 0; i < b.nlobes; ++i) r = r + lobe_albedo(sd, b.lobes[i], c, n) * b.lobes[i].weight;
+    return r;
+}
+
+// FrameBuffer::compile_compute_geom's kernel (frame_buffer.cpp:72-122) for
+// one pixel: the camera ray of the pixel's first sampler dimension, its
+// closest hit's shading normal and camera-space depth (Sensor::linear_depth,
+// sensor.cpp:198-201), the material albedo and the emitted radiance toward
+// the camera in linear sRGB.  geo: the PixelGeometry record of 8 floats
+// (fwidth and the depth gradient are filled by or_gbuffer_gradient).
+void gbuffer_px(const Ctx &c, const FisTable &fis, const RParams &rp, uint32_t x, uint32_t y, float *geo, float *alb,
+                float *emi) {
+    const Scene &s = *c.s;
+    const Spectral &sd = *c.sd;
+    const size_t pix = size_t(y) * rp.W + x;
+    Swl w = sample_wavelengths(sd, x, y, rp.frame);
+    Sampler sp;
+    sp.start(x, y, rp.frame, 0u);
+    V3 o, d;
+    float fw, pf[2];
+    camera_ray(s, fis, x, y, sp, o, d, fw, pf);
+    Hit h = c.trace({o, d, 0.f, FLT_MAX}, false);
+    float *g = geo + 8 * pix;
+    g[0] = g[1] = g[2] = g[3] = 0.f;
+    g[4] = pf[0], g[5] = pf[1], g[6] = 0.f, g[7] = -1.f;
+    V3 a = v3(0.f, 0.f, 0.f), e = v3(0.f, 0.f, 0.f);
+    if (h.tri != kInv) {
+        Surf it = surface(s, h.tri, h.b1, h.b2, true);
+        it.wo = unit(neg(d));
+        g[0] = it.fr.z.x, g[1] = it.fr.z.y, g[2] = it.fr.z.z;
+        g[7] = affine(rp.w2c, it.pos).z;
+        const uint32_t m = s.inst[4 * it.inst], light = s.inst[4 * it.inst + 1];
+        if (m != kInv) {
+            Bsdf b = make_bsdf(sd, s, m, it, w);
+            a = linear_srgb(sd, bsdf_albedo(sd, b, it.wo, w.n), w);
+        }
+        if (light != kInv) {
+            Spec Le;
+            float lpdf;
+            eval_hit(sd, s, *c.lt, it, light, h.tri, o, w, Le, lpdf);
+            e = linear_srgb(sd, Le, w);
+        }
+    }
+    alb[4 * pix] = a.x, alb[4 * pix + 1] = a.y, alb[4 * pix + 2] = a.z, alb[4 * pix + 3] = 1.f;
+    emi[4 * pix] = e.x, emi[4 * pix + 1] = e.y, emi[4 * pix + 2] = e.z, emi[4 * pix + 3] = 1.f;
+}
+
 }// namespace
 
 extern "C" {
+
+static float tone_map(float v, uint32_t tm);
+void or_gbuffer_gradient(float *gbuf, uint32_t W, uint32_t H);
+void or_variance_stats_add(float *stats, const float *x, uint32_t n);
 
 // Spectral inputs of or_render / or_bsdf_*; field order mirrored by
 // oracle/oracle.py (OrSpectral).
@@ -2943,7 +3032,7 @@ uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const 
                          const float *cam, const float *render, uint32_t nv, uint32_t nt, uint32_t ni, uint32_t nm,
                          uint32_t nl, const or_spectral *spec, const float *restir, const float *prev_w2c,
                          const float *prev_pos, uint32_t frame, uint32_t threads, float *rsv, float *surf, float *ext,
-                         float *rt, float *gi_rsv, float *gi_smp, float *gi_aux, float *gi_rad) {
+                         float *rt, float *gi_rsv, float *gi_smp, float *gi_aux, float *gi_rad, float *gbuf) {
     Scene s{P, N, UV, tri, tri_inst, inst, mat, light, cam, render, nv, nt, ni, nm, nl};
     Spectral sd = to_spectral(spec);
     Bvh2 b;
@@ -3017,7 +3106,17 @@ uint64_t or_restir_frame(const float *P, const float *N, const float *UV, const 
     GState g{gi_rsv, gi_smp, gi_aux, nullptr, gi_rad, rp.P};
     std::vector<float> direct;
     if (gp.open) direct.assign(4 * size_t(rp.P), 0.f);
-    rows([&](uint32_t x, uint32_t y) { restir_initial_px(c, fis, rp, st, x, y, gp.open ? gi_aux : nullptr); });
+    if (gbuf) {
+        // RealTimeIntegrator::render (rt.cpp:144-149): the G-buffer of this
+        // frame (geometry [P][8], albedo [P][4], emission [P][4]) before
+        // ReSTIR DI; DI's shader0 then writes the motion vectors (gi_aux[2])
+        float *geo = gbuf, *alb = gbuf + 8 * size_t(rp.P), *emi = gbuf + 12 * size_t(rp.P);
+        rows([&](uint32_t x, uint32_t y) { gbuffer_px(c, fis, rp, x, y, geo, alb, emi); });
+        or_gbuffer_gradient(geo, rp.W, rp.H);
+    }
+    rows([&](uint32_t x, uint32_t y) {
+        restir_initial_px(c, fis, rp, st, x, y, gp.open || gbuf ? gi_aux : nullptr);
+    });
     uint32_t in = 2;
     for (uint32_t k = 0; k < rp.extra; ++k) {
         rows([&](uint32_t x, uint32_t y) { restir_spatial_px(c, rp, st, x, y, in, in ^ 1u, 2u + k); });
@@ -3059,7 +3158,7 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
                    const uint32_t *inst, const uint32_t *mat, const uint32_t *light, const float *cam,
                    const float *render, uint32_t nv, uint32_t nt, uint32_t ni, uint32_t nm, uint32_t nl,
                    uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t frame_start, uint32_t spp, int mode,
-                   uint32_t threads, const or_spectral *spec, float *rt) {
+                   uint32_t threads, const or_spectral *spec, float *rt, float *vstats) {
     Scene s{P, N, UV, tri, tri_inst, inst, mat, light, cam, render, nv, nt, ni, nm, nl};
     Spectral sd = to_spectral(spec);
     Bvh2 b;
@@ -3071,6 +3170,12 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
     const FisTable fis = fis_build(cam);
     uint32_t W = uint32_t(cam[19]);
     bool accumulate = render[6] != 0.f;
+    // ConvergenceInspector (pt.cpp:43-48, :59-64; inspector.cpp:26-38):
+    // render[8] tone mapper, [9] on, [10] threshold, [11] min sample number;
+    // vstats (avg, var, N bits, 0) per pixel
+    const bool adaptive = vstats != nullptr && render[9] != 0.f;
+    const float exposure = render[7], threshold = render[10];
+    const uint32_t tm = uint32_t(render[8]), min_n = uint32_t(render[11]);
     std::atomic<uint32_t> next_row{y0};
     auto work = [&]() {
         while (true) {
@@ -3078,7 +3183,13 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
             if (y >= y1) break;
             for (uint32_t x = x0; x < x1; ++x) {
                 float *acc = rt + 4 * (size_t(y) * W + x);
+                float *vst = adaptive ? vstats + 4 * (size_t(y) * W + x) : nullptr;
                 for (uint32_t f = frame_start; f < frame_start + spp; ++f) {
+                    if (vst) {// is_convergence: the pixel adds no more samples
+                        uint32_t n;
+                        std::memcpy(&n, vst + 2, 4);
+                        if (n > min_n && vst[1] / vst[0] < threshold) continue;
+                    }
                     Swl w = sample_wavelengths(sd, x, y, f);
                     Sampler sp;
                     sp.start(x, y, f, 0u);
@@ -3092,6 +3203,12 @@ uint64_t or_render(const float *P, const float *N, const float *UV, const uint32
                         for (int k = 0; k < 4; ++k) acc[k] = acc[k] + a * (v[k] - acc[k]);
                     } else {
                         for (int k = 0; k < 4; ++k) acc[k] = v[k];
+                    }
+                    if (vst) {// the output buffer's value (normal.cpp:96-99) feeds the statistics
+                        float o[3];
+                        for (int k = 0; k < 3; ++k) o[k] = tone_map(1.f - vs::em::exp(-acc[k] * exposure), tm);
+                        float sum = o[0] + o[1] + o[2];
+                        or_variance_stats_add(vst, &sum, 1);
                     }
                 }
             }

@@ -67,10 +67,16 @@ def lib():
                                _u32p, _f32p, _f32p]
         h.or_render.restype = C.c_uint64
         h.or_render.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
-                                [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, C.c_void_p, _f32p])
+                                [C.c_uint32] * 5 + [C.c_uint32] * 6 + [C.c_int, C.c_uint32, C.c_void_p, _f32p,
+                                                                       C.c_void_p])
         h.or_restir_frame.argtypes = ([_f32p, _f32p, _f32p, _u32p, _u32p, _u32p, _u32p, _u32p, _f32p, _f32p] +
                                       [C.c_uint32] * 5 + [C.c_void_p, _f32p, _f32p, _f32p, C.c_uint32, C.c_uint32,
-                                                          _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p])
+                                                          _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                                          C.c_void_p])
+        h.or_gbuffer_gradient.argtypes = [_f32p, C.c_uint32, C.c_uint32]
+        h.or_svgf_frame.argtypes = [_f32p, C.c_uint32, C.c_uint32, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p, _f32p,
+                                    _f32p, _f32p]
+        h.or_variance_stats_add.argtypes = [_f32p, _f32p, C.c_uint32]
         h.or_reservoir_stream.restype = C.c_int32
         h.or_reservoir_stream.argtypes = [_f32p, _f32p, C.c_uint32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
         h.or_prev_raster.argtypes = [_f32p, _f32p, _f32p, C.c_uint32, _f32p]
@@ -242,9 +248,11 @@ def trace(positions: np.ndarray, triangles: np.ndarray, rays: np.ndarray, any_hi
 
 
 def render(scene: dict, frame_start: int = 0, spp: int = 1, window=None, bvh: bool = True, threads: int = 0,
-           film: np.ndarray | None = None) -> np.ndarray:
+           film: np.ndarray | None = None, vstats: np.ndarray | None = None) -> np.ndarray:
     """Render the exported scene (vision_amd.Scene.export()) on the CPU.
-    Returns the running-mean film float32[H, W, 4] (updated in place if given)."""
+    Returns the running-mean film float32[H, W, 4] (updated in place if given).
+    With adaptive sampling on in the export, `vstats` float32[H*W, 4] holds
+    the per-pixel statistics (updated in place; zero them at frame 0)."""
     spec = Spectral(scene) if "spd" in scene else None
     cam = np.ascontiguousarray(scene["camera"], np.float32)
     W, H = int(cam[19]), int(cam[20])
@@ -257,8 +265,46 @@ def render(scene: dict, frame_start: int = 0, spp: int = 1, window=None, bvh: bo
                     a["materials"].reshape(-1), a["lights"].reshape(-1), cam, a["render"],
                     len(a["positions"]), len(a["triangles"]), len(a["instances"]), len(a["materials"]),
                     len(a["lights"]), x0, y0, x1, y1, frame_start, spp, 1 if bvh else 0, threads,
-                    None if spec is None else spec.ptr, film)
+                    None if spec is None else spec.ptr, film, None if vstats is None else vstats.ctypes.data)
     return film
+
+
+def gbuffer_gradient(gbuf: np.ndarray, W: int, H: int) -> np.ndarray:
+    """FrameBuffer::compute_gradient over a PixelGeometry buffer float32[H*W, 8]
+    (normal.xyz fwidth | p_film.xy depth_gradient depth); returns a copy
+    with fwidth and the depth gradient filled."""
+    g = np.ascontiguousarray(gbuf, np.float32).reshape(-1).copy()
+    lib().or_gbuffer_gradient(g, W, H)
+    return g.reshape(H * W, 8)
+
+
+SVGF_DEFAULTS = dict(N=3, alpha=0.05, moments_alpha=0.2, history_limit=32, moments_filter_radius=3, sigma_rt=10.0,
+                     sigma_normal=30.0, reproject=True, filter_moments=True)
+
+
+def svgf_params(**kw) -> np.ndarray:
+    p = dict(SVGF_DEFAULTS, **kw)
+    return np.array([p["N"], p["alpha"], p["moments_alpha"], p["history_limit"], p["moments_filter_radius"],
+                     p["sigma_rt"], p["sigma_normal"], float(p["reproject"]), float(p["filter_moments"])], np.float32)
+
+
+def svgf_frame(params: np.ndarray, W: int, H: int, gcur, gprev, motion, rad, albedo, emission, prev_rec):
+    """One SVGF frame (or_svgf_frame): G-buffers float32[H*W, 8], motion
+    [H*W, 2], radiance / albedo / emission [H*W, 4], the previous frame's
+    records [H*W, 8].  Returns (records [H*W, 8], picture [H*W, 4])."""
+    f = lambda a, n: np.ascontiguousarray(a, np.float32).reshape(H * W, n)
+    cur = np.zeros((H * W, 8), np.float32)
+    out = np.zeros((H * W, 4), np.float32)
+    lib().or_svgf_frame(np.ascontiguousarray(params, np.float32), W, H, f(gcur, 8), f(gprev, 8), f(motion, 2),
+                        f(rad, 4), f(albedo, 4), f(emission, 4), f(prev_rec, 8), cur, out)
+    return cur, out
+
+
+def variance_stats_add(stats: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """VarianceStats::add on per-pixel statistics float32[n, 4] (avg, var, N
+    bits, 0), in place."""
+    lib().or_variance_stats_add(stats, np.ascontiguousarray(x, np.float32), len(x))
+    return stats
 
 
 def reservoir_stream(w: np.ndarray, u: np.ndarray):
@@ -303,6 +349,14 @@ class Restir:
         self.gi_smp = np.zeros((4, P, 4), np.float32)
         self.gi_aux = np.zeros((3, P, 4), np.float32)
         self.gi_rad = np.zeros((P, 4), np.float32)
+        # G-buffer [2][P][8] (PixelGeometry, depth -1 when empty), albedo /
+        # emission [P][4], SVGF records [2][P][8], the denoised picture
+        self.geo = np.zeros((2, P, 8), np.float32)
+        self.geo[:, :, 7] = -1.0
+        self.alb = np.zeros((P, 4), np.float32)
+        self.emi = np.zeros((P, 4), np.float32)
+        self.svgf_rec = np.zeros((2, P, 8), np.float32)
+        self.picture = np.zeros((P, 4), np.float32)
         self.frame = 0
         self.threads = threads
         self.scene = scene
@@ -316,7 +370,11 @@ class Restir:
         spec = Spectral(sc)
         cam = np.ascontiguousarray(sc["camera"], np.float32)
         a = {k: np.ascontiguousarray(v) for k, v in sc.items() if isinstance(v, np.ndarray)}
+        svgf = float(sc["restir"][48]) != 0.0 if len(sc["restir"]) > 48 else False
+        P = self.W * self.H
         for _ in range(frames):
+            cur, prev = (self.frame + 1) & 1, self.frame & 1
+            gblock = np.zeros(16 * P, np.float32) if svgf else None
             lib().or_restir_frame(a["positions"].reshape(-1), a["normals"].reshape(-1), a["uvs"].reshape(-1),
                                   a["triangles"].reshape(-1), a["tri_instance"], a["instances"].reshape(-1),
                                   a["materials"].reshape(-1), a["lights"].reshape(-1), cam, a["render"],
@@ -324,7 +382,21 @@ class Restir:
                                   len(a["materials"]), len(a["lights"]), spec.ptr,
                                   np.ascontiguousarray(a["restir"], np.float32), self.prev_w2c, self.prev_pos,
                                   self.frame, self.threads, self.rsv, self.surf, self.ext, self.film, self.gi_rsv,
-                                  self.gi_smp, self.gi_aux, self.gi_rad)
+                                  self.gi_smp, self.gi_aux, self.gi_rad,
+                                  None if gblock is None else gblock.ctypes.data)
+            if svgf:
+                # RealTimeIntegrator::render: the denoiser after the combine
+                # (rt.cpp:157-158)
+                self.geo[cur] = gblock[:8 * P].reshape(P, 8)
+                self.alb = gblock[8 * P:12 * P].reshape(P, 4).copy()
+                self.emi = gblock[12 * P:].reshape(P, 4).copy()
+                rs = np.asarray(sc["restir"], np.float32)
+                params = rs[49:58]
+                rec, pic = svgf_frame(params, self.W, self.H, self.geo[cur], self.geo[prev],
+                                      self.gi_aux[2, :, :2], self.film.reshape(P, 4), self.alb, self.emi,
+                                      self.svgf_rec[prev])
+                self.svgf_rec[cur] = rec
+                self.picture = pic
             self.frame += 1
             self.prev_w2c = np.ascontiguousarray(a["restir"][20:32], np.float32)
             self.prev_pos = np.array([cam[3], cam[7], cam[11]], np.float32)

@@ -1,0 +1,111 @@
+"""G-buffer, SVGF denoiser and adaptive sampling on the GPU (svgf.hip,
+kernels.hip) against the oracle on the same frames.
+
+SVGF: the rt integrator renders a few frames with a moving camera and the
+denoiser on; the G-buffer of the current frame (normal, fwidth, p_film,
+depth gradient, depth), the albedo and emission, the motion vectors, the
+SVGF records (illumination, variance, history, moments) and the denoised
+picture are compared with the oracle's.  The bar is the ReSTIR one: nearly
+all pixels within 1e-4 relative (a last-bit difference of a device math
+function can move a reservoir draw, which the filters then spread a little).
+
+Adaptive sampling: the pt integrator with the convergence inspector on, in
+one fused launch and frame by frame; the film and the per-pixel statistics
+equal the oracle's, and pixels stop at different counts.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle
+from vision_amd import scenegen
+
+pytestmark = pytest.mark.gpu
+
+REL_TOL = 1e-4
+ABS_FLOOR = 1e-6
+
+
+def _close(a, b, what, frac=0.999):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    ok = (np.abs(a - b) <= REL_TOL * np.maximum(np.abs(a), np.abs(b)) + ABS_FLOOR)
+    ok = ok.reshape(ok.shape[0], -1).all(axis=-1)
+    print(f"{what}: {ok.mean() * 100:.3f}% within {REL_TOL}")
+    assert ok.mean() >= frac, what
+
+
+def _svgf_scene(w, h, gi=False, **svgf):
+    s = scenegen.cornell_box(w, h, max_depth=3, light_sampler="power")
+    indirect = None
+    if gi:
+        indirect = {"open": True, "max_age": 6, "temporal": {"open": True, "history_limit": 8, "radius": 4, "N": 5},
+                    "spatial": {"open": True, "sample_num": 3, "radius": 6}}
+    s["integrator"] = scenegen.restir_integrator(M_light=4, M_bsdf=1, sample_num=2, radius=6, indirect=indirect,
+                                                 max_depth=3)
+    param = {"N": 3, "alpha": 0.05, "moments_alpha": 0.2, "history_limit": 32, "moments_filter_radius": 3,
+             "sigma_rt": 10, "sigma_normal": 30}
+    param.update(svgf)
+    s["integrator"]["param"]["denoiser"] = {"type": "svgf", "param": param}
+    return s
+
+
+@pytest.mark.parametrize("cfg", [dict(), dict(gi=True), dict(N=1, moments_filter_radius=1)],
+                         ids=["default", "with_gi", "one_pass"])
+def test_svgf_parity(gpu_ctx, cfg):
+    W, H = 96, 80
+    scene = _svgf_scene(W, H, **cfg)
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    sc.set_denoiser(True)
+    orc = oracle.Restir(sc.export())
+    cams = scenegen.jitter_path(6, amplitude=0.01, seed=5)
+    for f in range(6):
+        sc.update_camera(cams[f], 19.5)
+        sc.render(1)
+        orc.render(1, scene=sc.export())
+    st = sc.svgf_state()
+    picture = sc.film(1)
+    sc.close()
+    P = W * H
+    cur = orc.frame & 1  # the slot the last frame wrote: (frame + 1) & 1 of the last frame index
+    g = st["gbuffer"][cur]
+    geo = np.concatenate([g[0], g[1]], axis=1)
+    _close(geo, orc.geo[cur], "g-buffer")
+    _close(st["albedo"], orc.alb, "albedo")
+    _close(st["emission"], orc.emi, "emission")
+    _close(st["motion"][:, :2], orc.gi_aux[2, :, :2], "motion vectors")
+    r = st["records"][cur]
+    rec = np.concatenate([r[0], r[1]], axis=1)[:, :7]
+    _close(rec, orc.svgf_rec[cur][:, :7], "svgf records")
+    _close(picture.reshape(P, 4), orc.picture, "denoised picture")
+    assert np.isfinite(picture).all()
+    assert orc.picture[:, :3].mean() > 0
+    # the denoised picture is smoother than the noisy film
+    film = orc.film.reshape(H, W, 4)[..., 0]
+    den = orc.picture.reshape(H, W, 4)[..., 0]
+    assert np.abs(np.diff(den, axis=1)).mean() < np.abs(np.diff(film, axis=1)).mean()
+
+
+@pytest.mark.parametrize("fused", [1, 0], ids=["fused", "per_frame"])
+def test_adaptive_sampling_parity(gpu_ctx, fused):
+    W, H = 64, 48
+    scene = scenegen.cornell_box(W, H, max_depth=3)
+    sc = gpu_ctx.load_text(scene)
+    sc.build_accel()
+    sc.set_adaptive(True, 0.5, 4)
+    ex = sc.export()
+    sc.render(16, fused=bool(fused))
+    sc.render(8, fused=bool(fused))
+    film = sc.film(0)
+    stats = sc.variance_stats()
+    sc.close()
+    vst = np.zeros((W * H, 4), np.float32)
+    ref = oracle.render(ex, 0, 24, vstats=vst)
+    n_gpu, n_ref = stats[:, 2].view(np.uint32), vst[:, 2].view(np.uint32)
+    same_n = np.mean(n_gpu == n_ref)
+    print(f"adaptive: {same_n * 100:.3f}% pixels stopped at the same count; "
+          f"counts {n_ref.min()}..{n_ref.max()}")
+    assert same_n >= 0.999
+    assert n_ref.min() < 24
+    _close(film.reshape(W * H, 4), ref.reshape(W * H, 4), "adaptive film")
+    _close(stats[:, :2], vst[:, :2], "adaptive statistics")

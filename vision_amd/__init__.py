@@ -160,6 +160,10 @@ SIGNATURES = {
     "vs_scene_update_camera": (C.c_int32, [_P, _P, C.c_float]),
     "vs_scene_restir_state": (C.c_int32, [_P, _P, _P]),
     "vs_scene_restir_gi_state": (C.c_int32, [_P, _P, _P]),
+    "vs_scene_set_denoiser": (C.c_int32, [_P, C.c_int32, C.c_int32, C.c_int32]),
+    "vs_scene_set_adaptive": (C.c_int32, [_P, C.c_int32, C.c_float, C.c_uint32]),
+    "vs_scene_variance_stats": (C.c_int32, [_P, _P]),
+    "vs_scene_svgf_state": (C.c_int32, [_P, _P, _P, _P, _P, _P]),
     "vs_trace_closest": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_any": (C.c_int32, [_P, _P, C.c_uint32, _P]),
     "vs_trace_closest_device": (C.c_int32, [_P, _P, C.c_uint32, _P, _P]),
@@ -346,7 +350,7 @@ class Scene:
             "materials": arr(a.materials, 32 * a.num_materials, np.uint32).reshape(-1, 32),
             "lights": arr(a.lights, 28 * a.num_lights, np.uint32).reshape(-1, 28),
             "camera": arr(a.camera, 32, np.float32),
-            "render": arr(a.render, 8, np.float32),
+            "render": arr(a.render, 12, np.float32),
             "spd": arr(a.spd, a.num_spd, np.float32),
             "spectrum_rgb": int(a.spectrum_rgb),
             "spectrum_dim": int(a.spectrum_dim),
@@ -437,6 +441,40 @@ class Scene:
         rad = np.zeros((n, 4), np.float32)
         self.ctx._check(lib().vs_scene_restir_gi_state(self._h, rsv.ctypes.data, rad.ctypes.data))
         return rsv, rad
+
+    def set_adaptive(self, on: bool = True, threshold: float | None = None, min_samples: int | None = None):
+        """Switch adaptive sampling of the pt integrator on or off; threshold and
+        minimum sample number default to the scene's "adaptive" block."""
+        ex = self.export()["render"]
+        t = float(ex[10]) if threshold is None else float(threshold)
+        m = int(ex[11]) if min_samples is None else int(min_samples)
+        self.ctx._check(lib().vs_scene_set_adaptive(self._h, int(on), t, m))
+
+    def variance_stats(self) -> np.ndarray:
+        """Per-pixel VarianceStats float32[H*W, 4] (avg, var, N bits, 0)."""
+        info = self.info()
+        out = np.zeros((info["width"] * info["height"], 4), np.float32)
+        self.ctx._check(lib().vs_scene_variance_stats(self._h, out.ctypes.data))
+        return out
+
+    def set_denoiser(self, on: bool = True, reproject: bool = True, filter_moments: bool = True):
+        """Switch the rt integrator's SVGF denoiser (and its reprojection and
+        moment-filter passes) on or off, as the reference's GUI check boxes do."""
+        self.ctx._check(lib().vs_scene_set_denoiser(self._h, int(on), int(reproject), int(filter_moments)))
+
+    def svgf_state(self) -> dict:
+        """G-buffer and SVGF buffers after the last denoised frame: gbuffer
+        float32[2, 2, H*W, 4], albedo / emission [H*W, 4], records
+        [3, 2, H*W, 4], motion [H*W, 4]."""
+        info = self.info()
+        n = info["width"] * info["height"]
+        out = {"gbuffer": np.zeros((2, 2, n, 4), np.float32), "albedo": np.zeros((n, 4), np.float32),
+               "emission": np.zeros((n, 4), np.float32), "records": np.zeros((3, 2, n, 4), np.float32),
+               "motion": np.zeros((n, 4), np.float32)}
+        self.ctx._check(lib().vs_scene_svgf_state(self._h, out["gbuffer"].ctypes.data, out["albedo"].ctypes.data,
+                                                  out["emission"].ctypes.data, out["records"].ctypes.data,
+                                                  out["motion"].ctypes.data))
+        return out
 
     def set_kernel(self, mode: str = "megakernel", chunk_frames: int = 8, trace_block: int = 128):
         """Kernel form of render(): "megakernel" or "wavefront"."""

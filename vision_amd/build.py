@@ -8,6 +8,7 @@ from __future__ import annotations
 import concurrent.futures as cf
 import os
 import subprocess
+import tempfile
 import sys
 from pathlib import Path
 
@@ -65,12 +66,54 @@ def build_rgb2spec(verbose: bool = False) -> Path:
     return RGB2SPEC
 
 
+LLVM_BIN = Path(os.environ.get("ROCM_PATH", "/opt/rocm")) / "llvm" / "bin"
+
+
+def device_functions(obj: Path) -> list[tuple[str, int]]:
+    """Non-kernel functions (name, bytes) in the gfx950 code object of a
+    compiled HIP object.
+
+    Every device function is meant to be inlined into its kernels.  A
+    function the compiler keeps out of line returns through s[30:31]; when
+    it is longer than the short-branch range (+-128 KiB) the branch
+    relaxation of this toolchain builds its long jumps in s[30:31] too, the
+    return address is lost and the kernel jumps astray (a GPU memory fault),
+    so the build refuses any out-of-line device function."""
+    with tempfile.TemporaryDirectory() as td:
+        fb, co = Path(td) / "fatbin", Path(td) / "gfx950.co"
+        r = subprocess.run([str(LLVM_BIN / "llvm-objcopy"), f"--dump-section=.hip_fatbin={fb}", str(obj),
+                            str(Path(td) / "host.o")], capture_output=True, text=True)
+        if r.returncode != 0 or not fb.exists():
+            raise RuntimeError(f"no device code in {obj}: {r.stderr}")
+        r = subprocess.run([str(LLVM_BIN / "clang-offload-bundler"), "--unbundle", "--type=o", f"--input={fb}",
+                            "--targets=hipv4-amdgcn-amd-amdhsa--gfx950", f"--output={co}"],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"no gfx950 code object in {obj}: {r.stderr}")
+        r = subprocess.run([str(LLVM_BIN / "llvm-readelf"), "-s", "-W", str(co)], capture_output=True, text=True,
+                           check=True)
+    funcs, kernels = {}, set()
+    for line in r.stdout.splitlines():
+        f = line.split()
+        if len(f) >= 8 and f[3] == "FUNC":
+            funcs[f[7]] = int(f[2])
+        elif len(f) >= 8 and f[7].endswith(".kd"):
+            kernels.add(f[7][:-3])
+    return sorted((n, sz) for n, sz in funcs.items() if n not in kernels)
+
+
 def build(verbose: bool = False) -> Path:
     BUILD.mkdir(exist_ok=True)
     build_rgb2spec(verbose)
     sources = sorted(CSRC.glob("*.cpp")) + sorted(CSRC.glob("*.hip"))
     with cf.ThreadPoolExecutor(max_workers=min(8, len(sources))) as ex:
         objs = list(ex.map(_compile, sources))
+    for o in objs:
+        if o.name.endswith(".hip.o"):
+            calls = device_functions(o)
+            if calls:
+                raise RuntimeError(f"{o.name}: device functions kept out of line (mark them __forceinline__): "
+                                   + ", ".join(f"{n} ({sz} B)" for n, sz in calls))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [HIPCC, "-shared", "--offload-arch=gfx950", "-fPIC", "-o", str(tmp), *map(str, objs),
            "-Wl,--no-undefined", "-lpthread", "-lz"]

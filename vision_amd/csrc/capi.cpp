@@ -21,6 +21,7 @@
 #include "data_tables.h"
 #include "image_io.h"
 #include "restir.h"
+#include "svgf.h"
 #include "tables.h"
 #include "wavefront.h"
 
@@ -188,7 +189,7 @@ void affine_inverse(const float *m, float *out) {
 float radians_to_cos(float deg) { return std::cos(deg * (3.14159265358979323846f / 180.f)); }
 
 void fill_restir_export(vs_scene *s) {
-    s->exp_restir.assign(48, 0.f);
+    s->exp_restir.assign(64, 0.f);
     const vs::IntegratorHost &in = s->host.integrator;
     if (in.type != "rt") return;
     const vs::RestirHost &r = in.restir;
@@ -232,13 +233,29 @@ void fill_restir_export(vs_scene *s) {
     e[44] = g.t_depth;
     e[45] = g.t_radius;
     e[46] = float(std::max(int(in.max_depth) - 1, 0));
+    // SVGF (restir[48..57]): on, N, alpha, moments_alpha, history_limit,
+    // moments_filter_radius, sigma_rt, sigma_normal, reproject, filter moments
+    const vs::SvgfHost &d = in.svgf;
+    e[48] = d.svgf && d.on ? 1.f : 0.f;
+    e[49] = float(d.N);
+    e[50] = d.alpha;
+    e[51] = d.moments_alpha;
+    e[52] = float(d.history_limit);
+    e[53] = float(d.moments_radius);
+    e[54] = d.sigma_rt;
+    e[55] = d.sigma_normal;
+    e[56] = d.reproject ? 1.f : 0.f;
+    e[57] = d.filter_moments ? 1.f : 0.f;
 }
 
 void fill_render_export(vs_scene *s) {
     const vs::SceneHost &h = s->host;
+    const vs::AdaptiveHost &a = h.integrator.adaptive;
     s->exp_render = {float(h.integrator.max_depth), float(h.integrator.min_depth), h.integrator.rr_threshold,
                      float(h.integrator.mis_mode), h.light_sampler_type == "power" ? 1.f : 0.f, h.ray_offset_factor,
-                     float(h.camera.accumulation), h.camera.exposure};
+                     float(h.camera.accumulation), h.camera.exposure,
+                     float(h.camera.tone_mapper == "aces" ? 1 : (h.camera.tone_mapper == "reinhard" ? 2 : 0)),
+                     a.on && h.integrator.type == "pt" ? 1.f : 0.f, a.threshold, float(a.min_samples)};
 }
 
 void alloc_film(vs_scene *s) {
@@ -806,11 +823,42 @@ vs::RestirView restir_view(vs_scene *s, uint32_t frame) {
     }
     for (int i = 0; i < 12; ++i) v.w2c[i] = e[20 + i];
     affine_inverse(s->prev_c2w, v.prev_w2c);
-    if (s->host.integrator.gi.open && s->gi_pixels == v.pixels) {
-        v.gi_aux = s->gi_aux.as<float4>();
-        v.gi_direct = s->gi_direct.as<float4>();
-    }
+    if (s->aux_pixels == v.pixels) v.gi_aux = s->gi_aux.as<float4>();
+    if (s->host.integrator.gi.open && s->gi_pixels == v.pixels) v.gi_direct = s->gi_direct.as<float4>();
     return v;
+}
+
+bool svgf_active(const vs_scene *s) {
+    const vs::SvgfHost &d = s->host.integrator.svgf;
+    return s->host.integrator.type == "rt" && d.svgf && d.on;
+}
+
+vs::SvgfView svgf_view(vs_scene *s, uint32_t frame) {
+    vs::SvgfView v{};
+    v.geo = s->sv_geo.as<float4>();
+    v.alb = s->sv_alb.as<float4>();
+    v.emi = s->sv_emi.as<float4>();
+    v.rec = s->sv_rec.as<float4>();
+    v.width = s->cam.width;
+    v.height = s->cam.height;
+    v.pixels = v.width * v.height;
+    v.frame = frame;
+    return v;
+}
+
+vs::SvgfParams svgf_params(const vs_scene *s) {
+    const vs::SvgfHost &d = s->host.integrator.svgf;
+    vs::SvgfParams p{};
+    p.N = d.N;
+    p.alpha = d.alpha;
+    p.moments_alpha = d.moments_alpha;
+    p.history_limit = d.history_limit;
+    p.moments_radius = d.moments_radius;
+    p.sigma_rt = d.sigma_rt;
+    p.sigma_normal = d.sigma_normal;
+    p.reproject = d.reproject ? 1u : 0u;
+    p.filter_moments = d.filter_moments ? 1u : 0u;
+    return p;
 }
 
 // ReSTIR GI settings and state (construct_param, indirect.cpp:335-352, as
@@ -854,13 +902,26 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
         hip_check(vs::restir_clear(restir_view(s, 0), st), "restir clear");
     }
     const bool gi = s->host.integrator.gi.open;
+    const bool svgf = svgf_active(s);
+    if ((gi || svgf) && s->aux_pixels != P) {
+        s->gi_aux.alloc(size_t(P) * 3u * 16u);
+        hip_check(hipMemsetAsync(s->gi_aux.ptr, 0, s->gi_aux.bytes, st), "aux clear");
+        s->aux_pixels = P;
+    }
     if (gi && s->gi_pixels != P) {
         s->gi_rsv.alloc(size_t(P) * vs::kGiReservoirBuffers * 4u * 16u);
-        s->gi_aux.alloc(size_t(P) * 3u * 16u);
         s->gi_direct.alloc(size_t(P) * 16u);
         s->gi_rad.alloc(size_t(P) * 16u);
         s->gi_pixels = P;
         hip_check(vs::restir_gi_clear(restir_gi_view(s), P, st), "restir gi clear");
+    }
+    if (svgf && s->sv_pixels != P) {
+        s->sv_geo.alloc(size_t(P) * 2u * 2u * 16u);
+        s->sv_alb.alloc(size_t(P) * 16u);
+        s->sv_emi.alloc(size_t(P) * 16u);
+        s->sv_rec.alloc(size_t(P) * 3u * 2u * 16u);
+        s->sv_pixels = P;
+        hip_check(vs::svgf_clear(svgf_view(s, 0), st), "svgf clear");
     }
     const bool rgb = s->packed.rgb;
     const uint32_t dim = rv.spectrum_dim;
@@ -873,6 +934,10 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
     for (uint32_t f = 0; f < spp; ++f) {
         vs::RestirView v = restir_view(s, s->frame_index + f);
         hip_check(hipEventRecord(s->ctx->tr0, st), "event");
+        // the G-buffer pass (rt.cpp:147-148), then shader0
+        if (svgf)
+            hip_check(vs::gbuffer_compute(s->view, s->cam, v, svgf_view(s, v.frame), dim, rgb, ms, ctr, st),
+                      "g-buffer");
         hip_check(vs::restir_initial(s->view, s->cam, v, dim, rgb, ms, ctr, st), "restir initial");
         hip_check(hipEventRecord(s->ctx->tr1, st), "event");
         uint32_t in = 2u;
@@ -894,6 +959,17 @@ uint32_t render_restir(vs_scene *s, const vs::RenderView &rv, uint32_t spp, unsi
                       "restir gi spatial / shading");
             launches += 2u;
             ts.stage_launches[2] += 2u;
+        }
+        if (svgf) {
+            // the denoiser after the combine (rt.cpp:157-158): the film's
+            // running mean in, the modulated picture into the output buffer
+            hip_check(vs::svgf_denoise(svgf_view(s, v.frame), svgf_params(s), s->gi_aux.as<float4>() + 2u * size_t(P),
+                                       s->film_rt.as<float4>(), s->film_out.as<float4>(), st),
+                      "svgf");
+            const vs::SvgfParams sp = svgf_params(s);
+            const uint32_t n = 1u + sp.filter_moments + sp.N + (sp.N == 0 ? 1u : 0u);
+            launches += 2u + n;
+            ts.stage_launches[2] += 2u + n;
         }
         hip_check(hipEventRecord(s->ctx->ev1, st), "event");
         hip_check(hipEventSynchronize(s->ctx->ev1), "restir sync");
@@ -940,6 +1016,59 @@ vs_status vs_scene_restir_gi_state(vs_scene *s, float *reservoirs, float *radian
     });
 }
 
+vs_status vs_scene_set_adaptive(vs_scene *s, int32_t on, float threshold, uint32_t min_samples) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        if (on && s->host.integrator.type != "pt") throw std::invalid_argument("adaptive sampling needs the pt integrator");
+        if (!(threshold >= 0.f)) throw std::invalid_argument("threshold must be >= 0");
+        vs::AdaptiveHost &a = s->host.integrator.adaptive;
+        a.on = on != 0;
+        a.threshold = threshold;
+        a.min_samples = min_samples;
+        fill_render_export(s);
+    });
+}
+
+vs_status vs_scene_variance_stats(vs_scene *s, float *stats) {
+    if (!s || !stats) return fail(s ? s->ctx : nullptr, VS_ERROR_INVALID_ARGUMENT, "null argument");
+    return guarded(s->ctx, [&] {
+        if (s->vstats_pixels == 0) throw std::logic_error("no adaptive frame rendered");
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(s->ctx->stream), "stats sync");
+        hip_check(hipMemcpy(stats, s->vstats.ptr, s->vstats.bytes, hipMemcpyDeviceToHost), "stats D2H");
+    });
+}
+
+vs_status vs_scene_set_denoiser(vs_scene *s, int32_t on, int32_t reproject, int32_t filter_moments) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        vs::SvgfHost &d = s->host.integrator.svgf;
+        if (on && !d.svgf) throw std::invalid_argument("the scene's denoiser is not svgf");
+        d.on = on != 0;
+        d.reproject = reproject != 0;
+        d.filter_moments = filter_moments != 0;
+        fill_restir_export(s);
+    });
+}
+
+vs_status vs_scene_svgf_state(vs_scene *s, float *gbuffer, float *albedo, float *emission, float *records,
+                              float *motion) {
+    if (!s) return VS_ERROR_INVALID_ARGUMENT;
+    return guarded(s->ctx, [&] {
+        require_built(s);
+        if (s->sv_pixels == 0) throw std::logic_error("no denoised frame rendered");
+        hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
+        hip_check(hipStreamSynchronize(s->ctx->stream), "svgf sync");
+        const size_t P = s->sv_pixels;
+        if (gbuffer) hip_check(hipMemcpy(gbuffer, s->sv_geo.ptr, s->sv_geo.bytes, hipMemcpyDeviceToHost), "svgf D2H");
+        if (albedo) hip_check(hipMemcpy(albedo, s->sv_alb.ptr, s->sv_alb.bytes, hipMemcpyDeviceToHost), "svgf D2H");
+        if (emission) hip_check(hipMemcpy(emission, s->sv_emi.ptr, s->sv_emi.bytes, hipMemcpyDeviceToHost), "svgf D2H");
+        if (records) hip_check(hipMemcpy(records, s->sv_rec.ptr, s->sv_rec.bytes, hipMemcpyDeviceToHost), "svgf D2H");
+        if (motion)
+            hip_check(hipMemcpy(motion, s->gi_aux.as<float4>() + 2u * P, P * 16u, hipMemcpyDeviceToHost), "svgf D2H");
+    });
+}
+
 vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
     if (!s) return VS_ERROR_INVALID_ARGUMENT;
     return guarded(s->ctx, [&] {
@@ -957,8 +1086,27 @@ vs_status vs_render(vs_scene *s, uint32_t spp, int32_t fused) {
         uint32_t launches = 0;
         vs_render_stats ts{};
         const bool restir = h.integrator.type == "rt";
-        const bool wavefront = !restir && s->kernel_mode == VS_KERNEL_WAVEFRONT;
+        // adaptive sampling runs in the megakernel form (its statistics live
+        // with the pixel's running mean)
+        const bool adaptive = !restir && h.integrator.adaptive.on;
+        const bool wavefront = !restir && !adaptive && s->kernel_mode == VS_KERNEL_WAVEFRONT;
         const bool staged = restir || wavefront;// two counter sets, [0..2] and [4..6]
+        if (!restir) {
+            // ConvergenceInspector::reset at frame 0 (pt.cpp:93-95)
+            const uint32_t P = s->cam.width * s->cam.height;
+            if (adaptive && s->vstats_pixels != P) {
+                s->vstats.alloc(size_t(P) * 16u);
+                s->vstats_pixels = P;
+                hip_check(hipMemsetAsync(s->vstats.ptr, 0, s->vstats.bytes, st), "stats clear");
+            }
+            if (s->frame_index == 0 && s->vstats_pixels == P)
+                hip_check(hipMemsetAsync(s->vstats.ptr, 0, s->vstats.bytes, st), "stats clear");
+            if (adaptive) {
+                rv.vstats = s->vstats.as<float4>();
+                rv.adapt_threshold = h.integrator.adaptive.threshold;
+                rv.adapt_min_samples = h.integrator.adaptive.min_samples;
+            }
+        }
         hip_check(hipEventRecord(s->ctx->ev0, st), "event");
         if (restir) {
             launches = render_restir(s, rv, spp, ctr, ts);

@@ -142,6 +142,14 @@ struct vs_scene {
     // direct and indirect radiance
     DevBuf gi_rsv, gi_aux, gi_direct, gi_rad;
     uint32_t gi_pixels{0};
+    // shader0's per-pixel record (HitBSDF, motion vector) when GI or SVGF reads it
+    uint32_t aux_pixels{0};
+    // G-buffer and SVGF records (svgf.h), allocated on the first denoised frame
+    DevBuf sv_geo, sv_alb, sv_emi, sv_rec;
+    uint32_t sv_pixels{0};
+    // ConvergenceInspector's VarianceStats (avg, var, N, 0) per pixel
+    DevBuf vstats;
+    uint32_t vstats_pixels{0};
     float prev_c2w[12]{};
     vs::SceneView view{};
     vs::CameraView cam{};

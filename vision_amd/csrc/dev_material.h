@@ -696,5 +696,80 @@ __device__ __forceinline__ SampleN<NS> mat_sample_any(const SceneView &sc, const
     return mat_sample<NS, MS>(c, sp);
 }
 
+// ---- albedo (MaterialEvaluator::albedo, material.cpp:91-98) at the cosine
+// of wo against the shading normal.  Lobe::albedo of each kind:
+// MicrofacetLobe kr F(cos) (lobe.cpp:208-210), CoatLobe s kr and
+// SpecularLobe (F0 + s (1 - F0)) kr with s from their tables
+// (principled_bsdf.cpp:154-160, :198-205), DielectricLobe kt (1 - F(|cos|))
+// + F (lobe.cpp:308-313), DiffuseLobe / SheenLTC their R / tint; a lobe set
+// adds its lobes' albedos times their evaluation weights (LobeSet::albedo,
+// lobe.cpp:564-570).  The black body's albedo is zero.
+
+// the principled stack's lobes added to r, each weight times `scale` (a mix
+// component's sample weight when flattened, 1 on its own)
+template <int NS>
+__device__ __forceinline__ void principled_albedo_into(const SceneView &sc, const MatCtx<NS> &c, float ct,
+                                                       float scale, Spec<NS> &r) {
+    r = r + c.sh_tint * scale;
+    {
+        float s;
+        lut3<1>(sc.lut + LUT_COAT, 32, sqrtf(sqrtf(c.cc_alpha * c.cc_alpha)), ct, (c.cc_eta - 1.003f) / (4.f - 1.003f),
+                &s);
+        r = r + (c.cc_kr * s) * scale;
+    }
+    r = r + (c.mt_kr * fresnel_f82(c.mt_F0, c.mt_B, ct)) * scale;
+    {
+        Spec<NS> F = fresnel_schlick(c.tr_F0, c.tr_eta, fabsf(ct));
+        r = r + (c.tr_kt * (1.f - F) + F) * (c.tr_weight * scale);
+    }
+    {
+        float s;
+        lut3<1>(sc.lut + LUT_SPEC, 32, sqrtf(sqrtf(c.ggx.ax * c.ggx.ay)), ct,
+                sqrtf(fabsf((c.sp_eta - 1.f) / (c.sp_eta + 1.f))), &s);
+        r = r + ((c.sp_F0 + s * (1.f - c.sp_F0)) * c.sp_kr) * scale;
+    }
+    r = r + c.df_kr * scale;
+}
+
+// the albedo of a single-lobe material
+template <int NS, int MS>
+__device__ __forceinline__ Spec<NS> single_lobe_albedo(const MatCtx<NS> &c, float ct) {
+    if (ms_has<MS>(MS_METAL) && c.type == MAT_METAL) {
+        Spec<NS> F;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) F.v[i] = fresnel_conductor(ct, c.a0.v[i], c.a1.v[i]);
+        return Spec<NS>::fill(1.f) * F;
+    }
+    if (ms_has<MS>(MS_GLASS) && c.type == MAT_GLASS) {
+        Spec<NS> F;
+#pragma unroll
+        for (int i = 0; i < NS; ++i) F.v[i] = fresnel_dielectric(fabsf(ct), c.a1.v[i]);
+        return c.a0 * (1.f - F) + F;
+    }
+    if (c.type == MAT_BLACK) return Spec<NS>::fill(0.f);
+    return c.a0;// Lambert, Oren-Nayar, mirror (constant Fresnel)
+}
+
+template <int NS, bool RGB, int MS = MS_ALL>
+__device__ __forceinline__ Spec<NS> mat_albedo(const SceneView &sc, const MatCtx<NS> &c, const Swl<NS> &swl) {
+    const float ct = c.wo.z;
+    if (MS == MS_ALL && (c.type == MAT_MIX || c.type == MAT_ADD)) {
+        Spec<NS> r = Spec<NS>::fill(0.f);
+#pragma unroll
+        for (int k = 0; k < 2; ++k) {
+            MatCtx<NS> m = mix_component<NS, RGB>(sc, c, k, swl);
+            if (m.type == MAT_PRINCIPLED) principled_albedo_into(sc, m, ct, c.sub_s[k], r);
+            else r = r + single_lobe_albedo<NS, MS_ALL>(m, ct) * c.sub_e[k];
+        }
+        return r;
+    }
+    if (ms_has<MS>(MS_PRINCIPLED) && c.type == MAT_PRINCIPLED) {
+        Spec<NS> r = Spec<NS>::fill(0.f);
+        principled_albedo_into(sc, c, ct, 1.f, r);
+        return r;
+    }
+    return single_lobe_albedo<NS, MS>(c, ct);
+}
+
 }// namespace dev
 }// namespace vs

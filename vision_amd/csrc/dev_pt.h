@@ -338,7 +338,7 @@ struct ShadowRay {
 // one bounce of IlluminationIntegrator::Li (src/base/integrator.cpp:150-273)
 // given the closest hit of ps.o + t ps.d
 template <int NS, bool RGB, int MS = MS_ALL>
-__device__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<NS> &ps, const TraceHit &h,
+__device__ __forceinline__ void path_shade(const SceneView &sc, const RenderView &rv, PathState<NS> &ps, const TraceHit &h,
                            ShadowRay &sr) {
     sr.valid = 0u;
     const uint32_t bounces = ps.tail ? 0u : ps.bounce;
@@ -549,7 +549,7 @@ __device__ __forceinline__ void path_begin(const CameraView &cam, const RenderVi
 
 // the whole path in one thread (the megakernel form)
 template <int NS, bool RGB, int MS = MS_ALL>
-__device__ f3 trace_path(const SceneView &sc, const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
+__device__ __forceinline__ f3 trace_path(const SceneView &sc, const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
                          uint32_t frame, uint2 *stack, uint32_t stride, PathCounters &pc) {
     PathState<NS> ps;
     path_begin<NS, RGB>(cam, rv, px, py, frame, ps);
@@ -614,8 +614,28 @@ __device__ __forceinline__ float4 film_output(const RenderView &rv, float4 acc) 
 // instead of waiting at the end of each frame for the longest path of its
 // wave.  Every pixel's samples still run in frame order in one lane with the
 // same arithmetic, so the film equals the frame-by-frame loop's bit for bit.
-template <int NS, bool RGB, int MS = MS_ALL>
-__device__ void render_column(const SceneView &sc, const CameraView &cam, const RenderView &rv,
+// VarianceStats::add_sample_impl (adaptive/inspector.h:22-31) on (avg, var,
+// N bits, 0)
+__device__ __forceinline__ float4 vstats_add(float4 st, float x) {
+    const uint32_t N = __float_as_uint(st.z);
+    const float fN = float(N), fN1 = float(N + 1u);
+    const float new_avg = (st.x * fN + x) / fN1;
+    const float d = st.x - new_avg, e = x - st.x;
+    const float new_var = (fN * st.y + fN * (d * d) + e * e + 2.f * e * d + d * d) / fN1;
+    return make_float4(new_avg, new_var, __uint_as_float(N + 1u), 0.f);
+}
+
+// ConvergenceInspector::is_convergence (inspector.cpp:33-36)
+__device__ __forceinline__ bool vstats_converged(const RenderView &rv, float4 st) {
+    return __float_as_uint(st.z) > rv.adapt_min_samples && st.y / st.x < rv.adapt_threshold;
+}
+
+// ADAPT: adaptive sampling (pt.cpp:43-64): a pixel whose statistics have
+// converged adds no more samples (a skipped sample adds nothing, so the rest
+// of its frames are skipped too); each added sample feeds the statistics
+// the output buffer's value x + y + z
+template <int NS, bool RGB, int MS = MS_ALL, bool ADAPT = false>
+__device__ __forceinline__ void render_column(const SceneView &sc, const CameraView &cam, const RenderView &rv,
                               float4 *__restrict__ rt, float4 *__restrict__ out, uint32_t px, uint32_t y0,
                               uint32_t dy, uint32_t n_px, uint32_t frame_start, uint32_t frames, uint2 *stack,
                               uint32_t stride, PathCounters &pc) {
@@ -624,11 +644,17 @@ __device__ void render_column(const SceneView &sc, const CameraView &cam, const 
     uint32_t j = 0, s = 0, py = y0;
     uint32_t idx = py * cam.width + px;
     float4 acc = rt[idx];
+    float4 vst = ADAPT ? rv.vstats[idx] : make_float4(0.f, 0.f, 0.f, 0.f);
     PathState<NS> ps;
-    bool fresh = true;
+    bool fresh = true, skip = false;
     while (true) {
         if (fresh) {
-            path_begin<NS, RGB>(cam, rv, px, py, frame_start + s, ps);
+            if (ADAPT && vstats_converged(rv, vst)) {
+                skip = true;
+                ps.alive = 0u;
+            } else {
+                path_begin<NS, RGB>(cam, rv, px, py, frame_start + s, ps);
+            }
             fresh = false;
         }
         if (ps.alive) {
@@ -646,16 +672,25 @@ __device__ void render_column(const SceneView &sc, const CameraView &cam, const 
             }
         }
         if (!ps.alive) {
-            acc = film_add(rv, acc, ps.L * ps.fw, frame_start + s);
+            if (!ADAPT || !skip) {
+                acc = film_add(rv, acc, ps.L * ps.fw, frame_start + s);
+                if (ADAPT) {
+                    const float4 o = film_output(rv, acc);
+                    vst = vstats_add(vst, o.x + o.y + o.z);
+                }
+            }
             fresh = true;
-            if (++s == frames) {
+            if (++s == frames || (ADAPT && skip)) {
                 rt[idx] = acc;
                 if (out) out[idx] = film_output(rv, acc);
+                if (ADAPT) rv.vstats[idx] = vst;
                 s = 0;
+                skip = false;
                 if (++j == n) break;
                 py += dy;
                 idx = py * cam.width + px;
                 acc = rt[idx];
+                if (ADAPT) vst = rv.vstats[idx];
             }
         }
     }

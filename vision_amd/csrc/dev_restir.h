@@ -671,7 +671,7 @@ __device__ __forceinline__ Reservoir temporal_reuse(const SceneView &sc, const R
 
 // shader0 for one pixel (direct.cpp:519-555 with compute_hit, :434-517)
 template <int NS, bool RGB, int MS>
-__device__ void restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
+__device__ __forceinline__ void restir_initial(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
                                uint32_t py, uint2 *stack, uint32_t stride, PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;
     const uint32_t cslot = cur_slot(rv.frame);
@@ -879,7 +879,7 @@ __device__ __forceinline__ f3 restir_shading(const SceneView &sc, const RestirVi
 // `buf`, shading; the reservoir is kept for the next frame.  Returns the
 // pixel's direct radiance for the film.
 template <int NS, bool RGB, int MS>
-__device__ f3 restir_final(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
+__device__ __forceinline__ f3 restir_final(const SceneView &sc, const CameraView &cam, const RestirView &rv, uint32_t px,
                            uint32_t py, uint32_t buf, uint2 *stack, uint32_t stride, PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;
     Sampler sp;
@@ -908,7 +908,7 @@ __device__ f3 restir_final(const SceneView &sc, const CameraView &cam, const Res
 // an extra spatial pass (beyond the reference's one): buffer `in` to `out`,
 // its own sampler dimension
 template <int NS, bool RGB, int MS>
-__device__ void restir_spatial_pass(const SceneView &sc, const RestirView &rv, uint32_t px, uint32_t py,
+__device__ __forceinline__ void restir_spatial_pass(const SceneView &sc, const RestirView &rv, uint32_t px, uint32_t py,
                                     uint32_t in, uint32_t out, uint32_t dim) {
     const uint32_t pix = py * rv.width + px;
     Sampler sp;

@@ -222,7 +222,7 @@ __device__ __forceinline__ GiReservoir gi_combine_temporal(const SceneView &sc, 
 // 153-236); the reservoir goes to the pass-through buffer.  Pixels that see
 // nothing get zero indirect radiance and keep their reservoirs.
 template <int MS>
-__device__ void restir_gi_initial_temporal(const SceneView &sc, const CameraView &cam, const RenderView &rv0,
+__device__ __forceinline__ void restir_gi_initial_temporal(const SceneView &sc, const CameraView &cam, const RenderView &rv0,
                                            const RestirView &rv, const RestirGiView &g, uint32_t px, uint32_t py,
                                            uint2 *stack, uint32_t stride, PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;
@@ -304,7 +304,7 @@ __device__ __forceinline__ uint32_t gi_neighbor(const RestirView &rv, float radi
 // light.  Returns the pixel's indirect radiance; the reservoir is kept for
 // the next frame.
 template <int MS>
-__device__ f3 restir_gi_spatial_shade(const SceneView &sc, const RestirView &rv, const RestirGiView &g,
+__device__ __forceinline__ f3 restir_gi_spatial_shade(const SceneView &sc, const RestirView &rv, const RestirGiView &g,
                                       const SurfaceRec &cs, uint32_t px, uint32_t py, uint2 *stack, uint32_t stride,
                                       PathCounters &pc) {
     const uint32_t pix = py * rv.width + px;

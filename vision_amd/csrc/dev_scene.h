@@ -90,6 +90,11 @@ struct RenderView {
     uint32_t write_output;
     uint32_t count_rays;
     uint32_t spectrum_dim;// hero wavelengths per path (3 in sRGB mode)
+    // ConvergenceInspector (adaptive/inspector.h): VarianceStats (avg, var,
+    // N bits, 0) per pixel, null when adaptive sampling is off
+    float4 *vstats;
+    float adapt_threshold;
+    uint32_t adapt_min_samples;
 };
 
 // device ray / hit records of the trace entry points (32 B / 24 B)

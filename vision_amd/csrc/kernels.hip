@@ -55,7 +55,8 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 // budget of 80 that spills some state to scratch): traversal latency is
 // better hidden by more waves than lost to the spills (measured: 1.9x on
 // the mixed-material procedural scene, 1.25-1.55x on diffuse scenes).
-template <int NS, bool RGB, int MS>
+// ADAPT: adaptive sampling on (rv.vstats holds the per-pixel statistics).
+template <int NS, bool RGB, int MS, bool ADAPT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
                                                   float4 *__restrict__ rt, float4 *__restrict__ out,
                                                   unsigned long long *__restrict__ counters) {
@@ -64,8 +65,8 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) pt
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= cam.width || py >= cam.height) return;
     PathCounters pc{0, 0, 0};
-    render_column<NS, RGB, MS>(sc, cam, rv, rt, rv.write_output ? out : nullptr, px, py, 1u, 1u, rv.frame_start,
-                               rv.spp, lds_stack + threadIdx.x, blockDim.x, pc);
+    render_column<NS, RGB, MS, ADAPT>(sc, cam, rv, rt, rv.write_output ? out : nullptr, px, py, 1u, 1u,
+                                      rv.frame_start, rv.spp, lds_stack + threadIdx.x, blockDim.x, pc);
     if (counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
         atomicAdd(counters + 1, (unsigned long long)pc.nodes);
@@ -153,14 +154,19 @@ hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderVie
     dim3 block(64);
     dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8);
     size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
-#define VS_PT(NS_, RGB_)                                                                                        \
+#define VS_PT2(NS_, RGB_, AD_)                                                                                  \
     do {                                                                                                        \
         if (diffuse_only)                                                                                       \
-            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_DIFFUSE>), grid, block, lds, stream, sc, cam, rv, rt, out, \
-                               counters);                                                                       \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_DIFFUSE, AD_>), grid, block, lds, stream, sc, cam, rv, rt, \
+                               out, counters);                                                                  \
         else                                                                                                    \
-            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_ALL>), grid, block, lds, stream, sc, cam, rv, rt, out,     \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_ALL, AD_>), grid, block, lds, stream, sc, cam, rv, rt, out, \
                                counters);                                                                       \
+    } while (0)
+#define VS_PT(NS_, RGB_)                                                                                        \
+    do {                                                                                                        \
+        if (rv.vstats) VS_PT2(NS_, RGB_, true);                                                                 \
+        else VS_PT2(NS_, RGB_, false);                                                                          \
     } while (0)
     if (sc.spectrum_rgb) {
         VS_PT(3, true);
@@ -172,6 +178,7 @@ hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderVie
         return hipErrorInvalidValue;
     }
 #undef VS_PT
+#undef VS_PT2
     return hipGetLastError();
 }
 

@@ -548,6 +548,25 @@ SceneHost load_scene(const std::string &text, const std::string &base_dir) {
         s.integrator.mis_mode = p["mis_mode"].as_int(0);
         if (s.integrator.type != "pt" && s.integrator.type != "rt")
             throw std::runtime_error("unsupported integrator type: " + s.integrator.type);
+        {
+            SvgfHost &v = s.integrator.svgf;
+            const Json &dn = p["denoiser"];
+            v.svgf = dn["type"].as_string("svgf") == "svgf";
+            const Json &dp = dn["param"];
+            v.N = dp["N"].as_uint(3);
+            v.alpha = float(dp["alpha"].as_float(0.05));
+            v.moments_alpha = float(dp["moments_alpha"].as_float(0.2));
+            v.history_limit = dp["history_limit"].as_uint(32);
+            v.moments_radius = dp["moments_filter_radius"].as_int(3);
+            v.sigma_rt = float(dp["sigma_rt"].as_float(10.0));
+            v.sigma_normal = float(dp["sigma_normal"].as_float(30.0));
+            if (v.N > 16 || v.moments_radius < 0 || v.moments_radius > 16 || v.history_limit == 0)
+                throw std::runtime_error("SVGF settings out of range");
+            AdaptiveHost &a = s.integrator.adaptive;
+            const Json &ad = p["adaptive"];
+            a.threshold = float(ad["threshold"].as_float(0.001));
+            a.min_samples = ad["start_index"].as_uint(128);
+        }
         if (s.integrator.type == "rt") {
             RestirHost &r = s.integrator.restir;
             const Json &di = p["direct"];

@@ -121,6 +121,30 @@ struct RestirGiHost {
     float t_radius{8.f}, t_theta{35.f}, t_depth{0.3f};
 };
 
+// the integrator's "denoiser" block (IntegratorDesc::init, node_desc.cpp:102-111:
+// type "svgf" by default) with SVGF's parameters (SVGF::SVGF, SVGF/svgf.h:44-53).
+// Like the reference's GUI switches, `on`, `reproject` and `filter_moments`
+// are set from the host (vs_scene_set_denoiser); SVGF runs after each frame
+// of the rt integrator (rt.cpp:157-158).
+struct SvgfHost {
+    bool svgf{true};// false for another denoiser type (not run)
+    bool on{false}, reproject{true}, filter_moments{true};
+    uint32_t N{3};
+    float alpha{0.05f}, moments_alpha{0.2f};
+    uint32_t history_limit{32};
+    int moments_radius{3};
+    float sigma_rt{10.f}, sigma_normal{30.f};
+};
+
+// the pt integrator's "adaptive" block: ConvergenceInspector's threshold and
+// start_index (adaptive/inspector.cpp:9-12); switched on from the host
+// (vs_scene_set_adaptive), as the reference's GUI does
+struct AdaptiveHost {
+    bool on{false};
+    float threshold{0.001f};
+    uint32_t min_samples{128};
+};
+
 struct IntegratorHost {
     std::string type{"pt"};
     uint32_t max_depth{16}, min_depth{5};
@@ -129,6 +153,8 @@ struct IntegratorHost {
     Json param;
     RestirHost restir;
     RestirGiHost gi;
+    SvgfHost svgf;
+    AdaptiveHost adaptive;
 };
 
 struct SceneHost {
