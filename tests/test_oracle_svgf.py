@@ -308,3 +308,34 @@ def test_cornell_gbuffer_closed_forms():
     bw = back.reshape(H, W)
     inner = bw & ndimage.binary_erosion(bw, np.ones((3, 3)))
     assert np.abs(geo[:, 6].reshape(H, W)[inner]).max() < 1e-4
+
+
+def test_svgf_smooths_the_cornell_box_back_wall():
+    # ReSTIR DI frames of the Cornell box, the film not accumulating (the
+    # real-time setting), with the denoiser on: inside the back wall (flat,
+    # one albedo) the denoised picture varies much less from pixel to pixel
+    # than the frame it was computed from
+    W, H = 64, 48
+    scene = scenegen.cornell_box(W, H, max_depth=2, light_sampler="power")
+    scene["camera"]["param"]["radiance_collector"]["param"]["accumulation"] = 0
+    scene["integrator"] = scenegen.restir_integrator(M_light=2, M_bsdf=1)
+    with va.Context(-1) as ctx:
+        sc = ctx.load_text(scene).build_accel()
+        sc.set_denoiser(True)
+        ex = sc.export()
+    orc = oracle.Restir(ex)
+    orc.render(8)
+    cur = orc.frame & 1
+    n = orc.geo[cur][:, :3]
+    back = (orc.geo[cur][:, 7] > 0) & (n[:, 2] > 0.999)
+    inner = ndimage.binary_erosion(back.reshape(H, W), np.ones((3, 3)))
+    film = orc.film.reshape(H, W, 4)[..., :3]
+    pic = orc.picture.reshape(H, W, 4)[..., :3]
+    pair = inner[:, 1:] & inner[:, :-1]
+
+    def rough(img):
+        return np.abs(np.diff(img, axis=1))[pair].mean()
+    print(f"neighbour differences on the back wall: film {rough(film):.4f}, denoised {rough(pic):.4f}")
+    assert rough(pic) < 0.5 * rough(film)
+    # and it keeps the wall's brightness
+    assert abs(pic[inner].mean() / film[inner].mean() - 1) < 0.05
