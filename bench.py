@@ -420,7 +420,7 @@ def serve_main(args) -> None:
 
 def _compact(l: dict) -> dict:
     keep = ("value", "unit", "steps", "warmup", "ms_per_step", "mrays_per_s", "rays_per_sample", "bvh_build_s",
-            "relmse_vs_4096spp", "relmse_at", "relmse_reference", "stages_ms_per_step")
+            "relmse_vs_4096spp", "relmse_detail", "relmse_at", "relmse_reference", "stages_ms_per_step")
     out = {k: l[k] for k in keep if k in l}
     out["workload"] = l["config"]["workload"] + (f"; {l['config']['kernel']}" if "kernel" in l["config"] else "")
     rf = l["roofline"]
@@ -581,6 +581,20 @@ def relmse(x, ref) -> float:
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
+def relmse_detail(x, ref) -> dict:
+    """How the relMSE is spread over pixels: its median per pixel, the mean
+    without the 0.1% of pixels with the largest error, that 0.1%'s share of
+    the total, and the largest per-pixel term (a few fireflies of a bright
+    HDRI sun or emitter can carry the mean)."""
+    import numpy as np
+    x, r = x[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
+    e = ((x - r) ** 2 / (r * r + 1e-2)).mean(axis=-1).ravel()
+    order = np.sort(e)
+    k = max(1, int(len(e) * 0.001))
+    return {"median": float(np.median(e)), "mean_without_top_0.1pct": float(order[:-k].mean()),
+            "top_0.1pct_share": float(order[-k:].sum() / max(e.sum(), 1e-300)), "max_pixel": float(order[-1])}
+
+
 def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None, cameras=None,
                    fov_y: float = 19.5):
     """A plain path-traced film of `spp` samples per pixel (megakernel, in
@@ -626,6 +640,8 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
     """One of the plain (non-serving) workloads: `steps` timed jobs after
     `warmup` untimed ones, a counting run for the roofline's algorithmic
     bytes, relMSE against a 4096-spp plain render, the CPU baseline."""
+    import numpy as np
+
     import vision_amd as va
     from vision_amd import scenegen
     scene_desc, base_dir, spp, config = workload(name)
@@ -743,7 +759,9 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
         # scene; the 1080p workloads are judged at 480x270 (same scene, spp and
         # camera path), where the reference costs 0.5 G samples
         if name == "cbox":
-            line["relmse_vs_4096spp"] = relmse(film, reference_film(ctx, scene_desc, base_dir, ref_spp))
+            ref = reference_film(ctx, scene_desc, base_dir, ref_spp)
+            line["relmse_vs_4096spp"] = relmse(film, ref)
+            line["relmse_detail"] = relmse_detail(film, ref)
             line["relmse_at"] = f"{W}x{H}, the timed job's film"
         else:
             rw, rh = 480, 270
@@ -761,7 +779,10 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
                 test = sc.film(0)
                 ref = reference_film(ctx, scene_desc, base_dir, ref_spp, (rw, rh))
             line["relmse_vs_4096spp"] = relmse(test, ref)
+            line["relmse_detail"] = relmse_detail(test, ref)
             line["relmse_at"] = f"{rw}x{rh}, same scene and {spp} {'frames' if restir else 'spp'}"
+            if os.environ.get("VS_BENCH_DUMP"):
+                np.savez_compressed(Path(os.environ["VS_BENCH_DUMP"]) / f"relmse_{name}.npz", test=test, ref=ref)
             sc.set_resolution(W, H)
     if cpu:
         line["cpu_baseline"] = restir_cpu_baseline(sc) if restir else cpu_baseline(sc, spp)
