@@ -40,6 +40,10 @@ writes, cache hits included):
       material + CANDIDATE_BYTES per light candidate, per pixel;
   ReSTIR shader1: traversal + records (pass-through, current reservoir,
       own and neighbour surfaces and reservoirs) + film, per pixel.
+`traffic` is the HBM bytes the same kernel moved in rocprofv3 PMC runs of
+the workload (profiles/hbm_traffic.json: FETCH_SIZE x 2.0 + WRITE_SIZE x
+1.0, the factors a plain 256 MiB copy reads back), null where no PMC run
+covers the kernel and workload.
 """
 from __future__ import annotations
 
@@ -64,6 +68,17 @@ SHADE_BYTES = 240 + 16 + 8 + SURFACE_GATHER + MATERIAL_BYTES + CANDIDATE_BYTES
 
 
 SCENE_CACHE = ROOT / "vision_amd" / "_scenes"
+TRAFFIC = ROOT / "profiles" / "hbm_traffic.json"
+
+
+def measured_traffic(key: str):
+    """HBM bytes a kernel moved per sample or per launch in rocprofv3 PMC runs
+    of this workload (FETCH_SIZE and WRITE_SIZE, each scaled by what the same
+    counter reads for a plain 256 MiB copy); None where no run covers it."""
+    try:
+        return json.loads(TRAFFIC.read_text())["kernels"].get(key)
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def workload(name: str):
@@ -253,6 +268,17 @@ def serve_main(args) -> None:
             print(f"plain: {n + 1} jobs", file=sys.stderr, flush=True)
     plain_wall = time.perf_counter() - t0
     plain_msps = total_samples / plain_wall / 1e6
+    # what batching can gain over this plain form at most: every job at the
+    # best rate a plain job of its scene and spectrum mode reached (a large
+    # job keeps the GPU full, so that rate stands for the kernel's own
+    # throughput), against the plain form's measured service times
+    best = {}
+    for j, t in zip(jobs, service):
+        key = (j["scene"], j["spectral"])
+        best[key] = max(best.get(key, 0.0), j["width"] * j["height"] * j["spp"] / t)
+    ideal_s = sum(j["width"] * j["height"] * j["spp"] / best[(j["scene"], j["spectral"])] for j in jobs)
+    batching_bound = sum(service) / ideal_s
+    small = sum(j["width"] * j["height"] * j["spp"] for j in jobs if j["width"] * j["height"] * j["spp"] < 1 << 20)
     # plain films of a few jobs for the batched-vs-plain comparison
     check = list(range(min(args.check_jobs, len(jobs))))
     plain_films = {}
@@ -349,6 +375,7 @@ def serve_main(args) -> None:
     kernel_s = st_time["busy_ms"] / 1e3
     achieved = alg_bytes / kernel_s / 1e9
     rays_per_sample = st_cnt["rays"] / sub_samples
+    tr = measured_traffic("serve_kernel@serving_mix")
     line = {
         "metric": METRIC,
         "value": round(batched_msps, 3),
@@ -377,6 +404,10 @@ def serve_main(args) -> None:
         "autotune": tune,
         "plain_msamples_per_s": round(plain_msps, 3),
         "batched_over_plain": round(batched_msps / plain_msps, 3),
+        "batched_over_plain_bound": round(batching_bound, 3),
+        "batching_bound_note": ("the plain service time over the time of every job at the best plain rate of its "
+                                "scene and spectrum mode; jobs under 1 M samples carry "
+                                f"{100.0 * small / total_samples:.2f}% of the samples"),
         "relmse_batched_vs_plain": relmse,
         "bit_identical_jobs_checked": f"{identical}/{len(check)}",
         "latency": {
@@ -394,10 +425,14 @@ def serve_main(args) -> None:
         "busy_ms_persistent_steps": round(st_sat["busy_ms"], 3),
         "max_active_jobs": st_sat["max_active"], "hbm_peak_bytes": st_sat["hbm_peak"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": round(tr["hbm_bytes_per_sample"] * sub_samples) if tr else None,
+                     "traffic_unit": "HBM bytes over the counting jobs, the same unit as the algorithmic bytes",
                      "kernel": "serve_kernel (persistent multi-job megakernel)",
                      "kernel_ms": round(st_time["busy_ms"], 3), "kernel_jobs": len(sub),
-                     "algorithmic_bytes_per_sample": round(alg_bytes / sub_samples, 1)},
+                     "algorithmic_bytes": int(alg_bytes),
+                     "algorithmic_bytes_per_sample": round(alg_bytes / sub_samples, 1),
+                     "hbm_bytes_per_sample": tr["hbm_bytes_per_sample"] if tr else None},
         "wavefront_lane_routes": lane_pairs,
     }
     if not args.no_cpu:
@@ -725,6 +760,8 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
         kernel = "pt_kernel (megakernel)"
     achieved = alg_bytes / kernel_s / 1e9
     mrays = cst["rays"] / avg_kernel_s / 1e6
+    tag = {"procedural": "procedural_1080p_64spp_hero4"}.get(name)
+    tr = measured_traffic(kernel.split()[0] + "@" + tag) if tag else None
     film = sc.film(0)
 
     line = {
@@ -746,7 +783,8 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
         "kernel_ms_per_step": round(avg_kernel_s * 1e3, 3),
         "bvh_build_s": round(build_s, 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": tr["hbm_bytes_per_launch"] if tr else None, "kernel": kernel,
                      "launches_per_step": int(launches),
                      "avg_launch_ms": round(kernel_s / max(launches, 1) * 1e3, 4),
                      "algorithmic_bytes_per_launch": int(alg_bytes / max(launches, 1)),
