@@ -76,7 +76,6 @@ extern "C" void *dc_load(const char *text, const char *base) {
     const SceneHost &h = s->h;
     const PackedScene &pk = s->pk;
     v.nodes = reinterpret_cast<const uint32_t *>(s->bvh.nodes.data());
-    v.node_quads = 5;
     v.leaf_tris = reinterpret_cast<const float *>(s->bvh.tris.data());
     v.P = h.wP.data();
     v.N = h.wN.data();

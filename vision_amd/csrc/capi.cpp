@@ -6,7 +6,6 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
-#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -185,13 +184,6 @@ void affine_inverse(const float *m, float *out) {
         for (int c = 0; c < 3; ++c) out[r * 4 + c] = float(inv[r][c]);
         out[r * 4 + 3] = float(-(inv[r][0] * t[0] + inv[r][1] * t[1] + inv[r][2] * t[2]));
     }
-}
-
-// uint4s per device node record: VS_NODE_QUADS (5 or 8), default 5
-uint32_t node_quads_setting() {
-    const char *e = std::getenv("VS_NODE_QUADS");
-    if (e && std::strcmp(e, "8") == 0) return 8u;
-    return 5u;
 }
 
 float radians_to_cos(float deg) { return std::cos(deg * (3.14159265358979323846f / 180.f)); }
@@ -416,17 +408,7 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         hip_check(hipSetDevice(s->ctx->device), "hipSetDevice");
         hipStream_t st = s->ctx->stream;
         const vs::SceneHost &h = s->host;
-        // node records on the device: packed (80 B apart) or one per 128-B
-        // cache line (VS_NODE_QUADS=8), the layout the traversal reads
-        s->node_quads = node_quads_setting();
-        if (s->node_quads == 5) {
-            s->nodes.upload(s->bvh.nodes, st);
-        } else {
-            std::vector<uint32_t> padded(s->bvh.nodes.size() * s->node_quads * 4u, 0u);
-            for (size_t i = 0; i < s->bvh.nodes.size(); ++i)
-                std::memcpy(padded.data() + i * s->node_quads * 4u, s->bvh.nodes[i].w, sizeof(vs::Bvh8Node));
-            s->nodes.upload(padded, st);
-        }
+        s->nodes.upload(s->bvh.nodes, st);
         s->ltris.upload(s->bvh.tris, st);
         s->P.upload(h.wP, st);
         s->N.upload(h.wN, st);
@@ -453,7 +435,6 @@ vs_status vs_scene_build_accel(vs_scene *s) {
         hip_check(hipStreamSynchronize(st), "upload sync");
         vs::SceneView &v = s->view;
         v.nodes = s->nodes.as<uint32_t>();
-        v.node_quads = s->node_quads;
         v.leaf_tris = s->ltris.as<float>();
         v.P = s->P.as<vs::f3>();
         v.N = s->N.as<vs::f3>();

@@ -119,7 +119,6 @@ struct vs_scene {
     vs::PackedScene packed;
     // device buffers
     DevBuf nodes, ltris, P, N, UV, tri, tri_inst, inst, lut;
-    uint32_t node_quads{5};// uint4s per device node record
     vs_internal::ModeTables mt;// materials, lights and environment in the scene's spectrum mode
     // the other spectrum mode, packed and uploaded on first use; kept until
     // the scene is destroyed, so views that point at it stay valid

@@ -10,7 +10,7 @@
 namespace vs {
 
 struct SceneView {
-    const uint32_t *nodes;   // Bvh8Node[], 80 B each, node_quads x 16 B apart
+    const uint32_t *nodes;   // Bvh8Node[], 80 B each
     const float *leaf_tris;  // LeafTri[], 48 B each
     const f3 *P;             // world positions per vertex
     const f3 *N;             // world (unnormalized) normals per vertex
@@ -29,7 +29,7 @@ struct SceneView {
     uint32_t stack_size;     // traversal stack entries per lane
     float ray_offset_factor;
     uint32_t num_tris;
-    uint32_t node_quads;// uint4s per BVH8 node record: 5 (packed, 80 B) or 8 (128-B aligned)
+    uint32_t pad;
     // spectra: CIE X, Y, Z and D65 tables (94 samples each, at offsets 0, 94,
     // 188, 282) followed by the materials' ior / eta / k curves
     const float *spd;

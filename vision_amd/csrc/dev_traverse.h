@@ -114,10 +114,10 @@ __device__ __forceinline__ float qbyte(uint32_t lo_word, uint32_t hi_word, int s
 
 // intersect the 8 child boxes of node `idx`; outputs the internal-child hit
 // mask in visiting order and the leaf-triangle mask relative to tri_base
-__device__ __forceinline__ void process_node(const uint4 *__restrict__ nodes, uint32_t nq, uint32_t idx,
-                                             const RayPre &r, float tmax, uint32_t &cbase, uint32_t &chits,
-                                             uint32_t &cimask, uint32_t &tbase, uint32_t &tmask, uint32_t &nfetch) {
-    const uint4 *np = nodes + size_t(idx) * nq;
+__device__ __forceinline__ void process_node(const uint4 *__restrict__ nodes, uint32_t idx, const RayPre &r,
+                                             float tmax, uint32_t &cbase, uint32_t &chits, uint32_t &cimask,
+                                             uint32_t &tbase, uint32_t &tmask, uint32_t &nfetch) {
+    const uint4 *np = nodes + size_t(idx) * 5;
     uint4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3], n4 = np[4];
     ++nfetch;
     float sx = __uint_as_float((n0.w & 0xFFu) << 23) * r.idir.x;
@@ -172,8 +172,7 @@ __device__ __forceinline__ TraceHit traverse(const SceneView &sc, f3 o, f3 d, fl
     h.b1 = h.b2 = 0.f;
     uint32_t sp = 0;
     uint32_t gbase, ghits, gimask, tbase, tmask;
-    const uint32_t nq = sc.node_quads;
-    process_node(nodes, nq, 0, r, tmax, gbase, ghits, gimask, tbase, tmask, nfetch);
+    process_node(nodes, 0, r, tmax, gbase, ghits, gimask, tbase, tmask, nfetch);
     while (true) {
         while (tmask) {
             uint32_t j = __builtin_ctz(tmask);
@@ -206,7 +205,7 @@ __device__ __forceinline__ TraceHit traverse(const SceneView &sc, f3 o, f3 d, fl
         uint32_t slot = i ^ r.oct;
         uint32_t child = gbase + __popc(gimask & ((1u << slot) - 1u));
         uint32_t nb, nh, ni;
-        process_node(nodes, nq, child, r, ANY ? tmax : h.t, nb, nh, ni, tbase, tmask, nfetch);
+        process_node(nodes, child, r, ANY ? tmax : h.t, nb, nh, ni, tbase, tmask, nfetch);
         if (nh) {
             if (ghits) {
                 if (sp < sc.stack_size) stack[sp * stride] = make_uint2(gbase, ghits | (gimask << 8));

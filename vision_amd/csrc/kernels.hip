@@ -51,13 +51,16 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 // with the film's running-mean arithmetic (normal.cpp:87-101), each frame's
 // path starting as soon as the previous one ends (render_column).  NS: wavelengths
 // carried per path; RGB: sRGB mode (three fixed channels) vs hero wavelengths.
-// The kernel asks for at least six resident waves per SIMD (a register
-// budget of 80 that spills some state to scratch): traversal latency is
-// better hidden by more waves than lost to the spills (measured: 1.9x on
-// the mixed-material procedural scene, 1.25-1.55x on diffuse scenes).
+// The kernel asks for a number of resident waves per SIMD that spills some
+// state to scratch: traversal latency is better hidden by more waves than
+// lost to the spills.  Measured on one MI355X: the diffuse family at six
+// waves (80 VGPRs; 1.25-1.55x over the compiler's own choice on diffuse
+// scenes, 4 and 5 waves slower); every material at four (128 VGPRs; 6 waves:
+// 466 ms, 5: 441 ms, 4: 428 ms, 3: 473 ms for the procedural 1080p
+// 64-spp hero-4 job).
 // ADAPT: adaptive sampling on (rv.vstats holds the per-pixel statistics).
 template <int NS, bool RGB, int MS, bool ADAPT>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MS == MS_ALL ? 4 : 6))) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
                                                   float4 *__restrict__ rt, float4 *__restrict__ out,
                                                   unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];

@@ -34,8 +34,16 @@ __device__ __forceinline__ void add_counters(unsigned long long *counters, const
     atomicAdd(counters + 2, (unsigned long long)pc.tris);
 }
 
+// Register budget of shader0 and shader1: four resident waves per SIMD for
+// the diffuse family (128 VGPRs; the compiler alone takes 161-187 and
+// three or two waves).  Measured on the 1080p many-lights job (64 frames):
+// shader0 150.7 -> 139.6 ms, shader1 59.9 -> 54.2 ms; five and six waves
+// were slower, and the spatial pass is fastest left free.  Kernels for
+// every material keep the compiler's choice (not measured).
+constexpr int restir_waves(int ms) { return ms == MS_ALL ? 1 : 4; }
+
 template <int NS, bool RGB, int MS>
-__global__ void __launch_bounds__(64) initial_kernel(SceneView sc, CameraView cam, RestirView v,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(restir_waves(MS)))) initial_kernel(SceneView sc, CameraView cam, RestirView v,
                                                      unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
@@ -58,7 +66,7 @@ __global__ void __launch_bounds__(64) spatial_kernel(SceneView sc, RestirView v,
 // shader1 plus the film's running mean (RealTimeIntegrator's combine,
 // rt.cpp:121-130, with the indirect half absent)
 template <int NS, bool RGB, int MS>
-__global__ void __launch_bounds__(64) final_kernel(SceneView sc, CameraView cam, RestirView v, RenderView rv,
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(restir_waves(MS)))) final_kernel(SceneView sc, CameraView cam, RestirView v, RenderView rv,
                                                    uint32_t in, float4 *__restrict__ rt, float4 *__restrict__ out,
                                                    unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
