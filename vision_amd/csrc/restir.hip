@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "dev_restir_gi.h"
+#include "xcd.h"
 
 namespace vs {
 
@@ -46,8 +47,9 @@ template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(restir_waves(MS)))) initial_kernel(SceneView sc, CameraView cam, RestirView v,
                                                      unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    const uint2 tile = xcd_tile_2d();
+    uint32_t px = tile.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = tile.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     PathCounters pc{0, 0, 0};
     restir_initial<NS, RGB, MS>(sc, cam, v, px, py, lds_stack + threadIdx.x, blockDim.x, pc);
@@ -57,8 +59,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(restir_
 template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) spatial_kernel(SceneView sc, RestirView v, uint32_t in, uint32_t out,
                                                      uint32_t dim) {
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    const uint2 tile = xcd_tile_2d();
+    uint32_t px = tile.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = tile.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     restir_spatial_pass<NS, RGB, MS>(sc, v, px, py, in, out, dim);
 }
@@ -70,8 +73,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(restir_
                                                    uint32_t in, float4 *__restrict__ rt, float4 *__restrict__ out,
                                                    unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    const uint2 tile = xcd_tile_2d();
+    uint32_t px = tile.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = tile.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     PathCounters pc{0, 0, 0};
     f3 L = restir_final<NS, RGB, MS>(sc, cam, v, px, py, in, lds_stack + threadIdx.x, blockDim.x, pc);
@@ -102,6 +106,8 @@ __global__ void __launch_bounds__(64) gi_initial_temporal_kernel(SceneView sc, C
                                                                  RestirGiView g, RenderView rv,
                                                                  unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
+    // launch order: a path's cost varies across the image, and XCD order
+    // would leave the XCDs with the cheap bands idle
     uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
@@ -118,8 +124,9 @@ __global__ void __launch_bounds__(64) gi_spatial_shade_kernel(SceneView sc, Rest
                                                               float4 *__restrict__ out,
                                                               unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
-    uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    const uint2 tile = xcd_tile_2d();
+    uint32_t px = tile.x * 8u + (threadIdx.x & 7u);
+    uint32_t py = tile.y * 8u + (threadIdx.x >> 3);
     if (px >= v.width || py >= v.height) return;
     const uint32_t idx = py * v.width + px;
     PathCounters pc{0, 0, 0};

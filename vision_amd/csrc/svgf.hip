@@ -13,6 +13,7 @@
 #include "dev_pt.h"
 #include "dev_restir.h"
 #include "svgf.h"
+#include "xcd.h"
 
 namespace vs {
 
@@ -90,8 +91,9 @@ template <int NS, bool RGB, int MS>
 __global__ void __launch_bounds__(64) gbuffer_kernel(SceneView sc, CameraView cam, RestirView rv, SvgfView sv,
                                                      unsigned long long *__restrict__ counters) {
     extern __shared__ uint2 lds_stack[];
-    const uint32_t px = blockIdx.x * 8u + (threadIdx.x & 7u);
-    const uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
+    const uint2 tile = xcd_tile_2d();
+    const uint32_t px = tile.x * 8u + (threadIdx.x & 7u);
+    const uint32_t py = tile.y * 8u + (threadIdx.x >> 3);
     if (px >= sv.width || py >= sv.height) return;
     const uint32_t pix = py * sv.width + px;
     Swl<NS> swl = sample_wavelengths<NS, RGB>(px, py, sv.frame);
@@ -139,7 +141,8 @@ __global__ void __launch_bounds__(64) gbuffer_kernel(SceneView sc, CameraView ca
 // FrameBuffer::compute_gradient (frame_buffer.cpp:124-168): writes only the
 // fwidth and depth-gradient words, which no thread of the pass reads
 __global__ void __launch_bounds__(256) gradient_kernel(SvgfView sv) {
-    const int x = int(blockIdx.x * 16u + (threadIdx.x & 15u)), y = int(blockIdx.y * 16u + (threadIdx.x >> 4));
+    const uint2 tile = xcd_tile_2d();
+    const int x = int(tile.x * 16u + (threadIdx.x & 15u)), y = int(tile.y * 16u + (threadIdx.x >> 4));
     const int W = int(sv.width), H = int(sv.height);
     if (x >= W || y >= H) return;
     const uint32_t cur = (sv.frame + 1u) & 1u;
@@ -190,7 +193,8 @@ __global__ void __launch_bounds__(256) gradient_kernel(SvgfView sv) {
 // kernel (reproject.cpp:29-162); writes buffer `out`
 __global__ void __launch_bounds__(256) reproject_kernel(SvgfView sv, SvgfParams p, const float4 *__restrict__ motion,
                                                         const float4 *__restrict__ rad, uint32_t out) {
-    const int x = int(blockIdx.x * 16u + (threadIdx.x & 15u)), y = int(blockIdx.y * 16u + (threadIdx.x >> 4));
+    const uint2 tile = xcd_tile_2d();
+    const int x = int(tile.x * 16u + (threadIdx.x & 15u)), y = int(tile.y * 16u + (threadIdx.x >> 4));
     const int W = int(sv.width), H = int(sv.height);
     if (x >= W || y >= H) return;
     const size_t i = size_t(y) * W + x;
@@ -287,7 +291,8 @@ __global__ void __launch_bounds__(256) reproject_kernel(SvgfView sv, SvgfParams 
 
 // FilterMoments (filter_moments.cpp:13-60): buffer `in` -> `out`
 __global__ void __launch_bounds__(256) moments_kernel(SvgfView sv, SvgfParams p, uint32_t in, uint32_t out) {
-    const int x = int(blockIdx.x * 16u + (threadIdx.x & 15u)), y = int(blockIdx.y * 16u + (threadIdx.x >> 4));
+    const uint2 tile = xcd_tile_2d();
+    const int x = int(tile.x * 16u + (threadIdx.x & 15u)), y = int(tile.y * 16u + (threadIdx.x >> 4));
     const int W = int(sv.width), H = int(sv.height);
     if (x >= W || y >= H) return;
     const uint32_t i = uint32_t(y) * sv.width + uint32_t(x);
@@ -330,7 +335,8 @@ __global__ void __launch_bounds__(256) moments_kernel(SvgfView sv, SvgfParams p,
 // also runs the Modulator's modulate kernel (modulator.cpp:19-27) into `pic`
 __global__ void __launch_bounds__(256) atrous_kernel(SvgfView sv, SvgfParams p, uint32_t in, uint32_t out, int step,
                                                      float4 *__restrict__ pic) {
-    const int x = int(blockIdx.x * 16u + (threadIdx.x & 15u)), y = int(blockIdx.y * 16u + (threadIdx.x >> 4));
+    const uint2 tile = xcd_tile_2d();
+    const int x = int(tile.x * 16u + (threadIdx.x & 15u)), y = int(tile.y * 16u + (threadIdx.x >> 4));
     const int W = int(sv.width), H = int(sv.height);
     if (x >= W || y >= H) return;
     const uint32_t i = uint32_t(y) * sv.width + uint32_t(x);
