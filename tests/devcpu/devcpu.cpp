@@ -134,18 +134,14 @@ extern "C" void dc_set_camera(void *p, const float *c) {
     }
 }
 
+// every sample of one pixel through render_column, the loop the PT and
+// serving megakernels run
 template <int NS, bool RGB>
-static void render_px(DcScene *s, const RenderView &rv, uint32_t px, uint32_t py, float *out) {
+static void render_px(DcScene *s, const RenderView &rv, uint32_t px, uint32_t py, float4 *rt) {
     std::vector<uint2> stack(64);
-    float acc[4] = {0, 0, 0, 0};
     PathCounters pc{0, 0, 0};
-    for (uint32_t f = rv.frame_start; f < rv.frame_start + rv.spp; ++f) {
-        f3 L = trace_path<NS, RGB>(s->v, s->cam, rv, px, py, f, stack.data(), 1, pc);
-        float v[4] = {zero_if_nan_inf(L.x), zero_if_nan_inf(L.y), zero_if_nan_inf(L.z), 1.f};
-        float a = 1.f / float(f + 1u);
-        for (int k = 0; k < 4; ++k) acc[k] = acc[k] + a * (v[k] - acc[k]);
-    }
-    for (int k = 0; k < 4; ++k) out[k] = acc[k];
+    render_column<NS, RGB>(s->v, s->cam, rv, rt, nullptr, px, py, 1u, 1u, rv.frame_start, rv.spp, stack.data(), 1u,
+                           pc);
 }
 
 extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *film) {
@@ -159,13 +155,19 @@ extern "C" void dc_render(void *p, const float *render, uint32_t spp, float *fil
     rv.spp = spp;
     rv.accumulation = 1;
     rv.spectrum_dim = s->pk.rgb ? 3u : s->h.spectrum_dim;
+    std::vector<float4> rt(size_t(s->cam.width) * s->cam.height, make_float4(0.f, 0.f, 0.f, 0.f));
     for (uint32_t y = 0; y < s->cam.height; ++y)
         for (uint32_t x = 0; x < s->cam.width; ++x) {
-            float *o = film + 4 * (size_t(y) * s->cam.width + x);
-            if (s->pk.rgb) render_px<3, true>(s, rv, x, y, o);
-            else if (rv.spectrum_dim == 4) render_px<4, false>(s, rv, x, y, o);
-            else render_px<3, false>(s, rv, x, y, o);
+            if (s->pk.rgb) render_px<3, true>(s, rv, x, y, rt.data());
+            else if (rv.spectrum_dim == 4) render_px<4, false>(s, rv, x, y, rt.data());
+            else render_px<3, false>(s, rv, x, y, rt.data());
         }
+    for (size_t i = 0; i < rt.size(); ++i) {
+        film[4 * i + 0] = rt[i].x;
+        film[4 * i + 1] = rt[i].y;
+        film[4 * i + 2] = rt[i].z;
+        film[4 * i + 3] = rt[i].w;
+    }
 }
 
 // ---- ReSTIR DI frame of the device code (restir.hip's three kernels, in order)

@@ -547,29 +547,6 @@ __device__ __forceinline__ void path_begin(const CameraView &cam, const RenderVi
     ps.alive = rv.max_depth > 0u ? 1u : 0u;
 }
 
-// the whole path in one thread (the megakernel form)
-template <int NS, bool RGB, int MS = MS_ALL>
-__device__ __forceinline__ f3 trace_path(const SceneView &sc, const CameraView &cam, const RenderView &rv, uint32_t px, uint32_t py,
-                         uint32_t frame, uint2 *stack, uint32_t stride, PathCounters &pc) {
-    PathState<NS> ps;
-    path_begin<NS, RGB>(cam, rv, px, py, frame, ps);
-    while (ps.alive) {
-        TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
-        ++pc.rays;
-        ShadowRay sr;
-        path_shade<NS, RGB, MS>(sc, rv, ps, h, sr);
-        if (sr.valid) {
-            uint32_t dn = 0, dt = 0;
-            TraceHit sh = traverse<true>(sc, sr.o, sr.d, 0.f, 1.f - kShadowEpsilon, stack, stride, dn, dt);
-            pc.nodes += dn;
-            pc.tris += dt;
-            ++pc.rays;
-            if (sh.tri == kInvalid) ps.L += sr.c;
-        }
-    }
-    return ps.L * ps.fw;
-}
-
 __device__ __forceinline__ float zero_if_nan_inf(float v) { return (isnan(v) || isinf(v)) ? 0.f : v; }
 
 __device__ __forceinline__ float tone_map(float v, uint32_t tm) {
