@@ -760,7 +760,7 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
         kernel = "pt_kernel (megakernel)"
     achieved = alg_bytes / kernel_s / 1e9
     mrays = cst["rays"] / avg_kernel_s / 1e6
-    tag = {"procedural": "procedural_1080p_64spp_hero4"}.get(name)
+    tag = {"procedural": "procedural_1080p_64spp_hero4", "restir": "restir_many_lights_1080p_64frames"}.get(name)
     tr = measured_traffic(kernel.split()[0] + "@" + tag) if tag else None
     film = sc.film(0)
 
