@@ -101,12 +101,16 @@ def workload(name: str):
                         "hero spectral (4 wavelengths), box filter (plain megakernel PT)",
             "resolution": [1920, 1080], "spp": spp, "max_depth": 16, "triangles": 1310722,
             "spectrum": "hero4"}
-    if name == "restir":
+    if name in ("restir", "restir_pairwise"):
         frames = 64
-        return scenegen.many_lights_box(1920, 1080, seed=0), ".", frames, {
+        scene = scenegen.many_lights_box(1920, 1080, seed=0)
+        pairwise = name == "restir_pairwise"
+        scene["integrator"]["param"]["direct"]["pairwise"] = pairwise
+        return scene, ".", frames, {
             "workload": "ReSTIR DI: Cornell-box geometry with 10,000 emissive triangles (5,000 quads, radiance "
                         "log-uniform [1,100]) + 256 point/spot lights, power light sampler; 32 light + 1 BSDF "
-                        "candidates, temporal reuse (history 20), 5 spatial neighbours within 30 px in 2 passes; "
+                        "candidates, temporal reuse (history 20), 5 spatial neighbours within 30 px in 2 passes "
+                        f"({'pairwise MIS' if pairwise else 'constant 1/M combine, the default'}); "
                         "1920x1080, 1 spp/frame x 64 frames along a seeded camera jitter path, sRGB, fp32 "
                         "reservoirs",
             "resolution": [1920, 1080], "frames": frames, "M_light": 32, "M_bsdf": 1, "spatial_passes": 2,
@@ -468,10 +472,14 @@ def other_configs(args) -> dict:
     """The plain workloads beside the serving headline, each a few timed
     jobs plus relMSE against a 4096-spp plain render: the Cornell box 256^2 x
     16 spp (megakernel), the procedural 1080p x 64 spp hero-4 scene
-    (wavefront form) and ReSTIR DI over 64 jitter-path frames at 1080p."""
+    (wavefront form) and ReSTIR DI over 64 jitter-path frames at 1080p, with
+    the reference's default constant 1/M combine and with pairwise MIS (the
+    constant combine darkens where neighbours' samples lack support at the
+    pixel: one-sided emitters, spot cones; pairwise MIS does not)."""
     out = {}
     for key, name, steps, form in (("C1_cbox", "cbox", 5, "megakernel"), ("C2_procedural", "procedural", 2, "wavefront"),
-                                   ("C3_restir", "restir", 2, "wavefront")):
+                                   ("C3_restir", "restir", 2, "wavefront"),
+                                   ("C3_restir_pairwise", "restir_pairwise", 2, "wavefront")):
         t0 = time.perf_counter()
         l = plain_workload(name, steps, 1, form, args.chunk_frames, args.trace_block, cpu=False)
         out[key] = _compact(l)
@@ -561,7 +569,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="serve", choices=["serve", "procedural", "cbox", "restir", "nrc"])
+    ap.add_argument("--workload", default="serve",
+                    choices=["serve", "procedural", "cbox", "restir", "restir_pairwise", "nrc"])
     ap.add_argument("--nrc-frames", type=int, default=128, help="nrc: frames of 4 spp")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096, help="nrc: spp of the plain reference render")
     ap.add_argument("--jobs-per-step", type=int, default=50, help="serve: jobs in one step of the list")
@@ -681,7 +690,7 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
     import vision_amd as va
     from vision_amd import scenegen
     scene_desc, base_dir, spp, config = workload(name)
-    restir = name == "restir"
+    restir = name.startswith("restir")
     ctx = va.Context(0)
     sc = ctx.load_text(scene_desc, base_dir)
     t0 = time.perf_counter()
