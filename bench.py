@@ -630,14 +630,16 @@ def relmse_detail(x, ref) -> dict:
     """How the relMSE is spread over pixels: its median per pixel, the mean
     without the 0.1% of pixels with the largest error, that 0.1%'s share of
     the total, and the largest per-pixel term (a few fireflies of a bright
-    HDRI sun or emitter can carry the mean)."""
+    HDRI sun or emitter can carry the mean); the film's mean over the
+    reference's (a biased estimator shows here, fireflies hardly)."""
     import numpy as np
     x, r = x[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     e = ((x - r) ** 2 / (r * r + 1e-2)).mean(axis=-1).ravel()
     order = np.sort(e)
     k = max(1, int(len(e) * 0.001))
     return {"median": float(np.median(e)), "mean_without_top_0.1pct": float(order[:-k].mean()),
-            "top_0.1pct_share": float(order[-k:].sum() / max(e.sum(), 1e-300)), "max_pixel": float(order[-1])}
+            "top_0.1pct_share": float(order[-k:].sum() / max(e.sum(), 1e-300)), "max_pixel": float(order[-1]),
+            "mean_over_reference": float(x.mean() / max(r.mean(), 1e-300))}
 
 
 def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None, cameras=None,
@@ -802,6 +804,10 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
     }
     if staged:
         line["stages_ms_per_step"] = [round(x * 1e3, 3) for x in stage_s]
+    if restir:
+        line["roofline"]["note"] = ("algorithmic bytes count each light candidate's 184 B gather (32 per pixel) "
+                                    "from light tables that stay in L2, so achieved can pass the HBM peak; "
+                                    "traffic is what reached HBM")
     if quality:
         # relMSE of the job's film against a 4096-spp plain render of the same
         # scene; the 1080p workloads are judged at 480x270 (same scene, spp and
