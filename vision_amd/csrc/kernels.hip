@@ -55,7 +55,7 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 // state to scratch: traversal latency is better hidden by more waves than
 // lost to the spills.  Measured on one MI355X: the diffuse family at six
 // waves (80 VGPRs; 1.25-1.55x over the compiler's own choice on diffuse
-// scenes, 4 and 5 waves slower); every material at four (128 VGPRs; 6 waves:
+// scenes, four waves 5 % slower); every material at four (128 VGPRs; 6 waves:
 // 466 ms, 5: 441 ms, 4: 428 ms, 3: 473 ms for the procedural 1080p
 // 64-spp hero-4 job).
 // ADAPT: adaptive sampling on (rv.vstats holds the per-pixel statistics).
