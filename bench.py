@@ -464,7 +464,7 @@ def _compact(l: dict) -> dict:
     out = {k: l[k] for k in keep if k in l}
     out["workload"] = l["config"]["workload"] + (f"; {l['config']['kernel']}" if "kernel" in l["config"] else "")
     rf = l["roofline"]
-    out["roofline"] = {k: rf[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms", "traffic") if k in rf}
+    out["roofline"] = {k: rf[k] for k in ("kernel", "achieved", "frac", "avg_launch_ms", "traffic", "note") if k in rf}
     return out
 
 
