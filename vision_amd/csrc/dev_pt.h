@@ -607,11 +607,12 @@ __device__ __forceinline__ bool vstats_converged(const RenderView &rv, float4 st
     return __float_as_uint(st.z) > rv.adapt_min_samples && st.y / st.x < rv.adapt_threshold;
 }
 
+// COUNT: tally rays and BVH fetches into pc (the counting runs).
 // ADAPT: adaptive sampling (pt.cpp:43-64): a pixel whose statistics have
 // converged adds no more samples (a skipped sample adds nothing, so the rest
 // of its frames are skipped too); each added sample feeds the statistics
 // the output buffer's value x + y + z
-template <int NS, bool RGB, int MS = MS_ALL, bool ADAPT = false>
+template <int NS, bool RGB, int MS = MS_ALL, bool ADAPT = false, bool COUNT = true>
 __device__ __forceinline__ void render_column(const SceneView &sc, const CameraView &cam, const RenderView &rv,
                               float4 *__restrict__ rt, float4 *__restrict__ out, uint32_t px, uint32_t y0,
                               uint32_t dy, uint32_t n_px, uint32_t frame_start, uint32_t frames, uint2 *stack,
@@ -635,17 +636,21 @@ __device__ __forceinline__ void render_column(const SceneView &sc, const CameraV
             fresh = false;
         }
         if (ps.alive) {
-            TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, pc.nodes, pc.tris);
-            ++pc.rays;
+            // COUNT = false: the fetch counts are dead and their increments
+            // in the traversal loops compile away
+            uint32_t dn = 0, dt = 0;
+            TraceHit h = traverse<false>(sc, ps.o, ps.d, 0.f, 3.402823466e38f, stack, stride, dn, dt);
             ShadowRay sr;
             path_shade<NS, RGB, MS>(sc, rv, ps, h, sr);
             if (sr.valid) {
-                uint32_t dn = 0, dt = 0;
                 TraceHit sh = traverse<true>(sc, sr.o, sr.d, 0.f, 1.f - kShadowEpsilon, stack, stride, dn, dt);
+                if (COUNT) ++pc.rays;
+                if (sh.tri == kInvalid) ps.L += sr.c;
+            }
+            if (COUNT) {
                 pc.nodes += dn;
                 pc.tris += dt;
                 ++pc.rays;
-                if (sh.tri == kInvalid) ps.L += sr.c;
             }
         }
         if (!ps.alive) {

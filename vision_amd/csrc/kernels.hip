@@ -59,7 +59,8 @@ __global__ void __launch_bounds__(256) trace_kernel(SceneView sc, const RayRec *
 // 466 ms, 5: 441 ms, 4: 428 ms, 3: 473 ms for the procedural 1080p
 // 64-spp hero-4 job).
 // ADAPT: adaptive sampling on (rv.vstats holds the per-pixel statistics).
-template <int NS, bool RGB, int MS, bool ADAPT>
+// COUNT: rays and BVH fetches tallied into counters.
+template <int NS, bool RGB, int MS, bool ADAPT, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MS == MS_ALL ? 4 : 6))) pt_kernel(SceneView sc, CameraView cam, RenderView rv,
                                                   float4 *__restrict__ rt, float4 *__restrict__ out,
                                                   unsigned long long *__restrict__ counters) {
@@ -68,9 +69,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(MS == M
     uint32_t py = blockIdx.y * 8u + (threadIdx.x >> 3);
     if (px >= cam.width || py >= cam.height) return;
     PathCounters pc{0, 0, 0};
-    render_column<NS, RGB, MS, ADAPT>(sc, cam, rv, rt, rv.write_output ? out : nullptr, px, py, 1u, 1u,
-                                      rv.frame_start, rv.spp, lds_stack + threadIdx.x, blockDim.x, pc);
-    if (counters) {
+    render_column<NS, RGB, MS, ADAPT, COUNT>(sc, cam, rv, rt, rv.write_output ? out : nullptr, px, py, 1u, 1u,
+                                             rv.frame_start, rv.spp, lds_stack + threadIdx.x, blockDim.x, pc);
+    if (COUNT && counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
         atomicAdd(counters + 1, (unsigned long long)pc.nodes);
         atomicAdd(counters + 2, (unsigned long long)pc.tris);
@@ -157,19 +158,20 @@ hipError_t launch_pt(const SceneView &sc, const CameraView &cam, const RenderVie
     dim3 block(64);
     dim3 grid((cam.width + 7) / 8, (cam.height + 7) / 8);
     size_t lds = size_t(sc.stack_size) * 64 * sizeof(uint2);
-#define VS_PT2(NS_, RGB_, AD_)                                                                                  \
+#define VS_PT2(NS_, RGB_, AD_, CT_)                                                                             \
     do {                                                                                                        \
         if (diffuse_only)                                                                                       \
-            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_DIFFUSE, AD_>), grid, block, lds, stream, sc, cam, rv, rt, \
-                               out, counters);                                                                  \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_DIFFUSE, AD_, CT_>), grid, block, lds, stream, sc, cam, rv, \
+                               rt, out, counters);                                                              \
         else                                                                                                    \
-            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_ALL, AD_>), grid, block, lds, stream, sc, cam, rv, rt, out, \
-                               counters);                                                                       \
+            hipLaunchKernelGGL((pt_kernel<NS_, RGB_, MS_ALL, AD_, CT_>), grid, block, lds, stream, sc, cam, rv, rt, \
+                               out, counters);                                                                  \
     } while (0)
 #define VS_PT(NS_, RGB_)                                                                                        \
     do {                                                                                                        \
-        if (rv.vstats) VS_PT2(NS_, RGB_, true);                                                                 \
-        else VS_PT2(NS_, RGB_, false);                                                                          \
+        if (rv.vstats) VS_PT2(NS_, RGB_, true, true);                                                           \
+        else if (counters) VS_PT2(NS_, RGB_, false, true);                                                      \
+        else VS_PT2(NS_, RGB_, false, false);                                                                   \
     } while (0)
     if (sc.spectrum_rgb) {
         VS_PT(3, true);

@@ -18,7 +18,8 @@ using namespace dev;
 
 // OCC > 0 asks the compiler for at least OCC resident waves per SIMD (a
 // register budget of 512 / OCC, spilling the rest); 0 leaves it free
-template <int NS, bool RGB, int MS, int OCC>
+// COUNT: rays and BVH fetches tallied into counters (the counting runs)
+template <int NS, bool RGB, int MS, int OCC, bool COUNT>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC > 0 ? OCC : 1))) serve_kernel(const SceneView *__restrict__ scenes,
                                                      const ServeJob *__restrict__ jobs,
                                                      const StepTable *__restrict__ table, uint32_t *__restrict__ next,
@@ -51,10 +52,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(OCC > 0
         // one pixel of the item's tile per lane, its frames back to back
         const uint32_t px = (t % job.tiles_x) * tile_w + lx;
         const uint32_t py = (t / job.tiles_x) * tile_h + ly;
-        render_column<NS, RGB, MS>(sc, job.cam, job.rv, job.rt, nullptr, px, py, 1u, 1u, e.frame_start, e.frames,
-                                   lds_stack + threadIdx.x, 64u, pc);
+        render_column<NS, RGB, MS, false, COUNT>(sc, job.cam, job.rv, job.rt, nullptr, px, py, 1u, 1u, e.frame_start,
+                                                 e.frames, lds_stack + threadIdx.x, 64u, pc);
     }
-    if (counters) {
+    if (COUNT && counters) {
         atomicAdd(counters + 0, (unsigned long long)pc.rays);
         atomicAdd(counters + 1, (unsigned long long)pc.nodes);
         atomicAdd(counters + 2, (unsigned long long)pc.tris);
@@ -81,22 +82,24 @@ namespace {
 using ServeFn = void (*)(const SceneView *, const ServeJob *, const StepTable *, uint32_t *, uint32_t,
                          unsigned long long *);
 
-template <int OCC>
+template <int OCC, bool COUNT>
 ServeFn serve_fn_occ(uint32_t variant) {
     switch (variant) {
-    case SV_RGB: return serve_kernel<3, true, MS_ALL, OCC>;
-    case SV_RGB_DIFFUSE: return serve_kernel<3, true, MS_DIFFUSE, OCC>;
-    case SV_HERO4: return serve_kernel<4, false, MS_ALL, OCC>;
-    case SV_HERO4_DIFFUSE: return serve_kernel<4, false, MS_DIFFUSE, OCC>;
-    case SV_HERO3: return serve_kernel<3, false, MS_ALL, OCC>;
-    case SV_HERO3_DIFFUSE: return serve_kernel<3, false, MS_DIFFUSE, OCC>;
+    case SV_RGB: return serve_kernel<3, true, MS_ALL, OCC, COUNT>;
+    case SV_RGB_DIFFUSE: return serve_kernel<3, true, MS_DIFFUSE, OCC, COUNT>;
+    case SV_HERO4: return serve_kernel<4, false, MS_ALL, OCC, COUNT>;
+    case SV_HERO4_DIFFUSE: return serve_kernel<4, false, MS_DIFFUSE, OCC, COUNT>;
+    case SV_HERO3: return serve_kernel<3, false, MS_ALL, OCC, COUNT>;
+    case SV_HERO3_DIFFUSE: return serve_kernel<3, false, MS_DIFFUSE, OCC, COUNT>;
     default: return nullptr;
     }
 }
 
-ServeFn serve_fn(uint32_t variant, uint32_t occ) {
-    if (occ == 0) return serve_fn_occ<0>(variant);
-    if (occ == 6) return serve_fn_occ<6>(variant);
+// count: the counting form (occupancy queries use the other, the one steps
+// run when not counting)
+ServeFn serve_fn(uint32_t variant, uint32_t occ, bool count = false) {
+    if (occ == 0) return count ? serve_fn_occ<0, true>(variant) : serve_fn_occ<0, false>(variant);
+    if (occ == 6) return count ? serve_fn_occ<6, true>(variant) : serve_fn_occ<6, false>(variant);
     return nullptr;
 }
 
@@ -105,7 +108,7 @@ ServeFn serve_fn(uint32_t variant, uint32_t occ) {
 hipError_t serve_launch(uint32_t variant, uint32_t occ, const SceneView *scenes, const ServeJob *jobs,
                         const StepTable *table, uint32_t *next, uint32_t blocks, uint32_t tile_w,
                         uint32_t stack_entries, unsigned long long *counters, hipStream_t stream) {
-    ServeFn fn = serve_fn(variant, occ);
+    ServeFn fn = serve_fn(variant, occ, counters != nullptr);
     if (!fn || blocks == 0 || !(tile_w == 8 || tile_w == 16 || tile_w == 32 || tile_w == 64) || stack_entries == 0 ||
         stack_entries > 64)
         return hipErrorInvalidValue;
