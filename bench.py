@@ -573,22 +573,33 @@ def nrc_main(args) -> None:
     from vision_amd import scenegen
 
     W, H, spp, frames = 1920, 1080, 4, args.nrc_frames
-    scene = json.dumps(scenegen.cornell_box(W, H, max_depth=8))
+    base = "."
+    if args.nrc_scene == "procedural":
+        # the C2 scene in sRGB mode at depth 16 (glass / metal / Disney under
+        # the sky HDRI: long, costly paths)
+        base = str(SCENE_CACHE / "procedural_s0")
+        scene_d = scenegen.procedural_scene(base, seed=0, n_instances=64, level=5, width=W, height=H, spp=spp,
+                                            max_depth=16, spectrum="srgb", hdri=(2048, 1024))
+        label, depth = "procedural scene (64 icospheres, 1.31M triangles, sky HDRI), sRGB", 16
+    else:
+        scene_d = scenegen.cornell_box(W, H, max_depth=8)
+        label, depth = "Cornell box", 8
+    scene = json.dumps(scene_d)
     ctx = va.Context(0)
 
     def relmse(x, r):
         x, r = x[..., :3].astype(np.float64), r[..., :3].astype(np.float64)
         return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
-    ref = ctx.load_text(scene).build_accel()
+    ref = ctx.load_text(scene, base).build_accel()
     left = args.nrc_ref_spp
     while left > 0:
         ref.render(min(512, left), fused=True)
         left -= min(512, left)
     R = ref.film(0)
     ref.close()
-    plain = ctx.load_text(scene).build_accel()
-    cached = ctx.load_text(scene).build_accel()
+    plain = ctx.load_text(scene, base).build_accel()
+    cached = ctx.load_text(scene, base).build_accel()
     with va.RadianceCache(plain) as p0, va.RadianceCache(cached) as nrc:
         for _ in range(args.warmup):
             p0.render(1, spp, terminate=False, train=False)
@@ -604,6 +615,11 @@ def nrc_main(args) -> None:
         cached.film(0)
         t_nrc = time.perf_counter() - t0
         e_plain, e_nrc = relmse(plain.film(0), R), relmse(cached.film(0), R)
+        # the plain path tracer given the cache's wall time
+        frames_eq = max(1, int(round(frames * t_nrc / t_plain)))
+        plain.reset_film()
+        p0.render(frames_eq, spp, terminate=False, train=False)
+        e_plain_eq = relmse(plain.film(0), R)
         samples = W * H * spp * frames
         # MFMA work of the cache per frame: forward on the queries, forward
         # + backward (3x) on the training batches
@@ -615,13 +631,14 @@ def nrc_main(args) -> None:
             "metric": METRIC, "value": round(samples / t_nrc / 1e6, 3), "unit": "Msamples/s", "n_gpus": 1,
             "steps": 1, "warmup": args.warmup, "ms_per_step": round(t_nrc * 1e3, 3), "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "bf16 (MLP) / f32", "data": "synthetic",
-            "config": {"workload": f"radiance cache: Cornell box 1920x1080, {spp} spp/frame x {frames} frames, "
-                                   "depth 8, paths terminate into the cache at bounce 2; hash grid 16 levels x 2^19 x "
+            "config": {"workload": f"radiance cache: {label} 1920x1080, {spp} spp/frame x {frames} frames, "
+                                   f"depth {depth}, paths terminate into the cache at bounce 2; hash grid 16 levels x 2^19 x "
                                    "2 fp16, MLP 32-64-64-64-3 ReLU on bf16 MFMA, Adam lr 1e-2; 65 536 training "
                                    "paths/frame, 4 steps x 65 536 samples",
                        "reference_spp": args.nrc_ref_spp},
             "plain_msamples_per_s": round(samples / t_plain / 1e6, 3),
             "relmse_cache": e_nrc, "relmse_plain": e_plain,
+            "relmse_plain_equal_time": e_plain_eq, "plain_frames_equal_time": frames_eq,
             "stage_ms": {k: round(sn[k], 2) for k in ("render_ms", "train_paths_ms", "infer_ms", "train_ms",
                                                       "accumulate_ms")},
             "plain_render_ms": round(sp["render_ms"], 2),
@@ -647,6 +664,7 @@ def main():
                     choices=["serve", "procedural", "cbox", "restir", "restir_pairwise", "nrc"])
     ap.add_argument("--nrc-frames", type=int, default=128, help="nrc: frames of 4 spp")
     ap.add_argument("--nrc-ref-spp", type=int, default=4096, help="nrc: spp of the plain reference render")
+    ap.add_argument("--nrc-scene", default="cbox", choices=["cbox", "procedural"], help="nrc: scene")
     ap.add_argument("--jobs-per-step", type=int, default=50, help="serve: jobs in one step of the list")
     ap.add_argument("--no-configs", action="store_true",
                     help="serve: skip the mixed-material serving and the cbox / procedural / restir keys")

@@ -30,7 +30,9 @@ struct vs_nrc {
     uint32_t sample_cap{0}, staging_paths{0};
     vs::NrcView view{};
     uint32_t adam_t{0};
-    hipEvent_t ev[8]{};
+    // per-frame stage events of vs_nrc_render (6 a frame), made as needed:
+    // the frames run back to back and their times are read once at the end
+    std::vector<hipEvent_t> fev;
     vs_nrc_stats stats{};
 };
 
@@ -164,7 +166,6 @@ vs_status vs_nrc_create(vs_scene *s, const vs_nrc_config *cfg, vs_nrc **out) {
         std::uniform_real_distribution<float> dg(-1e-4f, 1e-4f);
         for (float &g : grid) g = dg(rng);
         upload_params(n.get(), mlp.data(), grid.data(), stream);
-        for (auto &e : n->ev) hip_check(hipEventCreate(&e), "event");
         hip_check(hipStreamSynchronize(stream), "nrc init");
     });
     if (st != VS_OK) return st;
@@ -174,7 +175,7 @@ vs_status vs_nrc_create(vs_scene *s, const vs_nrc_config *cfg, vs_nrc **out) {
 
 void vs_nrc_destroy(vs_nrc *n) {
     if (!n) return;
-    for (auto &e : n->ev)
+    for (auto &e : n->fev)
         if (e) (void)hipEventDestroy(e);
     delete n;
 }
@@ -309,26 +310,32 @@ vs_status vs_nrc_render(vs_nrc *n, uint32_t frames, uint32_t spp, uint32_t flags
         }
         uint32_t *qcount = n->counts.as<uint32_t>(), *tcount = qcount + 1;
         float ms[6] = {0, 0, 0, 0, 0, 0};
+        while (n->fev.size() < size_t(frames) * 6) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreate(&e), "event");
+            n->fev.push_back(e);
+        }
         for (uint32_t f = 0; f < frames; ++f) {
+            hipEvent_t *ev = n->fev.data() + size_t(f) * 6;
             const uint32_t frame0 = s->frame_index;
             hip_check(vs::nrc_zero(qcount, 2, st), "zero");
-            hip_check(hipEventRecord(n->ev[0], st), "event");
+            hip_check(hipEventRecord(ev[0], st), "event");
             hip_check(vs::nrc_render(s->view, s->cam, rv, frame0, spp, terminate ? n->cfg.term_bounce : 0u,
                                      n->L.as<float4>(), n->queries.as<vs::NrcQuery>(), qcount, uint32_t(slots),
                                      s->diffuse_only, st),
                       "nrc render");
-            hip_check(hipEventRecord(n->ev[1], st), "event");
+            hip_check(hipEventRecord(ev[1], st), "event");
             if (train)
                 hip_check(vs::nrc_train_paths(s->view, s->cam, rv, frame0 / spp, stride, n->staging.as<float>(),
                                               n->samples.as<vs::NrcSample>(), tcount, n->sample_cap, s->diffuse_only,
                                               st),
                           "train paths");
-            hip_check(hipEventRecord(n->ev[2], st), "event");
+            hip_check(hipEventRecord(ev[2], st), "event");
             if (terminate)
                 hip_check(vs::nrc_infer(n->view, n->queries.as<vs::NrcQuery>(), qcount, uint32_t(slots),
                                         n->L.as<float4>(), st),
                           "infer");
-            hip_check(hipEventRecord(n->ev[3], st), "event");
+            hip_check(hipEventRecord(ev[3], st), "event");
             if (train) {
                 // steps over consecutive batches of the frame's samples (all of
                 // them in each step when train_batch is 0)
@@ -342,16 +349,19 @@ vs_status vs_nrc_render(vs_nrc *n, uint32_t frames, uint32_t spp, uint32_t flags
                     adam_step(n, st);
                 }
             }
-            hip_check(hipEventRecord(n->ev[4], st), "event");
+            hip_check(hipEventRecord(ev[4], st), "event");
             hip_check(vs::nrc_accumulate(rv, n->L.as<float4>(), P, spp, frame0, s->film_rt.as<float4>(),
                                          s->film_out.as<float4>(), st),
                       "accumulate");
-            hip_check(hipEventRecord(n->ev[5], st), "event");
+            hip_check(hipEventRecord(ev[5], st), "event");
             s->frame_index += spp;
-            hip_check(hipEventSynchronize(n->ev[5]), "frame");
+        }
+        if (frames) hip_check(hipEventSynchronize(n->fev[size_t(frames) * 6 - 1]), "frames");
+        for (uint32_t f = 0; f < frames; ++f) {
+            const hipEvent_t *ev = n->fev.data() + size_t(f) * 6;
             for (int k = 0; k < 5; ++k) {
                 float t = 0.f;
-                hip_check(hipEventElapsedTime(&t, n->ev[k], n->ev[k + 1]), "event time");
+                hip_check(hipEventElapsedTime(&t, ev[k], ev[k + 1]), "event time");
                 ms[k] += t;
             }
         }
