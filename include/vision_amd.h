@@ -351,8 +351,9 @@ typedef struct vs_server_config {
     uint32_t waves_per_cu; /* persistent waves per CU for each kernel variant; 0: the occupancy limit */
     int32_t use_graphs;    /* 1: replay captured hipGraphs; 0: launch each step's kernels directly */
     int32_t count_rays;    /* 1: count rays and BVH fetches (slower) */
-    uint32_t min_waves_per_eu; /* register budget of the kernels: 0 the compiler's choice, 6 at least
-                                  six resident waves per SIMD (spilling what does not fit) */
+    uint32_t min_waves_per_eu; /* register budget of the kernels: 0 the compiler's choice, 6 the measured
+                                  budget: at least six resident waves per SIMD for the diffuse-family
+                                  kernels, four for the every-material ones (spilling what does not fit) */
 } vs_server_config;
 
 typedef struct vs_job_desc {

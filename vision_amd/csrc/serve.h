@@ -61,7 +61,8 @@ enum ServeVariant : uint32_t {
 };
 
 // register budgets the kernels are compiled for: 0 = the compiler's choice,
-// 6 = at least 6 resident waves per SIMD
+// 6 = the measured budget: at least 6 resident waves per SIMD for the
+// diffuse-family kernels, 4 for the every-material ones
 inline bool serve_occ_supported(uint32_t occ) { return occ == 0 || occ == 6; }
 
 // the persistent kernel of one variant: `blocks` one-wave workgroups claim
