@@ -682,8 +682,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-quality", action="store_true", help="skip the relMSE against 4096-spp references")
     ap.add_argument("--kernel", default="wavefront", choices=["megakernel", "wavefront"])
-    ap.add_argument("--chunk-frames", type=int, default=16)
-    ap.add_argument("--trace-block", type=int, default=128)
+    # wavefront launch shape, measured on the procedural job: frames per chunk
+    # 8 / 16 / 32 / 64 gave 477 / 514 / 531 / 538 Msamples/s at traversal
+    # blocks of 64 threads, which beat 128 and 256 at every chunk size
+    ap.add_argument("--chunk-frames", type=int, default=64)
+    ap.add_argument("--trace-block", type=int, default=64)
     ap.add_argument("--copy-probe", action="store_true",
                     help="only copy 256 MiB device to device three times (the reference point that scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiling run to bytes)")
@@ -867,6 +870,8 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
     mrays = cst["rays"] / avg_kernel_s / 1e6
     tag = {"procedural": "procedural_1080p_64spp_hero4", "restir": "restir_many_lights_1080p_64frames"}.get(name)
     tr = measured_traffic(kernel.split()[0] + "@" + tag) if tag else None
+    if tr and staged and not restir and (tr.get("chunk_frames"), tr.get("trace_block")) != (chunk_frames, trace_block):
+        tr = None  # measured at another launch shape: bytes per launch differ
     film = sc.film(0)
 
     line = {
