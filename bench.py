@@ -454,7 +454,9 @@ def serve_main(args) -> None:
         sc.close()
     ctx.close()
     if not args.no_configs:
-        line["mixed_material"] = serve_mixed(args)
+        # the launch shape the serving run used (autotuned or given)
+        shape = {k: line["config"][k] for k in ("tile_w", "waves_per_cu", "min_waves_per_eu")}
+        line["mixed_material"] = serve_mixed(args, shape)
         line["configs"] = other_configs(args)
     print(json.dumps(line))
 
@@ -469,12 +471,13 @@ def _compact(l: dict) -> dict:
     return out
 
 
-def serve_mixed(args, n_jobs: int = 100) -> dict:
+def serve_mixed(args, shape: dict, n_jobs: int = 100) -> dict:
     """Serving over mixed-material scenes (the procedural generator's matte /
     metal / glass / Disney icospheres under the sky HDRI, 102k-2.0M
     triangles: the every-material kernels): a seeded list of n_jobs jobs of
     the serving mix rendered the plain way one at a time, then all queued at
-    once through the server; two jobs' films compared bit for bit."""
+    once through the server at the launch shape `shape` (tile_w,
+    waves_per_cu, min_waves_per_eu); two jobs' films compared bit for bit."""
     import numpy as np
     import vision_amd as va
     from vision_amd import scenegen
@@ -502,8 +505,7 @@ def serve_mixed(args, n_jobs: int = 100) -> dict:
     for i in range(2):
         plain_job(scenes[jobs[i]["scene"]], jobs[i])
         check[i] = scenes[jobs[i]["scene"]].film(0)
-    srv = va.Server(ctx, max_jobs=args.max_jobs, step_ms=args.step_ms, tile_w=args.tile_w,
-                    waves_per_cu=args.waves_per_cu, min_waves_per_eu=args.min_waves_per_eu)
+    srv = va.Server(ctx, max_jobs=args.max_jobs, step_ms=args.step_ms, **shape)
     for sc in scenes:
         srv.add_scene(sc)
 
@@ -533,7 +535,7 @@ def serve_mixed(args, n_jobs: int = 100) -> dict:
     res = {"value": round(st["msamples_per_s"], 3), "unit": "Msamples/s", "plain_msamples_per_s": round(plain_msps, 3),
            "batched_over_plain": round(st["msamples_per_s"] / plain_msps, 3),
            "bit_identical_jobs_checked": f"{identical}/{len(check)}", "jobs": n_jobs, "samples": total,
-           "triangles": tris,
+           "triangles": tris, **shape,
            "workload": "mixed-material serving: jobs of the serving mix (256^2..1920x1080, 1-64 spp, depth "
                        "U[8, 16], sRGB or hero-4) over 4 procedural scenes (displaced level-5 icospheres of "
                        "matte / metal / glass / Disney under the sky + sun HDRI) of the listed triangle counts"}
