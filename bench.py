@@ -729,15 +729,19 @@ def relmse_detail(x, ref) -> dict:
     without the 0.1% of pixels with the largest error, that 0.1%'s share of
     the total, and the largest per-pixel term (a few fireflies of a bright
     HDRI sun or emitter can carry the mean); the film's mean over the
-    reference's (a biased estimator shows here, fireflies hardly)."""
+    reference's, over all pixels and without that 0.1% (a biased estimator
+    shows in the second, a few fireflies only in the first)."""
     import numpy as np
     x, r = x[..., :3].astype(np.float64), ref[..., :3].astype(np.float64)
     e = ((x - r) ** 2 / (r * r + 1e-2)).mean(axis=-1).ravel()
     order = np.sort(e)
     k = max(1, int(len(e) * 0.001))
+    keep = e < order[-k]
+    xm, rm = x.mean(axis=-1).ravel(), r.mean(axis=-1).ravel()
     return {"median": float(np.median(e)), "mean_without_top_0.1pct": float(order[:-k].mean()),
             "top_0.1pct_share": float(order[-k:].sum() / max(e.sum(), 1e-300)), "max_pixel": float(order[-1]),
-            "mean_over_reference": float(x.mean() / max(r.mean(), 1e-300))}
+            "mean_over_reference": float(x.mean() / max(r.mean(), 1e-300)),
+            "mean_over_reference_without_top_0.1pct": float(xm[keep].mean() / max(rm[keep].mean(), 1e-300))}
 
 
 def reference_film(ctx, scene_desc, base_dir, spp: int, resolution=None, pt_depth=None, cameras=None,
