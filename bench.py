@@ -463,7 +463,7 @@ def serve_main(args) -> None:
 
 def _compact(l: dict) -> dict:
     keep = ("value", "unit", "steps", "warmup", "ms_per_step", "mrays_per_s", "rays_per_sample", "bvh_build_s",
-            "relmse_vs_4096spp", "relmse_detail", "relmse_at", "relmse_reference", "stages_ms_per_step")
+            "relmse_vs_4096spp", "relmse_detail", "relmse_at", "relmse_reference", "stages_ms_per_step", "note")
     out = {k: l[k] for k in keep if k in l}
     out["workload"] = l["config"]["workload"] + (f"; {l['config']['kernel']}" if "kernel" in l["config"] else "")
     rf = l["roofline"]
@@ -908,6 +908,10 @@ def plain_workload(name: str, steps: int, warmup: int, kernel_form: str, chunk_f
     }
     if staged:
         line["stages_ms_per_step"] = [round(x * 1e3, 3) for x in stage_s]
+    if name == "restir_pairwise":
+        line["note"] = ("the reference's pairwise combine weights a neighbour's sample by its target at the "
+                        "neighbour's own pixel and normalises by it (ReSTIR/direct.cpp:248, 283; restated as is), "
+                        "so where the targets at the two pixels differ by orders of magnitude the estimate explodes")
     if restir:
         line["roofline"]["note"] = ("algorithmic bytes count each light candidate's 184 B gather (32 per pixel) "
                                     "from light tables that stay in L2, so achieved can pass the HBM peak; "
