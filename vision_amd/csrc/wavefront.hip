@@ -1,9 +1,9 @@
 // Wavefront path tracer for gfx950: the bounce loop of the megakernel split at
 // its traversals.  Per bounce: closest-hit over the live-path queue (lean
 // kernel, high occupancy), shading (path_shade: emission, NEE set-up, BSDF
-// sample, RR; appends live paths and shadow rays with one atomic per wave),
-// any-hit over the shadow rays.  Paths run the same path_shade arithmetic as
-// the megakernel, so both forms give the same film.
+// sample, RR; appends live paths and shadow rays, one atomic per block and
+// buffer), any-hit over the shadow rays.  Paths run the same path_shade
+// arithmetic as the megakernel, so both forms give the same film.
 #include <hip/hip_runtime.h>
 
 #include "dev_pt.h"
@@ -68,18 +68,6 @@ __device__ __forceinline__ void store_state(const WavefrontView &w, uint32_t i, 
     }
 }
 
-// a slot in an append buffer: one atomic per wave, lanes ranked by lane id
-__device__ __forceinline__ uint32_t append_slot(uint32_t *counter) {
-    unsigned long long mask = __ballot(1);
-    uint32_t lane = __lane_id();
-    uint32_t leader = uint32_t(__ffsll(mask) - 1);
-    uint32_t rank = uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
-    uint32_t base = 0u;
-    if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
-    base = __shfl(base, int(leader));
-    return base + rank;
-}
-
 template <int NS, bool RGB>
 __global__ void __launch_bounds__(256) begin_kernel(CameraView cam, RenderView rv, WavefrontView w, uint32_t frame0,
                                                     uint32_t n, uint32_t *__restrict__ queue) {
@@ -115,34 +103,62 @@ __global__ void __launch_bounds__(256) trace_queue_kernel(SceneView sc, Wavefron
 // left free the compiler takes 256 VGPRs and one wave.  Measured on the
 // procedural 1080p 64-spp hero-4 job: 2.64 ms per launch free, 2.06 ms at
 // three waves, 2.21 ms at four, 2.61 ms at five.
+// The survivors and shadow rays of a block are appended together: each wave
+// reserves its lanes' share in LDS, the block one range per buffer with one
+// global atomic.  Per-wave atomics on the two shared counters serialise in
+// L2: with them (64-thread blocks) the stage took 130.8 ms per procedural
+// job, with block ranges 106.1 ms.
+constexpr uint32_t kShadeBlock = 256;
+
 template <int NS, bool RGB>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(SceneView sc, RenderView rv, WavefrontView w,
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(SceneView sc, RenderView rv, WavefrontView w,
                                                    const uint32_t *__restrict__ qin, uint32_t count,
                                                    uint32_t *__restrict__ qout, uint32_t *__restrict__ counts) {
-    uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= count) return;
-    uint32_t i = qin[k];
-    PathState<NS> ps;
-    load_state<NS, RGB>(w, i, ps);
-    float4 hv = w.hit[i];
-    TraceHit h;
-    h.t = hv.x;
-    h.tri = __float_as_uint(hv.y);
-    h.b1 = hv.z;
-    h.b2 = hv.w;
+    __shared__ uint32_t s_n[2], s_base[2];
+    if (threadIdx.x < 2) s_n[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t i = 0u;
+    bool live = false;
     ShadowRay sr;
-    path_shade<NS, RGB>(sc, rv, ps, h, sr);
-    store_state<NS, RGB>(w, i, ps);
-    if (sr.valid) {
-        uint32_t slot = append_slot(counts + WF_SHADOW);
+    sr.valid = 0u;
+    if (k < count) {
+        i = qin[k];
+        PathState<NS> ps;
+        load_state<NS, RGB>(w, i, ps);
+        float4 hv = w.hit[i];
+        TraceHit h;
+        h.t = hv.x;
+        h.tri = __float_as_uint(hv.y);
+        h.b1 = hv.z;
+        h.b2 = hv.w;
+        path_shade<NS, RGB>(sc, rv, ps, h, sr);
+        store_state<NS, RGB>(w, i, ps);
+        live = ps.alive != 0u;
+    }
+    const bool shadow = sr.valid != 0u;
+    const unsigned long long below = (1ull << __lane_id()) - 1ull;
+    const unsigned long long ml = __ballot(live), ms = __ballot(shadow);
+    uint32_t wl = 0u, wsh = 0u;
+    if (__lane_id() == 0u) {
+        if (ml) wl = atomicAdd(&s_n[0], uint32_t(__popcll(ml)));
+        if (ms) wsh = atomicAdd(&s_n[1], uint32_t(__popcll(ms)));
+    }
+    wl = __shfl(wl, 0);
+    wsh = __shfl(wsh, 0);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        s_base[0] = s_n[0] ? atomicAdd(counts + WF_LIVE, s_n[0]) : 0u;
+        s_base[1] = s_n[1] ? atomicAdd(counts + WF_SHADOW, s_n[1]) : 0u;
+    }
+    __syncthreads();
+    if (shadow) {
+        const uint32_t slot = s_base[1] + wsh + uint32_t(__popcll(ms & below));
         w.sh_o[slot] = make_float4(sr.o.x, sr.o.y, sr.o.z, __uint_as_float(i));
         w.sh_d[slot] = make_float4(sr.d.x, sr.d.y, sr.d.z, 0.f);
         w.sh_c[slot] = make_float4(sr.c.x, sr.c.y, sr.c.z, 0.f);
     }
-    if (ps.alive) {
-        uint32_t slot = append_slot(counts + WF_LIVE);
-        qout[slot] = i;
-    }
+    if (live) qout[s_base[0] + wl + uint32_t(__popcll(ml & below))] = i;
 }
 
 __global__ void __launch_bounds__(256) shadow_kernel(SceneView sc, WavefrontView w,
@@ -218,7 +234,7 @@ hipError_t wf_shade(const SceneView &sc, const RenderView &rv, const WavefrontVi
                     bool rgb, const uint32_t *qin, uint32_t count, uint32_t *qout, uint32_t *counts,
                     hipStream_t stream) {
     if (count == 0) return hipSuccess;
-    dim3 g(blocks(count, 64)), b(64);
+    dim3 g(blocks(count, kShadeBlock)), b(kShadeBlock);
     if (rgb) hipLaunchKernelGGL((shade_kernel<3, true>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
     else if (spectrum_dim == 4)
         hipLaunchKernelGGL((shade_kernel<4, false>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
