@@ -108,25 +108,63 @@ __global__ void __launch_bounds__(256) trace_queue_kernel(SceneView sc, Wavefron
 // global atomic.  Per-wave atomics on the two shared counters serialise in
 // L2: with them (64-thread blocks) the stage took 130.8 ms per procedural
 // job, with block ranges 106.1 ms.
+// Before shading, a block sorts its paths by what they hit (the miss branch,
+// a material-less surface, or the material's kind) in LDS, so that a wave
+// mostly runs one branch of path_shade; the block's paths stay the same 256
+// queue entries, so their state loads stay within the same cache lines.
 constexpr uint32_t kShadeBlock = 256;
+constexpr uint32_t kShadeKeys = 16;// the last key: no path (past the queue's end)
+
+__device__ __forceinline__ uint32_t shade_key(const SceneView &sc, uint32_t tri) {
+    if (tri == kInvalid) return 0u;
+    const uint32_t m = sc.inst[sc.tri_inst[tri]].mat;
+    if (m == kInvalid) return 1u;
+    return 2u + min(sc.mat[m].type, kShadeKeys - 3u);
+}
 
 template <int NS, bool RGB>
 __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(SceneView sc, RenderView rv, WavefrontView w,
                                                    const uint32_t *__restrict__ qin, uint32_t count,
                                                    uint32_t *__restrict__ qout, uint32_t *__restrict__ counts) {
-    __shared__ uint32_t s_n[2], s_base[2];
+    __shared__ uint32_t s_n[2], s_base[2], s_key[kShadeKeys];
+    __shared__ uint32_t s_path[kShadeBlock];
+    __shared__ float4 s_hit[kShadeBlock];
     if (threadIdx.x < 2) s_n[threadIdx.x] = 0u;
+    if (threadIdx.x < kShadeKeys) s_key[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    uint32_t i = 0u;
+    {
+        const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+        uint32_t i0 = 0u, key = kShadeKeys - 1u;
+        float4 hv0 = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (k < count) {
+            i0 = qin[k];
+            hv0 = w.hit[i0];
+            key = shade_key(sc, __float_as_uint(hv0.y));
+        }
+        const uint32_t rank = atomicAdd(&s_key[key], 1u);
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t sum = 0u;
+            for (uint32_t j = 0; j < kShadeKeys; ++j) {
+                const uint32_t c = s_key[j];
+                s_key[j] = sum;
+                sum += c;
+            }
+        }
+        __syncthreads();
+        const uint32_t pos = s_key[key] + rank;
+        s_path[pos] = key == kShadeKeys - 1u ? kInvalid : i0;
+        s_hit[pos] = hv0;
+        __syncthreads();
+    }
+    const uint32_t i = s_path[threadIdx.x];
     bool live = false;
     ShadowRay sr;
     sr.valid = 0u;
-    if (k < count) {
-        i = qin[k];
+    if (i != kInvalid) {
         PathState<NS> ps;
         load_state<NS, RGB>(w, i, ps);
-        float4 hv = w.hit[i];
+        float4 hv = s_hit[threadIdx.x];
         TraceHit h;
         h.t = hv.x;
         h.tri = __float_as_uint(hv.y);
