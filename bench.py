@@ -27,9 +27,12 @@ per ray: BVH8 node fetches x 80 B + leaf-triangle fetches x 48 B + 48 B ray
 in / hit out.  Per-unit record bytes (the records the kernel reads and
 writes, cache hits included):
   megakernel PT: traversal + 32 B film read / write per pixel;
-  persistent serving kernel: traversal + 32 B film per sample (the counting
-      run covers the first --count-jobs jobs; the device time is the steps'
-      event time scaled to those jobs' share of the samples);
+  persistent serving kernel: traversal + 32 B film per sample (the first
+      --count-jobs jobs, routed as in the timed run, are run once timed and
+      once counting; jobs of --lane-min-samples samples or more go to the
+      wavefront lanes, whose stages are accounted as below, and the dominant
+      kernel is the one of the persistent kernel and the three lane stages
+      with the most event time);
   wavefront closest-hit: traversal;
   wavefront shading: SHADE_BYTES per shaded path (path state in / out,
       hit, queues, triangle, vertices, instance, material, light sample)
@@ -292,9 +295,12 @@ def serve_main(args) -> None:
         plain_films[i] = scenes[jobs[i]["scene"]].film(0)
 
     srv = va.Server(ctx, max_jobs=args.max_jobs, step_ms=args.step_ms, tile_w=args.tile_w,
-                    waves_per_cu=args.waves_per_cu, min_waves_per_eu=args.min_waves_per_eu)
+                    waves_per_cu=args.waves_per_cu, min_waves_per_eu=args.min_waves_per_eu,
+                    lane_min_samples=args.lane_min_samples)
     for sc in scenes:
         srv.add_scene(sc)
+
+    lane_total = [0]  # lane samples over every batch of this process (normalises PMC sums)
 
     def run_batch(js, arrivals=None):
         """Submit js (at their arrival offsets, or all at once) and wait for
@@ -323,7 +329,9 @@ def serve_main(args) -> None:
             if done % 100 == 0:
                 print(f"batched: {done} jobs", file=sys.stderr, flush=True)
         th.join()
-        return ids, srv.stats()
+        st = srv.stats()
+        lane_total[0] += st["lane_samples"]
+        return ids, st
 
     # warm-up steps (graph capture, first-step rate estimate)
     wids, _ = run_batch(warm_jobs)
@@ -359,12 +367,10 @@ def serve_main(args) -> None:
     for i in ids:
         srv.release(i)
     plain_lat = lindley(arrivals, service)
-    # roofline of the persistent kernel: the first count_jobs jobs, all on the
-    # persistent route, once timed (the steps' event time) and once counting
-    # rays and BVH fetches
+    # roofline of the dominant kernel: the first count_jobs jobs, routed as in
+    # the timed run, once timed (the persistent steps' event time and the
+    # lanes' per-stage event time) and once counting rays and BVH fetches
     lane_pairs = [(r["scene"], r["spectral"]) for r in (tune or {}).get("routes", []) if r["route"] == "wavefront"]
-    for sc_i, spectral in lane_pairs:
-        srv.set_route(sc_i, spectral, False)
     sub = jobs[:args.count_jobs]
     ids, st_time = run_batch(sub)
     for i in ids:
@@ -375,12 +381,29 @@ def serve_main(args) -> None:
         srv.release(i)
     srv.set_config(count_rays=False)
     sub_samples = sum(j["width"] * j["height"] * j["spp"] for j in sub)
-    alg_bytes = (NODE_BYTES * st_cnt["node_fetches"] + TRI_BYTES * st_cnt["tri_fetches"] +
-                 RAY_BYTES * st_cnt["rays"] + FILM_BYTES * sub_samples)
-    kernel_s = st_time["busy_ms"] / 1e3
-    achieved = alg_bytes / kernel_s / 1e9
-    rays_per_sample = st_cnt["rays"] / sub_samples
-    tr = measured_traffic("serve_kernel@serving_mix")
+    lane_samples = st_cnt["lane_samples"]
+    pers_samples = sub_samples - lane_samples
+    stage_names = ["trace_queue_kernel (wavefront lanes' closest-hit traversal)",
+                   "shade_kernel (wavefront lanes' shading)", "shadow_kernel (wavefront lanes' any-hit traversal)"]
+    cands = [("serve_kernel (persistent multi-job megakernel)", st_time["busy_ms"])]
+    cands += [(stage_names[k], st_time["lane_stage_ms"][k]) for k in range(3)]
+    kernel, kernel_ms = max(cands, key=lambda c: c[1])
+    rays_total = st_cnt["rays"] + sum(st_cnt["lane_rays"])
+    rays_per_sample = rays_total / sub_samples
+    if kernel.startswith("serve_kernel"):
+        alg_bytes = (NODE_BYTES * st_cnt["node_fetches"] + TRI_BYTES * st_cnt["tri_fetches"] +
+                     RAY_BYTES * st_cnt["rays"] + FILM_BYTES * pers_samples)
+        unit_samples, launches = pers_samples, None
+        tr = measured_traffic("serve_kernel@serving_mix") if not lane_samples else None
+    else:
+        k = stage_names.index(kernel)
+        trav = lambda t: (NODE_BYTES * st_cnt["lane_node_fetches"][t] + TRI_BYTES * st_cnt["lane_tri_fetches"][t] +
+                          RAY_BYTES * st_cnt["lane_rays"][t])
+        alg_bytes = [trav(0), SHADE_BYTES * st_cnt["lane_rays"][0] + RAY_BYTES * st_cnt["lane_rays"][1],
+                     trav(1) + 32 * st_cnt["lane_rays"][1]][k]
+        unit_samples, launches = lane_samples, st_time["lane_stage_launches"][k]
+        tr = None  # no PMC run of the lanes
+    achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     line = {
         "metric": METRIC,
         "value": round(batched_msps, 3),
@@ -431,13 +454,21 @@ def serve_main(args) -> None:
         "max_active_jobs": st_sat["max_active"], "hbm_peak_bytes": st_sat["hbm_peak"],
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": round(tr["hbm_bytes_per_sample"] * sub_samples) if tr else None,
+                     "traffic": round(tr["hbm_bytes_per_sample"] * unit_samples) if tr else None,
                      "traffic_unit": "HBM bytes over the counting jobs, the same unit as the algorithmic bytes",
-                     "kernel": "serve_kernel (persistent multi-job megakernel)",
-                     "kernel_ms": round(st_time["busy_ms"], 3), "kernel_jobs": len(sub),
+                     "kernel": kernel, "kernel_ms": round(kernel_ms, 3), "kernel_jobs": len(sub),
+                     "launches": launches,
+                     "avg_launch_ms": round(kernel_ms / launches, 4) if launches else None,
                      "algorithmic_bytes": int(alg_bytes),
-                     "algorithmic_bytes_per_sample": round(alg_bytes / sub_samples, 1),
-                     "hbm_bytes_per_sample": tr["hbm_bytes_per_sample"] if tr else None},
+                     "algorithmic_bytes_per_sample": round(alg_bytes / max(unit_samples, 1), 1),
+                     "hbm_bytes_per_sample": tr["hbm_bytes_per_sample"] if tr else None,
+                     "candidates_ms": {name.split()[0]: round(ms, 3) for name, ms in cands},
+                     "note": ("the lanes' stages run on three streams beside each other and the persistent "
+                              "kernel; a lane launch's event time includes the time it shares the device, so "
+                              "achieved is a lower bound on the stage's rate alone") if lane_samples else None},
+        "lane_min_samples": args.lane_min_samples,
+        "lane_sample_share": round(st_sat["lane_samples"] / max(st_sat["samples_done"], 1), 4),
+        "lane_samples_all_batches": lane_total[0],
         "wavefront_lane_routes": lane_pairs,
     }
     if not args.no_cpu:
@@ -456,6 +487,7 @@ def serve_main(args) -> None:
     if not args.no_configs:
         # the launch shape the serving run used (autotuned or given)
         shape = {k: line["config"][k] for k in ("tile_w", "waves_per_cu", "min_waves_per_eu")}
+        shape["lane_min_samples"] = args.lane_min_samples
         line["mixed_material"] = serve_mixed(args, shape)
         line["configs"] = other_configs(args)
     print(json.dumps(line))
@@ -677,6 +709,8 @@ def main():
     ap.add_argument("--tile-w", type=int, default=8)
     ap.add_argument("--waves-per-cu", type=int, default=0)
     ap.add_argument("--min-waves-per-eu", type=int, default=0, help="serve: kernel register budget (0 or 6)")
+    ap.add_argument("--lane-min-samples", type=int, default=1_000_000,
+                    help="serve: jobs of at least this many samples render on the wavefront lanes (0: by route only)")
     ap.add_argument("--check-jobs", type=int, default=16, help="serve: jobs compared with their plain film")
     ap.add_argument("--count-jobs", type=int, default=100, help="serve: jobs in the ray-counting run")
     ap.add_argument("--autotune", type=int, default=48, help="serve: probe jobs for the launch-shape autotuner "

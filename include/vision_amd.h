@@ -354,6 +354,8 @@ typedef struct vs_server_config {
     uint32_t min_waves_per_eu; /* register budget of the kernels: 0 the compiler's choice, 6 the measured
                                   budget: at least six resident waves per SIMD for the diffuse-family
                                   kernels, four for the every-material ones (spilling what does not fit) */
+    uint64_t lane_min_samples; /* jobs of at least this many samples (width * height * spp) render on the
+                                  wavefront lane whatever their route; 0: by route only */
 } vs_server_config;
 
 typedef struct vs_job_desc {
@@ -395,6 +397,14 @@ typedef struct vs_server_stats {
     uint32_t queued;       /* waiting for admission now */
     uint32_t pad;
     uint64_t hbm_budget, hbm_peak;
+    /* the wavefront lanes: samples rendered there, device time and launches
+     * of the closest-hit [0], shading [1] and any-hit [2] kernels, and (when
+     * count_rays) rays / node / triangle fetches of the closest-hit [0] and
+     * any-hit [1] traversals */
+    uint64_t lane_samples;
+    double lane_stage_ms[3];
+    uint64_t lane_stage_launches[3];
+    uint64_t lane_rays[2], lane_node_fetches[2], lane_tri_fetches[2];
 } vs_server_stats;
 
 /* Create a server on ctx's device (cfg NULL: defaults).  The worker thread

@@ -80,7 +80,8 @@ def _run(gpu_ctx, served, jobs, staggered=False, routes=(), **cfg):
         for jid, (name, w, h, spp, md, spectral, dim) in zip(ids, jobs):
             info = srv.info(jid)
             assert info["state"] == "done"
-            assert info["route"] == ("wavefront" if (name, spectral) in routes else "megakernel")
+            lane = (name, spectral) in routes or 0 < cfg.get("lane_min_samples", 0) <= w * h * spp
+            assert info["route"] == ("wavefront" if lane else "megakernel")
             res.append((srv.result(jid, 0, (h, w)), srv.result(jid, 1, (h, w))))
             srv.release(jid)
         stats = srv.stats()
@@ -177,3 +178,23 @@ def test_wavefront_lane_matches_the_plain_render(gpu_ctx, served):
         np.testing.assert_array_equal(a.view(np.uint32), c.view(np.uint32))
         np.testing.assert_array_equal(b.view(np.uint32), d.view(np.uint32))
     assert stats["jobs_done"] == len(JOBS)
+
+
+def test_large_jobs_go_to_the_lanes_and_match(gpu_ctx, served):
+    """Jobs of lane_min_samples samples or more render on the wavefront
+    lanes (several at once) and give the same films; the lane statistics
+    count their samples, stage times and, in a counting run, their rays."""
+    base, _ = _run(gpu_ctx, served, JOBS)
+    lane_min = 64 * 64 * 2
+    got, stats = _run(gpu_ctx, served, JOBS, lane_min_samples=lane_min)
+    for (a, b), (c, d) in zip(base, got):
+        np.testing.assert_array_equal(a.view(np.uint32), c.view(np.uint32))
+        np.testing.assert_array_equal(b.view(np.uint32), d.view(np.uint32))
+    big = sum(w * h * spp for _, w, h, spp, *_ in JOBS if w * h * spp >= lane_min)
+    assert 0 < big < stats["samples_done"]
+    assert stats["lane_samples"] == big
+    assert all(ms > 0 for ms in stats["lane_stage_ms"]) and all(n > 0 for n in stats["lane_stage_launches"])
+    _, cnt = _run(gpu_ctx, served, JOBS, lane_min_samples=lane_min, count_rays=True)
+    assert cnt["lane_rays"][0] >= big  # at least one camera ray per sample
+    assert cnt["lane_node_fetches"][0] > cnt["lane_rays"][0] > 0 and cnt["lane_tri_fetches"][0] > 0
+    assert cnt["lane_rays"][1] > 0 and cnt["lane_node_fetches"][1] > 0  # shadow rays of the lit scenes

@@ -85,7 +85,7 @@ class SceneArrays(C.Structure):
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_jobs", C.c_uint32), ("step_ms", C.c_float),
                 ("tile_w", C.c_uint32), ("waves_per_cu", C.c_uint32), ("use_graphs", C.c_int32),
-                ("count_rays", C.c_int32), ("min_waves_per_eu", C.c_uint32)]
+                ("count_rays", C.c_int32), ("min_waves_per_eu", C.c_uint32), ("lane_min_samples", C.c_uint64)]
 
 
 class JobDesc(C.Structure):
@@ -104,7 +104,10 @@ class ServerStats(C.Structure):
                 [(n, C.c_double) for n in ("busy_ms", "wall_ms", "msamples_per_s", "latency_p50_ms",
                                            "latency_p99_ms", "latency_mean_ms", "latency_max_ms")] +
                 [(n, C.c_uint32) for n in ("max_active", "active", "queued", "pad")] +
-                [("hbm_budget", C.c_uint64), ("hbm_peak", C.c_uint64)])
+                [("hbm_budget", C.c_uint64), ("hbm_peak", C.c_uint64), ("lane_samples", C.c_uint64),
+                 ("lane_stage_ms", C.c_double * 3), ("lane_stage_launches", C.c_uint64 * 3),
+                 ("lane_rays", C.c_uint64 * 2), ("lane_node_fetches", C.c_uint64 * 2),
+                 ("lane_tri_fetches", C.c_uint64 * 2)])
 
 
 class AutotuneResult(C.Structure):
@@ -529,15 +532,15 @@ class Server:
     an HBM budget, captured step graphs."""
 
     _CFG = ("hbm_budget", "max_jobs", "step_ms", "tile_w", "waves_per_cu", "use_graphs", "count_rays",
-            "min_waves_per_eu")
+            "min_waves_per_eu", "lane_min_samples")
 
     def __init__(self, ctx: "Context", hbm_budget: int = 0, max_jobs: int = 0, step_ms: float = 0.0,
                  tile_w: int = 0, waves_per_cu: int = 0, use_graphs: bool = True, count_rays: bool = False,
-                 min_waves_per_eu: int = 0):
+                 min_waves_per_eu: int = 0, lane_min_samples: int = 0):
         self.ctx = ctx
         self._h = _P()
         cfg = ServerConfig(hbm_budget, max_jobs, step_ms, tile_w, waves_per_cu, 1 if use_graphs else 0,
-                           1 if count_rays else 0, min_waves_per_eu)
+                           1 if count_rays else 0, min_waves_per_eu, lane_min_samples)
         _server_check(lib().vs_server_create(ctx._h, C.byref(cfg), C.byref(self._h)))
         self.scenes: list = []
 
@@ -590,7 +593,8 @@ class Server:
     def stats(self) -> dict:
         s = ServerStats()
         _server_check(lib().vs_server_stats_get(self._h, C.byref(s)))
-        return {f: getattr(s, f) for f, _ in ServerStats._fields_ if f != "pad"}
+        out = {f: getattr(s, f) for f, _ in ServerStats._fields_ if f != "pad"}
+        return {f: (list(v) if not isinstance(v, (int, float)) else v) for f, v in out.items()}
 
     def reset_stats(self):
         _server_check(lib().vs_server_reset_stats(self._h))

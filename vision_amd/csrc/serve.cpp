@@ -158,11 +158,21 @@ struct vs_server {
     std::deque<StepRec> inflight;
     std::vector<hipEvent_t> free_events;
     std::vector<JobRec *> copying;
-    // wavefront lane: its own thread, stream and path buffers; jobs routed to
-    // it render alone in the wavefront form beside the persistent steps
-    std::thread lane;
-    hipStream_t st_lane{nullptr};
-    WavefrontBuffers lane_wb;
+    // wavefront lanes: each its own thread, stream and path buffers; jobs
+    // routed to them render one at a time per lane in the wavefront form
+    // beside the persistent steps; several lanes fill each other's
+    // per-bounce host waits and short last bounces (measured on the serving
+    // mix, jobs of 1M samples and more on the lanes: one lane 583, two 635,
+    // three 651 Msamples/s)
+    static constexpr int kLanes = 3;
+    std::thread lane[kLanes];
+    hipStream_t st_lane[kLanes]{};
+    WavefrontBuffers lane_wb[kLanes];
+    hipEvent_t lane_ev[kLanes][4]{};// stage timing of each lane's bounces
+    DevBuf lane_ctr[kLanes];         // counting runs: 8 counters per lane
+    // lane totals (under mu)
+    uint64_t lane_samples{0}, lane_stage_launches[3]{}, lane_counts[6]{};
+    double lane_stage_ms[3]{};
     std::deque<JobRec *> lane_queue;   // under mu
     std::vector<JobRec *> lane_done;   // under mu
     std::condition_variable cv_lane;
@@ -389,6 +399,9 @@ void admit(vs_server *S) {
             const SceneSlot &sc = S->scenes[j->d.scene];
             cam = sc.cam;
             route = sc.route[j->d.spectral ? 1 : 0];
+            if (S->cfg.lane_min_samples &&
+                uint64_t(j->d.width) * j->d.height * j->d.spp >= S->cfg.lane_min_samples)
+                route = 1;
         }
         vs::ServeJob &sj = S->h_jobs[j->slot];
         std::memset(&sj, 0, sizeof(sj));
@@ -562,9 +575,9 @@ void retire_lane(vs_server *S) {
     }
 }
 
-// the wavefront lane: one job at a time, all its frames, output pass and
+// a wavefront lane: one job at a time, all its frames, output pass and
 // download on the lane's stream; the worker retires it
-void lane_main(vs_server *S) {
+void lane_main(vs_server *S, int lane) {
     (void)hipSetDevice(S->ctx->device);
     while (true) {
         JobRec *j = nullptr;
@@ -588,17 +601,36 @@ void lane_main(vs_server *S) {
             float4 *rt = reinterpret_cast<float4 *>(static_cast<char *>(S->arena_mem.ptr) + j->off);
             float4 *out = rt + px;
             camera_set_resolution(cam, j->d.width, j->d.height);
-            // frames per chunk: up to 16 M paths of state at a time
-            const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(16u, (16u << 20) / std::max(px, 1u)));
-            hip_check(hipStreamWaitEvent(S->st_lane, j->ev_ready, 0), "lane wait");
+            // frames per chunk: up to 32 M paths of state (5.9 GB) at a time
+            const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(64u, (32u << 20) / std::max(px, 1u)));
+            hipStream_t ls = S->st_lane[lane];
+            hip_check(hipStreamWaitEvent(ls, j->ev_ready, 0), "lane wait");
+            unsigned long long *ctr = nullptr;
+            if (S->cfg.count_rays) {
+                ctr = S->lane_ctr[lane].as<unsigned long long>();
+                hip_check(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), ls), "lane counters");
+            }
+            vs_render_stats ts{};
             wavefront_render(view, cam, j->rv, !j->d.spectral, rt, out, 0, j->d.spp, chunk,
-                             128, S->lane_wb, S->st_lane, nullptr, nullptr, nullptr);
-            hip_check(vs::serve_output(j->rv, rt, out, px, S->st_lane), "output pass");
-            hip_check(hipMemcpyAsync(j->h_rt, rt, size_t(px) * 16, hipMemcpyDeviceToHost, S->st_lane), "result D2H");
-            hip_check(hipMemcpyAsync(j->h_out, out, size_t(px) * 16, hipMemcpyDeviceToHost, S->st_lane), "result D2H");
-            hip_check(hipStreamSynchronize(S->st_lane), "lane sync");
+                             64, S->lane_wb[lane], ls, ctr, &ts, S->lane_ev[lane]);
+            unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+            if (ctr) hip_check(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, ls), "lane counters D2H");
+            hip_check(vs::serve_output(j->rv, rt, out, px, ls), "output pass");
+            hip_check(hipMemcpyAsync(j->h_rt, rt, size_t(px) * 16, hipMemcpyDeviceToHost, ls), "result D2H");
+            hip_check(hipMemcpyAsync(j->h_out, out, size_t(px) * 16, hipMemcpyDeviceToHost, ls), "result D2H");
+            hip_check(hipStreamSynchronize(ls), "lane sync");
             std::lock_guard<std::mutex> lk(S->mu);
             j->frames_done = j->d.spp;
+            S->lane_samples += uint64_t(px) * j->d.spp;
+            for (int k = 0; k < 3; ++k) {
+                S->lane_stage_ms[k] += ts.stage_ms[k];
+                S->lane_stage_launches[k] += ts.stage_launches[k];
+            }
+            // [0..2] closest-hit rays / nodes / triangles, [4..6] any-hit
+            for (int k = 0; k < 3; ++k) {
+                S->lane_counts[k] += c[k];
+                S->lane_counts[3 + k] += c[4 + k];
+            }
         } catch (const std::exception &e) {
             std::lock_guard<std::mutex> lk(S->mu);
             j->error = e.what();
@@ -726,9 +758,13 @@ vs_status vs_server_create(vs_context *ctx, const vs_server_config *cfg, vs_serv
         S->d_counters.alloc(8 * sizeof(unsigned long long));
         hip_check(hipMemset(S->d_counters.ptr, 0, 8 * sizeof(unsigned long long)), "counter clear");
         S->slot_job.assign(S->cfg.max_jobs, nullptr);
-        hip_check(hipStreamCreateWithFlags(&S->st_lane, hipStreamNonBlocking), "stream");
+        for (auto &ls : S->st_lane) hip_check(hipStreamCreateWithFlags(&ls, hipStreamNonBlocking), "stream");
+        for (int l = 0; l < vs_server::kLanes; ++l) {
+            for (auto &e : S->lane_ev[l]) hip_check(hipEventCreate(&e), "lane event");
+            S->lane_ctr[l].alloc(8 * sizeof(unsigned long long));
+        }
         S->worker = std::thread(worker_main, S.get());
-        S->lane = std::thread(lane_main, S.get());
+        for (int l = 0; l < vs_server::kLanes; ++l) S->lane[l] = std::thread(lane_main, S.get(), l);
     });
     if (st != VS_OK) {
         vs_server *raw = S.release();
@@ -928,6 +964,16 @@ vs_status vs_server_stats_get(vs_server *S, vs_server_stats *o) {
         o->active = running;
         o->hbm_budget = S->cfg.hbm_budget;
         o->hbm_peak = S->arena.peak();
+        o->lane_samples = S->lane_samples;
+        for (int k = 0; k < 3; ++k) {
+            o->lane_stage_ms[k] = S->lane_stage_ms[k];
+            o->lane_stage_launches[k] = S->lane_stage_launches[k];
+        }
+        for (int k = 0; k < 2; ++k) {
+            o->lane_rays[k] = S->lane_counts[3 * k];
+            o->lane_node_fetches[k] = S->lane_counts[3 * k + 1];
+            o->lane_tri_fetches[k] = S->lane_counts[3 * k + 2];
+        }
     }
     if (S->cfg.count_rays) {
         unsigned long long c[8];
@@ -950,6 +996,12 @@ vs_status vs_server_reset_stats(vs_server *S) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->n_submitted = S->n_done = S->n_failed = S->samples_done = S->graph_launches = S->steps = 0;
         S->busy_ms = 0.0;
+        S->lane_samples = 0;
+        for (int k = 0; k < 3; ++k) {
+            S->lane_stage_ms[k] = 0.0;
+            S->lane_stage_launches[k] = 0;
+        }
+        for (auto &c : S->lane_counts) c = 0;
         S->have_first = false;
         S->latencies.clear();
         S->max_active = 0;
@@ -1126,7 +1178,8 @@ void vs_server_destroy(vs_server *S) {
         std::lock_guard<std::mutex> lk(S->mu);
         S->cv_lane.notify_all();
     }
-    if (S->lane.joinable()) S->lane.join();
+    for (auto &t : S->lane)
+        if (t.joinable()) t.join();
     if (S->ctx && S->ctx->device >= 0) {
         (void)hipSetDevice(S->ctx->device);
         for (auto &s : S->st)
@@ -1152,10 +1205,14 @@ void vs_server_destroy(vs_server *S) {
         for (auto &s : S->st)
             if (s) (void)hipStreamDestroy(s);
         if (S->st_copy) (void)hipStreamDestroy(S->st_copy);
-        if (S->st_lane) {
-            (void)hipStreamSynchronize(S->st_lane);
-            (void)hipStreamDestroy(S->st_lane);
+        for (auto &ls : S->st_lane) {
+            if (!ls) continue;
+            (void)hipStreamSynchronize(ls);
+            (void)hipStreamDestroy(ls);
         }
+        for (auto &evs : S->lane_ev)
+            for (auto &e : evs)
+                if (e) (void)hipEventDestroy(e);
     }
     delete S;
 }
