@@ -402,7 +402,7 @@ def serve_main(args) -> None:
         alg_bytes = [trav(0), SHADE_BYTES * st_cnt["lane_rays"][0] + RAY_BYTES * st_cnt["lane_rays"][1],
                      trav(1) + 32 * st_cnt["lane_rays"][1]][k]
         unit_samples, launches = lane_samples, st_time["lane_stage_launches"][k]
-        tr = None  # no PMC run of the lanes
+        tr = measured_traffic(kernel.split()[0] + "@serving_mix_lanes")
     achieved = alg_bytes / (kernel_ms / 1e3) / 1e9
     line = {
         "metric": METRIC,
