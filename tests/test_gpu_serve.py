@@ -79,6 +79,8 @@ def _run(gpu_ctx, served, jobs, staggered=False, routes=(), **cfg):
         res = []
         for jid, (name, w, h, spp, md, spectral, dim) in zip(ids, jobs):
             info = srv.info(jid)
+            if info["state"] != "done":
+                srv.wait(jid)  # raises with the job's error
             assert info["state"] == "done"
             lane = (name, spectral) in routes or 0 < cfg.get("lane_min_samples", 0) <= w * h * spp
             assert info["route"] == ("wavefront" if lane else "megakernel")

@@ -694,7 +694,8 @@ namespace {
 }// extern "C"
 
 uint32_t vs_internal::wavefront_render(const vs::SceneView &view, const vs::CameraView &cam,
-                                       const vs::RenderView &rv0, bool rgb, float4 *rt, float4 *out,
+                                       const vs::RenderView &rv0, bool rgb, bool diffuse_only, float4 *rt,
+                                       float4 *out,
                                        uint32_t frame_index, uint32_t spp, uint32_t chunk_frames,
                                        uint32_t trace_block, WavefrontBuffers &wb, hipStream_t st,
                                        unsigned long long *ctr, vs_render_stats *ts, hipEvent_t *ev) {
@@ -748,7 +749,7 @@ uint32_t vs_internal::wavefront_render(const vs::SceneView &view, const vs::Came
             if (timed) hip_check(hipEventRecord(ev[0], st), "event");
             hip_check(vs::wf_trace(view, w, q[cur], count, ctr, trace_block, st), "wf trace");
             if (timed) hip_check(hipEventRecord(ev[1], st), "event");
-            hip_check(vs::wf_shade(view, rv0, w, dim, rgb, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
+            hip_check(vs::wf_shade(view, rv0, w, dim, rgb, diffuse_only ? 1 : 0, q[cur], count, q[cur ^ 1], counts, st), "wf shade");
             if (timed) hip_check(hipEventRecord(ev[2], st), "event");
             hip_check(vs::wf_shadow(view, w, counts, count, ctr ? ctr + 4 : nullptr, trace_block, st), "wf shadow");
             if (timed) hip_check(hipEventRecord(ev[3], st), "event");
@@ -783,7 +784,8 @@ namespace {
 uint32_t render_wavefront(vs_scene *s, const vs::RenderView &rv0, uint32_t spp, unsigned long long *ctr,
                           vs_render_stats &ts) {
     hipEvent_t ev[4] = {s->ctx->tr0, s->ctx->tr1, s->ctx->tr2, s->ctx->ev1};
-    return wavefront_render(s->view, s->cam, rv0, s->packed.rgb, s->film_rt.as<float4>(), s->film_out.as<float4>(),
+    return wavefront_render(s->view, s->cam, rv0, s->packed.rgb, s->diffuse_only, s->film_rt.as<float4>(),
+                            s->film_out.as<float4>(),
                             s->frame_index, spp, s->wf_chunk_frames, s->wf_trace_block, s->wf, s->ctx->stream, ctr,
                             &ts, ev);
 }

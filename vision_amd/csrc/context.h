@@ -226,7 +226,7 @@ inline void require_built(vs_scene *s) {
 // Synchronises `st` once per bounce (the live count).  Returns launches;
 // adds per-stage times to *ts when it and the three events are given.
 uint32_t wavefront_render(const vs::SceneView &view, const vs::CameraView &cam, const vs::RenderView &rv0,
-                          bool rgb, float4 *rt, float4 *out, uint32_t frame_index, uint32_t spp, uint32_t chunk,
+                          bool rgb, bool diffuse_only, float4 *rt, float4 *out, uint32_t frame_index, uint32_t spp, uint32_t chunk,
                           uint32_t trace_block, WavefrontBuffers &wb, hipStream_t st, unsigned long long *ctr,
                           vs_render_stats *ts, hipEvent_t *ev4);
 

@@ -591,11 +591,13 @@ void lane_main(vs_server *S, int lane) {
         try {
             vs::SceneView view;
             vs::CameraView cam;
+            bool diffuse_only;
             {
                 std::lock_guard<std::mutex> lk(S->mu);
                 const SceneSlot &sc = S->scenes[j->d.scene];
                 view = sc.view[j->d.spectral ? 1 : 0];
                 cam = sc.cam;
+                diffuse_only = sc.diffuse_only;
             }
             const uint32_t px = j->d.width * j->d.height;
             float4 *rt = reinterpret_cast<float4 *>(static_cast<char *>(S->arena_mem.ptr) + j->off);
@@ -611,7 +613,7 @@ void lane_main(vs_server *S, int lane) {
                 hip_check(hipMemsetAsync(ctr, 0, 8 * sizeof(unsigned long long), ls), "lane counters");
             }
             vs_render_stats ts{};
-            wavefront_render(view, cam, j->rv, !j->d.spectral, rt, out, 0, j->d.spp, chunk,
+            wavefront_render(view, cam, j->rv, !j->d.spectral, diffuse_only, rt, out, 0, j->d.spp, chunk,
                              64, S->lane_wb[lane], ls, ctr, &ts, S->lane_ev[lane]);
             unsigned long long c[8] = {0, 0, 0, 0, 0, 0, 0, 0};
             if (ctr) hip_check(hipMemcpyAsync(c, ctr, sizeof(c), hipMemcpyDeviceToHost, ls), "lane counters D2H");

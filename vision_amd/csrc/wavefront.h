@@ -37,9 +37,11 @@ hipError_t wf_begin(const CameraView &cam, const RenderView &rv, const Wavefront
                     bool rgb, uint32_t frame0, uint32_t n, uint32_t *queue, hipStream_t stream);
 hipError_t wf_trace(const SceneView &sc, const WavefrontView &w, const uint32_t *queue, uint32_t count,
                     unsigned long long *counters, uint32_t block, hipStream_t stream);
+// material_set: 0 every material, 1 the diffuse family only (a smaller
+// kernel for scenes of diffuse / Oren-Nayar / black materials)
 hipError_t wf_shade(const SceneView &sc, const RenderView &rv, const WavefrontView &w, uint32_t spectrum_dim,
-                    bool rgb, const uint32_t *qin, uint32_t count, uint32_t *qout, uint32_t *counts,
-                    hipStream_t stream);
+                    bool rgb, int material_set, const uint32_t *qin, uint32_t count, uint32_t *qout,
+                    uint32_t *counts, hipStream_t stream);
 hipError_t wf_shadow(const SceneView &sc, const WavefrontView &w, const uint32_t *counts, uint32_t max_count,
                      unsigned long long *counters, uint32_t block, hipStream_t stream);
 hipError_t wf_zero(uint32_t *counts, hipStream_t stream);

@@ -122,8 +122,14 @@ __device__ __forceinline__ uint32_t shade_key(const SceneView &sc, uint32_t tri)
     return 2u + min(sc.mat[m].type, kShadeKeys - 3u);
 }
 
-template <int NS, bool RGB>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(3))) shade_kernel(SceneView sc, RenderView rv, WavefrontView w,
+// register budget of the shading kernel, in waves per SIMD: three for every
+// material (measured on the procedural job: 2 -> 97.1, 3 -> 93.5, 4 -> 105.0
+// ms of shading per 64-frame job), more for the diffuse family, whose
+// smaller code needs fewer registers
+constexpr int shade_waves(int ms) { return ms == MS_ALL ? 3 : 4; }
+
+template <int NS, bool RGB, int MS>
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(shade_waves(MS)))) shade_kernel(SceneView sc, RenderView rv, WavefrontView w,
                                                    const uint32_t *__restrict__ qin, uint32_t count,
                                                    uint32_t *__restrict__ qout, uint32_t *__restrict__ counts) {
     __shared__ uint32_t s_n[2], s_base[2], s_key[kShadeKeys];
@@ -170,7 +176,7 @@ __global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_e
         h.tri = __float_as_uint(hv.y);
         h.b1 = hv.z;
         h.b2 = hv.w;
-        path_shade<NS, RGB>(sc, rv, ps, h, sr);
+        path_shade<NS, RGB, MS>(sc, rv, ps, h, sr);
         store_state<NS, RGB>(w, i, ps);
         live = ps.alive != 0u;
     }
@@ -268,18 +274,30 @@ hipError_t wf_trace(const SceneView &sc, const WavefrontView &w, const uint32_t 
     return hipGetLastError();
 }
 
-hipError_t wf_shade(const SceneView &sc, const RenderView &rv, const WavefrontView &w, uint32_t spectrum_dim,
+namespace {
+template <int MS>
+hipError_t shade_ms(const SceneView &sc, const RenderView &rv, const WavefrontView &w, uint32_t spectrum_dim,
                     bool rgb, const uint32_t *qin, uint32_t count, uint32_t *qout, uint32_t *counts,
                     hipStream_t stream) {
-    if (count == 0) return hipSuccess;
     dim3 g(blocks(count, kShadeBlock)), b(kShadeBlock);
-    if (rgb) hipLaunchKernelGGL((shade_kernel<3, true>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
+    if (rgb) hipLaunchKernelGGL((shade_kernel<3, true, MS>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
     else if (spectrum_dim == 4)
-        hipLaunchKernelGGL((shade_kernel<4, false>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
+        hipLaunchKernelGGL((shade_kernel<4, false, MS>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
     else if (spectrum_dim == 3)
-        hipLaunchKernelGGL((shade_kernel<3, false>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
+        hipLaunchKernelGGL((shade_kernel<3, false, MS>), g, b, 0, stream, sc, rv, w, qin, count, qout, counts);
     else return hipErrorInvalidValue;
     return hipGetLastError();
+}
+}// namespace
+
+hipError_t wf_shade(const SceneView &sc, const RenderView &rv, const WavefrontView &w, uint32_t spectrum_dim,
+                    bool rgb, int material_set, const uint32_t *qin, uint32_t count, uint32_t *qout,
+                    uint32_t *counts, hipStream_t stream) {
+    if (count == 0) return hipSuccess;
+    if (material_set == MS_DIFFUSE)
+        return shade_ms<MS_DIFFUSE>(sc, rv, w, spectrum_dim, rgb, qin, count, qout, counts, stream);
+    if (material_set != MS_ALL) return hipErrorInvalidValue;
+    return shade_ms<MS_ALL>(sc, rv, w, spectrum_dim, rgb, qin, count, qout, counts, stream);
 }
 
 hipError_t wf_shadow(const SceneView &sc, const WavefrontView &w, const uint32_t *counts, uint32_t max_count,
