@@ -108,7 +108,6 @@ struct JobRec {
     vs::RenderView rv{};
     float *h_rt{nullptr}, *h_out{nullptr};// pinned host copies of the films
     hipEvent_t ev_copy{nullptr};
-    hipEvent_t ev_ready{nullptr};// film cleared (wavefront lane)
     uint32_t route{0};           // 0 persistent megakernel, 1 wavefront lane
     std::string error;
 };
@@ -412,19 +411,20 @@ void admit(vs_server *S) {
         sj.scene = 2u * j->d.scene + (j->d.spectral ? 1u : 0u);
         sj.tiles_x = (j->d.width + S->cfg.tile_w - 1) / S->cfg.tile_w;
         uint32_t px = j->d.width * j->d.height;
-        hip_check(vs::serve_clear(sj.rt, px, S->st[0]), "film clear");
-        hip_check(hipMemcpyAsync(S->d_jobs.as<vs::ServeJob>() + j->slot, &sj, sizeof(sj), hipMemcpyHostToDevice,
-                                 S->st[0]),
-                  "job upload");
         j->route = route;
         if (j->route == 1) {
-            j->ev_ready = take_event(S);
-            hip_check(hipEventRecord(j->ev_ready, S->st[0]), "ready event");
+            // the lane clears the film on its own stream: nothing of a lane
+            // job waits on st[0], which may be capturing a step graph (a
+            // wait on an event of a capturing stream is an error)
             std::lock_guard<std::mutex> lk(S->mu);
             S->lane_queue.push_back(j);
             S->cv_lane.notify_one();
             continue;
         }
+        hip_check(vs::serve_clear(sj.rt, px, S->st[0]), "film clear");
+        hip_check(hipMemcpyAsync(S->d_jobs.as<vs::ServeJob>() + j->slot, &sj, sizeof(sj), hipMemcpyHostToDevice,
+                                 S->st[0]),
+                  "job upload");
         S->active.push_back(j);
         std::lock_guard<std::mutex> lk(S->mu);
         S->max_active = std::max<uint32_t>(S->max_active, uint32_t(S->active.size()));
@@ -566,10 +566,6 @@ void retire_lane(vs_server *S) {
         done.swap(S->lane_done);
     }
     for (JobRec *j : done) {
-        if (j->ev_ready) {
-            S->free_events.push_back(j->ev_ready);
-            j->ev_ready = nullptr;
-        }
         std::string err = j->error;
         finish_job(S, j, err.empty(), err);
     }
@@ -606,7 +602,7 @@ void lane_main(vs_server *S, int lane) {
             // frames per chunk: up to 32 M paths of state (5.9 GB) at a time
             const uint32_t chunk = std::max<uint32_t>(1u, std::min<uint32_t>(64u, (32u << 20) / std::max(px, 1u)));
             hipStream_t ls = S->st_lane[lane];
-            hip_check(hipStreamWaitEvent(ls, j->ev_ready, 0), "lane wait");
+            hip_check(vs::serve_clear(rt, px, ls), "film clear");
             unsigned long long *ctr = nullptr;
             if (S->cfg.count_rays) {
                 ctr = S->lane_ctr[lane].as<unsigned long long>();
